@@ -1,0 +1,84 @@
+"""ctypes bindings of the native libraries of the package.
+
+* ``libs2l_setup.so``: host setup stand-in (generators, partition, local factorizations),
+  declared in ``include/s2l_setup.h``.
+* ``libs2l_hip.so``: the MI355X apply path (device context, uploads, HIP kernels, device
+  Krylov loops), declared in ``include/s2l_hip.h``.
+
+Both are built in-tree by ``__graft_entry__.build()``. A missing library raises
+``RuntimeError`` at first use: there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class GenOpts(ctypes.Structure):
+    """Mirror of ``s2l_gen_opts`` (include/s2l_setup.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("nu", ctypes.c_double), ("ncomp", ctypes.c_int), ("time", ctypes.c_double),
+        ("pulse", ctypes.c_double), ("pulse_period", ctypes.c_int),
+        ("rho_ratio", ctypes.c_double), ("nbubbles", ctypes.c_int), ("radius", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("buoyancy", ctypes.c_int), ("seed", ctypes.c_uint),
+        ("rmin_cells", ctypes.c_double), ("rmax_cells", ctypes.c_double),
+        ("contrast", ctypes.c_double), ("channel_fraction", ctypes.c_double),
+        ("aniso", ctypes.c_double),
+    ]
+
+
+_setup = None
+_hip = None
+
+
+def _load(name):
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{name} is not built (expected at {path}); run __graft_entry__.build()")
+    return ctypes.CDLL(path)
+
+
+def setup_lib():
+    """Handle of libs2l_setup.so with argument types declared."""
+    global _setup
+    if _setup is not None:
+        return _setup
+    L = _load("libs2l_setup.so")
+    L.s2l_setup_last_error.restype = ctypes.c_char_p
+    L.s2l_gen_defaults.argtypes = [ctypes.POINTER(GenOpts)]
+    L.s2l_generate.restype = ctypes.c_void_p
+    L.s2l_generate.argtypes = [ctypes.c_char_p, ctypes.POINTER(GenOpts)]
+    L.s2l_problem_free.argtypes = [ctypes.c_void_p]
+    for f in ("s2l_problem_n", "s2l_problem_nnz"):
+        getattr(L, f).restype = ctypes.c_int64
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    for f in ("s2l_problem_symmetric", "s2l_problem_block_size", "s2l_problem_dim"):
+        getattr(L, f).restype = ctypes.c_int
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    L.s2l_problem_copy.argtypes = [ctypes.c_void_p, _c_i64p, _c_i64p, _c_dp, _c_dp, _c_dp]
+    L.s2l_random_rhs.argtypes = [ctypes.c_int64, ctypes.c_uint64, _c_dp]
+    L.s2l_random_rhs_seq.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _c_dp]
+    L.s2l_partition.restype = ctypes.c_int
+    L.s2l_partition.argtypes = [ctypes.c_int64, _c_i64p, _c_i64p, ctypes.c_int, _c_dp, ctypes.c_int, _c_i32p]
+    L.s2l_factorize.restype = ctypes.c_void_p
+    L.s2l_factorize.argtypes = [ctypes.c_int32, _c_i32p, _c_i32p, _c_dp, ctypes.c_int, ctypes.c_int]
+    L.s2l_factor_free.argtypes = [ctypes.c_void_p]
+    L.s2l_factor_info.argtypes = [ctypes.c_void_p, _c_i32p, _c_i64p, _c_i64p, _c_i32p,
+                                  ctypes.POINTER(ctypes.c_int), _c_dp]
+    L.s2l_factor_copy.argtypes = [ctypes.c_void_p, _c_i32p, _c_i64p, _c_i32p, _c_dp, _c_i64p, _c_i32p, _c_dp]
+    L.s2l_factor_supernodes.argtypes = [ctypes.c_void_p, _c_i32p]
+    _setup = L
+    return L
+
+
+def ptr(a, ctype):
+    """ctypes pointer to the data of a contiguous numpy array (None stays None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,1161 @@
+// api.hip -- device context, uploads, preconditioner application and device Krylov loops behind
+// the C-ABI of include/s2l_hip.h.
+#include "s2l_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+
+namespace s2l {
+template <typename T> const void* localSolveKernel();
+}
+
+namespace {
+
+using namespace s2l;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                         \
+    do {                                                                                                    \
+        hipError_t e_ = (x);                                                                                \
+        if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+/// Device buffer (hipMalloc), grown on demand.
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void reserve(size_t cnt)
+    {
+        if (cnt <= n && p) return;
+        release();
+        const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+        HIPCHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
+        n = std::max<size_t>(cnt, 1);
+    }
+    void upload(const T* h, size_t cnt, hipStream_t s)
+    {
+        reserve(cnt);
+        if (cnt) HIPCHECK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+/// Pinned host staging buffer.
+template <typename T> struct PinBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~PinBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void reserve(size_t cnt)
+    {
+        if (cnt <= n && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(cnt, 1) * sizeof(T), hipHostMallocDefault));
+        n = std::max<size_t>(cnt, 1);
+    }
+};
+
+// ---- dense FP64 factorizations of A_00 on the host (row-major)
+bool denseCholesky(int n, std::vector<double>& A)
+{
+    for (int j = 0; j < n; ++j) {
+        double* lj = A.data() + static_cast<size_t>(j) * n;
+        double d = lj[j];
+        for (int k = 0; k < j; ++k) d -= lj[k] * lj[k];
+        if (!(d > 0.0) || !std::isfinite(d)) return false;
+        const double ljj = std::sqrt(d);
+        lj[j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double* li = A.data() + static_cast<size_t>(i) * n;
+            double s = li[j];
+            for (int k = 0; k < j; ++k) s -= li[k] * lj[k];
+            li[j] = s / ljj;
+        }
+    }
+    return true;
+}
+
+// Inverse of the lower triangular part of L (unit diagonal if @p unit), row by row.
+std::vector<double> lowerInverse(int n, const std::vector<double>& L, bool unit)
+{
+    std::vector<double> X(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double* xi = X.data() + static_cast<size_t>(i) * n;
+        const double* li = L.data() + static_cast<size_t>(i) * n;
+        for (int k = 0; k < i; ++k) {
+            const double f = li[k];
+            if (f == 0.0) continue;
+            const double* xk = X.data() + static_cast<size_t>(k) * n;
+            for (int j = 0; j <= k; ++j) xi[j] -= f * xk[j];
+        }
+        xi[i] += 1.0;
+        const double d = unit ? 1.0 : li[i];
+        for (int j = 0; j <= i; ++j) xi[j] /= d;
+    }
+    return X;
+}
+
+bool denseLu(int n, std::vector<double>& A, std::vector<int32_t>& p)
+{
+    p.resize(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) p[static_cast<size_t>(i)] = i;
+    for (int j = 0; j < n; ++j) {
+        int piv = j;
+        double amax = std::fabs(A[static_cast<size_t>(j) * n + j]);
+        for (int i = j + 1; i < n; ++i)
+            if (std::fabs(A[static_cast<size_t>(i) * n + j]) > amax) {
+                amax = std::fabs(A[static_cast<size_t>(i) * n + j]);
+                piv = i;
+            }
+        if (amax == 0.0 || !std::isfinite(amax)) return false;
+        if (piv != j) {
+            std::swap_ranges(A.begin() + static_cast<ptrdiff_t>(j) * n, A.begin() + static_cast<ptrdiff_t>(j + 1) * n,
+                             A.begin() + static_cast<ptrdiff_t>(piv) * n);
+            std::swap(p[static_cast<size_t>(j)], p[static_cast<size_t>(piv)]);
+        }
+        const double d = A[static_cast<size_t>(j) * n + j];
+        const double* aj = A.data() + static_cast<size_t>(j) * n;
+        for (int i = j + 1; i < n; ++i) {
+            double* ai = A.data() + static_cast<size_t>(i) * n;
+            const double l = ai[j] / d;
+            ai[j] = l;
+            if (l != 0.0)
+                for (int c = j + 1; c < n; ++c) ai[c] -= l * aj[c];
+        }
+    }
+    return true;
+}
+
+} // namespace
+
+struct s2l_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t max_lds = 0;
+
+    // operator
+    int32_t n = 0, nsub = 0;
+    int64_t nnz = 0;
+    std::vector<int32_t> ownership;
+    DevBuf<int32_t> rp, col;
+    DevBuf<double> val;
+    bool system_ready = false;
+
+    // subdomains
+    std::vector<int32_t> local_ptr, gid_h;
+    std::vector<double> pou_h;
+    bool boolean = true;
+    int one_level = 0;
+    DevBuf<int32_t> rg_ptr, rg_idx;
+    DevBuf<double> pou_d;
+    bool subs_ready = false;
+
+    // local factors
+    std::vector<HostFactor> hf;
+    int precision = 0;
+    bool local_ready = false;
+    int launch_mode = 0;
+    Packed pk;
+    DevBuf<SubDesc> sub_d;
+    DevBuf<SnDesc> sn_d;
+    DevBuf<int32_t> lev_d, levsn_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
+    DevBuf<float> diag_f, below_f;
+    DevBuf<double> diag_dd, below_dd;
+    DevBuf<unsigned char> tbuf;
+    int64_t local_total = 0;
+    size_t lds_bytes = 0;
+
+    // coarse
+    int32_t n0 = 0;
+    int correction = 0;
+    bool coarse_ready = false;
+    CoarseArgs ca{};
+    std::vector<int64_t> off_h;
+    DevBuf<int64_t> off_d, voff_d;
+    DevBuf<int32_t> vrows_d, vrowptr_d, vrowg_d, blksub_d, blkrow_d, cperm_d;
+    DevBuf<double> V_d, T1_d, T2_d;
+    bool coarse_perm = false;
+
+    // work
+    DevBuf<double> x0, w, loc, cvec, tvec, yvec;
+    int work_nrhs = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    int64_t launches = 0;
+    double ms_total = 0.0;
+
+    // Krylov
+    DevBuf<double> kV, kw, kH, kcs, ksn, kg, kh, kh2, ky, khist, kpart, kr, kz, ku, kred;
+    DevBuf<GmState> kst;
+    DevBuf<int32_t> kflags;
+    DevBuf<unsigned> kcnt;
+    PinBuf<GmState> pst;
+    PinBuf<int32_t> pflags;
+    PinBuf<double> pred;
+
+    void ensureWork(int nrhs)
+    {
+        if (nrhs <= work_nrhs) return;
+        x0.reserve(static_cast<size_t>(n) * nrhs);
+        w.reserve(static_cast<size_t>(n) * nrhs);
+        loc.reserve(static_cast<size_t>(local_total) * nrhs);
+        cvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
+        tvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
+        yvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
+        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
+        tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nrhs * esz);
+        work_nrhs = nrhs;
+    }
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F> int guarded(s2l_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return S2L_OK;
+    } catch (const ApiError& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return e.code;
+    } catch (const HipError& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return S2L_EHIP;
+    } catch (const std::exception& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return S2L_EINVAL;
+    }
+}
+
+void require(bool c, int code, const std::string& m)
+{
+    if (!c) throw ApiError(code, m);
+}
+
+hipStream_t pick(s2l_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+
+// ---- local solve launch (all subdomains in one launch, or one launch per subdomain)
+template <typename T>
+void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip,
+                hipStream_t s)
+{
+    LocalArgs<T> a{};
+    a.sub = c->sub_d.p;
+    a.sn = c->sn_d.p;
+    a.lev = c->lev_d.p;
+    a.lev_sn = c->levsn_d.p;
+    if constexpr (std::is_same_v<T, float>) {
+        a.diag = c->diag_f.p;
+        a.below = c->below_f.p;
+    } else {
+        a.diag = c->diag_dd.p;
+        a.below = c->below_dd.p;
+    }
+    a.ridx = c->ridx_d.p;
+    a.cptr = c->cptr_d.p;
+    a.cidx = c->cidx_d.p;
+    a.tbuf = reinterpret_cast<T*>(c->tbuf.p);
+    a.gmap = c->gmap_d.p;
+    a.lperm = c->lperm_d.p;
+    a.rhs = rhs;
+    a.x0 = x0;
+    a.out = out;
+    a.pou = (mode == 1 && c->one_level == 0 && !c->boolean) ? c->pou_d.p : nullptr;
+    a.skip = skip;
+    a.n_global = c->n;
+    a.n_local_total = c->local_total;
+    a.t_total = c->pk.t_total;
+    a.mode = mode;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        if (c->ev_pool.size() < 2) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t e;
+                HIPCHECK(hipEventCreate(&e));
+                c->ev_pool.push_back(e);
+            }
+        }
+        e0 = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        e1 = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        HIPCHECK(hipEventRecord(e0, s));
+    }
+    if (c->launch_mode == 0) {
+        a.sub_first = 0;
+        launchLocalSolve<T>(a, c->nsub, nrhs, c->lds_bytes, s);
+    } else {
+        for (int i = 0; i < c->nsub; ++i) {
+            a.sub_first = i;
+            launchLocalSolve<T>(a, 1, nrhs, c->lds_bytes, s);
+        }
+    }
+    HIPCHECK(hipGetLastError());
+    if (c->timing) {
+        HIPCHECK(hipEventRecord(e1, s));
+        c->ev_pending.emplace_back(e0, e1);
+        c->launches += 1;
+    }
+}
+
+void collectTiming(s2l_ctx* c)
+{
+    for (auto& pr : c->ev_pending) {
+        HIPCHECK(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        c->ms_total += ms;
+        c->ev_pool.push_back(pr.first);
+        c->ev_pool.push_back(pr.second);
+    }
+    c->ev_pending.clear();
+}
+
+// one-level part: z = x0 + M1^{-1} rhs on the owned rows
+void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    const bool direct = (c->one_level == 0 && c->boolean);
+    const int mode = direct ? 0 : 1;
+    double* out = direct ? z : c->loc.p;
+    if (c->precision == 1)
+        localSolve<float>(c, rhs, x0, out, mode, nrhs, skip, s);
+    else
+        localSolve<double>(c, rhs, x0, out, mode, nrhs, skip, s);
+    if (!direct) launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, x0, z, nrhs, skip, s);
+}
+
+void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    require(c->system_ready && c->subs_ready && c->local_ready, S2L_ESTATE, "apply: upload the system, subdomains and factors first");
+    require(c->coarse_ready, S2L_ESTATE, "apply: upload the coarse space first (n0 = 0 for one level)");
+    c->ensureWork(nrhs);
+    if (c->n0 == 0) {
+        oneLevel(c, r, nullptr, z, nrhs, skip, s);
+        return;
+    }
+    // x0 = R_0^T A_00^{-1} R_0 r
+    launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, nrhs, skip, s);
+    launchTriGemv(c->n0, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
+    launchTriGemv(c->n0, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
+    if (c->boolean) {
+        launchCoarseProlong(c->ca, c->yvec.p, c->x0.p, c->n, 1, nrhs, skip, s);
+    } else {
+        launchCoarseProlong(c->ca, c->yvec.p, c->loc.p, c->local_total, 0, nrhs, skip, s);
+        launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, nullptr, c->x0.p, nrhs, skip, s);
+    }
+    if (c->correction == 1) {
+        oneLevel(c, r, c->x0.p, z, nrhs, skip, s); // z = x0 + M1^{-1} r
+    } else {
+        launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->x0.p, c->w.p, r, nrhs, skip, s); // w = r - A x0
+        oneLevel(c, c->w.p, c->x0.p, z, nrhs, skip, s);                                       // z = x0 + M1^{-1} w
+    }
+    HIPCHECK(hipGetLastError());
+}
+
+void spmvImpl(s2l_ctx* c, const double* x, double* y, const double* b, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    require(c->system_ready, S2L_ESTATE, "spmv: upload the system first");
+    launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, x, y, b, nrhs, skip, s);
+    HIPCHECK(hipGetLastError());
+}
+
+// ---- device GMRES (gmres.cpp:18-161), nrhs independent solves sharing every launch
+int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int nrhs, s2l_krylov_result* res,
+              double* history, hipStream_t s)
+{
+    const int m = o->restart;
+    require(m >= 1 && m + 1 <= kGmMaxK, S2L_ELIMIT, "GMRES restart must be in [1, 64] on the device");
+    require(o->stop_norm == 0, S2L_EINVAL, "the device Krylov loops implement the l2 stopping norm");
+    require(o->maxit >= 0, S2L_EINVAL, "maxit must be >= 0");
+    const int64_t n = c->n;
+    const int maxit = o->maxit;
+    const int nb = reduceBlocks(n);
+    c->kV.reserve(static_cast<size_t>(m + 1) * n * nrhs);
+    c->kw.reserve(static_cast<size_t>(n) * nrhs);
+    c->kr.reserve(static_cast<size_t>(n) * nrhs);
+    c->kz.reserve(static_cast<size_t>(n) * nrhs);
+    c->ku.reserve(static_cast<size_t>(n) * nrhs);
+    c->kH.reserve(static_cast<size_t>(m + 1) * m * nrhs);
+    c->kcs.reserve(static_cast<size_t>(m) * nrhs);
+    c->ksn.reserve(static_cast<size_t>(m) * nrhs);
+    c->kg.reserve(static_cast<size_t>(m + 1) * nrhs);
+    c->kh.reserve(static_cast<size_t>(m + 1) * nrhs);
+    c->kh2.reserve(static_cast<size_t>(m + 1) * nrhs);
+    c->ky.reserve(static_cast<size_t>(m) * nrhs);
+    c->khist.reserve(static_cast<size_t>(maxit + 1) * nrhs);
+    c->kpart.reserve(static_cast<size_t>(nb) * kGmMaxK * nrhs);
+    c->kred.reserve(static_cast<size_t>(3) * nrhs);
+    c->kst.reserve(static_cast<size_t>(nrhs));
+    c->kflags.reserve(static_cast<size_t>(2) * nrhs);
+    if (c->kcnt.n < static_cast<size_t>(nrhs)) {
+        c->kcnt.reserve(static_cast<size_t>(nrhs));
+        HIPCHECK(hipMemsetAsync(c->kcnt.p, 0, sizeof(unsigned) * c->kcnt.n, s));
+    }
+    c->pst.reserve(static_cast<size_t>(nrhs));
+    c->pflags.reserve(static_cast<size_t>(2) * nrhs);
+    c->pred.reserve(static_cast<size_t>(3) * nrhs);
+
+    GmArgs a{};
+    a.n = n;
+    a.m = m;
+    a.nrhs = nrhs;
+    a.V = c->kV.p;
+    a.w = c->kw.p;
+    a.H = c->kH.p;
+    a.cs = c->kcs.p;
+    a.sn = c->ksn.p;
+    a.g = c->kg.p;
+    a.h = c->kh.p;
+    a.h2 = c->kh2.p;
+    a.y = c->ky.p;
+    a.st = c->kst.p;
+    a.flags = c->kflags.p;
+    a.hist = c->khist.p;
+    a.hist_ld = maxit + 1;
+    a.rtol = o->rtol;
+    a.partial = c->kpart.p;
+    a.counter = c->kcnt.p;
+    a.nblocks = nb;
+
+    auto reduceToHost = [&](int kind, const double* x, const double* y, const int32_t* skip) {
+        launchReduce(kind, n, x, y, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+        HIPCHECK(hipMemcpyAsync(c->pred.p, c->kred.p, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    };
+
+    std::vector<double> bnorm(static_cast<size_t>(nrhs)), beta(static_cast<size_t>(nrhs));
+    std::vector<int> conv(static_cast<size_t>(nrhs), 0), iters(static_cast<size_t>(nrhs), 0);
+    std::vector<int> breakdown(static_cast<size_t>(nrhs), 0);
+    std::vector<double> rel(static_cast<size_t>(nrhs), 0.0);
+    std::vector<int32_t> idle(static_cast<size_t>(nrhs), 0);
+    std::vector<std::vector<double>> hist(static_cast<size_t>(nrhs));
+
+    reduceToHost(0, d_b, nullptr, nullptr);
+    for (int r = 0; r < nrhs; ++r) bnorm[static_cast<size_t>(r)] = std::sqrt(c->pred.p[3 * r]);
+    // bnorm == 0: x = 0, converged (gmres.cpp:101-108)
+    for (int r = 0; r < nrhs; ++r)
+        if (bnorm[static_cast<size_t>(r)] == 0.0) {
+            HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(r) * n, 0, sizeof(double) * n, s));
+            conv[static_cast<size_t>(r)] = 1;
+            hist[static_cast<size_t>(r)].push_back(0.0);
+        }
+    // r = b - A x
+    spmvImpl(c, d_x, c->kr.p, d_b, nrhs, nullptr, s);
+    reduceToHost(0, c->kr.p, nullptr, nullptr);
+    for (int r = 0; r < nrhs; ++r) {
+        if (bnorm[static_cast<size_t>(r)] == 0.0) continue;
+        beta[static_cast<size_t>(r)] = std::sqrt(c->pred.p[3 * r]);
+        rel[static_cast<size_t>(r)] = beta[static_cast<size_t>(r)] / bnorm[static_cast<size_t>(r)];
+        hist[static_cast<size_t>(r)].push_back(rel[static_cast<size_t>(r)]);
+        if (beta[static_cast<size_t>(r)] <= o->rtol * bnorm[static_cast<size_t>(r)]) conv[static_cast<size_t>(r)] = 1;
+    }
+    const int batch = 4;
+    for (;;) {
+        bool any = false;
+        for (int r = 0; r < nrhs; ++r) {
+            const bool act = !conv[static_cast<size_t>(r)] && iters[static_cast<size_t>(r)] < maxit;
+            idle[static_cast<size_t>(r)] = act ? 0 : 1;
+            any = any || act;
+            GmState& st = c->pst.p[r];
+            std::memset(&st, 0, sizeof(st));
+            st.bnorm = bnorm[static_cast<size_t>(r)];
+            st.beta = beta[static_cast<size_t>(r)];
+            st.iters = iters[static_cast<size_t>(r)];
+            st.maxit = maxit;
+            st.done = act ? 0 : 1;
+            c->pflags.p[r] = act ? 0 : 1;
+            c->pflags.p[nrhs + r] = act ? 0 : 1;
+        }
+        if (!any) break;
+        HIPCHECK(hipMemcpyAsync(c->kst.p, c->pst.p, sizeof(GmState) * nrhs, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(c->kflags.p, c->pflags.p, sizeof(int32_t) * 2 * nrhs, hipMemcpyHostToDevice, s));
+        gmStartCycle(a, c->kr.p, s);
+        const int32_t* live = c->kflags.p;
+        const int32_t* idl = c->kflags.p + nrhs;
+        for (int j0 = 0; j0 < m; j0 += batch) {
+            const int j1 = std::min(m, j0 + batch);
+            for (int j = j0; j < j1; ++j) {
+                const double* vj = c->kV.p + static_cast<int64_t>(j) * nrhs * n;
+                applyImpl(c, vj, c->kz.p, nrhs, live, s);               // z = M^{-1} v_j
+                spmvImpl(c, c->kz.p, c->kw.p, nullptr, nrhs, live, s);  // w = A z
+                gmDots(a, j, s);                                        // CGS2 pass 1
+                gmUpdateDots(a, j, s);                                  // w -= V h; pass 2
+                gmUpdateNorm(a, j, s);                                  // w -= V h2; ||w||; Givens
+                gmNormalize(a, j, s);                                   // v_{j+1} = w / ||w||
+            }
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(c->pflags.p, c->kflags.p, sizeof(int32_t) * nrhs, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            bool all = true;
+            for (int r = 0; r < nrhs; ++r) all = all && c->pflags.p[r];
+            if (all) break;
+        }
+        // x += M^{-1} V_k y_k; true residual (gmres.cpp:210-231)
+        gmLsq(a, s);
+        gmCombine(a, c->ku.p, s);
+        applyImpl(c, c->ku.p, c->kz.p, nrhs, idl, s);
+        launchAxpby(n, 1.0, c->kz.p, 1.0, d_x, nrhs, idl, s);
+        spmvImpl(c, d_x, c->kr.p, d_b, nrhs, idl, s);
+        HIPCHECK(hipMemcpyAsync(c->pst.p, c->kst.p, sizeof(GmState) * nrhs, hipMemcpyDeviceToHost, s));
+        reduceToHost(0, c->kr.p, nullptr, idl);
+        // history of this cycle
+        std::vector<double> hc(static_cast<size_t>(maxit + 1) * nrhs);
+        HIPCHECK(hipMemcpyAsync(hc.data(), c->khist.p, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        for (int r = 0; r < nrhs; ++r) {
+            if (idle[static_cast<size_t>(r)]) continue;
+            const GmState& st = c->pst.p[r];
+            for (int it = iters[static_cast<size_t>(r)] + 1; it <= st.iters; ++it)
+                hist[static_cast<size_t>(r)].push_back(hc[static_cast<size_t>(r) * (maxit + 1) + it]);
+            iters[static_cast<size_t>(r)] = st.iters;
+            rel[static_cast<size_t>(r)] = st.rel;
+            const double b2 = std::sqrt(c->pred.p[3 * r]);
+            beta[static_cast<size_t>(r)] = b2;
+            const double bn = bnorm[static_cast<size_t>(r)];
+            bool cv = st.conv != 0;
+            const bool within = b2 <= o->rtol * bn;
+            const bool within_slack = b2 <= 1.1 * o->rtol * bn;
+            if (st.breakdown) cv = within || cv;
+            if (cv && !within_slack) cv = false;
+            if (b2 == 0.0) cv = true;
+            conv[static_cast<size_t>(r)] = cv ? 1 : 0;
+            breakdown[static_cast<size_t>(r)] = breakdown[static_cast<size_t>(r)] || st.breakdown;
+        }
+    }
+    for (int r = 0; r < nrhs; ++r) {
+        s2l_krylov_result& R = res[r];
+        std::memset(&R, 0, sizeof(R));
+        R.converged = conv[static_cast<size_t>(r)];
+        R.breakdown = 0;
+        R.iterations = iters[static_cast<size_t>(r)];
+        R.rel_residual = rel[static_cast<size_t>(r)];
+        R.true_rel_residual = bnorm[static_cast<size_t>(r)] > 0.0 ? beta[static_cast<size_t>(r)] / bnorm[static_cast<size_t>(r)] : 0.0;
+        const auto& H = hist[static_cast<size_t>(r)];
+        R.history_len = static_cast<int>(H.size());
+        if (history)
+            for (size_t q = 0; q < H.size() && q < static_cast<size_t>(maxit + 1); ++q) history[static_cast<size_t>(r) * (maxit + 1) + q] = H[q];
+    }
+    return S2L_OK;
+}
+
+} // namespace
+
+// ============================================================================ C-ABI
+extern "C" {
+
+const char* s2l_last_error(const s2l_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int s2l_ctx_create(int device, s2l_ctx** out)
+{
+    return guarded(nullptr, [&] {
+        require(out != nullptr, S2L_EINVAL, "out is null");
+        int count = 0;
+        HIPCHECK(hipGetDeviceCount(&count));
+        require(device >= 0 && device < count, S2L_EINVAL, "no HIP device " + std::to_string(device));
+        HIPCHECK(hipSetDevice(device));
+        auto* c = new s2l_ctx;
+        c->device = device;
+        try {
+            HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            int lds = 0;
+            HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+            c->max_lds = static_cast<size_t>(lds);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int s2l_ctx_destroy(s2l_ctx* ctx)
+{
+    if (!ctx) return S2L_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& pr : ctx->ev_pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return S2L_OK;
+}
+
+int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_t* col, const double* val, int32_t nsub,
+                      const int32_t* ownership)
+{
+    return guarded(c, [&] {
+        require(c && n > 0 && rowptr && col && val && nsub >= 1 && ownership, S2L_EINVAL, "upload_system: bad arguments");
+        require(ownership[0] == 0 && ownership[nsub] == n, S2L_EINVAL, "ownership must cover [0, n)");
+        for (int p = 0; p < nsub; ++p) require(ownership[p + 1] > ownership[p], S2L_EINVAL, "empty subdomain in ownership");
+        HIPCHECK(hipSetDevice(c->device));
+        const int64_t nnz = rowptr[n];
+        std::vector<int32_t> cl(static_cast<size_t>(nnz));
+        std::vector<double> vl(static_cast<size_t>(nnz));
+        std::vector<std::pair<int64_t, int64_t>> key;
+        int p = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            while (i >= ownership[p + 1]) ++p;
+            const int32_t b = ownership[p], e = ownership[p + 1];
+            key.clear();
+            for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+                const int32_t cc = col[k];
+                require(cc >= 0 && cc < n, S2L_EINVAL, "column index out of range");
+                const bool own = cc >= b && cc < e;
+                key.emplace_back((own ? 0 : (int64_t)1 << 40) + cc, k);
+            }
+            std::sort(key.begin(), key.end());
+            int64_t q = rowptr[i];
+            for (const auto& kv : key) {
+                cl[static_cast<size_t>(q)] = col[kv.second];
+                vl[static_cast<size_t>(q)] = val[kv.second];
+                ++q;
+            }
+        }
+        if (c->system_ready && c->n == n && c->nnz == nnz && c->nsub == nsub &&
+            std::equal(c->ownership.begin(), c->ownership.end(), ownership)) {
+            c->val.upload(vl, c->stream); // same pattern: refresh the values
+        } else {
+            c->n = n;
+            c->nnz = nnz;
+            c->nsub = nsub;
+            c->ownership.assign(ownership, ownership + nsub + 1);
+            c->rp.upload(rowptr, static_cast<size_t>(n) + 1, c->stream);
+            c->col.upload(cl, c->stream);
+            c->val.upload(vl, c->stream);
+            c->subs_ready = c->local_ready = c->coarse_ready = false;
+            c->work_nrhs = 0;
+        }
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->system_ready = true;
+    });
+}
+
+int s2l_upload_subdomains(s2l_ctx* c, int32_t nsub, const int32_t* local_ptr, const int32_t* gid, const double* pou,
+                          int one_level)
+{
+    return guarded(c, [&] {
+        require(c && c->system_ready, S2L_ESTATE, "upload_subdomains: upload the system first");
+        require(nsub == c->nsub && local_ptr && gid, S2L_EINVAL, "upload_subdomains: bad arguments");
+        require(one_level == 0 || one_level == 1, S2L_EINVAL, "one_level must be 0 (RAS) or 1 (ASM)");
+        HIPCHECK(hipSetDevice(c->device));
+        const int64_t tot = local_ptr[nsub];
+        c->local_ptr.assign(local_ptr, local_ptr + nsub + 1);
+        c->gid_h.assign(gid, gid + tot);
+        // owned rows first, in order; ghosts sorted, outside the owned range
+        for (int i = 0; i < nsub; ++i) {
+            const int32_t b = c->ownership[i], e = c->ownership[i + 1], l0 = local_ptr[i], l1 = local_ptr[i + 1];
+            require(l1 - l0 >= e - b, S2L_EINVAL, "subdomain smaller than its owned rows");
+            for (int32_t l = 0; l < e - b; ++l) require(gid[l0 + l] == b + l, S2L_EINVAL, "owned rows must come first, in order");
+            for (int32_t l = l0 + (e - b); l < l1; ++l) {
+                require(gid[l] >= 0 && gid[l] < c->n && (gid[l] < b || gid[l] >= e), S2L_EINVAL, "bad ghost id");
+                if (l > l0 + (e - b)) require(gid[l] > gid[l - 1], S2L_EINVAL, "ghosts must be sorted by global id");
+            }
+        }
+        c->boolean = (pou == nullptr);
+        c->one_level = one_level;
+        c->local_total = tot;
+        if (pou) {
+            c->pou_h.assign(pou, pou + tot);
+            c->pou_d.upload(c->pou_h, c->stream);
+        }
+        // reverse-add plan: every global row lists the owner's copy, then the ghost copies of the
+        // other subdomains in increasing subdomain order (halo_exchange.cpp:98-106)
+        std::vector<int32_t> cnt(static_cast<size_t>(c->n) + 1, 1);
+        cnt[0] = 0;
+        for (int i = 0; i < nsub; ++i) {
+            const int32_t nI = c->ownership[i + 1] - c->ownership[i];
+            for (int32_t l = local_ptr[i] + nI; l < local_ptr[i + 1]; ++l) ++cnt[static_cast<size_t>(gid[l]) + 1];
+        }
+        std::vector<int32_t> ptr(static_cast<size_t>(c->n) + 1, 0);
+        for (int32_t g = 0; g < c->n; ++g) ptr[static_cast<size_t>(g) + 1] = ptr[static_cast<size_t>(g)] + cnt[static_cast<size_t>(g) + 1];
+        std::vector<int32_t> idx(static_cast<size_t>(ptr[static_cast<size_t>(c->n)]));
+        std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
+        for (int i = 0; i < nsub; ++i) {
+            const int32_t b = c->ownership[i], nI = c->ownership[i + 1] - b;
+            for (int32_t l = 0; l < nI; ++l) idx[static_cast<size_t>(next[static_cast<size_t>(b + l)]++)] = local_ptr[i] + l;
+        }
+        for (int i = 0; i < nsub; ++i) {
+            const int32_t nI = c->ownership[i + 1] - c->ownership[i];
+            for (int32_t l = local_ptr[i] + nI; l < local_ptr[i + 1]; ++l) idx[static_cast<size_t>(next[static_cast<size_t>(gid[l])]++)] = l;
+        }
+        c->rg_ptr.upload(ptr, c->stream);
+        c->rg_idx.upload(idx, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->hf.assign(static_cast<size_t>(nsub), HostFactor{});
+        c->subs_ready = true;
+        c->local_ready = c->coarse_ready = false;
+        c->work_nrhs = 0;
+    });
+}
+
+int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* perm, const int64_t* l_colptr, const int32_t* l_row,
+                            const double* l_val, const int64_t* ut_colptr, const int32_t* ut_row, const double* ut_val)
+{
+    return guarded(c, [&] {
+        require(c && c->subs_ready, S2L_ESTATE, "upload_local_factor: upload the subdomains first");
+        require(i >= 0 && i < c->nsub, S2L_EINVAL, "subdomain index out of range");
+        require(n == c->local_ptr[i + 1] - c->local_ptr[i], S2L_EINVAL, "factor size differs from n_i");
+        require(perm && l_colptr && l_row && l_val, S2L_EINVAL, "upload_local_factor: null factor");
+        HostFactor& F = c->hf[static_cast<size_t>(i)];
+        F.n = n;
+        F.perm.assign(perm, perm + n);
+        std::vector<char> seen(static_cast<size_t>(n), 0);
+        for (int32_t k = 0; k < n; ++k) {
+            require(perm[k] >= 0 && perm[k] < n && !seen[static_cast<size_t>(perm[k])], S2L_EINVAL, "perm is not a permutation");
+            seen[static_cast<size_t>(perm[k])] = 1;
+        }
+        F.lcp.assign(l_colptr, l_colptr + n + 1);
+        F.lr.assign(l_row, l_row + l_colptr[n]);
+        F.lv.assign(l_val, l_val + l_colptr[n]);
+        F.cholesky = (ut_colptr == nullptr);
+        if (!F.cholesky) {
+            require(ut_row && ut_val, S2L_EINVAL, "upload_local_factor: incomplete U^T");
+            F.ucp.assign(ut_colptr, ut_colptr + n + 1);
+            F.ur.assign(ut_row, ut_row + ut_colptr[n]);
+            F.uv.assign(ut_val, ut_val + ut_colptr[n]);
+        } else {
+            F.ucp.clear();
+            F.ur.clear();
+            F.uv.clear();
+        }
+        c->local_ready = false;
+    });
+}
+
+int s2l_finalize_local(s2l_ctx* c, int precision)
+{
+    return guarded(c, [&] {
+        require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
+        require(precision == 0 || precision == 1, S2L_EINVAL, "precision must be 0 (FP64) or 1 (FP32)");
+        for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
+        HIPCHECK(hipSetDevice(c->device));
+        std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
+        c->pk = packFactors(c->hf, moff);
+        Packed& P = c->pk;
+        for (int i = 0; i < c->nsub; ++i) {
+            P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
+            P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
+        }
+        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
+        c->lds_bytes = static_cast<size_t>(P.max_n) * esz;
+        require(c->lds_bytes <= c->max_lds, S2L_ELIMIT,
+                "subdomain of " + std::to_string(P.max_n) + " unknowns exceeds the LDS of one workgroup (" + std::to_string(c->max_lds) +
+                    " bytes)");
+        const void* kern = precision == 1 ? localSolveKernel<float>() : localSolveKernel<double>();
+        HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
+        // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
+        std::vector<int32_t> gmap(c->gid_h.size()), lperm(c->gid_h.size());
+        for (int i = 0; i < c->nsub; ++i) {
+            const int32_t off = c->local_ptr[i];
+            const HostFactor& F = c->hf[static_cast<size_t>(i)];
+            for (int32_t p = 0; p < F.n; ++p) {
+                lperm[static_cast<size_t>(off + p)] = F.perm[static_cast<size_t>(p)];
+                gmap[static_cast<size_t>(off + p)] = c->gid_h[static_cast<size_t>(off + F.perm[static_cast<size_t>(p)])];
+            }
+        }
+        c->sub_d.upload(P.sub, c->stream);
+        c->sn_d.upload(P.sn, c->stream);
+        c->lev_d.upload(P.lev, c->stream);
+        c->levsn_d.upload(P.lev_sn, c->stream);
+        c->ridx_d.upload(P.ridx, c->stream);
+        c->cptr_d.upload(P.cptr, c->stream);
+        c->cidx_d.upload(P.cidx, c->stream);
+        c->gmap_d.upload(gmap, c->stream);
+        c->lperm_d.upload(lperm, c->stream);
+        if (precision == 1) {
+            std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
+            c->diag_f.upload(d, c->stream);
+            c->below_f.upload(b, c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            c->diag_dd.release();
+            c->below_dd.release();
+        } else {
+            c->diag_dd.upload(P.diag, c->stream);
+            c->below_dd.upload(P.below, c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            c->diag_f.release();
+            c->below_f.release();
+        }
+        // the host copies of the values are no longer needed
+        std::vector<double>().swap(P.diag);
+        std::vector<double>().swap(P.below);
+        c->precision = precision;
+        c->work_nrhs = 0;
+        c->local_ready = true;
+    });
+}
+
+int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const double* V, const double* E, int symmetric,
+                      int correction)
+{
+    return guarded(c, [&] {
+        require(c && c->subs_ready, S2L_ESTATE, "upload_coarse: upload the subdomains first");
+        require(n0 >= 0 && offsets && (correction == 0 || correction == 1), S2L_EINVAL, "upload_coarse: bad arguments");
+        require(offsets[0] == 0 && offsets[c->nsub] == n0, S2L_EINVAL, "coarse offsets must cover [0, n0)");
+        HIPCHECK(hipSetDevice(c->device));
+        c->n0 = n0;
+        c->correction = correction;
+        c->off_h.assign(offsets, offsets + c->nsub + 1);
+        if (n0 > 0) {
+            require(V && E, S2L_EINVAL, "upload_coarse: null V or E");
+            std::vector<int64_t> voff(static_cast<size_t>(c->nsub));
+            std::vector<int32_t> vrows(static_cast<size_t>(c->nsub)), vrowptr(static_cast<size_t>(c->nsub)), vrowg, bsub, brow;
+            int64_t vtot = 0;
+            int mmax = 0;
+            for (int i = 0; i < c->nsub; ++i) {
+                const int32_t nI = c->ownership[i + 1] - c->ownership[i];
+                const int32_t rows = c->boolean ? nI : (c->local_ptr[i + 1] - c->local_ptr[i]);
+                const int64_t m = offsets[i + 1] - offsets[i];
+                require(m >= 0, S2L_EINVAL, "coarse offsets must be nondecreasing");
+                mmax = std::max<int>(mmax, static_cast<int>(m));
+                voff[static_cast<size_t>(i)] = vtot;
+                vrows[static_cast<size_t>(i)] = rows;
+                vrowptr[static_cast<size_t>(i)] = static_cast<int32_t>(vrowg.size());
+                for (int32_t l = 0; l < rows; ++l) vrowg.push_back(c->gid_h[static_cast<size_t>(c->local_ptr[i] + l)]);
+                vtot += static_cast<int64_t>(rows) * m;
+                // every subdomain gets prolongation blocks: those without coarse vectors write zeros
+                for (int32_t r0 = 0; r0 < rows; r0 += 256) {
+                    bsub.push_back(i);
+                    brow.push_back(r0);
+                }
+            }
+            c->V_d.upload(V, static_cast<size_t>(vtot), c->stream);
+            c->off_d.upload(c->off_h, c->stream);
+            c->voff_d.upload(voff, c->stream);
+            c->vrows_d.upload(vrows, c->stream);
+            c->vrowptr_d.upload(vrowptr, c->stream);
+            c->vrowg_d.upload(vrowg, c->stream);
+            c->blksub_d.upload(bsub, c->stream);
+            c->blkrow_d.upload(brow, c->stream);
+            // A_00: Cholesky (symmetric) or LU with partial pivoting, FP64; the device applies the
+            // inverses of the triangular factors as two triangular matrix-vector products.
+            std::vector<double> M(E, E + static_cast<size_t>(n0) * n0);
+            for (double v : M) require(std::isfinite(v), S2L_ENUMERIC, "A_00 has non-finite entries");
+            std::vector<double> T1, T2(static_cast<size_t>(n0) * n0, 0.0);
+            bool chol = symmetric && denseCholesky(n0, M);
+            c->coarse_perm = false;
+            if (chol) {
+                T1 = lowerInverse(n0, M, false);
+                for (int i = 0; i < n0; ++i)
+                    for (int j = i; j < n0; ++j) T2[static_cast<size_t>(i) * n0 + j] = T1[static_cast<size_t>(j) * n0 + i];
+            } else {
+                M.assign(E, E + static_cast<size_t>(n0) * n0);
+                std::vector<int32_t> piv;
+                require(denseLu(n0, M, piv), S2L_ENUMERIC, "A_00 is singular");
+                T1 = lowerInverse(n0, M, true);
+                // U^{-1} = (L'^{-1})^T with L' = U^T (lower)
+                std::vector<double> Ut(static_cast<size_t>(n0) * n0, 0.0);
+                for (int i = 0; i < n0; ++i)
+                    for (int j = i; j < n0; ++j) Ut[static_cast<size_t>(j) * n0 + i] = M[static_cast<size_t>(i) * n0 + j];
+                const std::vector<double> X = lowerInverse(n0, Ut, false);
+                for (int i = 0; i < n0; ++i)
+                    for (int j = i; j < n0; ++j) T2[static_cast<size_t>(i) * n0 + j] = X[static_cast<size_t>(j) * n0 + i];
+                c->cperm_d.upload(piv, c->stream);
+                c->coarse_perm = true;
+            }
+            for (double v : T1) require(std::isfinite(v), S2L_ENUMERIC, "A_00 factor is not finite");
+            c->T1_d.upload(T1, c->stream);
+            c->T2_d.upload(T2, c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            CoarseArgs& a = c->ca;
+            a.nsub = c->nsub;
+            a.n0 = n0;
+            a.mmax = mmax;
+            a.off = c->off_d.p;
+            a.voff = c->voff_d.p;
+            a.vrows = c->vrows_d.p;
+            a.vrow_ptr = c->vrowptr_d.p;
+            a.vrow_g = c->vrowg_d.p;
+            a.V = c->V_d.p;
+            a.blk_sub = c->blksub_d.p;
+            a.blk_row = c->blkrow_d.p;
+            a.nblk = static_cast<int32_t>(bsub.size());
+        }
+        c->work_nrhs = 0;
+        c->coarse_ready = true;
+    });
+}
+
+int s2l_apply(s2l_ctx* c, const double* d_r, double* d_z, int32_t nrhs, void* stream)
+{
+    return guarded(c, [&] {
+        require(c && d_r && d_z && nrhs >= 1, S2L_EINVAL, "apply: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        applyImpl(c, d_r, d_z, nrhs, nullptr, pick(c, stream));
+    });
+}
+
+int s2l_spmv(s2l_ctx* c, const double* d_x, double* d_y, int32_t nrhs, void* stream)
+{
+    return guarded(c, [&] {
+        require(c && d_x && d_y && nrhs >= 1, S2L_EINVAL, "spmv: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        spmvImpl(c, d_x, d_y, nullptr, nrhs, nullptr, pick(c, stream));
+    });
+}
+
+int s2l_krylov_solve(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int32_t nrhs,
+                     s2l_krylov_result* results, double* history, void* stream)
+{
+    int rc = S2L_OK;
+    const int g = guarded(c, [&] {
+        require(c && o && d_b && d_x && results && nrhs >= 1, S2L_EINVAL, "krylov_solve: bad arguments");
+        require(o->method == 0, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES)");
+        HIPCHECK(hipSetDevice(c->device));
+        rc = gmresImpl(c, o, d_b, d_x, nrhs, results, history, pick(c, stream));
+    });
+    return g != S2L_OK ? g : rc;
+}
+
+int s2l_apply_host(s2l_ctx* c, const double* r, double* z, int32_t nrhs)
+{
+    return guarded(c, [&] {
+        require(c && r && z && nrhs >= 1, S2L_EINVAL, "apply_host: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        DevBuf<double> dr, dz;
+        dr.upload(r, static_cast<size_t>(c->n) * nrhs, c->stream);
+        dz.reserve(static_cast<size_t>(c->n) * nrhs);
+        applyImpl(c, dr.p, dz.p, nrhs, nullptr, c->stream);
+        HIPCHECK(hipMemcpyAsync(z, dz.p, sizeof(double) * c->n * nrhs, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_spmv_host(s2l_ctx* c, const double* x, double* y, int32_t nrhs)
+{
+    return guarded(c, [&] {
+        require(c && x && y && nrhs >= 1, S2L_EINVAL, "spmv_host: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        DevBuf<double> dx, dy;
+        dx.upload(x, static_cast<size_t>(c->n) * nrhs, c->stream);
+        dy.reserve(static_cast<size_t>(c->n) * nrhs);
+        spmvImpl(c, dx.p, dy.p, nullptr, nrhs, nullptr, c->stream);
+        HIPCHECK(hipMemcpyAsync(y, dy.p, sizeof(double) * c->n * nrhs, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_krylov_solve_host(s2l_ctx* c, const s2l_krylov_opts* o, const double* b, double* x, int32_t nrhs,
+                          s2l_krylov_result* results, double* history)
+{
+    int rc = S2L_OK;
+    const int g = guarded(c, [&] {
+        require(c && o && b && x && results && nrhs >= 1, S2L_EINVAL, "krylov_solve_host: bad arguments");
+        require(o->method == 0, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES)");
+        HIPCHECK(hipSetDevice(c->device));
+        DevBuf<double> db, dx;
+        db.upload(b, static_cast<size_t>(c->n) * nrhs, c->stream);
+        dx.upload(x, static_cast<size_t>(c->n) * nrhs, c->stream);
+        rc = gmresImpl(c, o, db.p, dx.p, nrhs, results, history, c->stream);
+        HIPCHECK(hipMemcpyAsync(x, dx.p, sizeof(double) * c->n * nrhs, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+    return g != S2L_OK ? g : rc;
+}
+
+int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
+{
+    return guarded(c, [&] {
+        require(c && c->local_ready && b && y && i >= 0 && i < c->nsub, S2L_EINVAL, "local_solve_host: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        c->ensureWork(1);
+        // Scatter b into a global-length vector through the local -> global map, solve, read the
+        // local buffer (mode 1 without scaling).
+        const int32_t l0 = c->local_ptr[i], nl = c->local_ptr[i + 1] - l0;
+        std::vector<double> g(static_cast<size_t>(c->n), 0.0);
+        for (int32_t l = 0; l < nl; ++l) g[static_cast<size_t>(c->gid_h[static_cast<size_t>(l0 + l)])] = b[l];
+        DevBuf<double> dg;
+        dg.upload(g, c->stream);
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            LocalArgs<T> a{};
+            a.sub = c->sub_d.p;
+            a.sn = c->sn_d.p;
+            a.lev = c->lev_d.p;
+            a.lev_sn = c->levsn_d.p;
+            if constexpr (std::is_same_v<T, float>) {
+                a.diag = c->diag_f.p;
+                a.below = c->below_f.p;
+            } else {
+                a.diag = c->diag_dd.p;
+                a.below = c->below_dd.p;
+            }
+            a.ridx = c->ridx_d.p;
+            a.cptr = c->cptr_d.p;
+            a.cidx = c->cidx_d.p;
+            a.tbuf = reinterpret_cast<T*>(c->tbuf.p);
+            a.gmap = c->gmap_d.p;
+            a.lperm = c->lperm_d.p;
+            a.rhs = dg.p;
+            a.out = c->loc.p; // mode 1 without scaling: y by local index
+            a.n_global = c->n;
+            a.n_local_total = c->local_total;
+            a.t_total = c->pk.t_total;
+            a.sub_first = i;
+            a.mode = 1;
+            launchLocalSolve<T>(a, 1, 1, c->lds_bytes, c->stream);
+        };
+        if (c->precision == 1)
+            run(float{});
+        else
+            run(double{});
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(y, c->loc.p + l0, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_device_alloc(s2l_ctx* c, int64_t bytes, void** out)
+{
+    return guarded(c, [&] {
+        require(c && out && bytes >= 0, S2L_EINVAL, "device_alloc: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipMalloc(out, static_cast<size_t>(std::max<int64_t>(bytes, 1))));
+    });
+}
+
+int s2l_device_free(s2l_ctx* c, void* p)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "device_free: null context");
+        if (p) HIPCHECK(hipFree(p));
+    });
+}
+
+int s2l_memcpy_h2d(s2l_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    return guarded(c, [&] {
+        require(c && dst && src && bytes >= 0, S2L_EINVAL, "memcpy_h2d: bad arguments");
+        HIPCHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_memcpy_d2h(s2l_ctx* c, void* dst, const void* src, int64_t bytes)
+{
+    return guarded(c, [&] {
+        require(c && dst && src && bytes >= 0, S2L_EINVAL, "memcpy_d2h: bad arguments");
+        HIPCHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_synchronize(s2l_ctx* c)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "synchronize: null context");
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
+{
+    return guarded(c, [&] {
+        require(c && o, S2L_EINVAL, "get_stats: bad arguments");
+        std::memset(o, 0, sizeof(*o));
+        o->n = c->n;
+        o->nsub = c->nsub;
+        o->n0 = c->n0;
+        o->precision = c->precision;
+        o->nnz_a = c->nnz;
+        o->factor_entries_fwd = c->pk.entries_fwd;
+        o->factor_entries_bwd = c->pk.entries_bwd;
+        o->local_rows = c->local_total;
+        int64_t crows = 0;
+        for (int i = 0; i < c->nsub && c->n0 > 0; ++i) {
+            const int32_t nI = c->ownership[i + 1] - c->ownership[i];
+            const int32_t rows = c->boolean ? nI : (c->local_ptr[i + 1] - c->local_ptr[i]);
+            crows += static_cast<int64_t>(rows) * (c->off_h[static_cast<size_t>(i) + 1] - c->off_h[static_cast<size_t>(i)]);
+        }
+        o->coarse_rows = crows;
+        o->levels_fwd_max = c->pk.max_levels_fwd;
+        o->levels_bwd_max = c->pk.max_levels_bwd;
+        o->supernodes = static_cast<int32_t>(c->pk.sn.size());
+        const double sv = c->precision == 1 ? 4.0 : 8.0;
+        // local solve: factor values of both sweeps, index data, rhs gather (8 B value + 4 B map),
+        // permutation and owned-row output (read x0 8 B + write z 8 B)
+        const double local = (c->pk.entries_fwd + c->pk.entries_bwd) * sv + static_cast<double>(c->pk.bytes_index) +
+                              static_cast<double>(c->local_total) * (8.0 + 4.0 + 4.0) + static_cast<double>(c->n) * 16.0 +
+                              static_cast<double>(c->pk.t_total) * sv * 2.0;
+        o->bytes_local_solve = local;
+        const double spmv = static_cast<double>(c->nnz) * 12.0 + (static_cast<double>(c->n) + 1) * 4.0 + static_cast<double>(c->n) * 16.0;
+        o->bytes_spmv = spmv;
+        double apply = local;
+        if (c->n0 > 0) {
+            apply += static_cast<double>(crows) * 8.0 * 2.0 + static_cast<double>(c->n0) * (c->n0 + 1) * 8.0 +
+                     static_cast<double>(c->n) * 8.0 * 2.0; // V twice, both triangles, r read + x0 write
+            if (c->correction == 0) apply += spmv + static_cast<double>(c->n) * 8.0;
+        }
+        o->bytes_apply = apply;
+    });
+}
+
+int s2l_timing_enable(s2l_ctx* c, int on)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "timing_enable: null context");
+        c->timing = (on != 0);
+    });
+}
+
+int s2l_timing_read(s2l_ctx* c, int64_t* launches, double* ms_total, int reset)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "timing_read: null context");
+        collectTiming(c);
+        if (launches) *launches = c->launches;
+        if (ms_total) *ms_total = c->ms_total;
+        if (reset) {
+            c->launches = 0;
+            c->ms_total = 0.0;
+        }
+    });
+}
+
+int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
+{
+    return guarded(c, [&] {
+        require(c && (mode == 0 || mode == 1), S2L_EINVAL, "launch mode must be 0 or 1");
+        c->launch_mode = mode;
+    });
+}
+
+} // extern "C"

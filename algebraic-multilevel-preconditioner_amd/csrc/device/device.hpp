@@ -1,0 +1,124 @@
+// device.hpp -- kernel argument blocks and launchers of the MI355X apply path.
+#ifndef S2L_DEVICE_HPP
+#define S2L_DEVICE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pack.hpp"
+
+namespace s2l {
+
+constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per subdomain
+constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
+
+/// Arguments of the batched local solve (all pointers are device pointers).
+template <typename T> struct LocalArgs {
+    const SubDesc* sub;
+    const SnDesc* sn;
+    const int32_t* lev;
+    const int32_t* lev_sn;
+    const T* diag;
+    const T* below;
+    const int32_t* ridx;
+    const int32_t* cptr;
+    const int32_t* cidx;
+    T* tbuf;              ///< T buffer, t_total entries per right-hand side
+    const int32_t* gmap;  ///< factor position -> global row (gather)
+    const int32_t* lperm; ///< factor position -> local index
+    const double* rhs;    ///< global right-hand sides (n x nrhs)
+    const double* x0;     ///< added to the owned output rows (may be null)
+    double* out;          ///< mode 0: global output (n x nrhs); mode 1: local buffer (sum n_i x nrhs)
+    const double* pou;    ///< mode 1, RAS: D_i by local index (null = 1)
+    const int32_t* skip;  ///< per right-hand side: nonzero = nothing to do (may be null)
+    int64_t n_global;
+    int64_t n_local_total;
+    int64_t t_total;
+    int32_t sub_first;    ///< first subdomain of this launch
+    int32_t mode;         ///< 0 RAS with Boolean PoU (direct write), 1 local buffer for a reverse add
+};
+
+/// Coarse space arguments.
+struct CoarseArgs {
+    int32_t nsub, n0, mmax;
+    const int64_t* off;    ///< coarse offsets (nsub + 1)
+    const int64_t* voff;   ///< offset of each V_i block
+    const int32_t* vrows;  ///< rows of each V_i block
+    const int32_t* vrow_ptr; ///< offset of each block's rows in vrow_g
+    const int32_t* vrow_g; ///< global row of each V row
+    const double* V;
+    const int32_t* blk_sub; ///< prolongation blocks: subdomain and first row
+    const int32_t* blk_row;
+    int32_t nblk;
+};
+
+/// Per right-hand side state of the device GMRES loop.
+struct GmState {
+    double bnorm, beta, rel, hn;
+    int32_t k;          ///< columns of the current cycle
+    int32_t iters;      ///< total iterations
+    int32_t done;       ///< cycle finished (converged estimate, breakdown or maxit)
+    int32_t conv;       ///< estimate below the tolerance
+    int32_t breakdown;  ///< hn == 0 (happy breakdown)
+    int32_t maxit;
+    int32_t pad[2];
+};
+
+template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int nsub, int nrhs, size_t lds, hipStream_t s);
+
+void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                const double* b, int nrhs, const int32_t* skip, hipStream_t s);
+void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, int nrhs, const int32_t* skip,
+                          hipStream_t s);
+void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
+                   const int32_t* skip, hipStream_t s);
+void launchCoarseProlong(const CoarseArgs& c, const double* y, double* out, int64_t ld_out, int boolean, int nrhs,
+                         const int32_t* skip, hipStream_t s);
+void launchRevGather(int32_t n, const int32_t* ptr, const int32_t* idx, const double* src, int64_t src_ld,
+                     const double* x0, double* out, int nrhs, const int32_t* skip, hipStream_t s);
+
+// ---- vector kernels
+void launchAxpby(int64_t n, double a, const double* x, double b, double* y, int nrhs, const int32_t* skip, hipStream_t s);
+void launchCopy(int64_t n, const double* x, double* y, int nrhs, hipStream_t s);
+
+/// Deterministic sum reductions of nrhs columns: out[r * ncomp + c] for c < ncomp;
+/// kind 0: x.x ; 1: x.x, sum|x| ; 2: x.x, sum|x|, x.y ; 3: y.x (single) ; 4: x.x, x.y
+void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, unsigned* counter, double* out,
+                  int nrhs, const int32_t* skip, hipStream_t s);
+int reduceBlocks(int64_t n);
+
+// ---- GMRES
+struct GmArgs {
+    int64_t n;
+    int32_t m;          ///< restart
+    int32_t nrhs;
+    double* V;          ///< basis, V[(k * nrhs + r) * n + i]
+    double* w;          ///< n per rhs
+    double* H;          ///< (m + 1) x m per rhs, column-major
+    double* cs;
+    double* sn;
+    double* g;          ///< m + 1 per rhs
+    double* h;          ///< m + 1 per rhs
+    double* h2;
+    double* y;          ///< m per rhs
+    GmState* st;
+    int32_t* flags;     ///< [0, nrhs): 1 once the rhs has finished its cycle; [nrhs, 2 nrhs): 1 if idle this cycle
+    double* hist;       ///< (maxit + 1) per rhs
+    int32_t hist_ld;
+    double rtol;
+    double* partial;    ///< block partial sums
+    unsigned* counter;  ///< one per rhs
+    int32_t nblocks;
+};
+void gmDots(const GmArgs& a, int j, hipStream_t s);
+void gmUpdateDots(const GmArgs& a, int j, hipStream_t s);
+void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s);
+void gmNormalize(const GmArgs& a, int j, hipStream_t s);
+void gmStartCycle(const GmArgs& a, const double* r, hipStream_t s);
+void gmLsq(const GmArgs& a, hipStream_t s);
+void gmCombine(const GmArgs& a, double* u, hipStream_t s);
+
+} // namespace s2l
+
+#endif

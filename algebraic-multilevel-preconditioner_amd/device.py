@@ -1,0 +1,369 @@
+"""Python mirror of the reference's preconditioner / Krylov interface over the C-ABI of the HIP path.
+
+* ``HipTwoLevelSchwarz``  -- prec::Preconditioner (include/preconditioner/preconditioner.hpp:64-96)
+  implemented on the device: ``apply(r) -> z``, ``name()``, ``update(level)``, ``up_to_date()``.
+* ``HipGmres``            -- solv::Gmres (include/solver/gmres.hpp:39-71): ``solve(A, M, b, x)``,
+  ``name() == "GMRES(m)"``; the whole loop runs on the device.
+* ``make_krylov_solver``  -- solv::makeKrylovSolver (src/solver/krylov.cpp:23-36).
+* ``KrylovResult``        -- include/solver/krylov.hpp:43-54.
+
+Errors raise ``RuntimeError`` with the library's message, as the reference throws
+``std::runtime_error`` (src/preconditioner/one_level_schwarz.cpp:49-53). There is no CPU
+fallback: without ``libs2l_hip.so`` or a GPU every call fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+P = ctypes.POINTER
+_i32p, _i64p, _dp = P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_double)
+
+
+class KrylovOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("rtol", ctypes.c_double),
+                ("atol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_norm", ctypes.c_int)]
+
+
+class KrylovResultC(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("breakdown", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("true_rel_residual", ctypes.c_double), ("initial_residual_of", ctypes.c_double),
+                ("final_residual_of", ctypes.c_double), ("history_len", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("nsub", ctypes.c_int32), ("n0", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("nnz_a", ctypes.c_int64), ("factor_entries_fwd", ctypes.c_int64),
+                ("factor_entries_bwd", ctypes.c_int64), ("local_rows", ctypes.c_int64),
+                ("coarse_rows", ctypes.c_int64), ("levels_fwd_max", ctypes.c_int32),
+                ("levels_bwd_max", ctypes.c_int32), ("supernodes", ctypes.c_int32),
+                ("bytes_local_solve", ctypes.c_double), ("bytes_apply", ctypes.c_double),
+                ("bytes_spmv", ctypes.c_double)]
+
+
+_hip = None
+
+#: every symbol declared in include/s2l_hip.h
+EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_system", "s2l_upload_subdomains",
+           "s2l_upload_local_factor", "s2l_finalize_local", "s2l_upload_coarse", "s2l_apply", "s2l_spmv",
+           "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
+           "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_synchronize",
+           "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode"]
+
+
+def hip_lib():
+    """libs2l_hip.so with the argument types of include/s2l_hip.h (loading touches no GPU)."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    L = _lib._load("libs2l_hip.so")
+    vp = ctypes.c_void_p
+    sig = {
+        "s2l_ctx_create": [ctypes.c_int, P(vp)],
+        "s2l_ctx_destroy": [vp],
+        "s2l_upload_system": [vp, ctypes.c_int32, _i32p, _i32p, _dp, ctypes.c_int32, _i32p],
+        "s2l_upload_subdomains": [vp, ctypes.c_int32, _i32p, _i32p, _dp, ctypes.c_int],
+        "s2l_upload_local_factor": [vp, ctypes.c_int32, ctypes.c_int32, _i32p, _i64p, _i32p, _dp, _i64p, _i32p, _dp],
+        "s2l_finalize_local": [vp, ctypes.c_int],
+        "s2l_upload_coarse": [vp, ctypes.c_int32, _i64p, _dp, _dp, ctypes.c_int, ctypes.c_int],
+        "s2l_apply": [vp, vp, vp, ctypes.c_int32, vp],
+        "s2l_spmv": [vp, vp, vp, ctypes.c_int32, vp],
+        "s2l_krylov_solve": [vp, P(KrylovOpts), vp, vp, ctypes.c_int32, P(KrylovResultC), _dp, vp],
+        "s2l_apply_host": [vp, _dp, _dp, ctypes.c_int32],
+        "s2l_spmv_host": [vp, _dp, _dp, ctypes.c_int32],
+        "s2l_krylov_solve_host": [vp, P(KrylovOpts), _dp, _dp, ctypes.c_int32, P(KrylovResultC), _dp],
+        "s2l_local_solve_host": [vp, ctypes.c_int32, _dp, _dp],
+        "s2l_device_alloc": [vp, ctypes.c_int64, P(vp)],
+        "s2l_device_free": [vp, vp],
+        "s2l_memcpy_h2d": [vp, vp, vp, ctypes.c_int64],
+        "s2l_memcpy_d2h": [vp, vp, vp, ctypes.c_int64],
+        "s2l_synchronize": [vp],
+        "s2l_get_stats": [vp, P(Stats)],
+        "s2l_timing_enable": [vp, ctypes.c_int],
+        "s2l_timing_read": [vp, P(ctypes.c_int64), P(ctypes.c_double), ctypes.c_int],
+        "s2l_set_local_launch_mode": [vp, ctypes.c_int],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.s2l_last_error.argtypes = [vp]
+    L.s2l_last_error.restype = ctypes.c_char_p
+    _hip = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+class DeviceArray:
+    """A device buffer of float64 owned by a context (for inputs resident in HBM)."""
+
+    def __init__(self, ctx: "HipContext", n: int):
+        self.ctx = ctx
+        self.n = n
+        self.ptr = ctypes.c_void_p()
+        ctx._check(hip_lib().s2l_device_alloc(ctx.h, 8 * n, ctypes.byref(self.ptr)))
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        self.ctx._check(hip_lib().s2l_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(ctypes.c_void_p), 8 * self.n))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n)
+        self.ctx._check(hip_lib().s2l_memcpy_d2h(self.ctx.h, out.ctypes.data_as(ctypes.c_void_p), self.ptr, 8 * self.n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip_lib().s2l_device_free(self.ctx.h, self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HipContext:
+    """Owner of one s2l_ctx (all subdomains of one problem on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        self.L = hip_lib()
+        self.h = ctypes.c_void_p()
+        rc = self.L.s2l_ctx_create(device, ctypes.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"s2l_ctx_create: {self.L.s2l_last_error(None).decode()}")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"s2l error {rc}: {self.L.s2l_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.s2l_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ uploads
+    def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
+                     correction: str | None = None):
+        """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products)."""
+        dec = setup.dec
+        A = dec.A.tocsr()
+        n = A.shape[0]
+        rp = np.ascontiguousarray(A.indptr, np.int32)
+        col = np.ascontiguousarray(A.indices, np.int32)
+        val = np.ascontiguousarray(A.data, np.float64)
+        own = np.ascontiguousarray(dec.ownership, np.int32)
+        N = len(own) - 1
+        self._check(self.L.s2l_upload_system(self.h, n, _p(rp, ctypes.c_int32), _p(col, ctypes.c_int32),
+                                             _p(val, ctypes.c_double), N, _p(own, ctypes.c_int32)))
+        gids = [sd.global_ids() for sd in setup.subs]
+        lptr = np.zeros(N + 1, np.int32)
+        np.cumsum([len(g) for g in gids], out=lptr[1:])
+        gid = np.ascontiguousarray(np.concatenate(gids), np.int32)
+        boolean = setup.subs[0].boolean
+        pou = None if boolean else np.ascontiguousarray(np.concatenate([sd.D for sd in setup.subs]), np.float64)
+        ol = one_level or setup.one_level
+        self._check(self.L.s2l_upload_subdomains(self.h, N, _p(lptr, ctypes.c_int32), _p(gid, ctypes.c_int32),
+                                                 None if pou is None else _p(pou, ctypes.c_double),
+                                                 0 if ol == "ras" else 1))
+        for i, f in enumerate(setup.factors):
+            self.upload_factor(i, f)
+        self.finalize(precision)
+        self.upload_coarse(setup, correction or setup.correction)
+        self._keep = (rp, col, val, own, lptr, gid, pou)
+
+    def upload_factor(self, i, f):
+        perm = np.ascontiguousarray(f.perm, np.int32)
+        lcp = np.ascontiguousarray(f.l_colptr, np.int64)
+        lr = np.ascontiguousarray(f.l_row, np.int32)
+        lv = np.ascontiguousarray(f.l_val, np.float64)
+        if f.kind == 0:
+            args = (None, None, None)
+        else:
+            ucp = np.ascontiguousarray(f.ut_colptr, np.int64)
+            ur = np.ascontiguousarray(f.ut_row, np.int32)
+            uv = np.ascontiguousarray(f.ut_val, np.float64)
+            args = (_p(ucp, ctypes.c_int64), _p(ur, ctypes.c_int32), _p(uv, ctypes.c_double))
+        self._check(self.L.s2l_upload_local_factor(self.h, i, f.n, _p(perm, ctypes.c_int32), _p(lcp, ctypes.c_int64),
+                                                   _p(lr, ctypes.c_int32), _p(lv, ctypes.c_double), *args))
+
+    def finalize(self, precision: str = "double"):
+        self._check(self.L.s2l_finalize_local(self.h, 1 if precision == "single" else 0))
+
+    def upload_coarse(self, setup, correction: str = "deflated"):
+        cs = setup.coarse
+        n0 = cs.n0
+        off = np.ascontiguousarray(cs.offsets, np.int64)
+        if n0 > 0:
+            blocks = []
+            for sd, V in zip(setup.subs, cs.V):
+                rows = V[:sd.n_owned] if sd.boolean else V
+                blocks.append(np.asfortranarray(rows).reshape(-1, order="F"))
+            Vc = np.ascontiguousarray(np.concatenate(blocks), np.float64)
+            E = np.ascontiguousarray(cs.E, np.float64)
+            self._check(self.L.s2l_upload_coarse(self.h, n0, _p(off, ctypes.c_int64), _p(Vc, ctypes.c_double),
+                                                 _p(E, ctypes.c_double), 1 if setup.dec.symmetric else 0,
+                                                 1 if correction == "additive" else 0))
+            self._keep_coarse = (Vc, E, off)
+        else:
+            self._check(self.L.s2l_upload_coarse(self.h, 0, _p(off, ctypes.c_int64), None, None, 1,
+                                                 1 if correction == "additive" else 0))
+
+    # ------------------------------------------------------------------ operations (host memory)
+    def apply(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, np.float64)
+        nrhs = 1 if r.ndim == 1 else r.shape[0]
+        z = np.empty_like(r)
+        self._check(self.L.s2l_apply_host(self.h, _p(r, ctypes.c_double), _p(z, ctypes.c_double), nrhs))
+        return z
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        nrhs = 1 if x.ndim == 1 else x.shape[0]
+        y = np.empty_like(x)
+        self._check(self.L.s2l_spmv_host(self.h, _p(x, ctypes.c_double), _p(y, ctypes.c_double), nrhs))
+        return y
+
+    def local_solve(self, i: int, b: np.ndarray) -> np.ndarray:
+        b = np.ascontiguousarray(b, np.float64)
+        y = np.empty_like(b)
+        self._check(self.L.s2l_local_solve_host(self.h, i, _p(b, ctypes.c_double), _p(y, ctypes.c_double)))
+        return y
+
+    def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000):
+        b = np.ascontiguousarray(b, np.float64)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, np.float64).copy()
+        o = KrylovOpts(0, restart, rtol, 0.0, maxit, 0)
+        res = (KrylovResultC * nrhs)()
+        hist = np.zeros(nrhs * (maxit + 1))
+        self._check(self.L.s2l_krylov_solve_host(self.h, ctypes.byref(o), _p(b, ctypes.c_double),
+                                                 _p(x, ctypes.c_double), nrhs, res, _p(hist, ctypes.c_double)))
+        out = []
+        for r in range(nrhs):
+            R = res[r]
+            out.append(KrylovResult(bool(R.converged), bool(R.breakdown), R.iterations, R.restarts, R.rel_residual,
+                                    R.true_rel_residual, R.initial_residual_of, R.final_residual_of,
+                                    list(hist[r * (maxit + 1): r * (maxit + 1) + R.history_len])))
+        return x, (out[0] if nrhs == 1 else out)
+
+    # ------------------------------------------------------------------ operations (device memory)
+    def alloc(self, n: int) -> DeviceArray:
+        return DeviceArray(self, n)
+
+    def apply_device(self, r: DeviceArray, z: DeviceArray, nrhs: int = 1):
+        self._check(self.L.s2l_apply(self.h, r.ptr, z.ptr, nrhs, None))
+
+    def spmv_device(self, x: DeviceArray, y: DeviceArray, nrhs: int = 1):
+        self._check(self.L.s2l_spmv(self.h, x.ptr, y.ptr, nrhs, None))
+
+    def gmres_device(self, b: DeviceArray, x: DeviceArray, nrhs=1, restart=30, rtol=1e-8, maxit=1000):
+        o = KrylovOpts(0, restart, rtol, 0.0, maxit, 0)
+        res = (KrylovResultC * nrhs)()
+        self._check(self.L.s2l_krylov_solve(self.h, ctypes.byref(o), b.ptr, x.ptr, nrhs, res, None, None))
+        return [res[r] for r in range(nrhs)]
+
+    def synchronize(self):
+        self._check(self.L.s2l_synchronize(self.h))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._check(self.L.s2l_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def timing(self, on: bool):
+        self._check(self.L.s2l_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self, reset=True):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        self._check(self.L.s2l_timing_read(self.h, ctypes.byref(n), ctypes.byref(ms), 1 if reset else 0))
+        return n.value, ms.value
+
+    def set_launch_mode(self, mode: int):
+        self._check(self.L.s2l_set_local_launch_mode(self.h, mode))
+
+
+@dataclass
+class KrylovResult:
+    """include/solver/krylov.hpp:43-54."""
+
+    converged: bool = False
+    breakdown: bool = False
+    iterations: int = 0
+    restarts: int = 0
+    rel_residual: float = 0.0
+    true_rel_residual: float = 0.0
+    initial_residual_of: float = 0.0
+    final_residual_of: float = 0.0
+    history: list = field(default_factory=list)
+
+
+class HipTwoLevelSchwarz:
+    """prec::Preconditioner of the reference, applied on the GPU (two_level_schwarz.cpp:134-163)."""
+
+    def __init__(self, setup, precision: str = "double", device: int = 0, one_level: str | None = None,
+                 correction: str | None = None):
+        self.setup = setup
+        self.precision = precision
+        self.one_level = one_level or setup.one_level
+        self.correction = correction or setup.correction
+        self.ctx = HipContext(device)
+        self.ctx.upload_setup(setup, precision, self.one_level, self.correction)
+
+    def apply(self, r):
+        return self.ctx.apply(r)
+
+    def name(self) -> str:
+        """OneLevelSchwarz::name + ",deflated"/",additive" (two_level_schwarz.cpp:129-132)."""
+        return self.one_level.upper() + ("," + self.correction)
+
+    def update(self, level: str = "reuse") -> str:
+        """UpdateLevel semantics (preconditioner.hpp:30): nothing is recomputed on the device."""
+        if level not in ("reuse", "refactorize", "rebuild"):
+            raise ValueError(f"unknown update level '{level}'")
+        return "reuse"
+
+    def up_to_date(self) -> bool:
+        return True
+
+
+class HipGmres:
+    """solv::Gmres (gmres.hpp:39-71) with the loop on the device."""
+
+    def __init__(self, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
+        if stop != "l2":
+            raise ValueError("the device GMRES implements the l2 stopping norm")
+        self.m, self.rtol, self.maxit = restart, rtol, maxit
+
+    def name(self) -> str:
+        return f"GMRES({self.m})"
+
+    def solve(self, M: HipTwoLevelSchwarz, b, x=None):
+        return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit)
+
+
+def make_krylov_solver(krylov="gmres", restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
+    """makeKrylovSolver (src/solver/krylov.cpp:23-36)."""
+    if atol > 0.0 and stop != "openfoam":
+        raise ValueError("an absolute tolerance needs the OpenFOAM stopping norm")
+    if krylov == "gmres":
+        return HipGmres(restart, rtol, maxit, stop, atol)
+    raise ValueError(f"unknown Krylov solver '{krylov}' (gmres|bicgstab)")

@@ -1,0 +1,177 @@
+/*
+ * s2l_hip.h -- C-ABI of the MI355X apply path of schwarz2lvl's two-level overlapping Schwarz
+ * preconditioner inside right-preconditioned restarted GMRES / BiCGStab.
+ *
+ * One context holds every subdomain of the problem on one GPU. The host uploads once per setup
+ * (or refactorization) what the reference's setup produced; the apply, the SpMV and the Krylov
+ * iterations then run as HIP kernels. No torch types, plain pointers and sizes.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference
+ * repository). Conventions:
+ *   - return value: 0 on success, a negative S2L_E* code on failure; s2l_last_error(ctx) gives
+ *     the message (ctx may be NULL for failures of s2l_ctx_create). No exception crosses the ABI.
+ *   - ownership: the context copies every uploaded host array; the caller keeps its buffers.
+ *     Device buffers given to s2l_apply / s2l_spmv / s2l_krylov_solve belong to the caller.
+ *   - vectors are in the renumbered global order (subdomain p owns rows
+ *     [ownership[p], ownership[p+1]), src/partition/domain_decomposer.cpp:254-269); several
+ *     right-hand sides are stored one after the other (column-major n x nrhs).
+ *   - threading: a context is used by one host thread at a time (like the reference's
+ *     Preconditioner objects, whose apply() mutates work vectors); different contexts are
+ *     independent. All work of a context is ordered on the stream given (NULL = the context's).
+ */
+#ifndef S2L_HIP_H
+#define S2L_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    S2L_OK = 0,
+    S2L_EINVAL = -1,   /* invalid argument / shape */
+    S2L_ESTATE = -2,   /* call out of order (e.g. apply before upload) */
+    S2L_EHIP = -3,     /* HIP runtime error */
+    S2L_ELIMIT = -4,   /* a documented size limit is exceeded */
+    S2L_ENUMERIC = -5  /* singular coarse matrix, non-finite values */
+};
+
+typedef struct s2l_ctx s2l_ctx;
+
+/* Creates a context on HIP device @p device. Replaces the construction of the preconditioner
+ * objects (src/main.cpp:143-168); the process model of one rank per subdomain
+ * (README.md:281) becomes one context for all subdomains. */
+int s2l_ctx_create(int device, s2l_ctx** out);
+int s2l_ctx_destroy(s2l_ctx* ctx);
+const char* s2l_last_error(const s2l_ctx* ctx);
+
+/* Global operator A (n x n, CSR, int32 indices, FP64 values) in the renumbered order, together
+ * with the subdomain ownership (N + 1 offsets). Every row is reordered to the local column
+ * order of its owner (owned columns, then ghosts by increasing global id), which is the order
+ * of A(Omega_I, Omega_i) in DistributedMatrix::apply (include/linalg/distributed_matrix.hpp:33-38,
+ * include/partition/subdomain.hpp:112). Uploading again with the same pattern refreshes the values. */
+int s2l_upload_system(s2l_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
+                      int32_t nsub, const int32_t* ownership);
+
+/* Overlapping subdomains: local -> global maps Omega_i = [owned | ghosts sorted] concatenated
+ * (local_ptr: nsub + 1 offsets, gid: global ids), as Subdomain::globalId
+ * (src/partition/subdomain.cpp:150-161). @p pou: NULL for the Boolean partition of unity (1 on
+ * owned, 0 on ghosts), else D_i concatenated (src/partition/subdomain.cpp:281-302).
+ * @p one_level: 0 RAS, 1 ASM (src/preconditioner/one_level_schwarz.cpp:80-99). Replaces the
+ * HaloExchange pattern (src/linalg/halo_exchange.cpp): the forward exchange becomes the
+ * on-device gather x_loc[l] = x[gid[l]], the reverse add a deterministic gather-sum. */
+int s2l_upload_subdomains(s2l_ctx* ctx, int32_t nsub, const int32_t* local_ptr, const int32_t* gid, const double* pou,
+                          int one_level);
+
+/* Local factor of subdomain @p i: A_ii(perm, perm) = L U with perm[k] = local index of factor
+ * position k; L lower triangular in CSC (colptr n + 1, rows sorted, diagonal first), U given by
+ * U^T in the same form; ut_* = NULL means U = L^T (Cholesky). Replaces LocalSolver::factorize's
+ * result (include/linalg/local_solver.hpp:61-138; e.g. Eigen SimplicialLLT matrixL() with its
+ * permutation). The values are stored on the device in FP64 or FP32 according to
+ * s2l_finalize_local(). */
+int s2l_upload_local_factor(s2l_ctx* ctx, int32_t i, int32_t n, const int32_t* perm, const int64_t* l_colptr,
+                            const int32_t* l_row, const double* l_val, const int64_t* ut_colptr, const int32_t* ut_row,
+                            const double* ut_val);
+
+/* Packs every uploaded factor into the device layout (supernodes, level schedule) with values in
+ * @p precision: 0 FP64, 1 FP32 (--local-precision single, include/core/config.hpp:72: the
+ * right-hand side is rounded to float, both sweeps run in float, the result is widened, as
+ * solveIn<float> in src/linalg/local_solver.cpp:77-84). */
+int s2l_finalize_local(s2l_ctx* ctx, int precision);
+
+/* Coarse space: V_i blocks (rows x m_i, column-major, concatenated; rows = owned rows with the
+ * Boolean PoU, all local rows otherwise), offsets (nsub + 1) and A_00 dense row-major (n0 x n0).
+ * @p symmetric: Cholesky of A_00, else LU with partial pivoting (both FP64, factored on the host
+ * at upload). @p correction: 0 deflated, 1 additive (two_level_schwarz.hpp:69). n0 = 0 gives the
+ * one-level preconditioner (two_level_schwarz.cpp:136-139). Replaces CoarseSpace
+ * (src/preconditioner/coarse_space.cpp:208-246) and RootCoarseSolver
+ * (src/preconditioner/coarse_solver.cpp:94-179). */
+int s2l_upload_coarse(s2l_ctx* ctx, int32_t n0, const int64_t* offsets, const double* V, const double* E, int symmetric,
+                      int correction);
+
+/* z = M^{-1} r for @p nrhs right-hand sides (device pointers, n x nrhs). Replaces
+ * Preconditioner::apply (include/preconditioner/preconditioner.hpp:73) as implemented by
+ * TwoLevelSchwarz::apply (src/preconditioner/two_level_schwarz.cpp:134-163). @p stream: a
+ * hipStream_t or NULL. */
+int s2l_apply(s2l_ctx* ctx, const double* d_r, double* d_z, int32_t nrhs, void* stream);
+
+/* y = A x (device pointers). Replaces DistributedMatrix::apply
+ * (include/linalg/distributed_matrix.hpp:33-38). */
+int s2l_spmv(s2l_ctx* ctx, const double* d_x, double* d_y, int32_t nrhs, void* stream);
+
+/* Krylov solve with the uploaded operator and preconditioner. Replaces KrylovSolver::solve
+ * (include/solver/krylov.hpp:70-71) of Gmres (src/solver/gmres.cpp:18-161) and BiCgStab
+ * (src/solver/bicgstab.cpp:15-154), made by makeKrylovSolver (src/solver/krylov.cpp:23-36).
+ * d_x: initial guess in, solution out. Every right-hand side is an independent solve whose
+ * kernels are shared (batched launches); results[r] and history + r * (maxit + 1) hold its
+ * KrylovResult (include/solver/krylov.hpp:43-54). */
+typedef struct s2l_krylov_opts {
+    int method;       /* 0 GMRES(restart), 1 BiCGStab */
+    int restart;      /* GMRES restart length m (<= 64) */
+    double rtol;      /* relative tolerance */
+    double atol;      /* OpenFOAM norm only: absolute tolerance (0 = off) */
+    int maxit;        /* maximum applications of M^{-1} */
+    int stop_norm;    /* 0 l2, 1 openfoam (include/solver/krylov.hpp:29-45) */
+} s2l_krylov_opts;
+
+typedef struct s2l_krylov_result {
+    int converged;
+    int breakdown;
+    int iterations;
+    int restarts;
+    double rel_residual;
+    double true_rel_residual;
+    double initial_residual_of;
+    double final_residual_of;
+    int history_len;
+} s2l_krylov_result;
+
+int s2l_krylov_solve(s2l_ctx* ctx, const s2l_krylov_opts* opts, const double* d_b, double* d_x, int32_t nrhs,
+                     s2l_krylov_result* results, double* history, void* stream);
+
+/* ---- Host-memory conveniences (copy in, run on the context's stream, copy out, synchronize). */
+int s2l_apply_host(s2l_ctx* ctx, const double* r, double* z, int32_t nrhs);
+int s2l_spmv_host(s2l_ctx* ctx, const double* x, double* y, int32_t nrhs);
+int s2l_krylov_solve_host(s2l_ctx* ctx, const s2l_krylov_opts* opts, const double* b, double* x, int32_t nrhs,
+                          s2l_krylov_result* results, double* history);
+/* One local solve of subdomain i alone on the device, y = A_ii^{-1} b (local numbering); for
+ * parity tests of the batched triangular-solve kernel. */
+int s2l_local_solve_host(s2l_ctx* ctx, int32_t i, const double* b, double* y);
+
+/* ---- Device memory helpers (so that callers need no HIP headers). */
+int s2l_device_alloc(s2l_ctx* ctx, int64_t bytes, void** out);
+int s2l_device_free(s2l_ctx* ctx, void* p);
+int s2l_memcpy_h2d(s2l_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int s2l_memcpy_d2h(s2l_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int s2l_synchronize(s2l_ctx* ctx);
+
+/* ---- Introspection and measurement. */
+typedef struct s2l_stats {
+    int32_t n, nsub, n0, precision;
+    int64_t nnz_a;
+    int64_t factor_entries_fwd;   /* stored entries read by the forward sweeps */
+    int64_t factor_entries_bwd;   /* stored entries read by the backward sweeps */
+    int64_t local_rows;           /* sum of n_i */
+    int64_t coarse_rows;          /* rows of the V_i blocks */
+    int32_t levels_fwd_max, levels_bwd_max, supernodes;
+    double bytes_local_solve;     /* algorithmic HBM bytes of one batched local-solve launch (1 rhs) */
+    double bytes_apply;           /* algorithmic HBM bytes of one preconditioner application (1 rhs) */
+    double bytes_spmv;            /* algorithmic HBM bytes of one SpMV (1 rhs) */
+} s2l_stats;
+int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
+
+/* Event timing of the local-solve kernel: when enabled, every launch of it is bracketed by HIP
+ * events on its stream; s2l_timing_read returns the launch count and the summed milliseconds. */
+int s2l_timing_enable(s2l_ctx* ctx, int on);
+int s2l_timing_read(s2l_ctx* ctx, int64_t* launches, double* ms_total, int reset);
+
+/* Variant selection of the batched local solve (for the comparison with one launch per
+ * subdomain): 0 = one persistent launch for all subdomains (default), 1 = one launch per subdomain. */
+int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S2L_HIP_H */
