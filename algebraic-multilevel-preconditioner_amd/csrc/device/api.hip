@@ -188,10 +188,15 @@ struct s2l_ctx {
     Packed pk;
     DevBuf<SubDesc> sub_d;
     DevBuf<SnDesc> sn_d;
-    DevBuf<int32_t> lev_d, levsn_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
+    DevBuf<int32_t> lev_d, levsn_d, levw_d, levh_d, levc_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
+    size_t y_off = 0, st_off = 0;
     DevBuf<float> diag_f, below_f;
     DevBuf<double> diag_dd, below_dd;
-    DevBuf<unsigned char> tbuf;
+    DevBuf<unsigned char> tbuf, xgbuf;
+    DevBuf<TaskDesc> task_d;
+    DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
+    std::vector<int32_t> qsub_h;
+    int tasks_per_sub = 8;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
 
@@ -237,6 +242,7 @@ struct s2l_ctx {
         yvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
         tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nrhs * esz);
+        xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         work_nrhs = nrhs;
     }
 };
@@ -269,16 +275,21 @@ void require(bool c, int code, const std::string& m)
 
 hipStream_t pick(s2l_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
 
-// ---- local solve launch (all subdomains in one launch, or one launch per subdomain)
-template <typename T>
-void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip,
-                hipStream_t s)
+// ---- local solve launch: the persistent work-queue kernel over every subdomain (mode 0) or one
+//      launch per subdomain over its slice of the queue (mode 1)
+template <typename T> LocalArgs<T> localArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs,
+                                             const int32_t* skip)
 {
     LocalArgs<T> a{};
     a.sub = c->sub_d.p;
     a.sn = c->sn_d.p;
     a.lev = c->lev_d.p;
     a.lev_sn = c->levsn_d.p;
+    a.lev_wpre = c->levw_d.p;
+    a.lev_hpre = c->levh_d.p;
+    a.lev_chunk = c->levc_d.p;
+    a.y_off = static_cast<int32_t>(c->y_off);
+    a.st_off = static_cast<int32_t>(c->st_off);
     if constexpr (std::is_same_v<T, float>) {
         a.diag = c->diag_f.p;
         a.below = c->below_f.p;
@@ -301,6 +312,33 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
     a.n_local_total = c->local_total;
     a.t_total = c->pk.t_total;
     a.mode = mode;
+    a.task = c->task_d.p;
+    a.task_child = c->tchild_d.p;
+    a.task_rows = c->trows_d.p;
+    a.ext_rows = c->ext_d.p;
+    a.nrhs = nrhs;
+    a.xg = reinterpret_cast<T*>(c->xgbuf.p);
+    return a;
+}
+
+// One launch over @p count queue entries (x nrhs): a grid of one workgroup per entry, or a
+// single workgroup running them in order.
+template <typename T>
+void launchQueue(s2l_ctx* c, LocalArgs<T>& a, const int32_t* q, int32_t count, bool serial, hipStream_t s)
+{
+    if (count <= 0) return;
+    a.queue = q;
+    a.qcount = count;
+    a.serial = serial ? 1 : 0;
+    const int grid = serial ? 1 : count * a.nrhs;
+    launchLocalSolve<T>(a, grid, c->lds_bytes, s);
+}
+
+template <typename T>
+void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip,
+                hipStream_t s)
+{
+    LocalArgs<T> a = localArgs<T>(c, rhs, x0, out, mode, nrhs, skip);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         if (c->ev_pool.size() < 2) {
@@ -317,13 +355,13 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
         HIPCHECK(hipEventRecord(e0, s));
     }
     if (c->launch_mode == 0) {
-        a.sub_first = 0;
-        launchLocalSolve<T>(a, c->nsub, nrhs, c->lds_bytes, s);
+        // level-ordered queue: one launch per depth group, every subdomain batched
+        for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g)
+            launchQueue<T>(c, a, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
     } else {
-        for (int i = 0; i < c->nsub; ++i) {
-            a.sub_first = i;
-            launchLocalSolve<T>(a, 1, nrhs, c->lds_bytes, s);
-        }
+        for (int i = 0; i < c->nsub; ++i)
+            launchQueue<T>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
+                           c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, s);
     }
     HIPCHECK(hipGetLastError());
     if (c->timing) {
@@ -332,6 +370,7 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
         c->launches += 1;
     }
 }
+
 
 void collectTiming(s2l_ctx* c)
 {
@@ -768,14 +807,17 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
         HIPCHECK(hipSetDevice(c->device));
         std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
-        c->pk = packFactors(c->hf, moff);
+        c->pk = packFactors(c->hf, moff, c->tasks_per_sub);
         Packed& P = c->pk;
         for (int i = 0; i < c->nsub; ++i) {
             P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
             P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
         }
         const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
-        c->lds_bytes = static_cast<size_t>(P.max_n) * esz;
+        // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
+        c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
+        c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
+        c->lds_bytes = c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
         require(c->lds_bytes <= c->max_lds, S2L_ELIMIT,
                 "subdomain of " + std::to_string(P.max_n) + " unknowns exceeds the LDS of one workgroup (" + std::to_string(c->max_lds) +
                     " bytes)");
@@ -795,11 +837,21 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->sn_d.upload(P.sn, c->stream);
         c->lev_d.upload(P.lev, c->stream);
         c->levsn_d.upload(P.lev_sn, c->stream);
+        c->levw_d.upload(P.lev_wpre, c->stream);
+        c->levh_d.upload(P.lev_hpre, c->stream);
+        c->levc_d.upload(P.lev_chunk, c->stream);
         c->ridx_d.upload(P.ridx, c->stream);
         c->cptr_d.upload(P.cptr, c->stream);
         c->cidx_d.upload(P.cidx, c->stream);
         c->gmap_d.upload(gmap, c->stream);
         c->lperm_d.upload(lperm, c->stream);
+        c->task_d.upload(P.task, c->stream);
+        c->tchild_d.upload(P.task_child, c->stream);
+        c->trows_d.upload(P.task_rows, c->stream);
+        c->ext_d.upload(P.ext_rows, c->stream);
+        c->queue_d.upload(P.queue, c->stream);
+        c->qsubq_d.upload(P.queue_sub, c->stream);
+        c->qsub_h = P.qsub;
         if (precision == 1) {
             std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
             c->diag_f.upload(d, c->stream);
@@ -1006,38 +1058,17 @@ int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
         dg.upload(g, c->stream);
         auto run = [&](auto tag) {
             using T = decltype(tag);
-            LocalArgs<T> a{};
-            a.sub = c->sub_d.p;
-            a.sn = c->sn_d.p;
-            a.lev = c->lev_d.p;
-            a.lev_sn = c->levsn_d.p;
-            if constexpr (std::is_same_v<T, float>) {
-                a.diag = c->diag_f.p;
-                a.below = c->below_f.p;
-            } else {
-                a.diag = c->diag_dd.p;
-                a.below = c->below_dd.p;
-            }
-            a.ridx = c->ridx_d.p;
-            a.cptr = c->cptr_d.p;
-            a.cidx = c->cidx_d.p;
-            a.tbuf = reinterpret_cast<T*>(c->tbuf.p);
-            a.gmap = c->gmap_d.p;
-            a.lperm = c->lperm_d.p;
-            a.rhs = dg.p;
-            a.out = c->loc.p; // mode 1 without scaling: y by local index
-            a.n_global = c->n;
-            a.n_local_total = c->local_total;
-            a.t_total = c->pk.t_total;
-            a.sub_first = i;
-            a.mode = 1;
-            launchLocalSolve<T>(a, 1, 1, c->lds_bytes, c->stream);
+            LocalArgs<T> a = localArgs<T>(c, dg.p, nullptr, c->loc.p, 1, 1, nullptr);
+            a.pou = nullptr; // y by local index, no partition of unity
+            launchQueue<T>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
+                           c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, c->stream);
         };
         if (c->precision == 1)
             run(float{});
         else
             run(double{});
         HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(c->stream));
         HIPCHECK(hipMemcpyAsync(y, c->loc.p + l0, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
@@ -1147,6 +1178,15 @@ int s2l_timing_read(s2l_ctx* c, int64_t* launches, double* ms_total, int reset)
             c->launches = 0;
             c->ms_total = 0.0;
         }
+    });
+}
+
+int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
+{
+    return guarded(c, [&] {
+        require(c && tasks >= 1, S2L_EINVAL, "tasks per subdomain must be >= 1");
+        c->tasks_per_sub = tasks;
+        c->local_ready = false;
     });
 }
 

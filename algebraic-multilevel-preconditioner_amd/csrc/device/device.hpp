@@ -10,7 +10,7 @@
 
 namespace s2l {
 
-constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per subdomain
+constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per queue entry
 constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
 
 /// Arguments of the batched local solve (all pointers are device pointers).
@@ -19,11 +19,16 @@ template <typename T> struct LocalArgs {
     const SnDesc* sn;
     const int32_t* lev;
     const int32_t* lev_sn;
+    const int32_t* lev_wpre;  ///< per small supernode: widths of the small supernodes before it in its level
+    const int32_t* lev_hpre;  ///< same for the heights
+    const int32_t* lev_chunk; ///< chunk starts of every level
+    int32_t y_off;            ///< byte offset of the result buffer in the dynamic LDS
+    int32_t st_off;           ///< byte offset of the staged supernode descriptors
     const T* diag;
     const T* below;
     const int32_t* ridx;
     const int32_t* cptr;
-    const int32_t* cidx;
+    const int32_t* cidx;  ///< slot of every pushed product
     T* tbuf;              ///< T buffer, t_total entries per right-hand side
     const int32_t* gmap;  ///< factor position -> global row (gather)
     const int32_t* lperm; ///< factor position -> local index
@@ -35,8 +40,17 @@ template <typename T> struct LocalArgs {
     int64_t n_global;
     int64_t n_local_total;
     int64_t t_total;
-    int32_t sub_first;    ///< first subdomain of this launch
     int32_t mode;         ///< 0 RAS with Boolean PoU (direct write), 1 local buffer for a reverse add
+    // ---- level-ordered work queue
+    const TaskDesc* task;
+    const int32_t* task_child;
+    const int32_t* task_rows;
+    const int32_t* ext_rows;
+    const int32_t* queue;  ///< entries (task << 1 | phase) of this launch
+    int32_t qcount;        ///< entries per right-hand side in this launch
+    int32_t serial;        ///< 1: one workgroup runs all entries in order
+    int32_t nrhs;
+    T* xg;                 ///< per-task results shared between entries (n_local_total per rhs)
 };
 
 /// Coarse space arguments.
@@ -65,7 +79,8 @@ struct GmState {
     int32_t pad[2];
 };
 
-template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int nsub, int nrhs, size_t lds, hipStream_t s);
+template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int grid, size_t lds, hipStream_t s);
+template <typename T> const void* localSolveKernel();
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s);

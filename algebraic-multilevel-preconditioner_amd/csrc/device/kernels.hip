@@ -17,204 +17,294 @@ namespace {
 
 constexpr int kWave = 64;
 
-__device__ inline int pow2ceil(int w)
-{
-    int p = 1;
-    while (p < w) p <<= 1;
-    return p;
-}
-
 template <typename T> __device__ inline T waveSum(T v)
 {
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
 
-// ------------------------------------------------------------------ local solve: forward sweep
-template <typename T>
-__device__ void fwdSmall(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s, T* x, T* tb, int lane)
+// x[row] minus the products pushed to this row by its descendants: one contiguous slot range.
+template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const SubDesc& sd, const T* tb, const T* x, int row)
 {
-    const int w = s.w, h = s.h, c0 = s.c0;
-    T acc = T(0);
-    if (lane < w) {
-        const int row = c0 + lane;
-        T v = x[row];
-        const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
-        for (int q = cb; q < ce; ++q) v -= tb[a.cidx[q]];
-        acc = v;
+    T v = x[row];
+    const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
+    int q = cb;
+    for (; q + 4 <= ce; q += 4) {
+        const T t0 = tb[q], t1 = tb[q + 1], t2 = tb[q + 2], t3 = tb[q + 3];
+        v -= t0;
+        v -= t1;
+        v -= t2;
+        v -= t3;
     }
-    const T* F = a.diag + s.diag;
-    for (int c = 0; c < w; ++c) {
-        const T xc = __shfl(acc, c, kWave) / F[c * w + c];
-        if (lane == c)
-            acc = xc;
-        else if (lane > c && lane < w)
-            acc -= F[c * w + lane] * xc;
-    }
-    if (lane < w) x[c0 + lane] = acc;
-    if (h > 0) {
-        const int wp = pow2ceil(w), G = kWave / wp, g = lane / wp, c = lane % wp;
-        const T xc = __shfl(acc, c < w ? c : 0, kWave);
-        const T* B = a.below + s.below;
-        for (int kb = 0; kb < h; kb += G) {
-            const int k = kb + g;
-            T v = (k < h && c < w) ? B[static_cast<int64_t>(k) * w + c] * xc : T(0);
-            for (int off = wp >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-            if (c == 0 && k < h) tb[s.toff + k] = v;
-        }
-    }
+    for (; q < ce; ++q) v -= tb[q];
+    return v;
 }
 
+// ------------------------------------------------------------------ wide supernodes (w > 64): whole workgroup
+// The diagonal block is applied as its inverse (row-major, computed at upload): y = X x_C with a
+// wavefront per row of X, then the rows below push their products (forward), or the columns pull
+// the products with the rows below first (backward).
 template <typename T>
-__device__ void fwdBig(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s, T* x, T* tb, int tid, int lane, int wave,
-                       int nw)
+__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw)
 {
-    const int w = s.w, h = s.h, c0 = s.c0, nt = nw * kWave;
-    for (int j = tid; j < w; j += nt) {
-        const int row = c0 + j;
-        T v = x[row];
-        const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
-        for (int q = cb; q < ce; ++q) v -= tb[a.cidx[q]];
-        x[row] = v;
+    for (int j = wave; j < w; j += nw) {
+        const T* row = X + static_cast<int64_t>(j) * w;
+        const int lo = lower ? 0 : j, hi = lower ? j + 1 : w;
+        T v = T(0);
+        for (int c = lo + lane; c < hi; c += kWave) v += row[c] * x[c0 + c];
+        v = waveSum(v);
+        if (lane == 0) ybuf[j] = v;
     }
     __syncthreads();
-    const T* F = a.diag + s.diag;
-    for (int pc = 0; pc < w; pc += kWave) {
-        const int pw = min(kWave, w - pc);
-        if (wave == 0) {
-            T acc = lane < pw ? x[c0 + pc + lane] : T(0);
-            for (int c = 0; c < pw; ++c) {
-                const T* Fc = F + static_cast<int64_t>(pc + c) * w + pc;
-                const T xc = __shfl(acc, c, kWave) / Fc[c];
-                if (lane == c)
-                    acc = xc;
-                else if (lane > c && lane < pw)
-                    acc -= Fc[lane] * xc;
-            }
-            if (lane < pw) x[c0 + pc + lane] = acc;
-        }
-        __syncthreads();
-        for (int j = pc + pw + tid; j < w; j += nt) {
-            T sum = T(0);
-            for (int c = 0; c < pw; ++c) sum += F[static_cast<int64_t>(pc + c) * w + j] * x[c0 + pc + c];
-            x[c0 + j] -= sum;
-        }
-        __syncthreads();
-    }
+    for (int j = tid; j < w; j += nw * kWave) x[c0 + j] = ybuf[j];
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, T* tb, int tid, int lane,
+                        int wave, int nt)
+{
+    const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
+    for (int j = tid; j < w; j += nt) x[c0 + j] = pullRow(a, sd, tb, x, c0 + j);
+    __syncthreads();
+    wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, lane, wave, nw);
     const T* B = a.below + s.below;
-    for (int k = wave; k < h; k += nw) {
-        T v = T(0);
-        for (int c = lane; c < w; c += kWave) v += B[static_cast<int64_t>(k) * w + c] * x[c0 + c];
-        v = waveSum(v);
-        if (lane == 0) tb[s.toff + k] = v;
+    const int32_t* dst = a.cidx + sd.dst_off + s.toff;
+    for (int k = wave * 4; k < h; k += nw * 4) {
+        T v[4] = {T(0), T(0), T(0), T(0)};
+        for (int c = lane; c < w; c += kWave) {
+            const T xv = x[c0 + c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (k + q < h) v[q] += B[static_cast<int64_t>(k + q) * w + c] * xv;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T r = waveSum(v[q]);
+            if (lane == 0 && k + q < h) tb[dst[k + q]] = r;
+        }
     }
 }
 
-// ------------------------------------------------------------------ local solve: backward sweep
-template <typename T> __device__ void bwdSmall(const LocalArgs<T>& a, const SnDesc s, T* x, int lane)
+template <typename T> __device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
-    const int w = s.w, h = s.h, c0 = s.c0;
-    const int wp = pow2ceil(w), G = kWave / wp, g = lane / wp, c = lane % wp;
-    T part = T(0);
-    if (c < w && h > 0) {
-        const T* B = a.below + s.below;
-        const int32_t* R = a.ridx + s.ridx;
-        for (int k = g; k < h; k += G) part += B[static_cast<int64_t>(k) * w + c] * x[R[k]];
-    }
-    for (int off = wp; off < kWave; off <<= 1) part += __shfl_xor(part, off, kWave);
-    T acc = (c < w) ? x[c0 + c] - part : T(0);
-    const T* Gd = a.diag + s.diag;
-    for (int cc = w - 1; cc >= 0; --cc) {
-        const T xc = __shfl(acc, cc, kWave) / Gd[cc * w + cc];
-        if (c == cc)
-            acc = xc;
-        else if (c < cc)
-            acc -= Gd[cc * w + c] * xc;
-    }
-    if (lane < w) x[c0 + lane] = acc;
-}
-
-template <typename T> __device__ void bwdBig(const LocalArgs<T>& a, const SnDesc s, T* x, int tid, int lane, int wave, int nw)
-{
-    const int w = s.w, h = s.h, c0 = s.c0, nt = nw * kWave;
+    const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
     if (h > 0) {
+        // G groups of w threads, group g sums the rows k = g, g + G, ...; partial sums in ybuf,
+        // combined in group order
+        const int G = max(1, min(nt / w, kChunkRows / w));
         const T* B = a.below + s.below;
         const int32_t* R = a.ridx + s.ridx;
+        for (int u = tid; u < G * w; u += nt) {
+            const int g = u / w, c = u - g * w;
+            T sum = T(0);
+#pragma unroll 4
+            for (int k = g; k < h; k += G) sum += B[static_cast<int64_t>(k) * w + c] * x[R[k]];
+            ybuf[u] = sum;
+        }
+        __syncthreads();
         for (int c = tid; c < w; c += nt) {
             T sum = T(0);
-            for (int k = 0; k < h; ++k) sum += B[static_cast<int64_t>(k) * w + c] * x[R[k]];
+            for (int g = 0; g < G; ++g) sum += ybuf[g * w + c];
             x[c0 + c] -= sum;
         }
-    }
-    __syncthreads();
-    const T* Gd = a.diag + s.diag;
-    const int np = (w + kWave - 1) / kWave;
-    for (int pi = np - 1; pi >= 0; --pi) {
-        const int pc = pi * kWave, pw = min(kWave, w - pc);
-        if (wave == 0) {
-            T acc = lane < pw ? x[c0 + pc + lane] : T(0);
-            for (int cc = pw - 1; cc >= 0; --cc) {
-                const T* Gc = Gd + static_cast<int64_t>(pc + cc) * w + pc;
-                const T xc = __shfl(acc, cc, kWave) / Gc[cc];
-                if (lane == cc)
-                    acc = xc;
-                else if (lane < cc)
-                    acc -= Gc[lane] * xc;
-            }
-            if (lane < pw) x[c0 + pc + lane] = acc;
-        }
-        __syncthreads();
-        for (int j = tid; j < pc; j += nt) {
-            T sum = T(0);
-            for (int cc = 0; cc < pw; ++cc) sum += Gd[static_cast<int64_t>(pc + cc) * w + j] * x[c0 + pc + cc];
-            x[c0 + j] -= sum;
-        }
         __syncthreads();
     }
+    wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, lane, wave, nw);
 }
 
-template <typename T> __global__ __launch_bounds__(kLocalThreads) void k_local_solve(LocalArgs<T> a)
+// ------------------------------------------------------------------ small supernodes of one level, all at once
+// Index of the last prefix entry <= t (prefix[0] = 0, nondecreasing).
+__device__ inline int findItem(const int* prefix, int count, int t)
 {
-    const int r = blockIdx.y;
-    if (a.skip && a.skip[r]) return;
-    const SubDesc sd = a.sub[a.sub_first + blockIdx.x];
-    extern __shared__ unsigned char smem_raw[];
-    T* x = reinterpret_cast<T*>(smem_raw);
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave, nw = blockDim.x / kWave;
-    const double* rhs = a.rhs + static_cast<int64_t>(r) * a.n_global;
-    T* tb = a.tbuf + static_cast<int64_t>(r) * a.t_total + sd.t_off;
-    for (int p = tid; p < sd.n; p += blockDim.x) x[p] = static_cast<T>(rhs[a.gmap[sd.map_off + p]]);
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// One level of a sweep. The level's small supernodes are processed by chunks, each with every
+// thread of the workgroup on one row (or column) at a time:
+//   forward:  pull the slots of every row; y = inverse(diagonal block) * row values; push the
+//             products of the rows below to their slots;
+//   backward: every column subtracts the products with the rows below; y = inverse(diagonal) * x.
+// Then the level's wide supernodes, one after the other.
+template <typename T>
+__device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv, bool forward, T* x, T* ybuf, SnDesc* st,
+                         int* wp, int* hp, T* tb, int tid, int lane, int wave, int nt)
+{
+    const int end = lv[1], split = lv[2], wsum = lv[3], hsum = lv[4], coff = lv[5], nchunk = lv[6];
+    for (int ch = 0; ch < nchunk; ++ch) {
+        const int q0 = a.lev_chunk[coff + ch], q1 = a.lev_chunk[coff + ch + 1];
+        const int cnt = q1 - q0;
+        const int wbase = a.lev_wpre[q0], hbase = a.lev_hpre[q0];
+        const int W = (q1 < split ? a.lev_wpre[q1] : wsum) - wbase;
+        const int H = (q1 < split ? a.lev_hpre[q1] : hsum) - hbase;
+        for (int i = tid; i < cnt; i += nt) {
+            st[i] = a.sn[a.lev_sn[q0 + i]];
+            wp[i] = a.lev_wpre[q0 + i] - wbase;
+            hp[i] = a.lev_hpre[q0 + i] - hbase;
+        }
+        __syncthreads();
+        if (forward) {
+            for (int t = tid; t < W; t += nt) {
+                const int i = findItem(wp, cnt, t);
+                const int row = st[i].c0 + (t - wp[i]);
+                x[row] = pullRow(a, sd, tb, x, row);
+            }
+        } else {
+            for (int t = tid; t < W; t += nt) {
+                const int i = findItem(wp, cnt, t);
+                const SnDesc& s = st[i];
+                const int c = t - wp[i];
+                if (s.h > 0) {
+                    const T* B = a.below + s.below + c;
+                    const int32_t* R = a.ridx + s.ridx;
+                    T sum = T(0);
+#pragma unroll 8
+                    for (int k = 0; k < s.h; ++k) sum += B[static_cast<int64_t>(k) * s.w] * x[R[k]];
+                    x[s.c0 + c] -= sum;
+                }
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < W; t += nt) {
+            const int i = findItem(wp, cnt, t);
+            const SnDesc& s = st[i];
+            const int j = t - wp[i];
+            const T* D = a.diag + s.diag;
+            T y = T(0);
+            if (forward) {
+#pragma unroll 8
+                for (int c = 0; c <= j; ++c) y += D[c * s.w + j] * x[s.c0 + c];
+            } else {
+#pragma unroll 8
+                for (int c = j; c < s.w; ++c) y += D[c * s.w + j] * x[s.c0 + c];
+            }
+            ybuf[t] = y;
+        }
+        __syncthreads();
+        for (int t = tid; t < W; t += nt) {
+            const int i = findItem(wp, cnt, t);
+            x[st[i].c0 + (t - wp[i])] = ybuf[t];
+        }
+        __syncthreads();
+        if (forward) {
+            for (int t = tid; t < H; t += nt) {
+                const int i = findItem(hp, cnt, t);
+                const SnDesc& s = st[i];
+                const int k = t - hp[i];
+                const T* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
+                T v = T(0);
+#pragma unroll 8
+                for (int c = 0; c < s.w; ++c) v += B[c] * x[s.c0 + c];
+                tb[a.cidx[sd.dst_off + s.toff + k]] = v;
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = split; q < end; ++q) {
+        if (forward)
+            fwdWide(a, sd, a.sn[a.lev_sn[q]], x, ybuf, tb, tid, lane, wave, nt);
+        else
+            bwdWide(a, a.sn[a.lev_sn[q]], x, ybuf, tid, lane, wave, nt);
+    }
     __syncthreads();
-    for (int L = 0; L < sd.f_nlev; ++L) {
-        const int beg = a.lev[sd.f_lev + 2 * L], split = a.lev[sd.f_lev + 2 * L + 1], end = a.lev[sd.f_lev + 2 * L + 2];
-        for (int q = beg + wave; q < split; q += nw) fwdSmall(a, sd, a.sn[a.lev_sn[q]], x, tb, lane);
-        for (int q = split; q < end; ++q) fwdBig(a, sd, a.sn[a.lev_sn[q]], x, tb, tid, lane, wave, nw);
+}
+
+template <typename T>
+__device__ void runLevels(const LocalArgs<T>& a, const SubDesc& sd, int lev0, int nlev, bool forward, T* x, unsigned char* smem,
+                          T* tb, int tid, int lane, int wave, int nt)
+{
+    T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
+    SnDesc* st = reinterpret_cast<SnDesc*>(smem + a.st_off);
+    int* wp = reinterpret_cast<int*>(smem + a.st_off + kChunkSn * sizeof(SnDesc));
+    int* hp = wp + kChunkSn;
+    for (int L = 0; L < nlev; ++L) runLevel(a, sd, a.lev + lev0 + 7 * L, forward, x, ybuf, st, wp, hp, tb, tid, lane, wave, nt);
+}
+
+// One entry of the level-ordered queue: a task's forward sweep, its backward sweep, or both for a
+// task without parent. Entries of one launch are independent (one depth group of the queue, or
+// one subdomain's entries run in order by a single workgroup); a later launch on the same stream
+// sees the T buffer and the shared results of earlier launches.
+template <typename T>
+__device__ void runEntry(const LocalArgs<T>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
+{
+    const int r = static_cast<int>(e / a.qcount);
+    if (a.skip && a.skip[r]) return;
+    const int code = a.queue[e % a.qcount];
+    const int tk = code >> 1;
+    const bool backward = (code & 1) != 0;
+    const TaskDesc td = a.task[tk];
+    const bool combined = td.parent < 0;
+    const SubDesc sd = a.sub[td.sub];
+    T* tb = a.tbuf + static_cast<int64_t>(r) * a.t_total + sd.t_off;
+    T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
+    const int32_t* rows = a.task_rows + td.rows_first;
+    if (!backward) {
+        const double* rhs = a.rhs + static_cast<int64_t>(r) * a.n_global;
+        for (int q = tid; q < td.rows_count; q += blockDim.x) {
+            const int row = rows[q];
+            x[row] = static_cast<T>(rhs[a.gmap[sd.map_off + row]]);
+        }
+        __syncthreads();
+        runLevels(a, sd, td.f_lev, td.f_nlev, true, x, smem, tb, tid, lane, wave, nt);
+        if (!combined) {
+            for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
+            __syncthreads();
+            return;
+        }
+    } else {
+        for (int q = tid; q < td.rows_count; q += blockDim.x) x[rows[q]] = xg[rows[q]];
+        const int32_t* ext = a.ext_rows + td.ext_first;
+        for (int q = tid; q < td.ext_count; q += blockDim.x) x[ext[q]] = xg[ext[q]];
         __syncthreads();
     }
-    for (int L = 0; L < sd.b_nlev; ++L) {
-        const int beg = a.lev[sd.b_lev + 2 * L], split = a.lev[sd.b_lev + 2 * L + 1], end = a.lev[sd.b_lev + 2 * L + 2];
-        for (int q = beg + wave; q < split; q += nw) bwdSmall(a, a.sn[a.lev_sn[q]], x, lane);
-        for (int q = split; q < end; ++q) bwdBig(a, a.sn[a.lev_sn[q]], x, tid, lane, wave, nw);
-        __syncthreads();
-    }
+    runLevels(a, sd, td.b_lev, td.b_nlev, false, x, smem, tb, tid, lane, wave, nt);
+    if (td.nchild > 0)
+        for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
     if (a.mode == 0) {
         double* out = a.out + static_cast<int64_t>(r) * a.n_global;
         const double* x0 = a.x0 ? a.x0 + static_cast<int64_t>(r) * a.n_global : nullptr;
-        for (int p = tid; p < sd.n; p += blockDim.x) {
-            const int l = a.lperm[sd.map_off + p];
+        for (int q = tid; q < td.rows_count; q += blockDim.x) {
+            const int row = rows[q];
+            const int l = a.lperm[sd.map_off + row];
             if (l < sd.n_owned) {
                 const int64_t gi = sd.begin + l;
-                const double y = static_cast<double>(x[p]);
+                const double y = static_cast<double>(x[row]);
                 out[gi] = x0 ? x0[gi] + y : y;
             }
         }
     } else {
         double* out = a.out + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
-        for (int p = tid; p < sd.n; p += blockDim.x) {
-            const int l = a.lperm[sd.map_off + p];
-            const double y = static_cast<double>(x[p]);
+        for (int q = tid; q < td.rows_count; q += blockDim.x) {
+            const int row = rows[q];
+            const int l = a.lperm[sd.map_off + row];
+            const double y = static_cast<double>(x[row]);
             out[l] = a.pou ? a.pou[sd.map_off + l] * y : y;
         }
+    }
+    __syncthreads();
+}
+
+// Batched local solve. Grid mode: workgroup b runs entry b of the launch (all subdomains' tasks
+// of one depth group, for every right-hand side). Serial mode: one workgroup runs the entries of
+// one subdomain in queue order (the one-launch-per-subdomain comparison and single solves).
+template <typename T> __global__ __launch_bounds__(kLocalThreads) void k_local_solve(LocalArgs<T> a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* x = reinterpret_cast<T*>(smem_raw);
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave, nt = blockDim.x;
+    if (a.serial) {
+        const long long total = static_cast<long long>(a.qcount) * a.nrhs;
+        for (long long e = 0; e < total; ++e) runEntry(a, e, x, smem_raw, tid, lane, wave, nt);
+    } else {
+        runEntry(a, static_cast<long long>(blockIdx.x), x, smem_raw, tid, lane, wave, nt);
     }
 }
 
@@ -616,12 +706,12 @@ inline unsigned nblk(int64_t n, int t) { return static_cast<unsigned>((n + t - 1
 } // namespace
 
 // ------------------------------------------------------------------ launchers
-template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int nsub, int nrhs, size_t lds, hipStream_t s)
+template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_local_solve<T>, dim3(nsub, nrhs), dim3(kLocalThreads), lds, s, a);
+    hipLaunchKernelGGL(k_local_solve<T>, dim3(grid), dim3(kLocalThreads), lds, s, a);
 }
-template void launchLocalSolve<float>(const LocalArgs<float>&, int, int, size_t, hipStream_t);
-template void launchLocalSolve<double>(const LocalArgs<double>&, int, int, size_t, hipStream_t);
+template void launchLocalSolve<float>(const LocalArgs<float>&, int, size_t, hipStream_t);
+template void launchLocalSolve<double>(const LocalArgs<double>&, int, size_t, hipStream_t);
 
 template <typename T> const void* localSolveKernel() { return reinterpret_cast<const void*>(&k_local_solve<T>); }
 template const void* localSolveKernel<float>();

@@ -1,21 +1,23 @@
-// pack.cpp -- supernode detection, level schedule and value layout of the batched local solve.
+// pack.cpp -- supernode detection, task split, level schedule and value layout of the batched
+// local solve.
 //
 // Input: the factors of all subdomains in compressed-column form (L, and U^T for LU), the form
-// the reference's LocalSolver backends expose. Output: one table of supernodes per sweep, level
-// lists (forward: height above the leaves of the supernodal elimination tree; backward: depth
-// below the root), dense blocks (diagonal block column-major, rows below row-major) and, for the
-// forward sweep, the lists of products each row pulls from its descendants.
+// the reference's LocalSolver backends expose. Output: one table of supernodes per sweep; dense
+// blocks (diagonal block column-major, rows below row-major); for the forward sweep the lists of
+// products each row pulls from its descendants; and the tasks of the work queue: every subtree
+// below a split depth of the supernodal elimination tree is one task, every supernode above it
+// is one task. The queue lists the forward entries by decreasing task depth, then the backward
+// entries by increasing depth, all subdomains interleaved (a level-ordered work queue).
 #include "pack.hpp"
 
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 #include <string>
 
 namespace s2l {
 
 namespace {
-
-constexpr int32_t kSmallWidth = 64; // supernodes up to one wavefront wide are solved by one wave
 
 std::vector<int32_t> detectSupernodes(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& row)
 {
@@ -36,16 +38,110 @@ std::vector<int32_t> detectSupernodes(int32_t n, const std::vector<int64_t>& cp,
     return starts;
 }
 
-struct SweepBuild {
-    std::vector<int32_t> starts;
-    std::vector<int32_t> parent;
-};
+// Appends a level table, 7 ints per level: start and end of the level in lev_sn, start of its wide
+// supernodes (the small ones come first), sum of the small widths, sum of the small heights,
+// offset of its chunk list in lev_chunk and number of chunks. A chunk is a run of small
+// supernodes processed together: at most kChunkSn of them and at most kChunkRows rows in total
+// (the size of the kernel's staging area and result buffer in LDS). lev_wpre / lev_hpre hold, for
+// every small supernode, the sum of the widths / heights of the small supernodes before it in its
+// level.
+void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, int32_t& lev_off, int32_t& nlev)
+{
+    lev_off = static_cast<int32_t>(P.lev.size());
+    nlev = static_cast<int32_t>(byLevel.size());
+    for (const auto& list : byLevel) {
+        const int32_t start = static_cast<int32_t>(P.lev_sn.size());
+        int32_t wsum = 0, hsum = 0;
+        for (int wide = 0; wide < 2; ++wide)
+            for (int32_t s : list) {
+                const SnDesc& d = P.sn[static_cast<std::size_t>(s)];
+                if ((d.w > kSmallWidth) != (wide == 1)) continue;
+                P.lev_sn.push_back(s);
+                P.lev_wpre.push_back(wide ? 0 : wsum);
+                P.lev_hpre.push_back(wide ? 0 : hsum);
+                if (!wide) {
+                    wsum += d.w;
+                    hsum += d.h;
+                }
+            }
+        const int32_t end = static_cast<int32_t>(P.lev_sn.size());
+        int32_t split = start;
+        while (split < end && P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(split)])].w <= kSmallWidth) ++split;
+        const int32_t coff = static_cast<int32_t>(P.lev_chunk.size());
+        int32_t q = start;
+        while (q < split) {
+            P.lev_chunk.push_back(q);
+            int32_t rows = 0, cnt = 0;
+            while (q < split && cnt < kChunkSn &&
+                   rows + P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(q)])].w <= kChunkRows) {
+                rows += P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(q)])].w;
+                ++cnt;
+                ++q;
+            }
+        }
+        const int32_t nchunk = static_cast<int32_t>(P.lev_chunk.size()) - coff;
+        P.lev_chunk.push_back(split);
+        P.lev.insert(P.lev.end(), {start, end, split, wsum, hsum, coff, nchunk});
+    }
+}
+
+// In-place inverse of the lower triangular m x m block stored column-major at B with leading
+// dimension ld (B[c * ld + r] = L(r, c), r >= c), in FP64: column j of the inverse solves L x = e_j.
+void invertLower(double* B, int ld, int m)
+{
+    std::vector<double> X(static_cast<std::size_t>(m) * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        double* x = X.data() + static_cast<std::size_t>(j) * m; // column j of the inverse
+        x[j] = 1.0;
+        for (int c = j; c < m; ++c) {
+            x[c] /= B[static_cast<std::size_t>(c) * ld + c];
+            const double xc = x[c];
+            for (int r = c + 1; r < m; ++r) x[r] -= B[static_cast<std::size_t>(c) * ld + r] * xc;
+        }
+    }
+    for (int c = 0; c < m; ++c)
+        for (int r = 0; r < m; ++r) B[static_cast<std::size_t>(c) * ld + r] = (r >= c) ? X[static_cast<std::size_t>(c) * m + r] : 0.0;
+}
+
+// Same for an upper triangular block (B[c * ld + r] = U(r, c), r <= c): column j solves U x = e_j.
+void invertUpper(double* B, int ld, int m)
+{
+    std::vector<double> X(static_cast<std::size_t>(m) * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        double* x = X.data() + static_cast<std::size_t>(j) * m;
+        x[j] = 1.0;
+        for (int c = j; c >= 0; --c) {
+            x[c] /= B[static_cast<std::size_t>(c) * ld + c];
+            const double xc = x[c];
+            for (int r = 0; r < c; ++r) x[r] -= B[static_cast<std::size_t>(c) * ld + r] * xc;
+        }
+    }
+    for (int c = 0; c < m; ++c)
+        for (int r = 0; r < m; ++r) B[static_cast<std::size_t>(c) * ld + r] = (r <= c) ? X[static_cast<std::size_t>(c) * m + r] : 0.0;
+}
+
+// Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64;
+// column-major for a small supernode (a thread per row walks a column-major block: coalesced over
+// the rows), row-major for a wide one (a wavefront per row reads it contiguously).
+void invertDiagonal(double* B, int w, bool lower)
+{
+    if (lower)
+        invertLower(B, w, w);
+    else
+        invertUpper(B, w, w);
+    if (w > kSmallWidth)
+        for (int r = 0; r < w; ++r)
+            for (int c = r + 1; c < w; ++c) std::swap(B[static_cast<std::size_t>(c) * w + r], B[static_cast<std::size_t>(r) * w + c]);
+}
 
 } // namespace
 
-Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off)
+Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off, int tasks_per_sub)
 {
     Packed P;
+    // queue groups, keyed by (phase, depth): forward groups in decreasing depth, then backward
+    std::map<std::pair<int, int>, std::vector<int32_t>> groups; // key (0, -depth) / (1, depth)
+    std::vector<std::vector<int32_t>> perSub(fs.size());
     for (std::size_t i = 0; i < fs.size(); ++i) {
         const HostFactor& F = fs[i];
         const int32_t n = F.n;
@@ -66,9 +162,8 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         for (int32_t s = 0; s < nsf; ++s)
             for (int32_t j = fst[s]; j < fst[s + 1]; ++j) snodeOf[static_cast<std::size_t>(j)] = s;
         const int32_t f_base = static_cast<int32_t>(P.sn.size());
-        std::vector<int32_t> parentF(static_cast<std::size_t>(nsf), -1);
+        std::vector<int32_t> parent(static_cast<std::size_t>(nsf), -1);
         int32_t toff = 0;
-        // below blocks of the forward sweep, remembered for the Cholesky backward sweep
         std::vector<int64_t> belowOff(static_cast<std::size_t>(nsf)), ridxOff(static_cast<std::size_t>(nsf));
         for (int32_t s = 0; s < nsf; ++s) {
             const int32_t c0 = fst[s], w = fst[s + 1] - c0;
@@ -76,6 +171,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             const int32_t h = static_cast<int32_t>(F.lcp[c0 + 1] - p0) - w;
             for (int32_t r = 0; r < w; ++r)
                 if (F.lr[p0 + r] != c0 + r) throw std::invalid_argument("malformed supernode");
+            if (w > kChunkRows) throw std::invalid_argument("supernode wider than " + std::to_string(kChunkRows) + " columns");
             SnDesc d{};
             d.c0 = c0;
             d.w = w;
@@ -85,6 +181,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             d.diag = static_cast<int64_t>(P.diag.size());
             for (int32_t c = 0; c < w; ++c)
                 for (int32_t r = 0; r < w; ++r) P.diag.push_back(r >= c ? F.lv[F.lcp[c0 + c] + (r - c)] : 0.0);
+            invertDiagonal(P.diag.data() + d.diag, w, true);
             d.below = static_cast<int64_t>(P.below.size());
             for (int32_t k = 0; k < h; ++k)
                 for (int32_t c = 0; c < w; ++c) P.below.push_back(F.lv[F.lcp[c0 + c] + (w - c) + k]);
@@ -92,12 +189,13 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             for (int32_t k = 0; k < h; ++k) P.ridx.push_back(F.lr[p0 + w + k]);
             belowOff[static_cast<std::size_t>(s)] = d.below;
             ridxOff[static_cast<std::size_t>(s)] = d.ridx;
-            if (h > 0) parentF[static_cast<std::size_t>(s)] = snodeOf[static_cast<std::size_t>(F.lr[p0 + w])];
+            if (h > 0) parent[static_cast<std::size_t>(s)] = snodeOf[static_cast<std::size_t>(F.lr[p0 + w])];
             P.sn.push_back(d);
             P.entries_fwd += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
         }
         P.t_total += toff;
-        // contributions: row R_d[k] pulls T[toff_d + k]
+        // contributions: the product of row k below supernode d goes to a slot of its target row;
+        // the slots of one row are contiguous (ordered by d), so a row pulls one contiguous range
         {
             std::vector<int32_t> cnt(static_cast<std::size_t>(n) + 1, 0);
             for (int32_t s = 0; s < nsf; ++s) {
@@ -105,47 +203,25 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 for (int32_t k = 0; k < d.h; ++k) ++cnt[static_cast<std::size_t>(P.ridx[static_cast<std::size_t>(d.ridx + k)]) + 1];
             }
             sd.cptr_off = static_cast<int32_t>(P.cptr.size());
-            const int32_t cbase = static_cast<int32_t>(P.cidx.size());
+            sd.dst_off = static_cast<int32_t>(P.cidx.size());
             std::vector<int32_t> ptr(static_cast<std::size_t>(n) + 1, 0);
             for (int32_t r = 0; r < n; ++r) ptr[static_cast<std::size_t>(r) + 1] = ptr[static_cast<std::size_t>(r)] + cnt[static_cast<std::size_t>(r) + 1];
-            for (int32_t r = 0; r <= n; ++r) P.cptr.push_back(cbase + ptr[static_cast<std::size_t>(r)]);
-            P.cidx.resize(static_cast<std::size_t>(cbase) + static_cast<std::size_t>(ptr[static_cast<std::size_t>(n)]));
+            for (int32_t r = 0; r <= n; ++r) P.cptr.push_back(ptr[static_cast<std::size_t>(r)]);
+            P.cidx.resize(static_cast<std::size_t>(sd.dst_off) + static_cast<std::size_t>(toff));
             std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
             for (int32_t s = 0; s < nsf; ++s) {
                 const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
                 for (int32_t k = 0; k < d.h; ++k) {
                     const int32_t r = P.ridx[static_cast<std::size_t>(d.ridx + k)];
-                    P.cidx[static_cast<std::size_t>(cbase + next[static_cast<std::size_t>(r)]++)] = d.toff + k;
+                    P.cidx[static_cast<std::size_t>(sd.dst_off + d.toff + k)] = next[static_cast<std::size_t>(r)]++;
                 }
             }
-            P.bytes_index += static_cast<int64_t>(ptr[static_cast<std::size_t>(n)]) * 4 + (static_cast<int64_t>(n) + 1) * 4;
-        }
-        // forward levels: height above the leaves
-        {
-            std::vector<int32_t> height(static_cast<std::size_t>(nsf), 0);
-            int32_t nlev = 0;
-            for (int32_t s = 0; s < nsf; ++s) {
-                const int32_t p = parentF[static_cast<std::size_t>(s)];
-                if (p >= 0) height[static_cast<std::size_t>(p)] = std::max(height[static_cast<std::size_t>(p)], height[static_cast<std::size_t>(s)] + 1);
-                nlev = std::max(nlev, height[static_cast<std::size_t>(s)] + 1);
-            }
-            sd.f_lev = static_cast<int32_t>(P.lev.size());
-            sd.f_nlev = nlev;
-            for (int32_t L = 0; L < nlev; ++L) {
-                P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-                for (int pass = 0; pass < 2; ++pass) {
-                    if (pass == 1) P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-                    for (int32_t s = 0; s < nsf; ++s)
-                        if (height[static_cast<std::size_t>(s)] == L && ((P.sn[static_cast<std::size_t>(f_base + s)].w > kSmallWidth) == (pass == 1)))
-                            P.lev_sn.push_back(f_base + s);
-                }
-            }
-            P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-            P.max_levels_fwd = std::max(P.max_levels_fwd, nlev);
+            P.bytes_index += static_cast<int64_t>(toff) * 4 + (static_cast<int64_t>(n) + 1) * 4;
         }
 
         // ---------------- backward sweep (U, given by U^T)
         const std::vector<int32_t> bst = chol ? fst : detectSupernodes(n, ucp, ur);
+        const bool samePartition = (bst == fst);
         const int32_t nsb = static_cast<int32_t>(bst.size()) - 1;
         std::vector<int32_t> snodeB(static_cast<std::size_t>(n));
         for (int32_t s = 0; s < nsb; ++s)
@@ -156,15 +232,16 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             const int32_t c0 = bst[s], w = bst[s + 1] - c0;
             const int64_t p0 = ucp[c0];
             const int32_t h = static_cast<int32_t>(ucp[c0 + 1] - p0) - w;
+            if (w > kChunkRows) throw std::invalid_argument("supernode wider than " + std::to_string(kChunkRows) + " columns");
             SnDesc d{};
             d.c0 = c0;
             d.w = w;
             d.h = h;
-            d.toff = 0;
             d.diag = static_cast<int64_t>(P.diag.size());
             // G[c * w + r] = U(c0 + r, c0 + c) = U^T(c0 + c, c0 + r), r <= c
             for (int32_t c = 0; c < w; ++c)
                 for (int32_t r = 0; r < w; ++r) P.diag.push_back(r <= c ? uv[ucp[c0 + r] + (c - r)] : 0.0);
+            invertDiagonal(P.diag.data() + d.diag, w, false);
             if (chol) {
                 d.below = belowOff[static_cast<std::size_t>(s)];
                 d.ridx = ridxOff[static_cast<std::size_t>(s)];
@@ -180,31 +257,145 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             P.entries_bwd += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
             P.bytes_index += static_cast<int64_t>(h) * 4;
         }
-        // backward levels: depth below the root
-        {
-            std::vector<int32_t> depth(static_cast<std::size_t>(nsb), 0);
-            int32_t nlev = 0;
-            for (int32_t s = nsb - 1; s >= 0; --s) {
-                const int32_t p = parentB[static_cast<std::size_t>(s)];
-                depth[static_cast<std::size_t>(s)] = (p >= 0) ? depth[static_cast<std::size_t>(p)] + 1 : 0;
-                nlev = std::max(nlev, depth[static_cast<std::size_t>(s)] + 1);
+        P.bytes_index += static_cast<int64_t>(nsf + nsb) * static_cast<int64_t>(sizeof(SnDesc));
+
+        // ---------------- tasks
+        std::vector<int32_t> depth(static_cast<std::size_t>(nsf), 0);
+        int32_t maxDepth = 0;
+        for (int32_t s = nsf - 1; s >= 0; --s) {
+            const int32_t p = parent[static_cast<std::size_t>(s)];
+            depth[static_cast<std::size_t>(s)] = (p >= 0) ? depth[static_cast<std::size_t>(p)] + 1 : 0;
+            maxDepth = std::max(maxDepth, depth[static_cast<std::size_t>(s)]);
+        }
+        std::vector<int32_t> cntDepth(static_cast<std::size_t>(maxDepth) + 1, 0);
+        for (int32_t s = 0; s < nsf; ++s) ++cntDepth[static_cast<std::size_t>(depth[static_cast<std::size_t>(s)])];
+        int32_t dsplit = 0;
+        if (samePartition && tasks_per_sub > 1) {
+            for (int32_t d = 0; d <= maxDepth; ++d)
+                if (cntDepth[static_cast<std::size_t>(d)] >= tasks_per_sub) {
+                    dsplit = d;
+                    break;
+                }
+        }
+        // the backward sweep may use another supernode partition (LU with another structure):
+        // then every subdomain is a single task
+        if (!samePartition) dsplit = 0;
+        const int32_t t_base = static_cast<int32_t>(P.task.size());
+        std::vector<int32_t> taskOf(static_cast<std::size_t>(nsf), -1);
+        std::vector<int32_t> taskRoot;
+        for (int32_t s = nsf - 1; s >= 0; --s) {
+            const int32_t dpt = depth[static_cast<std::size_t>(s)];
+            if (!samePartition) {
+                if (taskRoot.empty()) taskRoot.push_back(nsf - 1);
+                taskOf[static_cast<std::size_t>(s)] = 0;
+            } else if (dpt <= dsplit) {
+                taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
+                taskRoot.push_back(s);
+            } else {
+                taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(parent[static_cast<std::size_t>(s)])];
             }
-            sd.b_lev = static_cast<int32_t>(P.lev.size());
-            sd.b_nlev = nlev;
-            for (int32_t L = 0; L < nlev; ++L) {
-                P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-                for (int pass = 0; pass < 2; ++pass) {
-                    if (pass == 1) P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-                    for (int32_t s = 0; s < nsb; ++s)
-                        if (depth[static_cast<std::size_t>(s)] == L && ((P.sn[static_cast<std::size_t>(b_base + s)].w > kSmallWidth) == (pass == 1)))
-                            P.lev_sn.push_back(b_base + s);
+        }
+        const int32_t nt = static_cast<int32_t>(taskRoot.size());
+        std::vector<std::vector<int32_t>> tsn(static_cast<std::size_t>(nt));
+        for (int32_t s = 0; s < nsf; ++s) tsn[static_cast<std::size_t>(taskOf[static_cast<std::size_t>(s)])].push_back(s);
+        std::vector<int32_t> heightT(static_cast<std::size_t>(nsf), 0);
+        for (int32_t s = 0; s < nsf; ++s) {
+            const int32_t p = parent[static_cast<std::size_t>(s)];
+            if (p >= 0 && taskOf[static_cast<std::size_t>(p)] == taskOf[static_cast<std::size_t>(s)])
+                heightT[static_cast<std::size_t>(p)] = std::max(heightT[static_cast<std::size_t>(p)], heightT[static_cast<std::size_t>(s)] + 1);
+        }
+        // backward level inside a task: depth in the backward tree relative to the task root
+        std::vector<int32_t> depthB(static_cast<std::size_t>(nsb), 0);
+        for (int32_t s = nsb - 1; s >= 0; --s) {
+            const int32_t p = parentB[static_cast<std::size_t>(s)];
+            depthB[static_cast<std::size_t>(s)] = (p >= 0) ? depthB[static_cast<std::size_t>(p)] + 1 : 0;
+        }
+        std::vector<std::vector<int32_t>> children(static_cast<std::size_t>(nt));
+        for (int32_t t = 0; t < nt; ++t) {
+            const int32_t r = taskRoot[static_cast<std::size_t>(t)];
+            TaskDesc td{};
+            td.sub = static_cast<int32_t>(i);
+            td.depth = samePartition ? depth[static_cast<std::size_t>(r)] : 0;
+            const int32_t pr = samePartition ? parent[static_cast<std::size_t>(r)] : -1;
+            td.parent = (pr >= 0) ? t_base + taskOf[static_cast<std::size_t>(pr)] : -1;
+            if (pr >= 0) children[static_cast<std::size_t>(taskOf[static_cast<std::size_t>(pr)])].push_back(t_base + t);
+            const auto& list = tsn[static_cast<std::size_t>(t)];
+            td.sn_first = static_cast<int32_t>(P.task_sn.size());
+            td.sn_count = static_cast<int32_t>(list.size());
+            td.rows_first = static_cast<int32_t>(P.task_rows.size());
+            for (int32_t s : list) {
+                P.task_sn.push_back(f_base + s);
+                for (int32_t j = fst[s]; j < fst[s + 1]; ++j) P.task_rows.push_back(j);
+            }
+            td.rows_count = static_cast<int32_t>(P.task_rows.size()) - td.rows_first;
+            // forward levels: height inside the task
+            int32_t hmax = 0;
+            for (int32_t s : list) hmax = std::max(hmax, heightT[static_cast<std::size_t>(s)]);
+            std::vector<std::vector<int32_t>> byLev(static_cast<std::size_t>(hmax) + 1);
+            for (int32_t s : list) byLev[static_cast<std::size_t>(heightT[static_cast<std::size_t>(s)])].push_back(f_base + s);
+            appendLevels(P, byLev, td.f_lev, td.f_nlev);
+            // backward levels: backward supernodes of the task (same columns when the partitions agree)
+            std::vector<int32_t> blist;
+            if (samePartition)
+                blist = list;
+            else
+                for (int32_t s = 0; s < nsb; ++s) blist.push_back(s);
+            int32_t dmin = 1 << 30, dmax = 0;
+            for (int32_t s : blist) {
+                dmin = std::min(dmin, depthB[static_cast<std::size_t>(s)]);
+                dmax = std::max(dmax, depthB[static_cast<std::size_t>(s)]);
+            }
+            std::vector<std::vector<int32_t>> byLevB(static_cast<std::size_t>(dmax - dmin) + 1);
+            for (int32_t s : blist) byLevB[static_cast<std::size_t>(depthB[static_cast<std::size_t>(s)] - dmin)].push_back(b_base + s);
+            appendLevels(P, byLevB, td.b_lev, td.b_nlev);
+            // rows of other tasks read by the backward sweep
+            td.ext_first = static_cast<int32_t>(P.ext_rows.size());
+            std::vector<int32_t> ext;
+            for (int32_t s : blist) {
+                const SnDesc& d = P.sn[static_cast<std::size_t>(b_base + s)];
+                for (int32_t k = 0; k < d.h; ++k) {
+                    const int32_t row = P.ridx[static_cast<std::size_t>(d.ridx + k)];
+                    if (taskOf[static_cast<std::size_t>(snodeOf[static_cast<std::size_t>(row)])] != t) ext.push_back(row);
                 }
             }
-            P.lev.push_back(static_cast<int32_t>(P.lev_sn.size()));
-            P.max_levels_bwd = std::max(P.max_levels_bwd, nlev);
+            std::sort(ext.begin(), ext.end());
+            ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+            td.ext_count = static_cast<int32_t>(ext.size());
+            P.ext_rows.insert(P.ext_rows.end(), ext.begin(), ext.end());
+            P.task.push_back(td);
         }
-        P.bytes_index += static_cast<int64_t>(nsf + nsb) * static_cast<int64_t>(sizeof(SnDesc));
+        for (int32_t t = 0; t < nt; ++t) {
+            TaskDesc& td = P.task[static_cast<std::size_t>(t_base + t)];
+            td.child_first = static_cast<int32_t>(P.task_child.size());
+            td.nchild = static_cast<int32_t>(children[static_cast<std::size_t>(t)].size());
+            P.task_child.insert(P.task_child.end(), children[static_cast<std::size_t>(t)].begin(), children[static_cast<std::size_t>(t)].end());
+        }
+        for (int32_t t = 0; t < nt; ++t) {
+            const TaskDesc& td = P.task[static_cast<std::size_t>(t_base + t)];
+            const int32_t id = t_base + t;
+            groups[{0, -td.depth}].push_back(id << 1);
+            if (td.parent >= 0) groups[{1, td.depth}].push_back((id << 1) | 1);
+        }
+        P.max_levels_fwd = std::max(P.max_levels_fwd, 0);
         P.sub.push_back(sd);
+    }
+    // level-ordered queue: forward groups by decreasing depth, then backward groups
+    for (const auto& kv : groups) {
+        P.qgroup.push_back(static_cast<int32_t>(P.queue.size()));
+        P.queue.insert(P.queue.end(), kv.second.begin(), kv.second.end());
+    }
+    P.qgroup.push_back(static_cast<int32_t>(P.queue.size()));
+    // per-subdomain slices in the same order
+    std::vector<std::vector<int32_t>> slice(fs.size());
+    for (int32_t e : P.queue) slice[static_cast<std::size_t>(P.task[static_cast<std::size_t>(e >> 1)].sub)].push_back(e);
+    for (const auto& sl : slice) {
+        P.qsub.push_back(static_cast<int32_t>(P.queue_sub.size()));
+        P.queue_sub.insert(P.queue_sub.end(), sl.begin(), sl.end());
+    }
+    P.qsub.push_back(static_cast<int32_t>(P.queue_sub.size()));
+    for (const TaskDesc& td : P.task) {
+        P.max_levels_fwd = std::max(P.max_levels_fwd, td.f_nlev);
+        P.max_levels_bwd = std::max(P.max_levels_bwd, td.b_nlev);
     }
     return P;
 }

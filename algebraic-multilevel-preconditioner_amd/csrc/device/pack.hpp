@@ -8,6 +8,10 @@
 
 namespace s2l {
 
+constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide are "small"
+constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
+constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
+
 /// One supernode of a sweep: columns [c0, c0 + w) of the factor, h rows below the diagonal block.
 struct SnDesc {
     int32_t c0, w, h, toff; ///< toff: forward only, offset of its h products in the T buffer
@@ -16,18 +20,30 @@ struct SnDesc {
     int64_t ridx;           ///< offset of the h row indices (factor positions)
 };
 
-/// Per-subdomain descriptor read by the kernel (one workgroup per subdomain).
+/// Per-subdomain descriptor.
 struct SubDesc {
     int32_t n;        ///< local size n_i
-    int32_t map_off;  ///< offset of the n_i entries of gmap / smap
-    int32_t f_lev;    ///< offset of the forward level table (2 * nlev + 1 ints)
-    int32_t f_nlev;
-    int32_t b_lev;    ///< offset of the backward level table
-    int32_t b_nlev;
-    int32_t cptr_off; ///< offset of the n_i + 1 contribution row pointers
+    int32_t map_off;  ///< offset of the n_i entries of gmap / lperm (= offset of Omega_i in the local buffers)
+    int32_t cptr_off; ///< offset of the n_i + 1 slot pointers (row r pulls T slots [cptr[r], cptr[r+1]))
     int32_t n_owned;  ///< n_I (set by the context)
     int64_t t_off;    ///< offset of the subdomain's T buffer region
     int32_t begin;    ///< first owned global row (set by the context)
+    int32_t dst_off;  ///< offset of the slot of every pushed product (indexed by toff + k)
+};
+
+/// A task of the work queue: a subtree of the supernodal elimination tree (every supernode at or
+/// below a split depth), or one supernode above it. Forward entries run bottom-up, backward
+/// entries top-down; a task without parent runs both sweeps in one entry.
+struct TaskDesc {
+    int32_t sub;
+    int32_t f_lev, f_nlev;          ///< forward level table of the task's supernodes
+    int32_t b_lev, b_nlev;          ///< backward level table
+    int32_t sn_first, sn_count;     ///< the task's supernodes (indices into task_sn)
+    int32_t ext_first, ext_count;   ///< rows of ancestor tasks read by the backward sweep
+    int32_t parent;                 ///< parent task (global index) or -1
+    int32_t child_first, nchild;    ///< children tasks (indices into task_child)
+    int32_t depth;                  ///< depth of the task's root supernode
+    int32_t rows_first, rows_count; ///< the task's rows (factor positions, indices into task_rows)
     int32_t pad;
 };
 
@@ -44,11 +60,22 @@ struct HostFactor {
 /// Packed factors of all subdomains (values still FP64).
 struct Packed {
     std::vector<SnDesc> sn;           ///< forward and backward supernodes of all subdomains
-    std::vector<int32_t> lev;         ///< level tables: [start0, split0, start1, split1, ..., start_nlev]
+    std::vector<TaskDesc> task;
+    std::vector<int32_t> task_sn;     ///< forward supernode ids of every task
+    std::vector<int32_t> task_child;
+    std::vector<int32_t> ext_rows;    ///< factor positions
+    std::vector<int32_t> task_rows;   ///< factor positions of every task's columns
+    std::vector<int32_t> queue;       ///< entries (task << 1 | phase), phase 0 forward/combined, 1 backward
+    std::vector<int32_t> qgroup;      ///< start of every depth group in the queue (+ end)
+    std::vector<int32_t> qsub;        ///< start of every subdomain's slice in queue_sub (+ end)
+    std::vector<int32_t> queue_sub;   ///< the same entries grouped by subdomain
+    std::vector<int32_t> lev;         ///< level tables (7 ints per level, see appendLevels)
     std::vector<int32_t> lev_sn;      ///< supernode indices listed by level
+    std::vector<int32_t> lev_wpre, lev_hpre; ///< prefix sums of small widths / heights per level
+    std::vector<int32_t> lev_chunk;   ///< chunk starts (positions in lev_sn) of every level, + end
     std::vector<double> diag, below;  ///< values
     std::vector<int32_t> ridx;        ///< below-row indices
-    std::vector<int32_t> cptr, cidx;  ///< forward contributions: rows -> T indices
+    std::vector<int32_t> cptr, cidx;  ///< forward contributions: slot ranges per row; slot of every product
     std::vector<SubDesc> sub;
     int64_t t_total = 0;
     int64_t entries_fwd = 0, entries_bwd = 0;
@@ -56,8 +83,9 @@ struct Packed {
     int64_t bytes_index = 0;          ///< bytes of index data read per solve (ridx + contributions)
 };
 
-/// Packs factors; map_off[i] gives the offset of subdomain i in the gmap/smap arrays.
-Packed packFactors(const std::vector<HostFactor>& f, const std::vector<int32_t>& map_off);
+/// Packs factors; map_off[i] gives the offset of subdomain i in the gmap/lperm arrays.
+/// @p tasks_per_sub: wanted subtree tasks per subdomain (1 = one task per subdomain).
+Packed packFactors(const std::vector<HostFactor>& f, const std::vector<int32_t>& map_off, int tasks_per_sub);
 
 } // namespace s2l
 

@@ -54,7 +54,8 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_upload_local_factor", "s2l_finalize_local", "s2l_upload_coarse", "s2l_apply", "s2l_spmv",
            "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
            "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_synchronize",
-           "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode"]
+           "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
+           "s2l_set_tasks_per_subdomain"]
 
 
 def hip_lib():
@@ -88,6 +89,7 @@ def hip_lib():
         "s2l_timing_enable": [vp, ctypes.c_int],
         "s2l_timing_read": [vp, P(ctypes.c_int64), P(ctypes.c_double), ctypes.c_int],
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
+        "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -161,8 +163,10 @@ class HipContext:
 
     # ------------------------------------------------------------------ uploads
     def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
-                     correction: str | None = None):
+                     correction: str | None = None, tasks_per_subdomain: int | None = None):
         """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products)."""
+        if tasks_per_subdomain is not None:
+            self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
         dec = setup.dec
         A = dec.A.tocsr()
         n = A.shape[0]
@@ -320,13 +324,13 @@ class HipTwoLevelSchwarz:
     """prec::Preconditioner of the reference, applied on the GPU (two_level_schwarz.cpp:134-163)."""
 
     def __init__(self, setup, precision: str = "double", device: int = 0, one_level: str | None = None,
-                 correction: str | None = None):
+                 correction: str | None = None, tasks_per_subdomain: int | None = None):
         self.setup = setup
         self.precision = precision
         self.one_level = one_level or setup.one_level
         self.correction = correction or setup.correction
         self.ctx = HipContext(device)
-        self.ctx.upload_setup(setup, precision, self.one_level, self.correction)
+        self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain)
 
     def apply(self, r):
         return self.ctx.apply(r)

@@ -170,6 +170,12 @@ int s2l_timing_read(s2l_ctx* ctx, int64_t* launches, double* ms_total, int reset
  * subdomain): 0 = one persistent launch for all subdomains (default), 1 = one launch per subdomain. */
 int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
 
+/* Work split of the batched local solve: the supernodal elimination tree of every subdomain is cut
+ * at the first depth holding at least @p tasks supernodes; each subtree below the cut and each
+ * supernode above it is one task of the level-ordered queue (1 = one task per subdomain). Takes
+ * effect at the next s2l_finalize_local(). Default 8. */
+int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
+
 #ifdef __cplusplus
 }
 #endif
