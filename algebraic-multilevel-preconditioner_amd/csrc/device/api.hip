@@ -196,7 +196,7 @@ struct s2l_ctx {
     DevBuf<TaskDesc> task_d;
     DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
     std::vector<int32_t> qsub_h;
-    int tasks_per_sub = 8;
+    int tasks_per_sub = 1;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
 
@@ -1106,6 +1106,16 @@ int s2l_memcpy_d2h(s2l_ctx* c, void* dst, const void* src, int64_t bytes)
         require(c && dst && src && bytes >= 0, S2L_EINVAL, "memcpy_d2h: bad arguments");
         HIPCHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_copy_d2d(s2l_ctx* c, double* dst, const double* src, int64_t count, void* stream)
+{
+    return guarded(c, [&] {
+        require(c && dst && src && count >= 0, S2L_EINVAL, "copy_d2d: bad arguments");
+        if (count == 0) return;
+        launchCopy(count, src, dst, 1, stream ? static_cast<hipStream_t>(stream) : c->stream);
+        HIPCHECK(hipGetLastError());
     });
 }
 

@@ -520,6 +520,7 @@ Problem generate(const std::string& name, const s2l_gen_opts& in)
     if (o.nx < 1) throw std::invalid_argument("generator: grid sizes must be positive");
     if (name.size() < 3) throw std::invalid_argument("unknown generator '" + name + "'");
     const std::string suffix = name.substr(name.size() - 2), kind = name.substr(0, name.size() - 2);
+    if (suffix != "2d" && suffix != "3d") throw std::invalid_argument("unknown generator '" + name + "'");
     const int dim = (suffix == "3d") ? 3 : 2;
     if (kind == "laplace" || kind == "convdiff" || kind == "coupled") return fvFamily(kind, dim, o);
     if (kind == "bubbles") return bubblesFamily(dim, o);

@@ -53,7 +53,7 @@ _hip = None
 EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_system", "s2l_upload_subdomains",
            "s2l_upload_local_factor", "s2l_finalize_local", "s2l_upload_coarse", "s2l_apply", "s2l_spmv",
            "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
-           "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_synchronize",
+           "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_copy_d2d", "s2l_synchronize",
            "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain"]
 
@@ -84,6 +84,7 @@ def hip_lib():
         "s2l_device_free": [vp, vp],
         "s2l_memcpy_h2d": [vp, vp, vp, ctypes.c_int64],
         "s2l_memcpy_d2h": [vp, vp, vp, ctypes.c_int64],
+        "s2l_copy_d2d": [vp, vp, vp, ctypes.c_int64, vp],
         "s2l_synchronize": [vp],
         "s2l_get_stats": [vp, P(Stats)],
         "s2l_timing_enable": [vp, ctypes.c_int],
@@ -283,6 +284,10 @@ class HipContext:
         res = (KrylovResultC * nrhs)()
         self._check(self.L.s2l_krylov_solve(self.h, ctypes.byref(o), b.ptr, x.ptr, nrhs, res, None, None))
         return [res[r] for r in range(nrhs)]
+
+    def copy_device(self, src: DeviceArray, dst: DeviceArray, count: int | None = None):
+        count = min(src.n, dst.n) if count is None else count
+        self._check(self.L.s2l_copy_d2d(self.h, dst.ptr, src.ptr, count, None))
 
     def synchronize(self):
         self._check(self.L.s2l_synchronize(self.h))

@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--tasks", type=int, default=8, help="tasks per subdomain of the batched local solve")
+    ap.add_argument("--tasks", type=int, default=1, help="tasks per subdomain of the batched local solve")
     ap.add_argument("--profile-applies", type=int, default=0,
                     help="only run this many FP32 applies after setup (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -93,6 +93,10 @@ def main():
         r.upload(st.dec.b)
         for _ in range(args.profile_applies):
             ctx.apply_device(r, z)
+        # calibration point of the HBM traffic counters: a plain copy of 2^25 doubles (512 MiB moved)
+        big_a, big_b = ctx.alloc(1 << 25), ctx.alloc(1 << 25)
+        for _ in range(3):
+            ctx.copy_device(big_a, big_b)
         ctx.synchronize()
         log(f"profile run: {args.profile_applies} applies done")
         return

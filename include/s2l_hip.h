@@ -144,6 +144,9 @@ int s2l_device_alloc(s2l_ctx* ctx, int64_t bytes, void** out);
 int s2l_device_free(s2l_ctx* ctx, void* p);
 int s2l_memcpy_h2d(s2l_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int s2l_memcpy_d2h(s2l_ctx* ctx, void* dst, const void* src, int64_t bytes);
+/* dst[k] = src[k], k < count, by a plain streaming kernel on @p stream (NULL = the context's):
+ * 16 bytes of HBM traffic per element, the calibration point of the traffic counters. */
+int s2l_copy_d2d(s2l_ctx* ctx, double* dst, const double* src, int64_t count, void* stream);
 int s2l_synchronize(s2l_ctx* ctx);
 
 /* ---- Introspection and measurement. */
@@ -173,7 +176,7 @@ int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
 /* Work split of the batched local solve: the supernodal elimination tree of every subdomain is cut
  * at the first depth holding at least @p tasks supernodes; each subtree below the cut and each
  * supernode above it is one task of the level-ordered queue (1 = one task per subdomain). Takes
- * effect at the next s2l_finalize_local(). Default 8. */
+ * effect at the next s2l_finalize_local(). Default 1. */
 int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
 
 #ifdef __cplusplus

@@ -1,0 +1,429 @@
+"""The oracle pinned against closed forms, dense sequential references and the invariants of the
+reference's own tests (CPU only, no GPU).
+
+Each test names the reference test it restates. Planted-mistake tests alter the oracle on the CPU
+and show that the corresponding check then fails."""
+import numpy as np
+import pytest
+
+from amp_amd import hostsetup as hs
+from oracle import oracle as orc
+
+from cases import brute_force_ghosts, dense_apply_reference, items, setup
+
+APPLY_CASES = [("laplace2d", dict(nx=12), 4), ("convdiff2d", dict(nx=14), 4)]
+
+
+# ---------------------------------------------------------------------- a1/a2/a3/a5: the apply
+@pytest.mark.parametrize("name,kw,N", APPLY_CASES)
+@pytest.mark.parametrize("pou", ["boolean", "multiplicity"])
+@pytest.mark.parametrize("one_level", ["ras", "asm"])
+@pytest.mark.parametrize("correction", ["deflated", "additive"])
+def test_apply_matches_dense_reference(name, kw, N, pou, one_level, correction):
+    """apply vs the sequential dense reference, rel. err < 1e-9, r = cos(linspace(0, 7, n))
+    (test/test_preconditioner.cpp:104-184)."""
+    st = setup(name, items(kw), N, items(dict(nev=6, pou=pou, one_level=one_level, correction=correction)))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    z = orc.OracleSchwarz(st).apply(r)
+    zr = dense_apply_reference(st, r, one_level, correction)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) < 1e-9
+
+
+@pytest.mark.parametrize("name,kw,N", [("coupled2d", dict(nx=10, ncomp=2), 4), ("convdiff3d", dict(nx=7), 4),
+                                       ("poisson3d", dict(nx=9), 8)])
+def test_apply_other_families(name, kw, N):
+    st = setup(name, items(kw), N, items(dict(nev=6)))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    z = orc.OracleSchwarz(st).apply(r)
+    zr = dense_apply_reference(st, r, "ras", "deflated")
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) < 1e-9
+
+
+def test_one_level_when_no_coarse_space():
+    """n0 = 0 gives the one-level preconditioner (two_level_schwarz.cpp:136-139)."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=0)))
+    assert st.coarse.n0 == 0
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    z = orc.OracleSchwarz(st).apply(r)
+    zr = dense_apply_reference(st, r, "ras", "deflated")
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) < 1e-9
+
+
+def test_deflation_consistency():
+    """R_0 (r - A x0) = 0 for x0 = R_0^T A_00^{-1} R_0 r (test/test_reuse.cpp:229,336-346)."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=6)))
+    O = orc.OracleSchwarz(st)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    x0 = O.coarse(r)
+    w = r - st.dec.A @ x0
+    for i, (sd, V) in enumerate(zip(st.subs, st.coarse.V)):
+        c = V.T @ w[sd.global_ids()]
+        assert np.abs(c).max() < 1e-10 * np.abs(r).max()
+
+
+def test_apply_is_bitwise_repeatable():
+    """Unchanged matrix, same input: bitwise-identical apply (test/test_reuse.cpp:309-316)."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=6)))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    O = orc.OracleSchwarz(st)
+    assert np.array_equal(O.apply(r), O.apply(r))
+    assert np.array_equal(O.apply(r), orc.OracleSchwarz(st, threads=4).apply(r))
+
+
+def test_planted_wrong_coarse_sign_is_caught(monkeypatch):
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=6)))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    zr = dense_apply_reference(st, r, "ras", "deflated")
+    good = orc.OracleSchwarz.coarse
+    monkeypatch.setattr(orc.OracleSchwarz, "coarse", lambda self, v: -good(self, v))
+    z = orc.OracleSchwarz(st).apply(r)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) > 1e-3
+
+
+def test_planted_missing_reverse_add_is_caught(monkeypatch):
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=6, pou="multiplicity", one_level="asm")))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    zr = dense_apply_reference(st, r, "asm", "deflated")
+    monkeypatch.setattr(orc.OracleSchwarz, "reverse_add", lambda self, y: [v.copy() for v in y])
+    z = orc.OracleSchwarz(st).apply(r)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) > 1e-3
+
+
+def test_planted_pou_on_ghosts_is_caught(monkeypatch):
+    """RAS with D_i = 1 everywhere is ASM, not RAS: the dense RAS reference must reject it."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=6, pou="multiplicity")))
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    zr = dense_apply_reference(st, r, "ras", "deflated")
+    O = orc.OracleSchwarz(st)
+    O.one_level_kind = "asm"
+    z = O.apply(r)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) > 1e-3
+
+
+# ---------------------------------------------------------------------- a4: SpMV
+@pytest.mark.parametrize("name,kw,N", APPLY_CASES + [("coupled2d", dict(nx=10, ncomp=2), 4)])
+def test_spmv_matches_global_product(name, kw, N):
+    """rel. diff < 1e-14 against the global product, x = sin(linspace(-1, 2, n))
+    (test/test_subdomain.cpp:77-83)."""
+    st = setup(name, items(kw), N, items(dict(nev=4)))
+    x = np.sin(np.linspace(-1.0, 2.0, st.n))
+    y = orc.OracleSchwarz(st).spmv(x)
+    yr = st.dec.A @ x
+    assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-14
+
+
+# ---------------------------------------------------------------------- a6: local solves
+@pytest.mark.parametrize("name,kw,fact", [("laplace2d", dict(nx=12), "auto"), ("laplace2d", dict(nx=12), "lu"),
+                                          ("convdiff2d", dict(nx=14), "auto"), ("poisson3d", dict(nx=9), "auto")])
+def test_local_solve_backward_error(name, kw, fact):
+    """||A x - b||_inf / (||A||_inf ||x||_inf + ||b||_inf) < 1e-12 in FP64 and < 1e-5 in FP32
+    (test/test_factorization.cpp:95-98,141), plus agreement with a dense solve."""
+    st = setup(name, items(kw), 4, items(dict(nev=2, factorization=fact)))
+    rng = np.random.default_rng(7)
+    for sd, f in zip(st.subs, st.factors):
+        Aii = sd.Aii.toarray()
+        b = rng.uniform(-1.0, 1.0, sd.n_local)
+        anorm = np.abs(Aii).sum(axis=1).max()
+        for prec, tol in (("double", 1e-12), ("single", 1e-5)):
+            x = orc.OracleLocalFactor(f, prec).solve(b)
+            be = np.abs(Aii @ x - b).max() / (anorm * np.abs(x).max() + np.abs(b).max())
+            assert be < tol, (prec, be)
+        xd = np.linalg.solve(Aii, b)
+        x = orc.OracleLocalFactor(f, "double").solve(b)
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-10
+
+
+def test_local_solve_single_rounds_rhs_to_float():
+    """The FP32 path rounds the right-hand side to float first (local_solver.cpp:77-84): a right-hand
+    side below float's smallest normal is lost entirely."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=2)))
+    f = st.factors[0]
+    b = np.full(st.subs[0].n_local, 1e-50)
+    assert np.all(orc.OracleLocalFactor(f, "single").solve(b) == 0.0)
+    assert np.all(orc.OracleLocalFactor(f, "double").solve(b) > 0.0)
+
+
+def test_planted_transposed_factor_is_caught():
+    """An LU solve with L used in place of U^T (a transposed operand) fails the backward-error check."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=2)))
+    sd, f = st.subs[0], st.factors[0]
+    assert f.kind == 1
+    bad = orc.OracleLocalFactor(f, "double")
+    bad.uv = np.ascontiguousarray(f.l_val, np.float64)
+    bad.ur = np.ascontiguousarray(f.l_row, np.int32)
+    bad.ucp = np.ascontiguousarray(f.l_colptr, np.int64)
+    Aii = sd.Aii.toarray()
+    b = np.cos(np.arange(sd.n_local))
+    x = bad.solve(b)
+    be = np.abs(Aii @ x - b).max() / (np.abs(Aii).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+    assert be > 1e-8
+
+
+# ---------------------------------------------------------------------- a12: index maps
+@pytest.mark.parametrize("overlap", [1, 2])
+@pytest.mark.parametrize("pou", ["boolean", "multiplicity"])
+def test_index_maps(overlap, pou):
+    """Overlap = distance-k neighbours in G(A + A^T); A_ii = R_i A R_i^T; lumping sums;
+    sum_i R_i^T D_i R_i = I; contiguous owned ranges; ghosts sorted (test/test_subdomain.cpp:44-76)."""
+    A, b, xyz, sym, bs = hs.generate("convdiff2d", nx=11)
+    part = hs.partition(A, 5, xyz, bs)
+    dec = hs.decompose(A, b, part, sym, bs)
+    subs = hs.build_subdomains(dec, overlap, pou)
+    Ag = dec.A
+    own = dec.ownership
+    assert own[0] == 0 and own[-1] == Ag.shape[0] and np.all(np.diff(own) > 0)
+    Ad = Ag.toarray()
+    total = np.zeros(Ag.shape[0])
+    for i, sd in enumerate(subs):
+        assert sd.begin == own[i] and sd.n_owned == own[i + 1] - own[i]
+        assert np.all(np.diff(sd.ghosts) > 0)
+        assert sd.ghosts.tolist() == brute_force_ghosts(Ag, int(own[i]), int(own[i + 1]), overlap)
+        gid = sd.global_ids()
+        assert np.array_equal(sd.Aii.toarray(), Ad[np.ix_(gid, gid)])
+        # lumping sums: sum of |A(j, k)| over k outside Omega_i, for the ghost rows j
+        outside = np.ones(Ag.shape[0], bool)
+        outside[gid] = False
+        sref = np.abs(Ad[np.ix_(sd.ghosts, np.nonzero(outside)[0])]).sum(axis=1)
+        assert np.allclose(sd.lump, sref, rtol=1e-14, atol=1e-14)
+        np.add.at(total, gid, sd.D)
+    assert np.abs(total - 1.0).max() < 1e-14
+
+
+def test_renumbering_round_trip():
+    """Ownership is contiguous after renumbering; new_to_old / old_to_new invert each other and the
+    renumbered matrix is P A P^T (domain_decomposer.cpp:254-269)."""
+    A, b, xyz, sym, bs = hs.generate("laplace2d", nx=13)
+    part = hs.partition(A, 6, xyz, bs)
+    dec = hs.decompose(A, b, part, sym, bs)
+    assert np.array_equal(dec.old_to_new[dec.new_to_old], np.arange(A.shape[0]))
+    assert np.array_equal(dec.b, b[dec.new_to_old])
+    assert abs(dec.A - A[dec.new_to_old][:, dec.new_to_old]).max() == 0.0
+    p_new = part[dec.new_to_old]
+    for k in range(6):
+        assert np.all(p_new[dec.ownership[k]:dec.ownership[k + 1]] == k)
+
+
+# ---------------------------------------------------------------------- a9: GMRES
+def test_gmres_converges_with_true_residual():
+    """GMRES(30) to 1e-8: converged, true residual <= 1.1e-8 and < 1e-7, history of one entry per
+    iteration plus the initial one (test/test_preconditioner.cpp:187-191, test/test_stopping.cpp:110-117)."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=6)))
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    x, res = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=500)
+    t2 = np.linalg.norm(b - st.dec.A @ x) / np.linalg.norm(b)
+    assert res.converged and t2 <= 1.1e-8 and res.initial_residual_of == 0.0
+    assert abs(t2 - res.true_rel_residual) < 1e-12
+    assert len(res.history) == res.iterations + 1
+
+
+def test_gmres_single_precision_factors():
+    """Single-precision local factors: true residual <= 1.1e-8 (test/test_factorization.cpp:296-306)."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=6)))
+    O = orc.OracleSchwarz(st, "single")
+    b = st.dec.b
+    x, res = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=500)
+    t2 = np.linalg.norm(b - st.dec.A @ x) / np.linalg.norm(b)
+    assert res.converged and t2 <= 1.1e-8
+
+
+@pytest.mark.parametrize("k", [1, 3, 6])
+def test_gmres_minimal_residual(k):
+    """After k iterations of the first cycle GMRES holds min over y of ||r0 - A M^{-1} K_k y||, K_k the
+    Krylov space of A M^{-1} and r0: compared with an explicit least-squares solve on a basis built
+    independently (Gram-Schmidt on the power sequence), and with the true residual of the returned x."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=4)))
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    x, res = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-14, maxit=k)
+    assert res.iterations == k
+    # Krylov basis of A M^{-1} from r0 = b (x0 = 0)
+    Q = np.zeros((st.n, k))
+    v = b / np.linalg.norm(b)
+    for j in range(k):
+        for _ in range(2):
+            v = v - Q[:, :j] @ (Q[:, :j].T @ v)
+        Q[:, j] = v / np.linalg.norm(v)
+        v = A @ O.apply(Q[:, j])
+    AMQ = np.column_stack([A @ O.apply(Q[:, j]) for j in range(k)])
+    y, *_ = np.linalg.lstsq(AMQ, b, rcond=None)
+    rmin = np.linalg.norm(b - AMQ @ y) / np.linalg.norm(b)
+    assert abs(res.history[k] - rmin) <= 1e-7 * rmin + 1e-14
+    assert abs(res.true_rel_residual - rmin) <= 1e-7 * rmin + 1e-14
+    assert np.all(np.diff(res.history) <= 1e-15)
+
+
+def test_gmres_restarts_keep_monotone_residual():
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=2)))
+    O = orc.OracleSchwarz(st)
+    x, res = orc.gmres(O.spmv, O.apply, st.dec.b, restart=4, rtol=1e-8, maxit=500)
+    assert res.converged and res.iterations > 4
+    assert len(res.history) == res.iterations + 1
+
+
+def test_planted_wrong_givens_sign_is_caught(monkeypatch):
+    """A sign error in the rotation of g (g[j+1] = +sn g[j]) leaves |g| unchanged, so the residual
+    estimate still falls, but the least-squares solution is wrong: the minimal-residual check fails."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=4)))
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    k = 4
+    calls = {"n": 0}
+
+    def wrong_solve(spmv, apply, b, **kw):
+        import math
+        # same loop as orc.gmres with one sign flipped, first cycle only
+        n = b.size
+        V = np.zeros((k + 1, n))
+        H = np.zeros((k + 1, k))
+        cs, sn, g = np.zeros(k), np.zeros(k), np.zeros(k + 1)
+        beta = np.linalg.norm(b)
+        V[0] = b / beta
+        g[0] = beta
+        for j in range(k):
+            w = spmv(apply(V[j]))
+            for _ in range(2):
+                h = V[:j + 1] @ w
+                w = w - V[:j + 1].T @ h
+                H[:j + 1, j] += h
+            H[j + 1, j] = np.linalg.norm(w)
+            V[j + 1] = w / H[j + 1, j]
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            den = math.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = H[j, j] / den, H[j + 1, j] / den
+            H[j, j], H[j + 1, j] = den, 0.0
+            g[j + 1] = +sn[j] * g[j]          # planted: the reference has -sn
+            g[j] = cs[j] * g[j]
+        y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+        calls["n"] += 1
+        return apply(V[:k].T @ y)
+
+    x = wrong_solve(O.spmv, O.apply, b)
+    _, res = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-14, maxit=k)
+    t_wrong = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert calls["n"] == 1
+    assert t_wrong > 1.01 * res.true_rel_residual
+
+
+# ---------------------------------------------------------------------- a9/a10: stopping norms, BiCGStab
+def _bubbles():
+    """bubbles3d 24^3, 20 bubbles of radius 0.1, generated right-hand side, tau 0.3, nev 8
+    (test/test_bicgstab.cpp:62-75, test/test_stopping.cpp:49-57)."""
+    return setup("bubbles3d", items(dict(nx=24, nbubbles=20, radius=0.1)), 4, items(dict(nev=8, tau=0.3)),
+                 rhs="generated")
+
+
+class _Counting:
+    def __init__(self, M):
+        self.M = M
+        self.count = 0
+
+    def __call__(self, r):
+        self.count += 1
+        return self.M(r)
+
+
+def test_bicgstab_accounting_and_convergence():
+    """iterations = applications of M; history of one entry per application; true residual
+    <= 1.1e-8 and equal to the reported one; at most 2 GMRES iterations + 4 (test/test_bicgstab.cpp:82-109)."""
+    st = _bubbles()
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    Mc = _Counting(O.apply)
+    x, r = orc.bicgstab(O.spmv, Mc, b, rtol=1e-8, maxit=1000)
+    t2 = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    _, g = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=1000)
+    assert r.converged and not r.breakdown and t2 <= 1.1e-8 and abs(t2 - r.true_rel_residual) < 1e-14
+    assert Mc.count == r.iterations and len(r.history) == r.iterations + 1
+    assert r.iterations <= 2 * g.iterations + 4
+
+
+@pytest.mark.parametrize("solver", ["gmres", "bicgstab"])
+@pytest.mark.parametrize("rtol", [1e-1, 1e-2, 1e-4, 1e-8])
+def test_openfoam_norm_stops_at_first_application_below_rtol(solver, rtol):
+    """OpenFOAM normalised residual: converged, sum|b - A x| <= rtol sum|b| and not one application
+    earlier; initial residual 1 for x0 = 0; final residual = the true one
+    (test/test_stopping.cpp:66-84, test/test_bicgstab.cpp:142-162)."""
+    st = _bubbles()
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    b1 = np.abs(b).sum()
+
+    def run(maxit):
+        if solver == "gmres":
+            return orc.gmres(O.spmv, O.apply, b, restart=30, rtol=rtol, maxit=maxit, stop="openfoam")
+        return orc.bicgstab(O.spmv, O.apply, b, rtol=rtol, maxit=maxit, stop="openfoam")
+
+    x, r = run(500)
+    t1 = np.abs(b - A @ x).sum() / b1
+    before = 1.0
+    if r.iterations > 1:
+        y, _ = run(r.iterations - 1)
+        before = np.abs(b - A @ y).sum() / b1
+    assert r.converged and t1 <= rtol * (1 + 1e-10) and before > rtol
+    assert abs(r.initial_residual_of - 1.0) < 1e-12
+    assert abs(r.final_residual_of - t1) < 1e-12 * max(1.0, t1 / rtol)
+    if solver == "bicgstab":
+        assert r.restarts == 0
+
+
+@pytest.mark.parametrize("solver", ["gmres", "bicgstab"])
+def test_openfoam_normalisation_nonzero_guess(solver):
+    """F = sum|A x0 - xm A 1| + sum|b - xm A 1| + 1e-20 with xm the mean of x0 = 0.7 + 0.3 sin(0.37 i)
+    (test/test_stopping.cpp:86-108, test/test_bicgstab.cpp:163-181)."""
+    st = _bubbles()
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    x0 = 0.7 + 0.3 * np.sin(0.37 * np.arange(st.n))
+    xm = x0.mean()
+    A1 = A @ np.ones(st.n)
+    F = np.abs(A @ x0 - xm * A1).sum() + np.abs(b - xm * A1).sum() + 1e-20
+    l10 = np.abs(b - A @ x0).sum()
+    if solver == "gmres":
+        x, r = orc.gmres(O.spmv, O.apply, b, x0, restart=30, rtol=1e-3, maxit=500, stop="openfoam")
+    else:
+        x, r = orc.bicgstab(O.spmv, O.apply, b, x0, rtol=1e-3, maxit=500, stop="openfoam")
+    t1 = np.abs(b - A @ x).sum()
+    assert r.converged and abs(r.initial_residual_of - l10 / F) < 1e-12 * (l10 / F)
+    assert abs(r.final_residual_of - t1 / F) < 1e-9 * (l10 / F) and t1 <= 1e-3 * l10 * (1 + 1e-10)
+
+
+def test_bicgstab_two_norm_and_budget():
+    """2-norm test (initial_residual_of = 0); a budget of 3 applications stops at the half step of
+    step 2 with a consistent x (test/test_bicgstab.cpp:183-211)."""
+    st = _bubbles()
+    O = orc.OracleSchwarz(st)
+    A = st.dec.A
+    b = st.dec.b
+    x, r = orc.bicgstab(O.spmv, O.apply, b, rtol=1e-8, maxit=500)
+    t2 = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert r.converged and t2 <= 1.1e-8 and r.initial_residual_of == 0.0 and abs(t2 - r.true_rel_residual) < 1e-14
+    xb, rb = orc.bicgstab(O.spmv, O.apply, b, rtol=1e-12, maxit=3)
+    tb = np.linalg.norm(b - A @ xb) / np.linalg.norm(b)
+    assert not rb.converged and rb.iterations == 3 and abs(tb - rb.true_rel_residual) < 1e-14 and tb < 1.0
+
+
+def test_bicgstab_unpreconditioned_laplace3d():
+    """Unpreconditioned BiCGStab on laplace3d 12^3 with b = sin(0.37 i) + 0.3 converges to 1.1e-8
+    (test/test_bicgstab.cpp:190-203)."""
+    A, _, _, _, _ = hs.generate("laplace3d", nx=12)
+    b = np.sin(0.37 * np.arange(A.shape[0])) + 0.3
+    x, r = orc.bicgstab(lambda v: A @ v, lambda v: v.copy(), b, rtol=1e-8, maxit=2000)
+    assert r.converged and r.true_rel_residual <= 1.1e-8
+
+
+def test_zero_rhs():
+    """b = 0: converged at once with x = 0 and one history entry (gmres.cpp:24-30, bicgstab.cpp:21-27)."""
+    st = setup("laplace2d", items(dict(nx=12)), 4, items(dict(nev=2)))
+    O = orc.OracleSchwarz(st)
+    for solve in (orc.gmres, orc.bicgstab):
+        x, r = solve(O.spmv, O.apply, np.zeros(st.n), np.ones(st.n))
+        assert r.converged and r.iterations == 0 and len(r.history) == 1 and not x.any()
