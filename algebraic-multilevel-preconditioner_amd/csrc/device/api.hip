@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -230,6 +231,11 @@ struct s2l_ctx {
     PinBuf<GmState> pst;
     PinBuf<int32_t> pflags;
     PinBuf<double> pred;
+    // BiCGStab
+    DevBuf<double> bR, bRh, bP, bV, bY, bS, bZ, bT, bcoef, bones;
+    DevBuf<int32_t> bskip;
+    PinBuf<double> pcoef;
+    PinBuf<int32_t> pskip;
 
     void ensureWork(int nrhs)
     {
@@ -612,6 +618,318 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
     return S2L_OK;
 }
 
+
+// ---- device BiCGStab (bicgstab.cpp:15-154), nrhs independent solves sharing every launch. The
+// scalar recurrences and every decision run on the host in FP64 exactly as in the reference; the
+// vector updates, products and stopping norms are fused kernels whose sums come back per stage.
+struct BicgRhs {
+    double bnorm = 0, rn = 0, l1 = 0, l1_0 = 0, thr = 0, normF = 1, rho = 0, rho_old = 1, alpha = 1, omega = 1, rhn = 0;
+    int its = 0, restarts = 0;
+    bool conv = false, brk = false, candidate = false, first = true, done = false;
+    int phase = 0; // 0 cycle start, 1 step, 2 true residual
+    std::vector<double> hist;
+};
+
+int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int nrhs, s2l_krylov_result* res,
+             double* history, hipStream_t s)
+{
+    require(o->maxit >= 0, S2L_EINVAL, "maxit must be >= 0");
+    require(o->stop_norm == 0 || o->stop_norm == 1, S2L_EINVAL, "stop_norm must be 0 (l2) or 1 (openfoam)");
+    const bool of = o->stop_norm == 1;
+    const int64_t n = c->n;
+    const int maxit = o->maxit;
+    const double rtol = o->rtol;
+    const size_t N = static_cast<size_t>(n) * nrhs;
+    for (DevBuf<double>* b : {&c->bR, &c->bRh, &c->bP, &c->bV, &c->bY, &c->bS, &c->bZ, &c->bT}) b->reserve(N);
+    c->bcoef.reserve(static_cast<size_t>(4) * nrhs);
+    c->bskip.reserve(static_cast<size_t>(nrhs));
+    c->pcoef.reserve(static_cast<size_t>(4) * nrhs);
+    c->pskip.reserve(static_cast<size_t>(nrhs));
+    c->kpart.reserve(static_cast<size_t>(reduceBlocks(n)) * 3 * nrhs);
+    c->kred.reserve(static_cast<size_t>(3) * nrhs);
+    c->pred.reserve(static_cast<size_t>(3) * nrhs);
+    if (c->kcnt.n < static_cast<size_t>(nrhs)) {
+        c->kcnt.reserve(static_cast<size_t>(nrhs));
+        HIPCHECK(hipMemsetAsync(c->kcnt.p, 0, sizeof(unsigned) * c->kcnt.n, s));
+    }
+    double* r = c->bR.p;
+    double* rh = c->bRh.p;
+    double* p = c->bP.p;
+    double* v = c->bV.p;
+    double* y = c->bY.p;
+    double* sv = c->bS.p;
+    double* z = c->bZ.p;
+    double* t = c->bT.p;
+    std::vector<BicgRhs> R(static_cast<size_t>(nrhs));
+
+    // Every stage: the host writes the participation mask and coefficients, copies them, launches,
+    // and (for reductions) reads the sums back. The stream is synchronized before the pinned
+    // staging buffers are written again.
+    auto stage = [&](const std::vector<char>& on, const std::function<double(int, int)>& coef) -> bool {
+        HIPCHECK(hipStreamSynchronize(s));  // the previous stage's copies have read the staging buffers
+        bool any = false;
+        for (int q = 0; q < nrhs; ++q) {
+            c->pskip.p[q] = on[static_cast<size_t>(q)] ? 0 : 1;
+            any = any || on[static_cast<size_t>(q)];
+            for (int k = 0; k < 4; ++k) c->pcoef.p[4 * q + k] = on[static_cast<size_t>(q)] ? coef(q, k) : 0.0;
+        }
+        if (!any) return false;
+        HIPCHECK(hipMemcpyAsync(c->bskip.p, c->pskip.p, sizeof(int32_t) * nrhs, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(c->bcoef.p, c->pcoef.p, sizeof(double) * 4 * nrhs, hipMemcpyHostToDevice, s));
+        return true;
+    };
+    auto readSums = [&] {
+        HIPCHECK(hipMemcpyAsync(c->pred.p, c->kred.p, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    };
+    auto sums = [&](int q, int k) { return c->pred.p[3 * q + k]; };
+    auto zero = [](int, int) { return 0.0; };
+    const int32_t* skip = c->bskip.p;
+    auto satisfied = [&](const BicgRhs& st, double slack) {
+        return of ? (st.l1 <= slack * st.thr) : (st.rn <= slack * rtol * st.bnorm);
+    };
+
+    // ||b|| for every right-hand side
+    std::vector<char> all(static_cast<size_t>(nrhs), 1);
+    stage(all, zero);
+    launchReduce(0, n, d_b, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+    readSums();
+    std::vector<char> live(static_cast<size_t>(nrhs), 0);
+    for (int q = 0; q < nrhs; ++q) {
+        BicgRhs& st = R[static_cast<size_t>(q)];
+        st.bnorm = std::sqrt(sums(q, 0));
+        if (st.bnorm == 0.0) { // x = 0, converged (bicgstab.cpp:24-29)
+            HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(q) * n, 0, sizeof(double) * n, s));
+            st.conv = st.done = true;
+            st.hist.push_back(0.0);
+        } else {
+            live[static_cast<size_t>(q)] = 1;
+        }
+    }
+    // r = A x0; OpenFOAM normalisation from A x0; r = b - r
+    if (stage(live, zero)) {
+        spmvImpl(c, d_x, r, nullptr, nrhs, skip, s);
+        if (of) {
+            launchReduce(5, n, d_x, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            readSums();
+            std::vector<double> xm(static_cast<size_t>(nrhs));
+            for (int q = 0; q < nrhs; ++q) xm[static_cast<size_t>(q)] = n ? sums(q, 1) / static_cast<double>(n) : 0.0;
+            std::vector<char> withm(static_cast<size_t>(nrhs), 0), nom(static_cast<size_t>(nrhs), 0);
+            for (int q = 0; q < nrhs; ++q) {
+                if (!live[static_cast<size_t>(q)]) continue;
+                (xm[static_cast<size_t>(q)] == 0.0 ? nom : withm)[static_cast<size_t>(q)] = 1;
+            }
+            if (stage(nom, zero)) {  // xm = 0: F = |A x0|_1 + |b|_1 + 1e-20
+                launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                readSums();
+                for (int q = 0; q < nrhs; ++q)
+                    if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 1);
+                launchReduce(1, n, d_b, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                readSums();
+                for (int q = 0; q < nrhs; ++q)
+                    if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF += sums(q, 1) + 1e-20;
+            }
+            if (stage(withm, [&](int q, int k) { return k == 0 ? xm[static_cast<size_t>(q)] : 0.0; })) {
+                c->bones.reserve(static_cast<size_t>(n) * 2);
+                launchFill(n, 1.0, c->bones.p, s);
+                spmvImpl(c, c->bones.p, c->bones.p + n, nullptr, 1, nullptr, s);
+                ofNorm(n, r, d_b, c->bones.p + n, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                readSums();
+                for (int q = 0; q < nrhs; ++q)
+                    if (withm[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 0) + sums(q, 1) + 1e-20;
+            }
+            stage(live, zero);
+        }
+        launchAxpby(n, 1.0, d_b, -1.0, r, nrhs, skip, s);  // r = b - A x0
+        launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+        readSums();
+        for (int q = 0; q < nrhs; ++q) {
+            if (!live[static_cast<size_t>(q)]) continue;
+            BicgRhs& st = R[static_cast<size_t>(q)];
+            st.rn = std::sqrt(sums(q, 0));
+            st.l1 = sums(q, 1);
+            st.l1_0 = st.l1;
+            if (of) st.thr = std::max(rtol * st.l1, o->atol * st.normF);
+            st.hist.push_back(st.rn / st.bnorm);
+        }
+    }
+
+    std::vector<char> on(static_cast<size_t>(nrhs)), half(static_cast<size_t>(nrhs)), full(static_cast<size_t>(nrhs));
+    for (;;) {
+        // cycle starts: convergence / budget tests, shadow residual r^ = r (bicgstab.cpp:73-82)
+        std::fill(on.begin(), on.end(), 0);
+        for (int q = 0; q < nrhs; ++q) {
+            BicgRhs& st = R[static_cast<size_t>(q)];
+            if (st.done || st.phase != 0) continue;
+            if (satisfied(st, 1.0)) {
+                st.conv = st.done = true;
+                continue;
+            }
+            if (st.its >= maxit) {
+                st.done = true;
+                continue;
+            }
+            st.rhn = st.rn;
+            st.rho = st.rn * st.rn;
+            st.rho_old = st.alpha = st.omega = 1.0;
+            st.first = true;
+            st.candidate = false;
+            st.phase = 1;
+            on[static_cast<size_t>(q)] = 1;
+        }
+        if (stage(on, zero)) launchAxpby(n, 1.0, r, 0.0, rh, nrhs, skip, s);
+        // step, first half: p, y = M p, v = A y, (r^, v)
+        std::fill(on.begin(), on.end(), 0);
+        for (int q = 0; q < nrhs; ++q) {
+            BicgRhs& st = R[static_cast<size_t>(q)];
+            if (st.done || st.phase != 1) continue;
+            if (st.its >= maxit) {
+                st.phase = 2;
+                continue;
+            }
+            on[static_cast<size_t>(q)] = 1;
+        }
+        bool anyDone = true;
+        for (const BicgRhs& st : R) anyDone = anyDone && st.done;
+        if (anyDone) break;
+        if (stage(on, [&](int q, int k) {
+                const BicgRhs& st = R[static_cast<size_t>(q)];
+                if (k == 0) return st.first ? 0.0 : (st.rho / st.rho_old) * (st.alpha / st.omega);
+                if (k == 1) return st.omega;
+                if (k == 2) return st.first ? 1.0 : 0.0;
+                return 0.0;
+            })) {
+            bicgP(n, r, p, v, c->bcoef.p, nrhs, skip, s);
+            applyImpl(c, p, y, nrhs, skip, s);
+            spmvImpl(c, y, v, nullptr, nrhs, skip, s);
+            launchReduce(3, n, v, rh, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            readSums();
+            std::fill(half.begin(), half.end(), 0);
+            for (int q = 0; q < nrhs; ++q) {
+                if (!on[static_cast<size_t>(q)]) continue;
+                BicgRhs& st = R[static_cast<size_t>(q)];
+                st.first = false;
+                ++st.its;
+                const double rhv = sums(q, 0);
+                if (rhv == 0.0 || !std::isfinite(rhv)) {
+                    st.brk = true;
+                    st.phase = 2;
+                    continue;
+                }
+                st.alpha = st.rho / rhv;
+                half[static_cast<size_t>(q)] = 1;
+            }
+            // s = r - alpha v, stopping norms of s (the half step)
+            if (stage(half, [&](int q, int k) { return k == 0 ? R[static_cast<size_t>(q)].alpha : 0.0; })) {
+                bicgS(n, r, v, sv, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                readSums();
+                std::fill(on.begin(), on.end(), 0);
+                std::fill(full.begin(), full.end(), 0);
+                for (int q = 0; q < nrhs; ++q) {
+                    if (!half[static_cast<size_t>(q)]) continue;
+                    BicgRhs& st = R[static_cast<size_t>(q)];
+                    st.rn = std::sqrt(sums(q, 0));
+                    st.l1 = sums(q, 1);
+                    st.hist.push_back(st.rn / st.bnorm);
+                    if (satisfied(st, 1.0) || st.its >= maxit) {
+                        st.candidate = satisfied(st, 1.0);
+                        st.phase = 2;
+                        on[static_cast<size_t>(q)] = 1;  // x += alpha y; r = s
+                    } else {
+                        full[static_cast<size_t>(q)] = 1;
+                    }
+                }
+                if (stage(on, [&](int q, int k) { return k == 0 ? R[static_cast<size_t>(q)].alpha : 0.0; }))
+                    bicgHalf(n, d_x, y, sv, r, c->bcoef.p, nrhs, skip, s);
+                // second half: z = M s, t = A z, omega = (t, s) / (t, t), x and r
+                if (stage(full, zero)) {
+                    applyImpl(c, sv, z, nrhs, skip, s);
+                    spmvImpl(c, z, t, nullptr, nrhs, skip, s);
+                    launchReduce(4, n, t, sv, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                    readSums();
+                    for (int q = 0; q < nrhs; ++q) {
+                        if (!full[static_cast<size_t>(q)]) continue;
+                        BicgRhs& st = R[static_cast<size_t>(q)];
+                        ++st.its;
+                        const double d0 = sums(q, 0), d1 = sums(q, 1);
+                        st.omega = d0 > 0.0 ? d1 / d0 : 0.0;
+                    }
+                    stage(full, [&](int q, int k) {
+                        const BicgRhs& st = R[static_cast<size_t>(q)];
+                        return k == 0 ? st.alpha : (k == 1 ? st.omega : 0.0);
+                    });
+                    bicgXR(n, d_x, y, z, sv, t, r, rh, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                    readSums();
+                    for (int q = 0; q < nrhs; ++q) {
+                        if (!full[static_cast<size_t>(q)]) continue;
+                        BicgRhs& st = R[static_cast<size_t>(q)];
+                        st.rho_old = st.rho;
+                        st.rho = sums(q, 2);
+                        st.rn = std::sqrt(sums(q, 0));
+                        st.l1 = sums(q, 1);
+                        st.hist.push_back(st.rn / st.bnorm);
+                        if (satisfied(st, 1.0)) {
+                            st.candidate = true;
+                            st.phase = 2;
+                        } else if (st.omega == 0.0 || std::abs(st.rho) < 1e-14 * st.rhn * st.rn) {
+                            st.phase = 2;  // restart with r^ = r
+                        } else if (st.its >= maxit) {
+                            st.phase = 2;
+                        }
+                    }
+                }
+            }
+        }
+        // end of a cycle: true residual r = b - A x (bicgstab.cpp:138-151)
+        std::fill(on.begin(), on.end(), 0);
+        for (int q = 0; q < nrhs; ++q) {
+            const BicgRhs& st = R[static_cast<size_t>(q)];
+            on[static_cast<size_t>(q)] = !st.done && st.phase == 2;
+        }
+        if (stage(on, zero)) {
+            spmvImpl(c, d_x, r, d_b, nrhs, skip, s);
+            launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            readSums();
+            for (int q = 0; q < nrhs; ++q) {
+                if (!on[static_cast<size_t>(q)]) continue;
+                BicgRhs& st = R[static_cast<size_t>(q)];
+                st.rn = std::sqrt(sums(q, 0));
+                st.l1 = sums(q, 1);
+                st.phase = 0;
+                if (st.brk) {
+                    st.done = true;
+                } else if (st.candidate && satisfied(st, 1.1)) {
+                    st.conv = st.done = true;
+                } else if (st.its >= maxit) {
+                    st.done = true;
+                } else {
+                    ++st.restarts;
+                }
+            }
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < nrhs; ++q) {
+        const BicgRhs& st = R[static_cast<size_t>(q)];
+        s2l_krylov_result& out = res[q];
+        std::memset(&out, 0, sizeof(out));
+        out.converged = st.conv;
+        out.breakdown = st.brk;
+        out.iterations = st.its;
+        out.restarts = st.restarts;
+        out.rel_residual = st.hist.empty() ? 0.0 : st.hist.back();
+        out.true_rel_residual = st.bnorm > 0.0 ? st.rn / st.bnorm : 0.0;
+        if (of && st.bnorm > 0.0) {
+            out.initial_residual_of = st.l1_0 / st.normF;
+            out.final_residual_of = st.l1 / st.normF;
+        }
+        out.history_len = static_cast<int>(st.hist.size());
+        if (history)
+            for (size_t k = 0; k < st.hist.size() && k < static_cast<size_t>(maxit + 1); ++k)
+                history[static_cast<size_t>(q) * (maxit + 1) + k] = st.hist[k];
+    }
+    return S2L_OK;
+}
 } // namespace
 
 // ============================================================================ C-ABI
@@ -990,9 +1308,10 @@ int s2l_krylov_solve(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, do
     int rc = S2L_OK;
     const int g = guarded(c, [&] {
         require(c && o && d_b && d_x && results && nrhs >= 1, S2L_EINVAL, "krylov_solve: bad arguments");
-        require(o->method == 0, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES)");
+        require(o->method == 0 || o->method == 1, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES) or 1 (BiCGStab)");
         HIPCHECK(hipSetDevice(c->device));
-        rc = gmresImpl(c, o, d_b, d_x, nrhs, results, history, pick(c, stream));
+        rc = o->method == 0 ? gmresImpl(c, o, d_b, d_x, nrhs, results, history, pick(c, stream))
+                            : bicgImpl(c, o, d_b, d_x, nrhs, results, history, pick(c, stream));
     });
     return g != S2L_OK ? g : rc;
 }
@@ -1031,12 +1350,13 @@ int s2l_krylov_solve_host(s2l_ctx* c, const s2l_krylov_opts* o, const double* b,
     int rc = S2L_OK;
     const int g = guarded(c, [&] {
         require(c && o && b && x && results && nrhs >= 1, S2L_EINVAL, "krylov_solve_host: bad arguments");
-        require(o->method == 0, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES)");
+        require(o->method == 0 || o->method == 1, S2L_EINVAL, "krylov_solve: method must be 0 (GMRES) or 1 (BiCGStab)");
         HIPCHECK(hipSetDevice(c->device));
         DevBuf<double> db, dx;
         db.upload(b, static_cast<size_t>(c->n) * nrhs, c->stream);
         dx.upload(x, static_cast<size_t>(c->n) * nrhs, c->stream);
-        rc = gmresImpl(c, o, db.p, dx.p, nrhs, results, history, c->stream);
+        rc = o->method == 0 ? gmresImpl(c, o, db.p, dx.p, nrhs, results, history, c->stream)
+                            : bicgImpl(c, o, db.p, dx.p, nrhs, results, history, c->stream);
         HIPCHECK(hipMemcpyAsync(x, dx.p, sizeof(double) * c->n * nrhs, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     });

@@ -99,6 +99,19 @@ void launchCopy(int64_t n, const double* x, double* y, int nrhs, hipStream_t s);
 
 /// Deterministic sum reductions of nrhs columns: out[r * ncomp + c] for c < ncomp;
 /// kind 0: x.x ; 1: x.x, sum|x| ; 2: x.x, sum|x|, x.y ; 3: y.x (single) ; 4: x.x, x.y
+void bicgP(int64_t n, const double* r, double* p, const double* v, const double* coef, int nrhs, const int32_t* skip,
+           hipStream_t s);
+void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial,
+           unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s);
+void bicgXR(int64_t n, double* x, const double* y, const double* z, const double* sv, const double* t, double* r,
+            const double* rh, const double* coef, double* partial, unsigned* counter, double* out, int nrhs,
+            const int32_t* skip, hipStream_t s);
+void bicgHalf(int64_t n, double* x, const double* y, const double* sv, double* r, const double* coef, int nrhs,
+              const int32_t* skip, hipStream_t s);
+void ofNorm(int64_t n, const double* ax, const double* b, const double* a1, const double* coef, double* partial,
+            unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s);
+void launchFill(int64_t n, double v, double* y, hipStream_t s);
+/// kind 0: (x.x); 1: (x.x, |x|_1); 2: (x.x, |x|_1, x.y); 3: (y.x); 4: (x.x, x.y); 5: (x.x, sum x)
 void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, unsigned* counter, double* out,
                   int nrhs, const int32_t* skip, hipStream_t s);
 int reduceBlocks(int64_t n);

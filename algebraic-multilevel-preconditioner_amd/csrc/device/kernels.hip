@@ -450,27 +450,10 @@ __device__ inline bool lastBlock(unsigned* counter, unsigned nblocks)
     return last;
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_reduce(int kind, int64_t n, const double* x, const double* y,
-                                                         double* partial, unsigned* counter, double* out,
-                                                         const int32_t* skip)
+// Three sums of one right-hand side (blockIdx.y): wave sums, one partial per block, and the last
+// block to finish adds the partials in block order (fixed order: bitwise repeatable).
+__device__ void finishReduce(double a0, double a1, double a2, double* partial, unsigned* counter, double* out, int rr)
 {
-    const int rr = blockIdx.y;
-    if (skip && skip[rr]) return;
-    const double* xv = x + static_cast<int64_t>(rr) * n;
-    const double* yv = y ? y + static_cast<int64_t>(rr) * n : nullptr;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const double v = xv[i];
-        if (kind == 3) {
-            a0 += yv[i] * v;
-        } else {
-            a0 += v * v;
-            if (kind == 1 || kind == 2) a1 += fabs(v);
-            if (kind == 2) a2 += v * yv[i];
-            if (kind == 4) a1 += v * yv[i];
-        }
-    }
     __shared__ double red[kRedThreads / kWave][kRedComp];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     a0 = waveSum(a0);
@@ -497,6 +480,129 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(int kind, int64_t n, con
         }
         if (threadIdx.x == 0) counter[rr] = 0;
     }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_reduce(int kind, int64_t n, const double* x, const double* y,
+                                                         double* partial, unsigned* counter, double* out,
+                                                         const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const double* xv = x + static_cast<int64_t>(rr) * n;
+    const double* yv = y ? y + static_cast<int64_t>(rr) * n : nullptr;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = xv[i];
+        if (kind == 3) {
+            a0 += yv[i] * v;
+        } else {
+            a0 += v * v;
+            if (kind == 1 || kind == 2) a1 += fabs(v);
+            if (kind == 2) a2 += v * yv[i];
+            if (kind == 4) a1 += v * yv[i];
+            if (kind == 5) a1 += v;
+        }
+    }
+    finishReduce(a0, a1, a2, partial, counter, out, rr);
+}
+
+// ------------------------------------------------------------------ BiCGStab (bicgstab.cpp:81-137)
+// Per right-hand side coefficients coef[4 r + k] are set by the host: the scalar recurrences of
+// BiCGStab stay on the host in FP64, the vector updates and their reductions are fused here.
+
+// p = r (first step of a cycle, coef[2] != 0) or p = r + beta (p - omega v)
+__global__ void k_bicg_p(int64_t n, const double* r, double* p, const double* v, const double* coef, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = static_cast<int64_t>(rr) * n + i;
+    const double beta = coef[4 * rr], omega = coef[4 * rr + 1];
+    p[o] = coef[4 * rr + 2] != 0.0 ? r[o] : r[o] + beta * (p[o] - omega * v[o]);
+}
+
+// s = r - alpha v; sums (s.s, |s|_1)
+__global__ __launch_bounds__(kRedThreads) void k_bicg_s(int64_t n, const double* r, const double* v, double* sv,
+                                                        const double* coef, double* partial, unsigned* counter,
+                                                        double* out, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t base = static_cast<int64_t>(rr) * n;
+    const double alpha = coef[4 * rr];
+    double a0 = 0.0, a1 = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double s = r[base + i] - alpha * v[base + i];
+        sv[base + i] = s;
+        a0 += s * s;
+        a1 += fabs(s);
+    }
+    finishReduce(a0, a1, 0.0, partial, counter, out, rr);
+}
+
+// x += alpha y + omega z; r = s - omega t; sums (r.r, |r|_1, rh.r)
+__global__ __launch_bounds__(kRedThreads) void k_bicg_xr(int64_t n, double* x, const double* y, const double* z,
+                                                         const double* sv, const double* t, double* r, const double* rh,
+                                                         const double* coef, double* partial, unsigned* counter,
+                                                         double* out, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t base = static_cast<int64_t>(rr) * n;
+    const double alpha = coef[4 * rr], omega = coef[4 * rr + 1];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t o = base + i;
+        x[o] = x[o] + (alpha * y[o] + omega * z[o]);
+        const double rv = sv[o] - omega * t[o];
+        r[o] = rv;
+        a0 += rv * rv;
+        a1 += fabs(rv);
+        a2 += rh[o] * rv;
+    }
+    finishReduce(a0, a1, a2, partial, counter, out, rr);
+}
+
+// OpenFOAM normalisation (krylov.cpp:47-57): sums (|A x0 - xm A 1|_1, |b - xm A 1|_1), xm = coef[4 r]
+__global__ __launch_bounds__(kRedThreads) void k_of_norm(int64_t n, const double* ax, const double* b, const double* a1,
+                                                         const double* coef, double* partial, unsigned* counter,
+                                                         double* out, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t base = static_cast<int64_t>(rr) * n;
+    const double xm = coef[4 * rr];
+    double a0 = 0.0, b0 = 0.0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double m = xm * a1[i];
+        a0 += fabs(ax[base + i] - m);
+        b0 += fabs(b[base + i] - m);
+    }
+    finishReduce(a0, b0, 0.0, partial, counter, out, rr);
+}
+
+__global__ void k_fill(int64_t n, double v, double* y)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = v;
+}
+
+// stop at the half step: x += alpha y; r = s
+__global__ void k_bicg_half(int64_t n, double* x, const double* y, const double* sv, double* r, const double* coef,
+                            const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = static_cast<int64_t>(rr) * n + i;
+    x[o] = x[o] + coef[4 * rr] * y[o];
+    r[o] = sv[o];
 }
 
 // ------------------------------------------------------------------ GMRES
@@ -768,6 +874,41 @@ void launchReduce(int kind, int64_t n, const double* x, const double* y, double*
 {
     hipLaunchKernelGGL(k_reduce, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, kind, n, x, y, partial, counter, out,
                        skip);
+}
+
+void bicgP(int64_t n, const double* r, double* p, const double* v, const double* coef, int nrhs, const int32_t* skip,
+           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, r, p, v, coef, skip);
+}
+void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial,
+           unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_s, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, r, v, sv, coef, partial, counter,
+                       out, skip);
+}
+void bicgXR(int64_t n, double* x, const double* y, const double* z, const double* sv, const double* t, double* r,
+            const double* rh, const double* coef, double* partial, unsigned* counter, double* out, int nrhs,
+            const int32_t* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_xr, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, x, y, z, sv, t, r, rh, coef,
+                       partial, counter, out, skip);
+}
+void bicgHalf(int64_t n, double* x, const double* y, const double* sv, double* r, const double* coef, int nrhs,
+              const int32_t* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_half, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, x, y, sv, r, coef, skip);
+}
+
+void ofNorm(int64_t n, const double* ax, const double* b, const double* a1, const double* coef, double* partial,
+            unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_of_norm, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, ax, b, a1, coef, partial,
+                       counter, out, skip);
+}
+void launchFill(int64_t n, double v, double* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, y);
 }
 
 void gmDots(const GmArgs& a, int j, hipStream_t s)

@@ -48,6 +48,8 @@ class Stats(ctypes.Structure):
 
 
 _hip = None
+_METHODS = {"gmres": 0, "bicgstab": 1}
+_STOPS = {"l2": 0, "openfoam": 1}
 
 #: every symbol declared in include/s2l_hip.h
 EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_system", "s2l_upload_subdomains",
@@ -252,11 +254,13 @@ class HipContext:
         self._check(self.L.s2l_local_solve_host(self.h, i, _p(b, ctypes.c_double), _p(y, ctypes.c_double)))
         return y
 
-    def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000):
+    def krylov(self, b: np.ndarray, x: np.ndarray | None = None, method: str = "gmres", restart=30, rtol=1e-8,
+               maxit=1000, stop: str = "l2", atol: float = 0.0):
+        """KrylovSolver::solve on the device through s2l_krylov_solve_host (host arrays in and out)."""
         b = np.ascontiguousarray(b, np.float64)
         nrhs = 1 if b.ndim == 1 else b.shape[0]
         x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, np.float64).copy()
-        o = KrylovOpts(0, restart, rtol, 0.0, maxit, 0)
+        o = KrylovOpts(_METHODS[method], restart, rtol, atol, maxit, _STOPS[stop])
         res = (KrylovResultC * nrhs)()
         hist = np.zeros(nrhs * (maxit + 1))
         self._check(self.L.s2l_krylov_solve_host(self.h, ctypes.byref(o), _p(b, ctypes.c_double),
@@ -269,6 +273,12 @@ class HipContext:
                                     list(hist[r * (maxit + 1): r * (maxit + 1) + R.history_len])))
         return x, (out[0] if nrhs == 1 else out)
 
+    def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000):
+        return self.krylov(b, x, "gmres", restart, rtol, maxit)
+
+    def bicgstab(self, b: np.ndarray, x: np.ndarray | None = None, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
+        return self.krylov(b, x, "bicgstab", 0, rtol, maxit, stop, atol)
+
     # ------------------------------------------------------------------ operations (device memory)
     def alloc(self, n: int) -> DeviceArray:
         return DeviceArray(self, n)
@@ -279,8 +289,8 @@ class HipContext:
     def spmv_device(self, x: DeviceArray, y: DeviceArray, nrhs: int = 1):
         self._check(self.L.s2l_spmv(self.h, x.ptr, y.ptr, nrhs, None))
 
-    def gmres_device(self, b: DeviceArray, x: DeviceArray, nrhs=1, restart=30, rtol=1e-8, maxit=1000):
-        o = KrylovOpts(0, restart, rtol, 0.0, maxit, 0)
+    def gmres_device(self, b: DeviceArray, x: DeviceArray, nrhs=1, restart=30, rtol=1e-8, maxit=1000, method="gmres"):
+        o = KrylovOpts(_METHODS[method], restart, rtol, 0.0, maxit, 0)
         res = (KrylovResultC * nrhs)()
         self._check(self.L.s2l_krylov_solve(self.h, ctypes.byref(o), b.ptr, x.ptr, nrhs, res, None, None))
         return [res[r] for r in range(nrhs)]
@@ -369,10 +379,27 @@ class HipGmres:
         return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit)
 
 
+class HipBiCgStab:
+    """solv::BiCgStab (bicgstab.hpp:14-40, bicgstab.cpp:15-154) with the vector work on the device."""
+
+    def __init__(self, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
+        if stop not in _STOPS:
+            raise ValueError(f"unknown stopping norm '{stop}' (l2|openfoam)")
+        self.rtol, self.maxit, self.stop, self.atol = rtol, maxit, stop, atol
+
+    def name(self) -> str:
+        return "BiCGStab"
+
+    def solve(self, M: HipTwoLevelSchwarz, b, x=None):
+        return M.ctx.bicgstab(b, x, self.rtol, self.maxit, self.stop, self.atol)
+
+
 def make_krylov_solver(krylov="gmres", restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
     """makeKrylovSolver (src/solver/krylov.cpp:23-36)."""
     if atol > 0.0 and stop != "openfoam":
         raise ValueError("an absolute tolerance needs the OpenFOAM stopping norm")
     if krylov == "gmres":
         return HipGmres(restart, rtol, maxit, stop, atol)
+    if krylov == "bicgstab":
+        return HipBiCgStab(rtol, maxit, stop, atol)
     raise ValueError(f"unknown Krylov solver '{krylov}' (gmres|bicgstab)")

@@ -43,6 +43,92 @@ WORKLOADS = {
 }
 
 
+COPY_COUNT = 1 << 25   # doubles of the calibration copy (256 MiB read + 256 MiB written)
+
+
+def time_applies(ctx, r, z, count, warm=3):
+    """(average ms of the local-solve launch set per apply from HIP events on its stream,
+    wall ms per apply)"""
+    for _ in range(warm):
+        ctx.apply_device(r, z)
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.timing_read(reset=True)
+    t = time.perf_counter()
+    for _ in range(count):
+        ctx.apply_device(r, z)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t) / count
+    nl, ms = ctx.timing_read(reset=True)
+    ctx.timing(False)
+    return ms / max(nl, 1), wall * 1e3
+
+
+def roofline(bytes_per_launch, avg_ms, launches):
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_local_solve<float>", "launches": int(launches), "avg_us": round(avg_ms * 1e3, 2),
+            "bytes_per_launch": int(bytes_per_launch)}
+
+
+def measure_traffic(args):
+    """HBM bytes of one k_local_solve<float> launch from two rocprofv3 --pmc passes (FETCH_SIZE, then
+    WRITE_SIZE) over a child run of this script with --profile-applies, each scaled by what the counter
+    reads for the calibration copy of COPY_COUNT doubles. The child skips the coarse space (the local
+    solve then reads no x0: 8 n bytes less than in the two-level apply). Returns (bytes, note)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    kib = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="s2l_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
+               os.path.abspath(__file__), "--profile-applies", "3", "--profile-one-level", "--setup-workers", "1",
+               "--workload", args.workload, "--tasks", str(args.tasks)]
+        log(f"traffic pass {counter}")
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd="/tmp",
+                               env={**os.environ, "TMPDIR": "/tmp"})
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if p.returncode != 0 or not files:
+            return None, f"{counter} pass failed (exit {p.returncode})"
+        local, copy = [], []
+        for f in files:
+            for row in csv.DictReader(open(f)):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row["Kernel_Name"]
+                if "k_local_solve<float>" in name:
+                    local.append(float(row["Counter_Value"]))
+                elif "k_copy" in name:
+                    copy.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not local or not copy:
+            return None, f"{counter}: kernels not found in the counter output"
+        kib[counter] = (float(np.mean(local)), float(np.mean(copy)))
+    # the copy moves COPY_COUNT * 8 bytes each way
+    ref_kib = COPY_COUNT * 8 / 1024.0
+    ff = ref_kib / kib["FETCH_SIZE"][1]
+    wf = ref_kib / kib["WRITE_SIZE"][1]
+    traffic = (kib["FETCH_SIZE"][0] * ff + kib["WRITE_SIZE"][0] * wf) * 1024.0
+    note = (f"FETCH_SIZE {kib['FETCH_SIZE'][0]:.0f} KiB x {ff:.3f}, WRITE_SIZE {kib['WRITE_SIZE'][0]:.0f} KiB x "
+            f"{wf:.3f} per launch (factors from a copy of {COPY_COUNT} doubles)")
+    return int(traffic), note
+
+
+def _us(ms):
+    """milliseconds -> microseconds rounded, None when not measured"""
+    return None if ms != ms else round(ms * 1e3, 2)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -60,6 +146,17 @@ def main():
     ap.add_argument("--tasks", type=int, default=1, help="tasks per subdomain of the batched local solve")
     ap.add_argument("--profile-applies", type=int, default=0,
                     help="only run this many FP32 applies after setup (for rocprofv3 runs)")
+    ap.add_argument("--no-launch-compare", action="store_true",
+                    help="skip the comparison with one launch per subdomain (keeps every k_local_solve<float> "
+                         "launch of the run the same work, for profiler averages)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) that measure the HBM traffic "
+                         "of the local-solve launch")
+    ap.add_argument("--profile-one-level", action="store_true",
+                    help="with --profile-applies: skip the coarse space (nev 0), for the counter passes")
+    ap.add_argument("--setup-workers", type=int, default=0,
+                    help="processes of the host eigensolves (0 = min(16, cores); 1 = no fork, needed under "
+                         "rocprofv3 --pmc, which initialises the GPU before the program starts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -69,12 +166,14 @@ def main():
     from amp_amd import device, hostsetup as hs
 
     gen, kw, N, overlap, nev, restart, desc = WORKLOADS[args.workload]
+    if args.profile_applies > 0 and args.profile_one_level:
+        nev = 0
     correction = "additive" if args.workload == "poisson2d-256" else "deflated"
     t0 = time.perf_counter()
     A, _, xyz, sym, bs = hs.generate(gen, **kw)
     n = A.shape[0]
     b0 = hs.random_rhs(n, args.seed)
-    workers = min(16, os.cpu_count() or 1)
+    workers = args.setup_workers or min(16, os.cpu_count() or 1)
     st = hs.build_setup(A, b0, N, xyz=xyz, symmetric=sym, overlap=overlap, nev=nev, eig_workers=workers,
                         eig_tol=1e-6, correction=correction,
                         factorization="lu" if args.workload == "poisson2d-256" else "auto")
@@ -88,17 +187,18 @@ def main():
         f"{stats.levels_fwd_max}/{stats.levels_bwd_max}, bytes/local solve {stats.bytes_local_solve / 1e6:.1f} MB")
 
     if args.profile_applies > 0:
+        # only FP32 applies (every k_local_solve<float> launch does the same work), event-timed
         r = ctx.alloc(n)
         z = ctx.alloc(n)
         r.upload(st.dec.b)
-        for _ in range(args.profile_applies):
-            ctx.apply_device(r, z)
+        avg_ms, _ = time_applies(ctx, r, z, args.profile_applies, warm=2)
         # calibration point of the HBM traffic counters: a plain copy of 2^25 doubles (512 MiB moved)
-        big_a, big_b = ctx.alloc(1 << 25), ctx.alloc(1 << 25)
+        big_a, big_b = ctx.alloc(COPY_COUNT), ctx.alloc(COPY_COUNT)
         for _ in range(3):
             ctx.copy_device(big_a, big_b)
         ctx.synchronize()
-        log(f"profile run: {args.profile_applies} applies done")
+        print(json.dumps({"mode": "profile", "applies": args.profile_applies,
+                          "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}), flush=True)
         return
 
     M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks)
@@ -122,8 +222,6 @@ def main():
     for k in range(args.warmup):
         run(ctx, k)
     ctx.synchronize()
-    ctx.timing(True)
-    ctx.timing_read(reset=True)
     lat32, its32, conv32 = [], [], []
     t_start = time.perf_counter()
     for k in range(args.warmup, nsteps):
@@ -133,8 +231,6 @@ def main():
         conv32.append(bool(res.converged))
     ctx.synchronize()
     t_total = time.perf_counter() - t_start
-    launches, ms_local = ctx.timing_read(reset=True)
-    ctx.timing(False)
     x32 = dX[args.warmup].download()
 
     # ---- plain FP64
@@ -154,35 +250,22 @@ def main():
     err32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
     err64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
 
-    # ---- batched persistent launch vs one launch per subdomain (FP32 applies)
+    # ---- batched launches vs one launch per subdomain (FP32 applies, dedicated loop: every timed
+    # k_local_solve<float> launch does the full work, unlike the GMRES loop where a launch after
+    # convergence is skipped by its mask)
     r = ctx.alloc(n)
     z = ctx.alloc(n)
     r.upload(B[0])
-    mode_ms = {}
-    for mode in (0, 1):
+    mode_ms = {0: (float("nan"), float("nan")), 1: (float("nan"), float("nan"))}
+    for mode in ((0,) if args.no_launch_compare else (0, 1)):
         ctx.set_launch_mode(mode)
-        for _ in range(3):
-            ctx.apply_device(r, z)
-        ctx.synchronize()
-        ctx.timing(True)
-        ctx.timing_read(reset=True)
-        t = time.perf_counter()
-        for _ in range(20):
-            ctx.apply_device(r, z)
-        ctx.synchronize()
-        wall = (time.perf_counter() - t) / 20
-        nl, ms = ctx.timing_read(reset=True)
-        ctx.timing(False)
-        mode_ms[mode] = (ms / max(nl, 1), wall * 1e3)
+        mode_ms[mode] = time_applies(ctx, r, z, 20, warm=3)
     ctx.set_launch_mode(0)
 
     # ---- roofline of the dominant kernel (batched local solve, FP32 factors)
-    avg_ms = ms_local / max(launches, 1)
-    achieved = stats.bytes_local_solve / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_local_solve<float>", "launches": int(launches), "avg_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": int(stats.bytes_local_solve)}
+    roof = roofline(stats.bytes_local_solve, mode_ms[0][0], 20)
+    if not args.no_traffic:
+        roof["traffic"], roof["traffic_note"] = measure_traffic(args)
 
     # ---- CPU baseline: the oracle (FP64 restatement) on the host cores, one solve
     cpu = None
@@ -226,11 +309,11 @@ def main():
         "speedup_fp32_vs_fp64": round((args.steps / t_total) / (args.steps / t64), 3),
         "solution_rel_err_fp32": err32,
         "solution_rel_err_fp64": err64,
-        "local_solve_us": {"batched_one_launch": round(mode_ms[0][0] * 1e3, 2),
-                           "one_launch_per_subdomain": round(mode_ms[1][0] * 1e3, 2),
-                           "apply_wall_us_batched": round(mode_ms[0][1] * 1e3, 2),
-                           "apply_wall_us_per_subdomain": round(mode_ms[1][1] * 1e3, 2)},
-        "roofline": roofline,
+        "local_solve_us": {"batched_one_launch": _us(mode_ms[0][0]),
+                           "one_launch_per_subdomain": _us(mode_ms[1][0]),
+                           "apply_wall_us_batched": _us(mode_ms[0][1]),
+                           "apply_wall_us_per_subdomain": _us(mode_ms[1][1])},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "setup_s": round(t_setup, 2),
     }

@@ -1,0 +1,108 @@
+"""GPU parity of the device Krylov loops (BiCGStab, several right-hand sides) against the CPU oracle,
+through the C-ABI."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+from cases import items, setup
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("laplace2d", dict(nx=20), 4, "random"), ("convdiff2d", dict(nx=24), 4, "random"),
+         ("bubbles3d", dict(nx=16, nbubbles=6, radius=0.12), 4, "generated")]
+
+
+def _ctx(st, precision="double"):
+    from amp_amd import device
+    return device.HipTwoLevelSchwarz(st, precision)
+
+
+def _st(name, kw, N, rhs):
+    print(f"case {name} {kw} N={N}", flush=True)
+    return setup(name, items(kw), N, items(dict(nev=8)), rhs=rhs)
+
+
+@pytest.mark.parametrize("name,kw,N,rhs", CASES)
+def test_bicgstab_fp64_matches_oracle(name, kw, N, rhs):
+    st = _st(name, kw, N, rhs)
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    xo, ro = orc.bicgstab(O.spmv, O.apply, b, rtol=1e-8, maxit=1000)
+    x, r = M.ctx.bicgstab(b, None, 1e-8, 1000)
+    assert r.converged == ro.converged and r.breakdown == ro.breakdown
+    assert r.iterations == ro.iterations and r.restarts == ro.restarts
+    assert len(r.history) == r.iterations + 1 == len(ro.history)
+    assert np.allclose(r.history, ro.history, rtol=1e-6, atol=1e-14)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    t2 = np.linalg.norm(b - st.dec.A @ x) / np.linalg.norm(b)
+    assert t2 <= 1.1e-8 and abs(t2 - r.true_rel_residual) < 1e-12
+
+
+@pytest.mark.parametrize("rtol", [1e-1, 1e-2, 1e-4, 1e-8])
+def test_bicgstab_openfoam_matches_oracle(rtol):
+    st = _st(*CASES[2])
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    xo, ro = orc.bicgstab(O.spmv, O.apply, b, rtol=rtol, maxit=500, stop="openfoam")
+    x, r = M.ctx.bicgstab(b, None, rtol, 500, "openfoam")
+    assert r.converged and r.iterations == ro.iterations and r.restarts == ro.restarts == 0
+    assert abs(r.initial_residual_of - 1.0) < 1e-12
+    t1 = np.abs(b - st.dec.A @ x).sum() / np.abs(b).sum()
+    assert t1 <= rtol * (1 + 1e-10) and abs(r.final_residual_of - t1) < 1e-12 * max(1.0, t1 / rtol)
+
+
+def test_bicgstab_openfoam_normalisation_nonzero_guess():
+    st = _st(*CASES[2])
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    x0 = 0.7 + 0.3 * np.sin(0.37 * np.arange(st.n))
+    xo, ro = orc.bicgstab(O.spmv, O.apply, b, x0, rtol=1e-3, maxit=500, stop="openfoam")
+    x, r = M.ctx.bicgstab(b, x0, 1e-3, 500, "openfoam")
+    assert r.converged and r.iterations == ro.iterations
+    assert abs(r.initial_residual_of - ro.initial_residual_of) < 1e-12 * ro.initial_residual_of
+    assert abs(r.final_residual_of - ro.final_residual_of) < 1e-9 * ro.initial_residual_of
+
+
+def test_bicgstab_budget_and_zero_rhs():
+    st = _st(*CASES[0])
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    xo, ro = orc.bicgstab(O.spmv, O.apply, b, rtol=1e-12, maxit=3)
+    x, r = M.ctx.bicgstab(b, None, 1e-12, 3)
+    assert not r.converged and r.iterations == 3 == ro.iterations
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-10
+    tb = np.linalg.norm(b - st.dec.A @ x) / np.linalg.norm(b)
+    assert abs(tb - r.true_rel_residual) < 1e-12 and tb < 1.0
+    x, r = M.ctx.bicgstab(np.zeros(st.n), np.ones(st.n), 1e-8, 100)
+    assert r.converged and r.iterations == 0 and len(r.history) == 1 and not x.any()
+
+
+def test_bicgstab_single_precision_factors():
+    st = _st(*CASES[1])
+    M32, M64 = _ctx(st, "single"), _ctx(st, "double")
+    b = st.dec.b
+    x32, r32 = M32.ctx.bicgstab(b, None, 1e-8, 1000)
+    x64, r64 = M64.ctx.bicgstab(b, None, 1e-8, 1000)
+    assert r32.converged and r32.true_rel_residual <= 1.1e-8
+    assert r32.iterations <= 1.1 * r64.iterations + 2
+    assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
+
+
+@pytest.mark.parametrize("method", ["gmres", "bicgstab"])
+def test_several_rhs_equal_single_solves(method):
+    """Each right-hand side of a batched solve gets the same result as when it is solved alone."""
+    from amp_amd import hostsetup as hs
+    st = _st(*CASES[1])
+    M = _ctx(st)
+    B = np.stack([hs.random_rhs(st.n, 100 + k) for k in range(3)])
+    B[1] *= 1e-3
+    X, R = M.ctx.krylov(B, None, method, 30, 1e-8, 1000)
+    for k in range(3):
+        x, r = M.ctx.krylov(B[k], None, method, 30, 1e-8, 1000)
+        assert R[k].iterations == r.iterations and R[k].converged == r.converged
+        assert np.array_equal(X[k], x)
