@@ -36,7 +36,7 @@ def build_device():
     objdir = os.path.join(HERE, "csrc", "device", "_obj")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for f in ("kernels.hip", "api.hip", "pack.cpp"):
+    for f in ("kernels.hip", "lsolve.hip", "api.hip", "pack.cpp", "lsched.cpp"):
         o = os.path.join(objdir, f + ".o")
         lang = ["-x", "hip"] if f.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", src] + lang + ["-c", os.path.join(src, f), "-o", o])

@@ -186,6 +186,14 @@ struct s2l_ctx {
     int precision = 0;
     bool local_ready = false;
     int launch_mode = 0;
+    // level-synchronous schedule (launch mode 2)
+    LsPlan ls;
+    DevBuf<LsSn> ls_sn;
+    DevBuf<LsItem> ls_items;
+    DevBuf<unsigned char> ls_diag, ls_bf, ls_bb, ls_slots, ls_part, ls_yf, ls_xb;
+    DevBuf<int32_t> ls_ridx, ls_cptr, ls_cidx, ls_owned, ls_begin;
+    // task-queue schedule (launch modes 0 and 1); needs a subdomain vector in the LDS of a workgroup
+    bool queue_ok = false;
     Packed pk;
     DevBuf<SubDesc> sub_d;
     DevBuf<SnDesc> sn_d;
@@ -213,7 +221,7 @@ struct s2l_ctx {
     bool coarse_perm = false;
 
     // work
-    DevBuf<double> x0, w, loc, cvec, tvec, yvec;
+    DevBuf<double> x0, w, loc, cvec, tvec, yvec, cpart;
     int work_nrhs = 0;
 
     // timing
@@ -227,7 +235,6 @@ struct s2l_ctx {
     DevBuf<double> kV, kw, kH, kcs, ksn, kg, kh, kh2, ky, khist, kpart, kr, kz, ku, kred;
     DevBuf<GmState> kst;
     DevBuf<int32_t> kflags;
-    DevBuf<unsigned> kcnt;
     PinBuf<GmState> pst;
     PinBuf<int32_t> pflags;
     PinBuf<double> pred;
@@ -246,9 +253,14 @@ struct s2l_ctx {
         cvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         tvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         yvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
+        cpart.reserve(static_cast<size_t>(std::max(nsub, 1)) * std::max(ca.rchunks, 1) * std::max(ca.mmax, 1) * nrhs);
         const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
         tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nrhs * esz);
         xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
+        ls_slots.reserve(static_cast<size_t>(std::max<int64_t>(ls.slots, 1)) * nrhs * esz);
+        ls_part.reserve(static_cast<size_t>(std::max<int64_t>(ls.partials, 1)) * nrhs * esz);
+        ls_yf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
+        ls_xb.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         work_nrhs = nrhs;
     }
 };
@@ -340,6 +352,46 @@ void launchQueue(s2l_ctx* c, LocalArgs<T>& a, const int32_t* q, int32_t count, b
     launchLocalSolve<T>(a, grid, c->lds_bytes, s);
 }
 
+template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode,
+                                       const int32_t* skip)
+{
+    LsArgs<T> a{};
+    a.sn = c->ls_sn.p;
+    a.items = c->ls_items.p;
+    a.diag = reinterpret_cast<const T*>(c->ls_diag.p);
+    a.belowF = reinterpret_cast<const T*>(c->ls_bf.p);
+    a.belowB = reinterpret_cast<const T*>(c->ls_bb.p);
+    a.ridx = c->ls_ridx.p;
+    a.cptr = c->ls_cptr.p;
+    a.cidx = c->ls_cidx.p;
+    a.gmap = c->gmap_d.p;
+    a.lperm = c->lperm_d.p;
+    a.owned = c->ls_owned.p;
+    a.begin = c->ls_begin.p;
+    a.rhs = rhs;
+    a.x0 = x0;
+    a.out = out;
+    a.pou = (mode == 1 && c->one_level == 0 && !c->boolean) ? c->pou_d.p : nullptr;
+    a.skip = skip;
+    a.slots = reinterpret_cast<T*>(c->ls_slots.p);
+    a.part = reinterpret_cast<T*>(c->ls_part.p);
+    a.yf = reinterpret_cast<T*>(c->ls_yf.p);
+    a.xb = reinterpret_cast<T*>(c->ls_xb.p);
+    a.n_global = c->n;
+    a.n_local_total = c->local_total;
+    a.nslots = c->ls.slots;
+    a.npart = c->ls.partials;
+    a.mode = mode;
+    return a;
+}
+
+template <typename T> void lsLaunches(s2l_ctx* c, const LsArgs<T>& a, int nrhs, hipStream_t s)
+{
+    const LsPlan& P = c->ls;
+    for (size_t q = 0; q + 1 < P.launch.size(); ++q)
+        launchLs<T>(a, P.launch_kind[q], P.launch[q], P.launch[q + 1] - P.launch[q], P.launch_w[q], nrhs, s);
+}
+
 template <typename T>
 void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip,
                 hipStream_t s)
@@ -360,7 +412,10 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
         c->ev_pool.pop_back();
         HIPCHECK(hipEventRecord(e0, s));
     }
-    if (c->launch_mode == 0) {
+    if (c->launch_mode == 2) {
+        // level-synchronous: one launch per level of all subdomains' trees
+        lsLaunches<T>(c, lsArgs<T>(c, rhs, x0, out, mode, skip), nrhs, s);
+    } else if (c->launch_mode == 0) {
         // level-ordered queue: one launch per depth group, every subdomain batched
         for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g)
             launchQueue<T>(c, a, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
@@ -414,7 +469,7 @@ void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* 
         return;
     }
     // x0 = R_0^T A_00^{-1} R_0 r
-    launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, nrhs, skip, s);
+    launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, c->cpart.p, nrhs, skip, s);
     launchTriGemv(c->n0, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
     launchTriGemv(c->n0, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
     if (c->boolean) {
@@ -467,10 +522,6 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
     c->kred.reserve(static_cast<size_t>(3) * nrhs);
     c->kst.reserve(static_cast<size_t>(nrhs));
     c->kflags.reserve(static_cast<size_t>(2) * nrhs);
-    if (c->kcnt.n < static_cast<size_t>(nrhs)) {
-        c->kcnt.reserve(static_cast<size_t>(nrhs));
-        HIPCHECK(hipMemsetAsync(c->kcnt.p, 0, sizeof(unsigned) * c->kcnt.n, s));
-    }
     c->pst.reserve(static_cast<size_t>(nrhs));
     c->pflags.reserve(static_cast<size_t>(2) * nrhs);
     c->pred.reserve(static_cast<size_t>(3) * nrhs);
@@ -494,11 +545,10 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
     a.hist_ld = maxit + 1;
     a.rtol = o->rtol;
     a.partial = c->kpart.p;
-    a.counter = c->kcnt.p;
     a.nblocks = nb;
 
     auto reduceToHost = [&](int kind, const double* x, const double* y, const int32_t* skip) {
-        launchReduce(kind, n, x, y, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+        launchReduce(kind, n, x, y, c->kpart.p, c->kred.p, nrhs, skip, s);
         HIPCHECK(hipMemcpyAsync(c->pred.p, c->kred.p, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
     };
@@ -648,10 +698,6 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     c->kpart.reserve(static_cast<size_t>(reduceBlocks(n)) * 3 * nrhs);
     c->kred.reserve(static_cast<size_t>(3) * nrhs);
     c->pred.reserve(static_cast<size_t>(3) * nrhs);
-    if (c->kcnt.n < static_cast<size_t>(nrhs)) {
-        c->kcnt.reserve(static_cast<size_t>(nrhs));
-        HIPCHECK(hipMemsetAsync(c->kcnt.p, 0, sizeof(unsigned) * c->kcnt.n, s));
-    }
     double* r = c->bR.p;
     double* rh = c->bRh.p;
     double* p = c->bP.p;
@@ -692,7 +738,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     // ||b|| for every right-hand side
     std::vector<char> all(static_cast<size_t>(nrhs), 1);
     stage(all, zero);
-    launchReduce(0, n, d_b, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+    launchReduce(0, n, d_b, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
     readSums();
     std::vector<char> live(static_cast<size_t>(nrhs), 0);
     for (int q = 0; q < nrhs; ++q) {
@@ -710,7 +756,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     if (stage(live, zero)) {
         spmvImpl(c, d_x, r, nullptr, nrhs, skip, s);
         if (of) {
-            launchReduce(5, n, d_x, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            launchReduce(5, n, d_x, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
             readSums();
             std::vector<double> xm(static_cast<size_t>(nrhs));
             for (int q = 0; q < nrhs; ++q) xm[static_cast<size_t>(q)] = n ? sums(q, 1) / static_cast<double>(n) : 0.0;
@@ -720,11 +766,11 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                 (xm[static_cast<size_t>(q)] == 0.0 ? nom : withm)[static_cast<size_t>(q)] = 1;
             }
             if (stage(nom, zero)) {  // xm = 0: F = |A x0|_1 + |b|_1 + 1e-20
-                launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
                 readSums();
                 for (int q = 0; q < nrhs; ++q)
                     if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 1);
-                launchReduce(1, n, d_b, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                launchReduce(1, n, d_b, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
                 readSums();
                 for (int q = 0; q < nrhs; ++q)
                     if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF += sums(q, 1) + 1e-20;
@@ -733,7 +779,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                 c->bones.reserve(static_cast<size_t>(n) * 2);
                 launchFill(n, 1.0, c->bones.p, s);
                 spmvImpl(c, c->bones.p, c->bones.p + n, nullptr, 1, nullptr, s);
-                ofNorm(n, r, d_b, c->bones.p + n, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                ofNorm(n, r, d_b, c->bones.p + n, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
                 readSums();
                 for (int q = 0; q < nrhs; ++q)
                     if (withm[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 0) + sums(q, 1) + 1e-20;
@@ -741,7 +787,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
             stage(live, zero);
         }
         launchAxpby(n, 1.0, d_b, -1.0, r, nrhs, skip, s);  // r = b - A x0
-        launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+        launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
         readSums();
         for (int q = 0; q < nrhs; ++q) {
             if (!live[static_cast<size_t>(q)]) continue;
@@ -802,7 +848,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
             bicgP(n, r, p, v, c->bcoef.p, nrhs, skip, s);
             applyImpl(c, p, y, nrhs, skip, s);
             spmvImpl(c, y, v, nullptr, nrhs, skip, s);
-            launchReduce(3, n, v, rh, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            launchReduce(3, n, v, rh, c->kpart.p, c->kred.p, nrhs, skip, s);
             readSums();
             std::fill(half.begin(), half.end(), 0);
             for (int q = 0; q < nrhs; ++q) {
@@ -821,7 +867,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
             }
             // s = r - alpha v, stopping norms of s (the half step)
             if (stage(half, [&](int q, int k) { return k == 0 ? R[static_cast<size_t>(q)].alpha : 0.0; })) {
-                bicgS(n, r, v, sv, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                bicgS(n, r, v, sv, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
                 readSums();
                 std::fill(on.begin(), on.end(), 0);
                 std::fill(full.begin(), full.end(), 0);
@@ -845,7 +891,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                 if (stage(full, zero)) {
                     applyImpl(c, sv, z, nrhs, skip, s);
                     spmvImpl(c, z, t, nullptr, nrhs, skip, s);
-                    launchReduce(4, n, t, sv, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                    launchReduce(4, n, t, sv, c->kpart.p, c->kred.p, nrhs, skip, s);
                     readSums();
                     for (int q = 0; q < nrhs; ++q) {
                         if (!full[static_cast<size_t>(q)]) continue;
@@ -858,7 +904,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                         const BicgRhs& st = R[static_cast<size_t>(q)];
                         return k == 0 ? st.alpha : (k == 1 ? st.omega : 0.0);
                     });
-                    bicgXR(n, d_x, y, z, sv, t, r, rh, c->bcoef.p, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+                    bicgXR(n, d_x, y, z, sv, t, r, rh, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
                     readSums();
                     for (int q = 0; q < nrhs; ++q) {
                         if (!full[static_cast<size_t>(q)]) continue;
@@ -888,7 +934,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
         }
         if (stage(on, zero)) {
             spmvImpl(c, d_x, r, d_b, nrhs, skip, s);
-            launchReduce(1, n, r, nullptr, c->kpart.p, c->kcnt.p, c->kred.p, nrhs, skip, s);
+            launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
             readSums();
             for (int q = 0; q < nrhs; ++q) {
                 if (!on[static_cast<size_t>(q)]) continue;
@@ -1125,22 +1171,56 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
         HIPCHECK(hipSetDevice(c->device));
         std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
+        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
+        // level-synchronous schedule (default launch mode)
+        c->ls = buildLsPlan(c->hf, c->local_ptr);
+        {
+            LsPlan& L = c->ls;
+            c->ls_sn.upload(L.sn, c->stream);
+            c->ls_items.upload(L.items, c->stream);
+            c->ls_ridx.upload(L.ridx, c->stream);
+            c->ls_cptr.upload(L.cptr, c->stream);
+            c->ls_cidx.upload(L.cidx, c->stream);
+            std::vector<int32_t> owned(static_cast<size_t>(c->nsub)), begin(static_cast<size_t>(c->nsub));
+            for (int i = 0; i < c->nsub; ++i) {
+                owned[static_cast<size_t>(i)] = c->ownership[i + 1] - c->ownership[i];
+                begin[static_cast<size_t>(i)] = c->ownership[i];
+            }
+            c->ls_owned.upload(owned, c->stream);
+            c->ls_begin.upload(begin, c->stream);
+            auto values = [&](DevBuf<unsigned char>& dst, std::vector<double>& src) {
+                if (precision == 1) {
+                    std::vector<float> f(src.begin(), src.end());
+                    dst.upload(reinterpret_cast<const unsigned char*>(f.data()), f.size() * sizeof(float), c->stream);
+                    HIPCHECK(hipStreamSynchronize(c->stream));
+                } else {
+                    dst.upload(reinterpret_cast<const unsigned char*>(src.data()), src.size() * sizeof(double), c->stream);
+                    HIPCHECK(hipStreamSynchronize(c->stream));
+                }
+                std::vector<double>().swap(src);
+            };
+            values(c->ls_diag, L.diag);
+            values(c->ls_bf, L.belowF);
+            values(c->ls_bb, L.belowB);
+        }
+        // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
         c->pk = packFactors(c->hf, moff, c->tasks_per_sub);
         Packed& P = c->pk;
         for (int i = 0; i < c->nsub; ++i) {
             P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
             P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
         }
-        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
         // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
         c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
         c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
         c->lds_bytes = c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
-        require(c->lds_bytes <= c->max_lds, S2L_ELIMIT,
-                "subdomain of " + std::to_string(P.max_n) + " unknowns exceeds the LDS of one workgroup (" + std::to_string(c->max_lds) +
-                    " bytes)");
-        const void* kern = precision == 1 ? localSolveKernel<float>() : localSolveKernel<double>();
-        HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
+        c->queue_ok = c->lds_bytes <= c->max_lds;
+        if (c->queue_ok) {
+            const void* kern = precision == 1 ? localSolveKernel<float>() : localSolveKernel<double>();
+            HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
+        } else if (c->launch_mode != 2) {
+            c->launch_mode = 2;  // the task queue cannot hold these subdomains in LDS
+        }
         // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
         std::vector<int32_t> gmap(c->gid_h.size()), lperm(c->gid_h.size());
         for (int i = 0; i < c->nsub; ++i) {
@@ -1170,7 +1250,12 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->queue_d.upload(P.queue, c->stream);
         c->qsubq_d.upload(P.queue_sub, c->stream);
         c->qsub_h = P.qsub;
-        if (precision == 1) {
+        if (!c->queue_ok) {
+            c->diag_f.release();
+            c->below_f.release();
+            c->diag_dd.release();
+            c->below_dd.release();
+        } else if (precision == 1) {
             std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
             c->diag_f.upload(d, c->stream);
             c->below_f.upload(b, c->stream);
@@ -1269,6 +1354,9 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
             a.nsub = c->nsub;
             a.n0 = n0;
             a.mmax = mmax;
+            int32_t maxrows = 0;
+            for (int32_t v : vrows) maxrows = std::max(maxrows, v);
+            a.rchunks = std::max(1, (maxrows + 255) / 256);
             a.off = c->off_d.p;
             a.voff = c->voff_d.p;
             a.vrows = c->vrows_d.p;
@@ -1378,6 +1466,13 @@ int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
         dg.upload(g, c->stream);
         auto run = [&](auto tag) {
             using T = decltype(tag);
+            if (c->launch_mode == 2) {
+                // every subdomain is solved; the others see their share of the scattered b
+                LsArgs<T> a = lsArgs<T>(c, dg.p, nullptr, c->loc.p, 1, nullptr);
+                a.pou = nullptr;
+                lsLaunches<T>(c, a, 1, c->stream);
+                return;
+            }
             LocalArgs<T> a = localArgs<T>(c, dg.p, nullptr, c->loc.p, 1, 1, nullptr);
             a.pou = nullptr; // y by local index, no partition of unity
             launchQueue<T>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
@@ -1457,8 +1552,9 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
         o->n0 = c->n0;
         o->precision = c->precision;
         o->nnz_a = c->nnz;
-        o->factor_entries_fwd = c->pk.entries_fwd;
-        o->factor_entries_bwd = c->pk.entries_bwd;
+        const bool lsm = c->launch_mode == 2;
+        o->factor_entries_fwd = lsm ? c->ls.entries : c->pk.entries_fwd;
+        o->factor_entries_bwd = lsm ? 0 : c->pk.entries_bwd;
         o->local_rows = c->local_total;
         int64_t crows = 0;
         for (int i = 0; i < c->nsub && c->n0 > 0; ++i) {
@@ -1467,15 +1563,29 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
             crows += static_cast<int64_t>(rows) * (c->off_h[static_cast<size_t>(i) + 1] - c->off_h[static_cast<size_t>(i)]);
         }
         o->coarse_rows = crows;
-        o->levels_fwd_max = c->pk.max_levels_fwd;
-        o->levels_bwd_max = c->pk.max_levels_bwd;
-        o->supernodes = static_cast<int32_t>(c->pk.sn.size());
+        o->levels_fwd_max = lsm ? c->ls.levels_fwd : c->pk.max_levels_fwd;
+        o->levels_bwd_max = lsm ? c->ls.levels_bwd : c->pk.max_levels_bwd;
+        o->supernodes = static_cast<int32_t>(lsm ? c->ls.sn.size() : c->pk.sn.size());
         const double sv = c->precision == 1 ? 4.0 : 8.0;
-        // local solve: factor values of both sweeps, index data, rhs gather (8 B value + 4 B map),
-        // permutation and owned-row output (read x0 8 B + write z 8 B)
-        const double local = (c->pk.entries_fwd + c->pk.entries_bwd) * sv + static_cast<double>(c->pk.bytes_index) +
-                              static_cast<double>(c->local_total) * (8.0 + 4.0 + 4.0) + static_cast<double>(c->n) * 16.0 +
-                              static_cast<double>(c->pk.t_total) * sv * 2.0;
+        double local = 0.0;
+        if (lsm) {
+            // level-synchronous: factor values of both sweeps (triangles of the inverted diagonal
+            // blocks and the blocks below), index data (descriptors, row indices, slot tables),
+            // rhs gather (8 B value + 4 B map), slots and backward partials written and read once,
+            // forward results written and read, backward results written and read by the
+            // descendants (one read per row index below a block), permutation and owned-row output
+            // (x0 read 8 B + z write 8 B)
+            const double lt = static_cast<double>(c->local_total);
+            local = static_cast<double>(c->ls.entries) * sv + static_cast<double>(c->ls.bytes_index) + lt * (8.0 + 4.0 + 4.0) +
+                    static_cast<double>(c->n) * 16.0 + static_cast<double>(c->ls.slots + c->ls.partials) * sv * 2.0 + lt * sv * 3.0 +
+                    static_cast<double>(c->ls.bwd_below_rows) * sv;
+        } else {
+            // task queue: factor values of both sweeps, index data, rhs gather (8 B value + 4 B map),
+            // permutation and owned-row output (read x0 8 B + write z 8 B)
+            local = (c->pk.entries_fwd + c->pk.entries_bwd) * sv + static_cast<double>(c->pk.bytes_index) +
+                    static_cast<double>(c->local_total) * (8.0 + 4.0 + 4.0) + static_cast<double>(c->n) * 16.0 +
+                    static_cast<double>(c->pk.t_total) * sv * 2.0;
+        }
         o->bytes_local_solve = local;
         const double spmv = static_cast<double>(c->nnz) * 12.0 + (static_cast<double>(c->n) + 1) * 4.0 + static_cast<double>(c->n) * 16.0;
         o->bytes_spmv = spmv;
@@ -1523,7 +1633,9 @@ int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
 int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
 {
     return guarded(c, [&] {
-        require(c && (mode == 0 || mode == 1), S2L_EINVAL, "launch mode must be 0 or 1");
+        require(c && mode >= 0 && mode <= 2, S2L_EINVAL, "launch mode must be 0, 1 or 2");
+        require(mode == 2 || !c->local_ready || c->queue_ok, S2L_ELIMIT,
+                "launch modes 0 and 1 need a subdomain vector in the LDS of one workgroup");
         c->launch_mode = mode;
     });
 }

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 
+#include "lsolve.hpp"
 #include "pack.hpp"
 
 namespace s2l {
@@ -56,6 +57,7 @@ template <typename T> struct LocalArgs {
 /// Coarse space arguments.
 struct CoarseArgs {
     int32_t nsub, n0, mmax;
+    int32_t rchunks;       ///< chunks of 256 rows of the largest V_i block (restriction grid)
     const int64_t* off;    ///< coarse offsets (nsub + 1)
     const int64_t* voff;   ///< offset of each V_i block
     const int32_t* vrows;  ///< rows of each V_i block
@@ -82,10 +84,16 @@ struct GmState {
 template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int grid, size_t lds, hipStream_t s);
 template <typename T> const void* localSolveKernel();
 
+/// One launch of the level-synchronous local solve: items [first, first + count) of kind LsKind
+/// (widest supernode maxw) for nrhs right-hand sides.
+template <typename T>
+void launchLs(const LsArgs<T>& a, int kind, int32_t first, int32_t count, int32_t maxw, int nrhs, hipStream_t s);
+
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s);
-void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, int nrhs, const int32_t* skip,
-                          hipStream_t s);
+/// c = R_0 r; part: nrhs x nsub x rchunks x mmax partials of the first stage
+void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
+                          const int32_t* skip, hipStream_t s);
 void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
                    const int32_t* skip, hipStream_t s);
 void launchCoarseProlong(const CoarseArgs& c, const double* y, double* out, int64_t ld_out, int boolean, int nrhs,
@@ -101,19 +109,19 @@ void launchCopy(int64_t n, const double* x, double* y, int nrhs, hipStream_t s);
 /// kind 0: x.x ; 1: x.x, sum|x| ; 2: x.x, sum|x|, x.y ; 3: y.x (single) ; 4: x.x, x.y
 void bicgP(int64_t n, const double* r, double* p, const double* v, const double* coef, int nrhs, const int32_t* skip,
            hipStream_t s);
-void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial,
-           unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s);
+void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial, double* out,
+           int nrhs, const int32_t* skip, hipStream_t s);
 void bicgXR(int64_t n, double* x, const double* y, const double* z, const double* sv, const double* t, double* r,
-            const double* rh, const double* coef, double* partial, unsigned* counter, double* out, int nrhs,
-            const int32_t* skip, hipStream_t s);
+            const double* rh, const double* coef, double* partial, double* out, int nrhs, const int32_t* skip,
+            hipStream_t s);
 void bicgHalf(int64_t n, double* x, const double* y, const double* sv, double* r, const double* coef, int nrhs,
               const int32_t* skip, hipStream_t s);
 void ofNorm(int64_t n, const double* ax, const double* b, const double* a1, const double* coef, double* partial,
-            unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s);
+            double* out, int nrhs, const int32_t* skip, hipStream_t s);
 void launchFill(int64_t n, double v, double* y, hipStream_t s);
 /// kind 0: (x.x); 1: (x.x, |x|_1); 2: (x.x, |x|_1, x.y); 3: (y.x); 4: (x.x, x.y); 5: (x.x, sum x)
-void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, unsigned* counter, double* out,
-                  int nrhs, const int32_t* skip, hipStream_t s);
+void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, double* out, int nrhs,
+                  const int32_t* skip, hipStream_t s);
 int reduceBlocks(int64_t n);
 
 // ---- GMRES
@@ -136,7 +144,6 @@ struct GmArgs {
     int32_t hist_ld;
     double rtol;
     double* partial;    ///< block partial sums
-    unsigned* counter;  ///< one per rhs
     int32_t nblocks;
 };
 void gmDots(const GmArgs& a, int j, hipStream_t s);

@@ -325,42 +325,61 @@ __global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const d
 }
 
 // ------------------------------------------------------------------ coarse space
-constexpr int kCq = 8; // coarse columns per workgroup of the restriction
+constexpr int kCq = 8;       // coarse columns per pass of the restriction
+constexpr int kCrows = 256;  // rows of V_i per workgroup of the restriction (one per thread)
 
-__global__ __launch_bounds__(256) void k_coarse_restrict(CoarseArgs c, const double* r, int64_t n, double* cvec,
-                                                          const int32_t* skip)
+
+// c_i = V_i^T r(rows of V_i), first stage: workgroup (i, chunk) sums its 256 rows for every column.
+__global__ __launch_bounds__(kCrows) void k_coarse_restrict(CoarseArgs c, const double* r, int64_t n, double* part,
+                                                             const int32_t* skip)
 {
     const int rr = blockIdx.z;
     if (skip && skip[rr]) return;
-    const int i = blockIdx.x, cq = blockIdx.y * kCq;
-    const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
-    if (cq >= m) return;
+    const int i = blockIdx.x, ch = blockIdx.y;
     const int rows = c.vrows[i];
+    const int nch = (rows + kCrows - 1) / kCrows;
+    if (ch >= nch) return;
+    const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
     const double* V = c.V + c.voff[i];
     const int32_t* g = c.vrow_g + c.vrow_ptr[i];
     const double* rv = r + static_cast<int64_t>(rr) * n;
-    double acc[kCq];
-#pragma unroll
-    for (int q = 0; q < kCq; ++q) acc[q] = 0.0;
-    const int nq = min(kCq, m - cq);
-    for (int l = threadIdx.x; l < rows; l += blockDim.x) {
-        const double xv = rv[g[l]];
-#pragma unroll
-        for (int q = 0; q < kCq; ++q)
-            if (q < nq) acc[q] += V[static_cast<int64_t>(cq + q) * rows + l] * xv;
-    }
-    __shared__ double red[256 / kWave][kCq];
+    const int l = ch * kCrows + threadIdx.x;
+    const double xv = l < rows ? rv[g[l]] : 0.0;
+    __shared__ double red[kCrows / kWave][kCq];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    double* pr = part + ((static_cast<int64_t>(rr) * c.nsub + i) * c.rchunks + ch) * c.mmax;
+    for (int cq = 0; cq < m; cq += kCq) {
+        double acc[kCq];
 #pragma unroll
-    for (int q = 0; q < kCq; ++q) {
-        const double v = waveSum(acc[q]);
-        if (lane == 0) red[wave][q] = v;
+        for (int q = 0; q < kCq; ++q) acc[q] = (l < rows && cq + q < m) ? V[static_cast<int64_t>(cq + q) * rows + l] * xv : 0.0;
+#pragma unroll
+        for (int q = 0; q < kCq; ++q) {
+            const double v = waveSum(acc[q]);
+            if (lane == 0) red[wave][q] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < kCq && cq + threadIdx.x < m) {
+            double t = 0.0;
+            for (int wv = 0; wv < kCrows / kWave; ++wv) t += red[wv][threadIdx.x];
+            pr[cq + threadIdx.x] = t;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < nq) {
-        double s = 0.0;
-        for (int wv = 0; wv < static_cast<int>(blockDim.x / kWave); ++wv) s += red[wv][threadIdx.x];
-        cvec[static_cast<int64_t>(rr) * c.n0 + c.off[i] + cq + threadIdx.x] = s;
+}
+
+// c_i[q] = sum over the chunks of subdomain i, in chunk order.
+__global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int i = blockIdx.x;
+    const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
+    const int nch = (c.vrows[i] + kCrows - 1) / kCrows;
+    const double* base = part + (static_cast<int64_t>(rr) * c.nsub + i) * c.rchunks * c.mmax;
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        double t = 0.0;
+        for (int b = 0; b < nch; ++b) t += base[static_cast<int64_t>(b) * c.mmax + q];
+        cvec[static_cast<int64_t>(rr) * c.n0 + c.off[i] + q] = t;
     }
 }
 
@@ -439,22 +458,18 @@ __global__ void k_copy(int64_t n, const double* x, double* y)
 constexpr int kRedThreads = 256;
 constexpr int kRedComp = 3;
 
-__device__ inline bool lastBlock(unsigned* counter, unsigned nblocks)
-{
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == nblocks - 1);
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
 
-// Three sums of one right-hand side (blockIdx.y): wave sums, one partial per block, and the last
-// block to finish adds the partials in block order (fixed order: bitwise repeatable).
-__device__ void finishReduce(double a0, double a1, double a2, double* partial, unsigned* counter, double* out, int rr)
+// Two-stage sums without atomics: every block of a reduction kernel writes its partial sums
+// (a wave tree, then the waves in order), and a second launch adds the partials of all blocks, one
+// block per sum: thread t takes blocks t, t + blockDim.x, ... in order, then a wave tree and the
+// waves in order. The order is fixed, so results are bitwise repeatable.
+constexpr int kBlockThreads = 256;
+constexpr int kBlockWaves = kBlockThreads / kWave;
+
+// Partial sums (a0, a1, a2) of this block for right-hand side rr: partial[(rr * gridDim.x + b) * 3 + k].
+__device__ void writePartial3(double a0, double a1, double a2, double* partial, int rr)
 {
-    __shared__ double red[kRedThreads / kWave][kRedComp];
+    __shared__ double red[kBlockWaves][kRedComp];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     a0 = waveSum(a0);
     a1 = waveSum(a1);
@@ -465,26 +480,42 @@ __device__ void finishReduce(double a0, double a1, double a2, double* partial, u
         red[wave][2] = a2;
     }
     __syncthreads();
-    double* pr = partial + (static_cast<int64_t>(rr) * gridDim.x + blockIdx.x) * kRedComp;
     if (threadIdx.x < kRedComp) {
-        double s = 0.0;
-        for (int wv = 0; wv < kRedThreads / kWave; ++wv) s += red[wv][threadIdx.x];
-        pr[threadIdx.x] = s;
-    }
-    if (lastBlock(counter + rr, gridDim.x)) {
-        if (threadIdx.x < kRedComp) {
-            const double* base = partial + static_cast<int64_t>(rr) * gridDim.x * kRedComp;
-            double s = 0.0;
-            for (unsigned b = 0; b < gridDim.x; ++b) s += base[b * kRedComp + threadIdx.x];
-            out[rr * kRedComp + threadIdx.x] = s;
-        }
-        if (threadIdx.x == 0) counter[rr] = 0;
+        double t = 0.0;
+        for (int wv = 0; wv < kBlockWaves; ++wv) t += red[wv][threadIdx.x];
+        partial[(static_cast<int64_t>(rr) * gridDim.x + blockIdx.x) * kRedComp + threadIdx.x] = t;
     }
 }
 
+// Block-wide sum of one value per thread (fixed order); the result is valid in thread 0.
+__device__ double blockSum(double v)
+{
+    __shared__ double red[kBlockWaves];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    v = waveSum(v);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int wv = 0; wv < kBlockWaves; ++wv) t += red[wv];
+    return t;
+}
+
+// out[rr * out_ld + k] = sum over b < nb of partial[(rr * nb + b) * ld + k]; grid (K, nrhs).
+__global__ __launch_bounds__(kBlockThreads) void k_final_sum(const double* partial, int ld, int nb, double* out, int out_ld,
+                                                             const int32_t* skip)
+{
+    const int k = blockIdx.x, rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const double* base = partial + static_cast<int64_t>(rr) * nb * ld + k;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += base[static_cast<int64_t>(b) * ld];
+    const double t = blockSum(v);
+    if (threadIdx.x == 0) out[static_cast<int64_t>(rr) * out_ld + k] = t;
+}
+
 __global__ __launch_bounds__(kRedThreads) void k_reduce(int kind, int64_t n, const double* x, const double* y,
-                                                         double* partial, unsigned* counter, double* out,
-                                                         const int32_t* skip)
+                                                         double* partial, const int32_t* skip)
 {
     const int rr = blockIdx.y;
     if (skip && skip[rr]) return;
@@ -504,7 +535,7 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(int kind, int64_t n, con
             if (kind == 5) a1 += v;
         }
     }
-    finishReduce(a0, a1, a2, partial, counter, out, rr);
+    writePartial3(a0, a1, a2, partial, rr);
 }
 
 // ------------------------------------------------------------------ BiCGStab (bicgstab.cpp:81-137)
@@ -525,8 +556,7 @@ __global__ void k_bicg_p(int64_t n, const double* r, double* p, const double* v,
 
 // s = r - alpha v; sums (s.s, |s|_1)
 __global__ __launch_bounds__(kRedThreads) void k_bicg_s(int64_t n, const double* r, const double* v, double* sv,
-                                                        const double* coef, double* partial, unsigned* counter,
-                                                        double* out, const int32_t* skip)
+                                                        const double* coef, double* partial, const int32_t* skip)
 {
     const int rr = blockIdx.y;
     if (skip && skip[rr]) return;
@@ -540,14 +570,13 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_s(int64_t n, const double*
         a0 += s * s;
         a1 += fabs(s);
     }
-    finishReduce(a0, a1, 0.0, partial, counter, out, rr);
+    writePartial3(a0, a1, 0.0, partial, rr);
 }
 
 // x += alpha y + omega z; r = s - omega t; sums (r.r, |r|_1, rh.r)
 __global__ __launch_bounds__(kRedThreads) void k_bicg_xr(int64_t n, double* x, const double* y, const double* z,
                                                          const double* sv, const double* t, double* r, const double* rh,
-                                                         const double* coef, double* partial, unsigned* counter,
-                                                         double* out, const int32_t* skip)
+                                                         const double* coef, double* partial, const int32_t* skip)
 {
     const int rr = blockIdx.y;
     if (skip && skip[rr]) return;
@@ -564,13 +593,12 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_xr(int64_t n, double* x, c
         a1 += fabs(rv);
         a2 += rh[o] * rv;
     }
-    finishReduce(a0, a1, a2, partial, counter, out, rr);
+    writePartial3(a0, a1, a2, partial, rr);
 }
 
 // OpenFOAM normalisation (krylov.cpp:47-57): sums (|A x0 - xm A 1|_1, |b - xm A 1|_1), xm = coef[4 r]
 __global__ __launch_bounds__(kRedThreads) void k_of_norm(int64_t n, const double* ax, const double* b, const double* a1,
-                                                         const double* coef, double* partial, unsigned* counter,
-                                                         double* out, const int32_t* skip)
+                                                         const double* coef, double* partial, const int32_t* skip)
 {
     const int rr = blockIdx.y;
     if (skip && skip[rr]) return;
@@ -583,7 +611,7 @@ __global__ __launch_bounds__(kRedThreads) void k_of_norm(int64_t n, const double
         a0 += fabs(ax[base + i] - m);
         b0 += fabs(b[base + i] - m);
     }
-    finishReduce(a0, b0, 0.0, partial, counter, out, rr);
+    writePartial3(a0, b0, 0.0, partial, rr);
 }
 
 __global__ void k_fill(int64_t n, double v, double* y)
@@ -613,35 +641,35 @@ constexpr int kGmThreads = 256;
 __device__ inline const double* basis(const GmArgs& a, int r, int k) { return a.V + (static_cast<int64_t>(k) * a.nrhs + r) * a.n; }
 __device__ inline double* basisW(const GmArgs& a, int r, int k) { return a.V + (static_cast<int64_t>(k) * a.nrhs + r) * a.n; }
 
-// h[k] = V_k . w for k <= j; one partial per block, the last block sums them in order.
-__device__ void gmDotsBody(const GmArgs& a, int j, int r, const double* w, double* hout)
+// Per-block partials of h[k] = V_k . w for k <= j (summed by k_gm_final).
+__device__ void gmDotsBody(const GmArgs& a, int j, int r, const double* w)
 {
-    __shared__ double red[kGmThreads / kWave][kGmMaxK];
+    __shared__ double red[kGmThreads / kWave * kGmMaxK];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (int k = 0; k <= j; ++k) {
-        const double* v = basis(a, r, k);
-        double s = 0.0;
-        for (int64_t i = i0; i < a.n; i += stride) s += v[i] * w[i];
-        s = waveSum(s);
-        if (lane == 0) red[wave][k] = s;
+    for (int k0 = 0; k0 <= j; k0 += 8) {
+        double s[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] = 0.0;
+        for (int64_t i = i0; i < a.n; i += stride) {
+            const double wi = w[i];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (k0 + q <= j) s[q] += basis(a, r, k0 + q)[i] * wi;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double v = waveSum(s[q]);
+            if (lane == 0 && k0 + q <= j) red[wave * kGmMaxK + k0 + q] = v;
+        }
     }
     __syncthreads();
     double* pr = a.partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kGmMaxK;
     for (int k = threadIdx.x; k <= j; k += blockDim.x) {
         double s = 0.0;
-        for (int wv = 0; wv < kGmThreads / kWave; ++wv) s += red[wv][k];
+        for (int wv = 0; wv < kGmThreads / kWave; ++wv) s += red[wv * kGmMaxK + k];
         pr[k] = s;
-    }
-    if (lastBlock(a.counter + r, gridDim.x)) {
-        const double* base = a.partial + static_cast<int64_t>(r) * gridDim.x * kGmMaxK;
-        for (int k = threadIdx.x; k <= j; k += blockDim.x) {
-            double s = 0.0;
-            for (unsigned b = 0; b < gridDim.x; ++b) s += base[b * kGmMaxK + k];
-            hout[k] = s;
-        }
-        if (threadIdx.x == 0) a.counter[r] = 0;
     }
 }
 
@@ -649,10 +677,10 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_dots(GmArgs a, int j)
 {
     const int r = blockIdx.y;
     if (a.flags[r]) return;
-    gmDotsBody(a, j, r, a.w + static_cast<int64_t>(r) * a.n, a.h + r * (a.m + 1));
+    gmDotsBody(a, j, r, a.w + static_cast<int64_t>(r) * a.n);
 }
 
-// w -= V h, then h2 = V^T w.
+// w -= V h, then partials of h2 = V^T w.
 __global__ __launch_bounds__(kGmThreads) void k_gm_update_dots(GmArgs a, int j)
 {
     const int r = blockIdx.y;
@@ -665,14 +693,14 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_update_dots(GmArgs a, int j)
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         double s = 0.0;
+#pragma unroll 8
         for (int k = 0; k <= j; ++k) s += basis(a, r, k)[i] * hs[k];
         w[i] -= s;
     }
-    gmDotsBody(a, j, r, w, a.h2 + r * (a.m + 1));
+    gmDotsBody(a, j, r, w);
 }
 
-// w -= V h2, hn = ||w||; the last block finishes the Arnoldi step: Hessenberg column, Givens
-// rotations, residual estimate and the stopping test (gmres.cpp:149-208).
+// w -= V h2 and the partials of ||w||^2 (summed by k_gm_givens).
 __global__ __launch_bounds__(kGmThreads) void k_gm_update_norm(GmArgs a, int j)
 {
     const int r = blockIdx.y;
@@ -686,6 +714,7 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_update_norm(GmArgs a, int j)
     double nrm = 0.0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
         double s = 0.0;
+#pragma unroll 8
         for (int k = 0; k <= j; ++k) s += basis(a, r, k)[i] * hs[k];
         const double v = w[i] - s;
         w[i] = v;
@@ -702,12 +731,32 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_update_norm(GmArgs a, int j)
         for (int wv = 0; wv < kGmThreads / kWave; ++wv) s += red[wv];
         pr[0] = s;
     }
-    if (!lastBlock(a.counter + r, gridDim.x)) return;
+}
+
+// h (second == 0) or h2 (second == 1) = sums of the partials of k_gm_dots / k_gm_update_dots;
+// grid (j + 1, nrhs).
+__global__ __launch_bounds__(kBlockThreads) void k_gm_final(GmArgs a, int j, int second)
+{
+    const int k = blockIdx.x, r = blockIdx.y;
+    if (a.flags[r]) return;
+    const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK + k;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+    const double t = blockSum(v);
+    if (threadIdx.x == 0) (second ? a.h2 : a.h)[r * (a.m + 1) + k] = t;
+}
+
+// hn = ||w|| from the partials of k_gm_update_norm, then the end of the Arnoldi step: Hessenberg
+// column, Givens rotations, residual estimate and the stopping test (gmres.cpp:149-208); grid nrhs.
+__global__ __launch_bounds__(kBlockThreads) void k_gm_givens(GmArgs a, int j)
+{
+    const int r = blockIdx.x;
+    if (a.flags[r]) return;
+    const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+    const double ss = blockSum(v);
     if (threadIdx.x != 0) return;
-    a.counter[r] = 0;
-    const double* base = a.partial + static_cast<int64_t>(r) * gridDim.x * kGmMaxK;
-    double ss = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) ss += base[b * kGmMaxK];
     const double hn = sqrt(ss);
     const int m = a.m;
     double* H = a.H + static_cast<int64_t>(r) * (m + 1) * m;
@@ -716,7 +765,8 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_update_norm(GmArgs a, int j)
     double* sn = a.sn + r * m;
     double* g = a.g + r * (m + 1);
     const double* h = a.h + r * (m + 1);
-    for (int i = 0; i <= j; ++i) Hc[i] = h[i] + hs[i];
+    const double* h2 = a.h2 + r * (m + 1);
+    for (int i = 0; i <= j; ++i) Hc[i] = h[i] + h2[i];
     Hc[j + 1] = hn;
     GmState& st = a.st[r];
     st.hn = hn;
@@ -803,6 +853,7 @@ __global__ void k_gm_combine(GmArgs a, double* u, const int32_t* idle)
     const int k = a.st[r].k;
     const double* y = a.y + r * a.m;
     double s = 0.0;
+#pragma unroll 8
     for (int l = 0; l < k; ++l) s += basis(a, r, l)[i] * y[l];
     u[static_cast<int64_t>(r) * a.n + i] = s;
 }
@@ -829,10 +880,11 @@ void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* 
     hipLaunchKernelGGL(k_spmv, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, rp, col, val, x, y, b, skip);
 }
 
-void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, int nrhs, const int32_t* skip,
-                          hipStream_t s)
+void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
+                          const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_coarse_restrict, dim3(c.nsub, (c.mmax + kCq - 1) / kCq, nrhs), dim3(256), 0, s, c, r, n, cvec, skip);
+    hipLaunchKernelGGL(k_coarse_restrict, dim3(c.nsub, c.rchunks, nrhs), dim3(kCrows), 0, s, c, r, n, part, skip);
+    hipLaunchKernelGGL(k_coarse_final, dim3(c.nsub, nrhs), dim3(64), 0, s, c, part, cvec, skip);
 }
 
 void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
@@ -866,14 +918,23 @@ void launchCopy(int64_t n, const double* x, double* y, int nrhs, hipStream_t s)
 int reduceBlocks(int64_t n)
 {
     const int64_t b = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
-    return static_cast<int>(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+    return static_cast<int>(b < 1 ? 1 : (b > 256 ? 256 : b));
 }
 
-void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, unsigned* counter, double* out,
-                  int nrhs, const int32_t* skip, hipStream_t s)
+namespace {
+void finalSum3(const double* partial, int nb, double* out, int nrhs, const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, kind, n, x, y, partial, counter, out,
+    hipLaunchKernelGGL(k_final_sum, dim3(kRedComp, nrhs), dim3(kBlockThreads), 0, s, partial, kRedComp, nb, out, kRedComp,
                        skip);
+}
+} // namespace
+
+void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, double* out, int nrhs,
+                  const int32_t* skip, hipStream_t s)
+{
+    const int nb = reduceBlocks(n);
+    hipLaunchKernelGGL(k_reduce, dim3(nb, nrhs), dim3(kRedThreads), 0, s, kind, n, x, y, partial, skip);
+    finalSum3(partial, nb, out, nrhs, skip, s);
 }
 
 void bicgP(int64_t n, const double* r, double* p, const double* v, const double* coef, int nrhs, const int32_t* skip,
@@ -881,30 +942,32 @@ void bicgP(int64_t n, const double* r, double* p, const double* v, const double*
 {
     hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, r, p, v, coef, skip);
 }
-void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial,
-           unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s)
+void bicgS(int64_t n, const double* r, const double* v, double* sv, const double* coef, double* partial, double* out,
+           int nrhs, const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bicg_s, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, r, v, sv, coef, partial, counter,
-                       out, skip);
+    const int nb = reduceBlocks(n);
+    hipLaunchKernelGGL(k_bicg_s, dim3(nb, nrhs), dim3(kRedThreads), 0, s, n, r, v, sv, coef, partial, skip);
+    finalSum3(partial, nb, out, nrhs, skip, s);
 }
 void bicgXR(int64_t n, double* x, const double* y, const double* z, const double* sv, const double* t, double* r,
-            const double* rh, const double* coef, double* partial, unsigned* counter, double* out, int nrhs,
-            const int32_t* skip, hipStream_t s)
+            const double* rh, const double* coef, double* partial, double* out, int nrhs, const int32_t* skip,
+            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bicg_xr, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, x, y, z, sv, t, r, rh, coef,
-                       partial, counter, out, skip);
+    const int nb = reduceBlocks(n);
+    hipLaunchKernelGGL(k_bicg_xr, dim3(nb, nrhs), dim3(kRedThreads), 0, s, n, x, y, z, sv, t, r, rh, coef, partial, skip);
+    finalSum3(partial, nb, out, nrhs, skip, s);
 }
 void bicgHalf(int64_t n, double* x, const double* y, const double* sv, double* r, const double* coef, int nrhs,
               const int32_t* skip, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bicg_half, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, x, y, sv, r, coef, skip);
 }
-
 void ofNorm(int64_t n, const double* ax, const double* b, const double* a1, const double* coef, double* partial,
-            unsigned* counter, double* out, int nrhs, const int32_t* skip, hipStream_t s)
+            double* out, int nrhs, const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_of_norm, dim3(reduceBlocks(n), nrhs), dim3(kRedThreads), 0, s, n, ax, b, a1, coef, partial,
-                       counter, out, skip);
+    const int nb = reduceBlocks(n);
+    hipLaunchKernelGGL(k_of_norm, dim3(nb, nrhs), dim3(kRedThreads), 0, s, n, ax, b, a1, coef, partial, skip);
+    finalSum3(partial, nb, out, nrhs, skip, s);
 }
 void launchFill(int64_t n, double v, double* y, hipStream_t s)
 {
@@ -914,14 +977,17 @@ void launchFill(int64_t n, double v, double* y, hipStream_t s)
 void gmDots(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_dots, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);
+    hipLaunchKernelGGL(k_gm_final, dim3(j + 1, a.nrhs), dim3(kBlockThreads), 0, s, a, j, 0);
 }
 void gmUpdateDots(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_update_dots, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);
+    hipLaunchKernelGGL(k_gm_final, dim3(j + 1, a.nrhs), dim3(kBlockThreads), 0, s, a, j, 1);
 }
 void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_update_norm, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);
+    hipLaunchKernelGGL(k_gm_givens, dim3(a.nrhs), dim3(kBlockThreads), 0, s, a, j);
 }
 void gmNormalize(const GmArgs& a, int j, hipStream_t s)
 {

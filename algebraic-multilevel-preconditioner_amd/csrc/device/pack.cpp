@@ -17,8 +17,6 @@
 
 namespace s2l {
 
-namespace {
-
 std::vector<int32_t> detectSupernodes(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& row)
 {
     for (int32_t j = 0; j < n; ++j) {
@@ -37,6 +35,8 @@ std::vector<int32_t> detectSupernodes(int32_t n, const std::vector<int64_t>& cp,
     starts.push_back(n);
     return starts;
 }
+
+namespace {
 
 // Appends a level table, 7 ints per level: start and end of the level in lev_sn, start of its wide
 // supernodes (the small ones come first), sum of the small widths, sum of the small heights,
@@ -85,6 +85,8 @@ void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, i
     }
 }
 
+} // namespace
+
 // In-place inverse of the lower triangular m x m block stored column-major at B with leading
 // dimension ld (B[c * ld + r] = L(r, c), r >= c), in FP64: column j of the inverse solves L x = e_j.
 void invertLower(double* B, int ld, int m)
@@ -119,6 +121,8 @@ void invertUpper(double* B, int ld, int m)
     for (int c = 0; c < m; ++c)
         for (int r = 0; r < m; ++r) B[static_cast<std::size_t>(c) * ld + r] = (r <= c) ? X[static_cast<std::size_t>(c) * m + r] : 0.0;
 }
+
+namespace {
 
 // Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64;
 // column-major for a small supernode (a thread per row walks a column-major block: coalesced over

@@ -84,6 +84,14 @@ struct Packed {
 };
 
 /// Packs factors; map_off[i] gives the offset of subdomain i in the gmap/lperm arrays.
+/// Supernode partition of a factor in CSC form (diagonal first, rows sorted): columns with the
+/// same structure below the diagonal form one supernode. Returns the first column of every
+/// supernode and n.
+std::vector<int32_t> detectSupernodes(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& row);
+/// In-place FP64 inverses of a lower / upper triangular m x m block, column-major with leading dimension ld.
+void invertLower(double* B, int ld, int m);
+void invertUpper(double* B, int ld, int m);
+
 /// @p tasks_per_sub: wanted subtree tasks per subdomain (1 = one task per subdomain).
 Packed packFactors(const std::vector<HostFactor>& f, const std::vector<int32_t>& map_off, int tasks_per_sub);
 

@@ -44,6 +44,9 @@ WORKLOADS = {
 
 
 COPY_COUNT = 1 << 25   # doubles of the calibration copy (256 MiB read + 256 MiB written)
+PMC_APPLIES = 3        # applies of a counter pass
+PROFILE_WARM = 2       # untimed applies before the timed ones in --profile-applies runs
+LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float>")
 
 
 def time_applies(ctx, r, z, count, warm=3):
@@ -68,15 +71,16 @@ def roofline(bytes_per_launch, avg_ms, launches):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_local_solve<float>", "launches": int(launches), "avg_us": round(avg_ms * 1e3, 2),
+            "kernel": "k_local_solve<float> (task queue, the launches of one apply)", "launches": int(launches),
+            "avg_us": round(avg_ms * 1e3, 2),
             "bytes_per_launch": int(bytes_per_launch)}
 
 
 def measure_traffic(args):
-    """HBM bytes of one k_local_solve<float> launch from two rocprofv3 --pmc passes (FETCH_SIZE, then
-    WRITE_SIZE) over a child run of this script with --profile-applies, each scaled by what the counter
-    reads for the calibration copy of COPY_COUNT doubles. The child skips the coarse space (the local
-    solve then reads no x0: 8 n bytes less than in the two-level apply). Returns (bytes, note)."""
+    """HBM bytes of the local-solve launches of one apply from two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE) over a child run of this script with --profile-applies, each scaled by what the
+    counter reads for the calibration copy of COPY_COUNT doubles. The child skips the coarse space (the
+    local solve then reads no x0: 8 n bytes less than in the two-level apply). Returns (bytes, note)."""
     import csv
     import glob
     import shutil
@@ -89,7 +93,7 @@ def measure_traffic(args):
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="s2l_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
-               os.path.abspath(__file__), "--profile-applies", "3", "--profile-one-level", "--setup-workers", "1",
+               os.path.abspath(__file__), "--profile-applies", str(PMC_APPLIES), "--profile-one-level", "--setup-workers", "1",
                "--workload", args.workload, "--tasks", str(args.tasks)]
         log(f"traffic pass {counter}")
         try:
@@ -100,27 +104,28 @@ def measure_traffic(args):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if p.returncode != 0 or not files:
             return None, f"{counter} pass failed (exit {p.returncode})"
-        local, copy = [], []
+        local, copy = 0.0, []
         for f in files:
             for row in csv.DictReader(open(f)):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row["Kernel_Name"]
-                if "k_local_solve<float>" in name:
-                    local.append(float(row["Counter_Value"]))
+                if any(k in name for k in LOCAL_KERNELS):
+                    local += float(row["Counter_Value"])
                 elif "k_copy" in name:
                     copy.append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
-        if not local or not copy:
+        if local == 0.0 or not copy:
             return None, f"{counter}: kernels not found in the counter output"
-        kib[counter] = (float(np.mean(local)), float(np.mean(copy)))
+        # the child runs PMC_APPLIES + PROFILE_WARM applies
+        kib[counter] = (local / (PMC_APPLIES + PROFILE_WARM), float(np.mean(copy)))
     # the copy moves COPY_COUNT * 8 bytes each way
     ref_kib = COPY_COUNT * 8 / 1024.0
     ff = ref_kib / kib["FETCH_SIZE"][1]
     wf = ref_kib / kib["WRITE_SIZE"][1]
     traffic = (kib["FETCH_SIZE"][0] * ff + kib["WRITE_SIZE"][0] * wf) * 1024.0
     note = (f"FETCH_SIZE {kib['FETCH_SIZE'][0]:.0f} KiB x {ff:.3f}, WRITE_SIZE {kib['WRITE_SIZE'][0]:.0f} KiB x "
-            f"{wf:.3f} per launch (factors from a copy of {COPY_COUNT} doubles)")
+            f"{wf:.3f} per apply's local solve (factors from a copy of {COPY_COUNT} doubles)")
     return int(traffic), note
 
 
@@ -147,8 +152,7 @@ def main():
     ap.add_argument("--profile-applies", type=int, default=0,
                     help="only run this many FP32 applies after setup (for rocprofv3 runs)")
     ap.add_argument("--no-launch-compare", action="store_true",
-                    help="skip the comparison with one launch per subdomain (keeps every k_local_solve<float> "
-                         "launch of the run the same work, for profiler averages)")
+                    help="skip the comparison with one launch per subdomain")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) that measure the HBM traffic "
                          "of the local-solve launch")
@@ -187,11 +191,11 @@ def main():
         f"{stats.levels_fwd_max}/{stats.levels_bwd_max}, bytes/local solve {stats.bytes_local_solve / 1e6:.1f} MB")
 
     if args.profile_applies > 0:
-        # only FP32 applies (every k_local_solve<float> launch does the same work), event-timed
+        # only FP32 applies (every local-solve launch set does the same work), event-timed
         r = ctx.alloc(n)
         z = ctx.alloc(n)
         r.upload(st.dec.b)
-        avg_ms, _ = time_applies(ctx, r, z, args.profile_applies, warm=2)
+        avg_ms, _ = time_applies(ctx, r, z, args.profile_applies, warm=PROFILE_WARM)
         # calibration point of the HBM traffic counters: a plain copy of 2^25 doubles (512 MiB moved)
         big_a, big_b = ctx.alloc(COPY_COUNT), ctx.alloc(COPY_COUNT)
         for _ in range(3):
@@ -250,14 +254,16 @@ def main():
     err32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
     err64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
 
-    # ---- batched launches vs one launch per subdomain (FP32 applies, dedicated loop: every timed
-    # k_local_solve<float> launch does the full work, unlike the GMRES loop where a launch after
-    # convergence is skipped by its mask)
+    # ---- local-solve schedules (FP32 applies, dedicated loop: every timed local-solve launch does
+    # the full work, unlike the GMRES loop where a launch after convergence is skipped by its mask):
+    # 0 = task queue in one launch per depth group (default), 2 = level-synchronous, 1 = one launch
+    # per subdomain
     r = ctx.alloc(n)
     z = ctx.alloc(n)
     r.upload(B[0])
-    mode_ms = {0: (float("nan"), float("nan")), 1: (float("nan"), float("nan"))}
-    for mode in ((0,) if args.no_launch_compare else (0, 1)):
+    nan = (float("nan"), float("nan"))
+    mode_ms = {2: nan, 0: nan, 1: nan}
+    for mode in ((0, 2) if args.no_launch_compare else (0, 2, 1)):
         ctx.set_launch_mode(mode)
         mode_ms[mode] = time_applies(ctx, r, z, 20, warm=3)
     ctx.set_launch_mode(0)
@@ -309,9 +315,11 @@ def main():
         "speedup_fp32_vs_fp64": round((args.steps / t_total) / (args.steps / t64), 3),
         "solution_rel_err_fp32": err32,
         "solution_rel_err_fp64": err64,
-        "local_solve_us": {"batched_one_launch": _us(mode_ms[0][0]),
+        "local_solve_us": {"level_synchronous": _us(mode_ms[2][0]),
+                           "task_queue_per_depth_group": _us(mode_ms[0][0]),
                            "one_launch_per_subdomain": _us(mode_ms[1][0]),
-                           "apply_wall_us_batched": _us(mode_ms[0][1]),
+                           "apply_wall_us_level_synchronous": _us(mode_ms[2][1]),
+                           "apply_wall_us_task_queue": _us(mode_ms[0][1]),
                            "apply_wall_us_per_subdomain": _us(mode_ms[1][1])},
         "roofline": roof,
         "cpu_baseline": cpu,

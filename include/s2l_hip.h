@@ -169,8 +169,11 @@ int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
 int s2l_timing_enable(s2l_ctx* ctx, int on);
 int s2l_timing_read(s2l_ctx* ctx, int64_t* launches, double* ms_total, int reset);
 
-/* Variant selection of the batched local solve (for the comparison with one launch per
- * subdomain): 0 = one persistent launch for all subdomains (default), 1 = one launch per subdomain. */
+/* Schedule of the batched local solve: 0 = task queue, one launch per depth group of all
+ * subdomains' task trees, a subtree per workgroup with its vector in LDS (default); 1 = the same
+ * tasks with one launch per subdomain (the comparison point); 2 = level-synchronous, one launch per
+ * level of all subdomains' supernodal trees with one wavefront per supernode or row block (no LDS
+ * limit on the subdomain size; chosen automatically when a subdomain does not fit mode 0). */
 int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
 
 /* Work split of the batched local solve: the supernodal elimination tree of every subdomain is cut

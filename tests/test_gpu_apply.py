@@ -91,19 +91,51 @@ def test_apply_fp32_as_accurate_as_oracle_float_path(name, kw, N):
 
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_apply_bitwise_repeatable_and_launch_modes_agree(precision):
-    """Unchanged matrix -> bitwise identical apply (test/test_reuse.cpp:309-316); one persistent
-    launch and one launch per subdomain give the same bits."""
+    """Unchanged matrix -> bitwise identical apply (test/test_reuse.cpp:309-316) in every launch
+    mode; the task queue in one launch per depth group and in one launch per subdomain give the
+    same bits; the level-synchronous launches agree with them to rounding."""
     from amp_amd import device
     st = _setup("poisson3d", dict(nx=16), 8, nev=8, overlap=2)
     M = device.HipTwoLevelSchwarz(st, precision)
     r = np.cos(np.linspace(0.0, 7.0, st.n))
-    z0 = M.apply(r)
-    z1 = M.apply(r)
-    assert np.array_equal(z0, z1)
-    M.ctx.set_launch_mode(1)
-    z2 = M.apply(r)
-    M.ctx.set_launch_mode(0)
-    assert np.array_equal(z0, z2)
+    out = {}
+    for mode in (2, 0, 1):
+        M.ctx.set_launch_mode(mode)
+        z0 = M.apply(r)
+        z1 = M.apply(r)
+        assert np.array_equal(z0, z1), mode
+        out[mode] = z0
+    M.ctx.set_launch_mode(2)
+    assert np.array_equal(out[0], out[1])
+    tol = 1e-13 if precision == "double" else 1e-5
+    assert np.linalg.norm(out[2] - out[0]) / np.linalg.norm(out[0]) < tol
+
+
+@pytest.mark.parametrize("mode", [2, 0])
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_wide_supernodes_and_partials(mode, precision):
+    """Subdomains whose separators exceed a wavefront (row blocks, backward partial products)."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=24), 4, nev=8, overlap=2)
+    print(f"case poisson3d 24^3 N=4 mode {mode} {precision}", flush=True)
+    M = device.HipTwoLevelSchwarz(st, precision)
+    M.ctx.set_launch_mode(mode)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    z = M.apply(r)
+    z64 = orc.OracleSchwarz(st, "double").apply(r)
+    err = np.linalg.norm(z - z64) / np.linalg.norm(z64)
+    if precision == "double":
+        assert err < 1e-12
+    else:
+        z32o = orc.OracleSchwarz(st, "single").apply(r)
+        assert err <= 3.0 * np.linalg.norm(z32o - z64) / np.linalg.norm(z64) + 1e-7
+    rng = np.random.default_rng(11)
+    for i in range(4):
+        f = st.factors[i]
+        b = rng.standard_normal(f.n)
+        y = M.ctx.local_solve(i, b)
+        yo = orc.OracleLocalFactor(f, "double").solve(b)
+        assert np.linalg.norm(y - yo) / np.linalg.norm(yo) < (1e-12 if precision == "double" else 1e-5), i
 
 
 def test_gmres_fp32_local_within_target():
