@@ -161,6 +161,11 @@ bool denseLu(int n, std::vector<double>& A, std::vector<int32_t>& p)
 struct s2l_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of the level-synchronous local solve: the large supernodes of a level run on
+    // it beside the small ones (fork and join by events on the caller's stream)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool ls_fork = false;  // measured: the cross-stream fork and join cost as much as the overlap gains
     std::string err;
     size_t max_lds = 0;
 
@@ -189,9 +194,10 @@ struct s2l_ctx {
     // level-synchronous schedule (launch mode 2)
     LsPlan ls;
     DevBuf<LsSn> ls_sn;
+    DevBuf<LsUnit> ls_units;
     DevBuf<LsItem> ls_items;
-    DevBuf<unsigned char> ls_diag, ls_bf, ls_bb, ls_slots, ls_part, ls_yf, ls_xb;
-    DevBuf<int32_t> ls_ridx, ls_cptr, ls_cidx, ls_owned, ls_begin;
+    DevBuf<unsigned char> ls_diag, ls_bf, ls_bb, ls_slots, ls_part, ls_b, ls_yf, ls_xb;
+    DevBuf<int32_t> ls_ridx, ls_cptr, ls_cidx, ls_owned, ls_begin, ls_pull;
     // task-queue schedule (launch modes 0 and 1); needs a subdomain vector in the LDS of a workgroup
     bool queue_ok = false;
     Packed pk;
@@ -259,6 +265,7 @@ struct s2l_ctx {
         xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         ls_slots.reserve(static_cast<size_t>(std::max<int64_t>(ls.slots, 1)) * nrhs * esz);
         ls_part.reserve(static_cast<size_t>(std::max<int64_t>(ls.partials, 1)) * nrhs * esz);
+        ls_b.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         ls_yf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         ls_xb.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         work_nrhs = nrhs;
@@ -357,7 +364,9 @@ template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const doub
 {
     LsArgs<T> a{};
     a.sn = c->ls_sn.p;
+    a.units = c->ls_units.p;
     a.items = c->ls_items.p;
+    a.pull_rows = c->ls_pull.p;
     a.diag = reinterpret_cast<const T*>(c->ls_diag.p);
     a.belowF = reinterpret_cast<const T*>(c->ls_bf.p);
     a.belowB = reinterpret_cast<const T*>(c->ls_bb.p);
@@ -375,6 +384,7 @@ template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const doub
     a.skip = skip;
     a.slots = reinterpret_cast<T*>(c->ls_slots.p);
     a.part = reinterpret_cast<T*>(c->ls_part.p);
+    a.bbuf = reinterpret_cast<T*>(c->ls_b.p);
     a.yf = reinterpret_cast<T*>(c->ls_yf.p);
     a.xb = reinterpret_cast<T*>(c->ls_xb.p);
     a.n_global = c->n;
@@ -388,8 +398,29 @@ template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const doub
 template <typename T> void lsLaunches(s2l_ctx* c, const LsArgs<T>& a, int nrhs, hipStream_t s)
 {
     const LsPlan& P = c->ls;
-    for (size_t q = 0; q + 1 < P.launch.size(); ++q)
-        launchLs<T>(a, P.launch_kind[q], P.launch[q], P.launch[q + 1] - P.launch[q], P.launch_w[q], nrhs, s);
+    const size_t nl = P.launch_kind.size();
+    auto onAux = [](int kind) { return kind == kLsPull || kind == kLsForward || kind == kLsBackPartial || kind == kLsBackward; };
+    for (size_t q = 0; q < nl;) {
+        size_t q1 = q;
+        bool aux = false, main = false;
+        while (q1 < nl && P.launch_level[q1] == P.launch_level[q]) {
+            (onAux(P.launch_kind[q1]) ? aux : main) = true;
+            ++q1;
+        }
+        const bool fork = aux && main && c->ls_fork;
+        if (fork) {
+            HIPCHECK(hipEventRecord(c->ev_fork, s));
+            HIPCHECK(hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+        }
+        for (size_t k = q; k < q1; ++k)
+            launchLs<T>(a, P.launch_kind[k], P.launch_first[k], P.launch_count[k], P.launch_w[k], nrhs,
+                        (fork && onAux(P.launch_kind[k])) ? c->aux : s);
+        if (fork) {
+            HIPCHECK(hipEventRecord(c->ev_join, c->aux));
+            HIPCHECK(hipStreamWaitEvent(s, c->ev_join, 0));
+        }
+        q = q1;
+    }
 }
 
 template <typename T>
@@ -995,6 +1026,9 @@ int s2l_ctx_create(int device, s2l_ctx** out)
         c->device = device;
         try {
             HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            HIPCHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+            HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
             int lds = 0;
             HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
             c->max_lds = static_cast<size_t>(lds);
@@ -1016,6 +1050,10 @@ int s2l_ctx_destroy(s2l_ctx* ctx)
         (void)hipEventDestroy(pr.second);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return S2L_OK;
@@ -1177,7 +1215,9 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         {
             LsPlan& L = c->ls;
             c->ls_sn.upload(L.sn, c->stream);
+            c->ls_units.upload(L.units, c->stream);
             c->ls_items.upload(L.items, c->stream);
+            c->ls_pull.upload(L.pull_rows, c->stream);
             c->ls_ridx.upload(L.ridx, c->stream);
             c->ls_cptr.upload(L.cptr, c->stream);
             c->ls_cidx.upload(L.cidx, c->stream);

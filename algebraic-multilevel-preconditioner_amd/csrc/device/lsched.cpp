@@ -50,25 +50,24 @@ Sweep structure(int32_t n, const std::vector<int64_t>& cp, const std::vector<int
 
 } // namespace
 
+namespace {
+bool isLarge(int32_t w, int32_t h) { return w > kLsRows || static_cast<int64_t>(w) * h > kLsLargeWork; }
+} // namespace
+
 LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off)
 {
     LsPlan P;
     const std::size_t nsub = fs.size();
-    // items per level, kept apart until all subdomains are packed
-    std::vector<std::vector<LsItem>> fwd, bpart, bwd;
-    std::vector<int32_t> fwdW, bpartW, bwdW; // widest supernode of every level
-    // the item list of level lev, recording a supernode of width w in it
-    auto at = [](std::vector<std::vector<LsItem>>& v, std::vector<int32_t>& maxw, int32_t lev, int32_t w) -> std::vector<LsItem>& {
-        if (static_cast<int32_t>(v.size()) <= lev) {
-            v.resize(static_cast<std::size_t>(lev) + 1);
-            maxw.resize(static_cast<std::size_t>(lev) + 1, 0);
-        }
-        maxw[static_cast<std::size_t>(lev)] = std::max(maxw[static_cast<std::size_t>(lev)], w);
+    // supernodes of every level, all subdomains, kept apart until all are packed
+    std::vector<std::vector<int32_t>> fwd, bwd;
+    auto at = [](std::vector<std::vector<int32_t>>& v, int32_t lev) -> std::vector<int32_t>& {
+        if (static_cast<int32_t>(v.size()) <= lev) v.resize(static_cast<std::size_t>(lev) + 1);
         return v[static_cast<std::size_t>(lev)];
     };
-    // pushes (for the slot tables): per subdomain, every (row, push index) in push order
+    // pushes (for the slot tables): every (row, push index) in push order
     std::vector<std::pair<int32_t, int64_t>> pushes;
     int64_t toff = 0;
+    std::vector<double> blk;
     for (std::size_t i = 0; i < nsub; ++i) {
         const HostFactor& F = fs[i];
         const int32_t n = F.n;
@@ -89,7 +88,6 @@ LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 height[static_cast<std::size_t>(p)] = std::max(height[static_cast<std::size_t>(p)], height[static_cast<std::size_t>(s)] + 1);
             }
         }
-        std::vector<double> blk;
         for (int32_t s = 0; s < nsf; ++s) {
             const int32_t c0 = S.st[static_cast<std::size_t>(s)], w = S.st[static_cast<std::size_t>(s) + 1] - c0;
             const int64_t p0 = F.lcp[static_cast<std::size_t>(c0)];
@@ -122,16 +120,16 @@ LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             for (int32_t j = 0; j < d.nrb; ++j)
                 for (int32_t k = 0; k < h; ++k) pushes.emplace_back(row0 + P.ridx[static_cast<std::size_t>(d.ridx + k)], toff + static_cast<int64_t>(j) * h + k);
             toff += static_cast<int64_t>(h) * d.nrb;
-            const int32_t id = static_cast<int32_t>(P.sn.size());
+            at(fwd, height[static_cast<std::size_t>(s)]).push_back(static_cast<int32_t>(P.sn.size()));
             P.sn.push_back(d);
-            auto& L = at(fwd, fwdW, height[static_cast<std::size_t>(s)], w);
-            for (int32_t j = 0; j < d.nrb; ++j) L.push_back({id, j});
             P.entries += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
-            P.bytes_index += static_cast<int64_t>(h) * 4 * (1 + d.nrb) + static_cast<int64_t>(sizeof(LsSn)) * d.nrb;
+            // descriptor reads, row indices and slot indices of the pushes, the pulled rows' slot pointers
+            P.bytes_index += static_cast<int64_t>(sizeof(LsSn)) * d.nrb + static_cast<int64_t>(h) * 4 * d.nrb +
+                             static_cast<int64_t>(w) * 8;
         }
 
         // ---------------- backward sweep: U (given by U^T), levels by depth (a supernode after its
-        // ancestors); a wide one gets a launch of partial dot products with its ancestors' values
+        // ancestors); a large one gets partial dot products with its ancestors' values first
         const Sweep B = chol ? S : structure(n, ucp, ur);
         const int32_t nsb = static_cast<int32_t>(B.st.size()) - 1;
         std::vector<int32_t> depth(static_cast<std::size_t>(nsb), 0);
@@ -149,9 +147,8 @@ LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             d.c0 = c0;
             d.w = w;
             d.h = h;
-            const bool wd = w > kLsWide;
-            d.nrb = wd ? ceilDiv(w, kLsRows) : 1;
-            d.nkc = (wd && h > 0) ? ceilDiv(h, kLsKChunk) : 0;
+            d.nrb = ceilDiv(w, kLsRows);
+            d.nkc = (isLarge(w, h) && h > 0) ? ceilDiv(h, kLsKChunk) : 0;
             d.toff = 0;
             d.poff = P.partials;
             P.partials += static_cast<int64_t>(d.nkc) * w;
@@ -170,16 +167,8 @@ LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                     P.belowB.push_back(uv[static_cast<std::size_t>(ucp[static_cast<std::size_t>(c0 + c)] + (w - c) + k)]);
             d.ridx = static_cast<int64_t>(P.ridx.size());
             for (int32_t k = 0; k < h; ++k) P.ridx.push_back(ur[static_cast<std::size_t>(p0 + w + k)]);
-            const int32_t id = static_cast<int32_t>(P.sn.size());
+            at(bwd, depth[static_cast<std::size_t>(s)]).push_back(static_cast<int32_t>(P.sn.size()));
             P.sn.push_back(d);
-            const int32_t lev = depth[static_cast<std::size_t>(s)];
-            auto& G = at(bwd, bwdW, lev, w);
-            for (int32_t j = 0; j < d.nrb; ++j) G.push_back({id, j});
-            if (d.nkc > 0) {
-                auto& Q = at(bpart, bpartW, lev, w);
-                for (int32_t j = 0; j < d.nrb; ++j)
-                    for (int32_t kc = 0; kc < d.nkc; ++kc) Q.push_back({id, j * d.nkc + kc});
-            }
             P.entries += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
             P.bytes_index += static_cast<int64_t>(h) * 4 + static_cast<int64_t>(sizeof(LsSn)) * d.nrb;
             P.bwd_below_rows += h;
@@ -197,26 +186,77 @@ LsPlan buildLsPlan(const std::vector<HostFactor>& fs, const std::vector<int32_t>
     P.cidx.assign(static_cast<std::size_t>(toff), 0);
     std::vector<int32_t> next(P.cptr.begin(), P.cptr.end() - 1);
     for (const auto& pr : pushes) P.cidx[static_cast<std::size_t>(pr.second)] = next[static_cast<std::size_t>(pr.first)]++;
-    P.bytes_index += (static_cast<int64_t>(nrows) + 1) * 4;
 
-    // launches: forward levels by height, then for every backward depth its partial dots and the level;
-    // in every launch the wide items come first (the longest work starts first)
-    auto emit = [&](std::vector<LsItem>& items, int32_t kind, int32_t maxw) {
-        if (items.empty()) return;
-        std::stable_sort(items.begin(), items.end(), [&](const LsItem& a, const LsItem& b) {
-            return P.sn[static_cast<std::size_t>(a.sn)].nrb > P.sn[static_cast<std::size_t>(b.sn)].nrb;
-        });
-        P.launch.push_back(static_cast<int32_t>(P.items.size()));
+    // launches
+    auto launch = [&](int32_t kind, std::size_t first, std::size_t count, int32_t maxw) {
+        if (count == 0) return;
         P.launch_kind.push_back(kind);
+        P.launch_first.push_back(static_cast<int32_t>(first));
+        P.launch_count.push_back(static_cast<int32_t>(count));
         P.launch_w.push_back(maxw);
-        P.items.insert(P.items.end(), items.begin(), items.end());
     };
-    for (std::size_t l = 0; l < fwd.size(); ++l) emit(fwd[l], kLsForward, fwdW[l]);
-    for (std::size_t l = 0; l < bwd.size(); ++l) {
-        if (l < bpart.size()) emit(bpart[l], kLsBackPartial, bpartW[l]);
-        emit(bwd[l], kLsBackward, bwdW[l]);
+    // units of a level: a launch of the row blocks of its large supernodes, then a launch of its
+    // small ones
+    auto units = [&](const std::vector<int32_t>& list, bool forward) {
+        std::size_t first = P.units.size();
+        int32_t maxw = 0;
+        std::vector<int32_t> small;
+        for (int32_t id : list) {
+            const LsSn& d = P.sn[static_cast<std::size_t>(id)];
+            const bool large = forward ? isLarge(d.w, d.h) : (d.nkc > 0 || d.w > kLsRows);
+            if (!large) {
+                small.push_back(id);
+                continue;
+            }
+            maxw = std::max(maxw, d.w);
+            for (int32_t j = 0; j < d.nrb; ++j) {
+                LsUnit u{};
+                u.large = 1;
+                u.part = j;
+                u.sn[0] = id;
+                for (int q = 1; q < kLsWaves; ++q) u.sn[q] = -1;
+                P.units.push_back(u);
+            }
+        }
+        launch(forward ? kLsForward : kLsBackward, first, P.units.size() - first, maxw);
+        first = P.units.size();
+        for (int32_t id : small) {
+            LsUnit u{};
+            u.large = 0;
+            u.part = 0;
+            u.sn[0] = id;
+            for (int q = 1; q < kLsWaves; ++q) u.sn[q] = -1;
+            P.units.push_back(u);
+        }
+        launch(forward ? kLsForwardSmall : kLsBackwardSmall, first, P.units.size() - first, kLsRows);
+    };
+    for (const auto& list : fwd) {
+        // the pull launch covers the rows of the large supernodes (the small ones pull their own)
+        const std::size_t first = P.pull_rows.size();
+        for (int32_t id : list) {
+            const LsSn& d = P.sn[static_cast<std::size_t>(id)];
+            if (!isLarge(d.w, d.h)) continue;
+            for (int32_t r = 0; r < d.w; ++r) P.pull_rows.push_back(d.row0 + d.c0 + r);
+        }
+        launch(kLsPull, first, P.pull_rows.size() - first, 0);
+        units(list, true);
+        P.launch_level.resize(P.launch_kind.size(), static_cast<int32_t>(&list - fwd.data()));
     }
-    P.launch.push_back(static_cast<int32_t>(P.items.size()));
+    for (const auto& list : bwd) {
+        const std::size_t first = P.items.size();
+        int32_t maxw = 0;
+        for (int32_t id : list) {
+            const LsSn& d = P.sn[static_cast<std::size_t>(id)];
+            for (int32_t j = 0; j < d.nrb && d.nkc > 0; ++j)
+                for (int32_t kc = 0; kc < d.nkc; ++kc) P.items.push_back({id, j * d.nkc + kc});
+            if (d.nkc > 0) maxw = std::max(maxw, d.w);
+        }
+        launch(kLsBackPartial, first, P.items.size() - first, maxw);
+        units(list, false);
+        P.launch_level.resize(P.launch_kind.size(), static_cast<int32_t>(fwd.size() + (&list - bwd.data())));
+    }
+    P.bytes_index += static_cast<int64_t>(P.pull_rows.size()) * 4 + static_cast<int64_t>(P.units.size()) * sizeof(LsUnit) +
+                     static_cast<int64_t>(P.items.size()) * sizeof(LsItem);
     P.levels_fwd = static_cast<int32_t>(fwd.size());
     P.levels_bwd = static_cast<int32_t>(bwd.size());
     return P;

@@ -149,6 +149,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tasks", type=int, default=1, help="tasks per subdomain of the batched local solve")
+    ap.add_argument("--local-mode", type=int, default=None, choices=(0, 1, 2),
+                    help="schedule of the local solves (s2l_set_local_launch_mode; default: the library's)")
     ap.add_argument("--profile-applies", type=int, default=0,
                     help="only run this many FP32 applies after setup (for rocprofv3 runs)")
     ap.add_argument("--no-launch-compare", action="store_true",
@@ -186,6 +188,8 @@ def main():
 
     M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks)
     ctx = M32.ctx
+    if args.local_mode is not None:
+        ctx.set_launch_mode(args.local_mode)
     stats = ctx.stats()
     log(f"factor entries fwd {stats.factor_entries_fwd} bwd {stats.factor_entries_bwd}, levels "
         f"{stats.levels_fwd_max}/{stats.levels_bwd_max}, bytes/local solve {stats.bytes_local_solve / 1e6:.1f} MB")
