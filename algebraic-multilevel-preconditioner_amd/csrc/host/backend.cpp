@@ -1,0 +1,204 @@
+// backend.cpp -- C++17 host classes over the C-ABI (include/s2l_backend.hpp).
+#include "s2l_backend.hpp"
+
+#include <stdexcept>
+
+namespace s2l {
+
+namespace {
+
+void check(s2l_ctx* ctx, int rc)
+{
+    if (rc == S2L_OK) return;
+    const std::string msg = s2l_last_error(ctx);
+    if (rc == S2L_EINVAL) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+} // namespace
+
+namespace prec {
+
+const char* toString(UpdateLevel l)
+{
+    switch (l) {
+    case UpdateLevel::Reuse: return "reuse";
+    case UpdateLevel::Refactorize: return "refactorize";
+    case UpdateLevel::Rebuild: return "rebuild";
+    }
+    return "?";
+}
+
+HipTwoLevelSchwarz::HipTwoLevelSchwarz(const SetupData& s, int device)
+{
+    if (s.one_level != "ras" && s.one_level != "asm") throw std::invalid_argument("one-level variant must be ras or asm");
+    if (s.correction != "deflated" && s.correction != "additive")
+        throw std::invalid_argument("coarse correction must be deflated or additive");
+    if (s.precision != "double" && s.precision != "single") throw std::invalid_argument("precision must be double or single");
+    check(nullptr, s2l_ctx_create(device, &ctx_));
+    try {
+        upload(s);
+    } catch (...) {
+        s2l_ctx_destroy(ctx_);
+        ctx_ = nullptr;
+        throw;
+    }
+    name_ = std::string(s.one_level == "ras" ? "RAS" : "ASM") + (s.correction == "deflated" ? ",deflated" : ",additive");
+}
+
+HipTwoLevelSchwarz::~HipTwoLevelSchwarz()
+{
+    if (ctx_) s2l_ctx_destroy(ctx_);
+}
+
+void HipTwoLevelSchwarz::upload(const SetupData& s)
+{
+    const int nsub = s.nsub();
+    if (nsub < 1 || static_cast<int>(s.ownership.size()) != nsub + 1 || static_cast<int>(s.factors.size()) != nsub)
+        throw std::invalid_argument("setup: ownership, local_ptr and factors must describe the same subdomains");
+    if (static_cast<int32_t>(s.rowptr.size()) != s.n + 1) throw std::invalid_argument("setup: rowptr must have n + 1 entries");
+    check(ctx_, s2l_upload_system(ctx_, s.n, s.rowptr.data(), s.col.data(), s.val.data(), nsub, s.ownership.data()));
+    check(ctx_, s2l_upload_subdomains(ctx_, nsub, s.local_ptr.data(), s.gid.data(), s.pou.empty() ? nullptr : s.pou.data(),
+                                      s.one_level == "asm" ? 1 : 0));
+    for (int i = 0; i < nsub; ++i) {
+        const SetupData::Factor& f = s.factors[static_cast<size_t>(i)];
+        const bool lu = !f.ut_colptr.empty();
+        check(ctx_, s2l_upload_local_factor(ctx_, i, f.n, f.perm.data(), f.l_colptr.data(), f.l_row.data(), f.l_val.data(),
+                                            lu ? f.ut_colptr.data() : nullptr, lu ? f.ut_row.data() : nullptr,
+                                            lu ? f.ut_val.data() : nullptr));
+    }
+    check(ctx_, s2l_finalize_local(ctx_, s.precision == "single" ? 1 : 0));
+    check(ctx_, s2l_upload_coarse(ctx_, s.n0, s.n0 > 0 ? s.coarse_offsets.data() : nullptr, s.n0 > 0 ? s.V.data() : nullptr,
+                                  s.n0 > 0 ? s.E.data() : nullptr, s.symmetric ? 1 : 0, s.correction == "additive" ? 1 : 0));
+    n_ = s.n;
+}
+
+void HipTwoLevelSchwarz::apply(const core::Vec& r, core::Vec& z) const
+{
+    if (static_cast<int32_t>(r.size()) != n_) throw std::invalid_argument("apply: r must have n entries");
+    z.resize(static_cast<size_t>(n_));
+    check(ctx_, s2l_apply_host(ctx_, r.data(), z.data(), 1));
+}
+
+void HipTwoLevelSchwarz::applyDevice(const double* d_r, double* d_z, int nrhs, void* stream) const
+{
+    check(ctx_, s2l_apply(ctx_, d_r, d_z, nrhs, stream));
+}
+
+void HipTwoLevelSchwarz::spmv(const core::Vec& x, core::Vec& y) const
+{
+    if (static_cast<int32_t>(x.size()) != n_) throw std::invalid_argument("spmv: x must have n entries");
+    y.resize(static_cast<size_t>(n_));
+    check(ctx_, s2l_spmv_host(ctx_, x.data(), y.data(), 1));
+}
+
+std::string HipTwoLevelSchwarz::name() const { return name_; }
+
+UpdateLevel HipTwoLevelSchwarz::refresh(const SetupData& s)
+{
+    if (s.n != n_) throw std::invalid_argument("refresh: the system size changed (a new setup is a Rebuild)");
+    upload(s);
+    up_to_date_ = true;
+    return UpdateLevel::Refactorize;
+}
+
+UpdateLevel HipTwoLevelSchwarz::update(UpdateLevel requested)
+{
+    if (requested == UpdateLevel::Rebuild)
+        throw std::logic_error("Rebuild needs a new host setup: construct a new HipTwoLevelSchwarz");
+    if (up_to_date_) return UpdateLevel::Reuse;
+    if (requested == UpdateLevel::Reuse) return UpdateLevel::Reuse;
+    throw std::logic_error("the preconditioner is stale: refresh() it with the refactorized values");
+}
+
+} // namespace prec
+
+namespace solv {
+
+StopNorm parseStopNorm(const std::string& s)
+{
+    if (s == "l2") return StopNorm::L2;
+    if (s == "openfoam") return StopNorm::OpenFOAM;
+    throw std::invalid_argument("unknown stopping norm '" + s + "' (l2|openfoam)");
+}
+
+const char* toString(StopNorm s) { return s == StopNorm::L2 ? "l2" : "openfoam"; }
+
+namespace {
+
+const prec::HipTwoLevelSchwarz& deviceSide(const HipOperator& A, const prec::Preconditioner& M)
+{
+    const auto* hm = dynamic_cast<const prec::HipTwoLevelSchwarz*>(&M);
+    if (!hm) throw std::invalid_argument("the device Krylov solvers need a HipTwoLevelSchwarz preconditioner");
+    if (&A.owner() != hm) throw std::invalid_argument("the operator and the preconditioner must share their context");
+    return *hm;
+}
+
+KrylovResult run(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x, int method,
+                 int restart, double rtol, double atol, int maxit, StopNorm stop)
+{
+    const prec::HipTwoLevelSchwarz& hm = deviceSide(A, M);
+    const size_t n = static_cast<size_t>(hm.n());
+    if (b.size() != n) throw std::invalid_argument("solve: b must have n entries");
+    if (x.size() != n) x.assign(n, 0.0);
+    s2l_krylov_opts o{};
+    o.method = method;
+    o.restart = restart;
+    o.rtol = rtol;
+    o.atol = atol;
+    o.maxit = maxit;
+    o.stop_norm = stop == StopNorm::OpenFOAM ? 1 : 0;
+    s2l_krylov_result r{};
+    std::vector<double> hist(static_cast<size_t>(maxit) + 1);
+    check(hm.context(), s2l_krylov_solve_host(hm.context(), &o, b.data(), x.data(), 1, &r, hist.data()));
+    KrylovResult out;
+    out.converged = r.converged != 0;
+    out.breakdown = r.breakdown != 0;
+    out.iterations = r.iterations;
+    out.restarts = r.restarts;
+    out.rel_residual = r.rel_residual;
+    out.true_rel_residual = r.true_rel_residual;
+    out.initial_residual_of = r.initial_residual_of;
+    out.final_residual_of = r.final_residual_of;
+    out.history.assign(hist.begin(), hist.begin() + r.history_len);
+    return out;
+}
+
+} // namespace
+
+HipGmres::HipGmres(int restart, double rtol, int maxit, StopNorm stop, double atol)
+    : m_(restart), maxit_(maxit), rtol_(rtol), atol_(atol), stop_(stop)
+{
+    if (restart < 1) throw std::invalid_argument("GMRES restart must be >= 1");
+    if (stop != StopNorm::L2) throw std::invalid_argument("the device GMRES implements the l2 stopping norm");
+}
+
+KrylovResult HipGmres::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x) const
+{
+    return run(A, M, b, x, 0, m_, rtol_, atol_, maxit_, stop_);
+}
+
+HipBiCgStab::HipBiCgStab(double rtol, int maxit, StopNorm stop, double atol)
+    : maxit_(maxit), rtol_(rtol), atol_(atol), stop_(stop)
+{
+}
+
+KrylovResult HipBiCgStab::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+                                core::Vec& x) const
+{
+    return run(A, M, b, x, 1, 0, rtol_, atol_, maxit_, stop_);
+}
+
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const std::string& krylov, int restart, double rtol, double atol, int maxit,
+                                               const std::string& stop_norm)
+{
+    const StopNorm stop = parseStopNorm(stop_norm);
+    if (atol > 0.0 && stop != StopNorm::OpenFOAM) throw std::invalid_argument("an absolute tolerance needs the OpenFOAM stopping norm");
+    if (krylov == "gmres") return std::make_unique<HipGmres>(restart, rtol, maxit, stop, atol);
+    if (krylov == "bicgstab") return std::make_unique<HipBiCgStab>(rtol, maxit, stop, atol);
+    throw std::invalid_argument("unknown Krylov solver '" + krylov + "' (gmres|bicgstab)");
+}
+
+} // namespace solv
+
+} // namespace s2l

@@ -108,6 +108,51 @@ def _p(a, t):
     return a.ctypes.data_as(P(t))
 
 
+def setup_arrays(setup, precision: str = "double", one_level: str | None = None,
+                 correction: str | None = None) -> dict:
+    """The arrays of the C-ABI upload (include/s2l_hip.h) from a hostsetup.SchwarzSetup: the
+    operator (CSR, renumbered order), ownership, Omega_i maps, D_i (absent for the Boolean
+    partition of unity), factors f{i}_* (U^T only for LU), coarse offsets, V blocks and A_00."""
+    dec = setup.dec
+    A = dec.A.tocsr()
+    d = {"n": np.int64(A.shape[0]),
+         "rowptr": np.ascontiguousarray(A.indptr, np.int32),
+         "col": np.ascontiguousarray(A.indices, np.int32),
+         "val": np.ascontiguousarray(A.data, np.float64),
+         "ownership": np.ascontiguousarray(dec.ownership, np.int32)}
+    gids = [sd.global_ids() for sd in setup.subs]
+    lptr = np.zeros(len(gids) + 1, np.int32)
+    np.cumsum([len(g) for g in gids], out=lptr[1:])
+    d["local_ptr"] = lptr
+    d["gid"] = np.ascontiguousarray(np.concatenate(gids), np.int32)
+    if not setup.subs[0].boolean:
+        d["pou"] = np.ascontiguousarray(np.concatenate([sd.D for sd in setup.subs]), np.float64)
+    for i, f in enumerate(setup.factors):
+        d[f"f{i}_perm"] = np.ascontiguousarray(f.perm, np.int32)
+        d[f"f{i}_lcp"] = np.ascontiguousarray(f.l_colptr, np.int64)
+        d[f"f{i}_lrow"] = np.ascontiguousarray(f.l_row, np.int32)
+        d[f"f{i}_lval"] = np.ascontiguousarray(f.l_val, np.float64)
+        if f.kind != 0:
+            d[f"f{i}_ucp"] = np.ascontiguousarray(f.ut_colptr, np.int64)
+            d[f"f{i}_urow"] = np.ascontiguousarray(f.ut_row, np.int32)
+            d[f"f{i}_uval"] = np.ascontiguousarray(f.ut_val, np.float64)
+    cs = setup.coarse
+    d["n0"] = np.int64(cs.n0)
+    d["coarse_offsets"] = np.ascontiguousarray(cs.offsets, np.int64)
+    if cs.n0 > 0:
+        blocks = []
+        for sd, V in zip(setup.subs, cs.V):
+            rows = V[:sd.n_owned] if sd.boolean else V
+            blocks.append(np.asfortranarray(rows).reshape(-1, order="F"))
+        d["V"] = np.ascontiguousarray(np.concatenate(blocks), np.float64)
+        d["E"] = np.ascontiguousarray(cs.E, np.float64)
+    d["symmetric"] = np.int64(1 if dec.symmetric else 0)
+    d["one_level"] = one_level or setup.one_level
+    d["correction"] = correction or setup.correction
+    d["precision"] = precision
+    return d
+
+
 class DeviceArray:
     """A device buffer of float64 owned by a context (for inputs resident in HBM)."""
 
@@ -170,31 +215,20 @@ class HipContext:
         """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products)."""
         if tasks_per_subdomain is not None:
             self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
-        dec = setup.dec
-        A = dec.A.tocsr()
-        n = A.shape[0]
-        rp = np.ascontiguousarray(A.indptr, np.int32)
-        col = np.ascontiguousarray(A.indices, np.int32)
-        val = np.ascontiguousarray(A.data, np.float64)
-        own = np.ascontiguousarray(dec.ownership, np.int32)
+        d = setup_arrays(setup, precision, one_level, correction)
+        rp, col, val, own = d["rowptr"], d["col"], d["val"], d["ownership"]
         N = len(own) - 1
-        self._check(self.L.s2l_upload_system(self.h, n, _p(rp, ctypes.c_int32), _p(col, ctypes.c_int32),
+        self._check(self.L.s2l_upload_system(self.h, int(d["n"]), _p(rp, ctypes.c_int32), _p(col, ctypes.c_int32),
                                              _p(val, ctypes.c_double), N, _p(own, ctypes.c_int32)))
-        gids = [sd.global_ids() for sd in setup.subs]
-        lptr = np.zeros(N + 1, np.int32)
-        np.cumsum([len(g) for g in gids], out=lptr[1:])
-        gid = np.ascontiguousarray(np.concatenate(gids), np.int32)
-        boolean = setup.subs[0].boolean
-        pou = None if boolean else np.ascontiguousarray(np.concatenate([sd.D for sd in setup.subs]), np.float64)
-        ol = one_level or setup.one_level
+        lptr, gid, pou = d["local_ptr"], d["gid"], d.get("pou")
         self._check(self.L.s2l_upload_subdomains(self.h, N, _p(lptr, ctypes.c_int32), _p(gid, ctypes.c_int32),
                                                  None if pou is None else _p(pou, ctypes.c_double),
-                                                 0 if ol == "ras" else 1))
+                                                 0 if d["one_level"] == "ras" else 1))
         for i, f in enumerate(setup.factors):
             self.upload_factor(i, f)
         self.finalize(precision)
-        self.upload_coarse(setup, correction or setup.correction)
-        self._keep = (rp, col, val, own, lptr, gid, pou)
+        self.upload_coarse(setup, d["correction"])
+        self._keep = d
 
     def upload_factor(self, i, f):
         perm = np.ascontiguousarray(f.perm, np.int32)

@@ -1,0 +1,196 @@
+// s2l_backend.hpp -- C++17 host classes of the MI355X apply path behind the reference's
+// preconditioner and Krylov interfaces, over the C-ABI of s2l_hip.h.
+//
+// They mirror, with the same names, argument meaning and error behaviour:
+//   prec::Preconditioner   include/preconditioner/preconditioner.hpp:64-96 (UpdateLevel :30)
+//   prec::TwoLevelSchwarz  src/preconditioner/two_level_schwarz.cpp:129-163 (here HipTwoLevelSchwarz)
+//   solv::KrylovResult     include/solver/krylov.hpp:43-54
+//   solv::KrylovSolver     include/solver/krylov.hpp:60-76, Gmres / BiCgStab (gmres.hpp, bicgstab.hpp)
+//   solv::makeKrylovSolver src/solver/krylov.cpp:23-36
+// Vectors are std::vector<double> with the owned entries of every subdomain in the renumbered
+// global order (all subdomains live in one process and on one GPU). Errors of the C-ABI become
+// std::runtime_error with the library's message; invalid arguments std::invalid_argument.
+#ifndef S2L_BACKEND_HPP
+#define S2L_BACKEND_HPP
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "s2l_hip.h"
+
+namespace s2l {
+
+namespace core {
+using Vec = std::vector<double>;
+}
+
+/// What the host setup produced, in the form the device layout is built from (the reference's
+/// setup objects: LocalSystem rows, Subdomain index maps and partition of unity, LocalSolver
+/// factors, CoarseSpace basis and A_00). Arrays are concatenated over the subdomains.
+struct SetupData {
+    // operator A in the renumbered order, every row in its owner's local column order
+    int32_t n = 0;
+    std::vector<int32_t> rowptr, col;
+    std::vector<double> val;
+    std::vector<int32_t> ownership;     ///< nsub + 1 offsets (subdomain p owns [ownership[p], ownership[p+1]))
+    // overlapping subdomains Omega_i = [owned | ghosts sorted]
+    std::vector<int32_t> local_ptr;     ///< nsub + 1 offsets into gid
+    std::vector<int32_t> gid;           ///< global ids of the local unknowns
+    std::vector<double> pou;            ///< D_i concatenated; empty = Boolean partition of unity
+    // local factors A_ii(perm, perm) = L U (U^T empty = Cholesky, U = L^T)
+    struct Factor {
+        int32_t n = 0;
+        std::vector<int32_t> perm;
+        std::vector<int64_t> l_colptr, ut_colptr;
+        std::vector<int32_t> l_row, ut_row;
+        std::vector<double> l_val, ut_val;
+    };
+    std::vector<Factor> factors;
+    // coarse space: V_i blocks (rows x m_i column-major; rows = owned rows for the Boolean PoU, all
+    // local rows otherwise) and A_00 (n0 x n0 row-major)
+    int32_t n0 = 0;
+    std::vector<int64_t> coarse_offsets; ///< nsub + 1
+    std::vector<double> V, E;
+    bool symmetric = true;
+    std::string one_level = "ras";       ///< "ras" | "asm"
+    std::string correction = "deflated"; ///< "deflated" | "additive"
+    std::string precision = "double";    ///< local factors: "double" | "single"
+    int nsub() const { return static_cast<int>(local_ptr.size()) - 1; }
+};
+
+namespace prec {
+
+/// Update levels of a preconditioner for new values of A with the same pattern.
+enum class UpdateLevel : int { Reuse = 0, Refactorize = 1, Rebuild = 2 };
+const char* toString(UpdateLevel l);
+
+/// Application of a (right) preconditioner z = M^{-1} r (owned entries).
+class Preconditioner {
+public:
+    virtual ~Preconditioner() = default;
+    virtual void apply(const core::Vec& r, core::Vec& z) const = 0;
+    virtual std::string name() const = 0;
+    virtual UpdateLevel update(UpdateLevel /*requested*/) { return UpdateLevel::Reuse; }
+    virtual bool upToDate() const { return true; }
+    virtual long localFallbacks() const { return 0; }
+    virtual long coarseFallbacks() const { return 0; }
+};
+
+/// The two-level Schwarz preconditioner with every subdomain resident on one GPU.
+class HipTwoLevelSchwarz final : public Preconditioner {
+public:
+    /// Creates a context on @p device and uploads the setup (copies it; @p s may be freed).
+    explicit HipTwoLevelSchwarz(const SetupData& s, int device = 0);
+    ~HipTwoLevelSchwarz() override;
+    HipTwoLevelSchwarz(const HipTwoLevelSchwarz&) = delete;
+    HipTwoLevelSchwarz& operator=(const HipTwoLevelSchwarz&) = delete;
+
+    /// z = M^{-1} r with host vectors of n owned entries (z is resized).
+    void apply(const core::Vec& r, core::Vec& z) const override;
+    /// z = M^{-1} r for nrhs device vectors (n x nrhs) on @p stream (a hipStream_t or null).
+    void applyDevice(const double* d_r, double* d_z, int nrhs = 1, void* stream = nullptr) const;
+    /// "RAS,deflated" and the like (two_level_schwarz.cpp:129-132).
+    std::string name() const override;
+    /// The preconditioner was built for other values of A (Subdomain::updateValues in the reference).
+    void markStale() { up_to_date_ = false; }
+    /// Refactorize on the device side: uploads the new values of A, of the local factors and of
+    /// A_00 computed by the host (same structure, same coarse basis). Returns Refactorize.
+    UpdateLevel refresh(const SetupData& s);
+    /// Reuse when up to date; a stale preconditioner needs refresh() (Refactorize) or a new object
+    /// (Rebuild: the host setup is not part of this library): std::logic_error.
+    UpdateLevel update(UpdateLevel requested) override;
+    bool upToDate() const override { return up_to_date_; }
+
+    int32_t n() const { return n_; }
+    s2l_ctx* context() const { return ctx_; }
+    /// y = A x with host vectors (the operator of the uploaded system).
+    void spmv(const core::Vec& x, core::Vec& y) const;
+
+private:
+    void upload(const SetupData& s);
+    s2l_ctx* ctx_ = nullptr;
+    int32_t n_ = 0;
+    std::string name_;
+    bool up_to_date_ = true;
+};
+
+} // namespace prec
+
+namespace solv {
+
+enum class StopNorm { L2, OpenFOAM };
+StopNorm parseStopNorm(const std::string& s);
+const char* toString(StopNorm s);
+
+/// Outcome of a Krylov solve (include/solver/krylov.hpp:43-54).
+struct KrylovResult {
+    bool converged = false;
+    bool breakdown = false;
+    int iterations = 0;
+    int restarts = 0;
+    double rel_residual = 0.0;
+    double true_rel_residual = 0.0;
+    double initial_residual_of = 0.0;
+    double final_residual_of = 0.0;
+    std::vector<double> history;
+};
+
+/// The operator A of a HipTwoLevelSchwarz (the system uploaded with its setup).
+class HipOperator {
+public:
+    explicit HipOperator(const prec::HipTwoLevelSchwarz& M) : M_(M) {}
+    void apply(const core::Vec& x, core::Vec& y) const { M_.spmv(x, y); }
+    const prec::HipTwoLevelSchwarz& owner() const { return M_; }
+
+private:
+    const prec::HipTwoLevelSchwarz& M_;
+};
+
+/// Solves A x = b with a preconditioner M; x is the initial guess in and the solution out.
+class KrylovSolver {
+public:
+    virtual ~KrylovSolver() = default;
+    virtual KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+                               core::Vec& x) const = 0;
+    virtual std::string name() const = 0;
+};
+
+/// Right-preconditioned restarted GMRES(m) with the whole loop on the device (gmres.cpp:18-161).
+class HipGmres final : public KrylovSolver {
+public:
+    HipGmres(int restart, double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0);
+    KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+                       core::Vec& x) const override;
+    std::string name() const override { return "GMRES(" + std::to_string(m_) + ")"; }
+
+private:
+    int m_, maxit_;
+    double rtol_, atol_;
+    StopNorm stop_;
+};
+
+/// Right-preconditioned BiCGStab, OpenFOAM's PBiCGStab variant (bicgstab.cpp:15-154).
+class HipBiCgStab final : public KrylovSolver {
+public:
+    HipBiCgStab(double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0);
+    KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+                       core::Vec& x) const override;
+    std::string name() const override { return "BiCGStab"; }
+
+private:
+    int maxit_;
+    double rtol_, atol_;
+    StopNorm stop_;
+};
+
+/// --krylov gmres (with restart) | bicgstab; an absolute tolerance needs the OpenFOAM norm.
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const std::string& krylov, int restart, double rtol, double atol,
+                                               int maxit, const std::string& stop_norm = "l2");
+
+} // namespace solv
+
+} // namespace s2l
+
+#endif

@@ -32,6 +32,26 @@ def test_library_exports_every_declared_symbol(header, lib):
     assert not missing, missing
 
 
+def test_cpp_backend_links_and_runs_without_a_gpu_call():
+    """libs2l_backend.so (the C++ host classes) loads; the C++ test program prints its usage."""
+    ctypes.CDLL(os.path.join(PKG, "libs2l_backend.so"))
+    exe = os.path.join(REPO, "tests", "cpp", "test_backend")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 2 and "usage" in p.stderr
+
+
+def test_setup_file_round_trip(tmp_path):
+    """The setup-file writer produces the layout the C++ reader expects (header, records)."""
+    import struct
+    from setup_io import write_arrays
+    path = tmp_path / "s.bin"
+    write_arrays(str(path), {"a": np.arange(3, dtype=np.int32), "b": np.float64(2.5), "t": "ras"})
+    raw = path.read_bytes()
+    assert raw[:8] == b"S2LSETUP" and struct.unpack("<I", raw[8:12])[0] == 3
+    assert raw[12:16] == struct.pack("<I", 1) and raw[16:17] == b"a" and raw[17] == 0
+    assert struct.unpack("<Q", raw[18:26])[0] == 3
+
+
 def test_python_binding_covers_the_header():
     assert sorted(device.EXPORTS) == declared("s2l_hip.h")
     device.hip_lib()   # sets every signature; loading touches no GPU
