@@ -238,7 +238,7 @@ struct s2l_ctx {
     double ms_total = 0.0;
 
     // Krylov
-    DevBuf<double> kV, kw, kH, kcs, ksn, kg, kh, kh2, ky, khist, kpart, kr, kz, ku, kred;
+    DevBuf<double> kV, kZ, kw, kH, kcs, ksn, kg, kh, kh2, ky, khist, kpart, kr, kz, ku, kred;
     DevBuf<GmState> kst;
     DevBuf<int32_t> kflags;
     PinBuf<GmState> pst;
@@ -537,6 +537,8 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
     const int maxit = o->maxit;
     const int nb = reduceBlocks(n);
     c->kV.reserve(static_cast<size_t>(m + 1) * n * nrhs);
+    const bool flex = o->flexible != 0;
+    if (flex) c->kZ.reserve(static_cast<size_t>(m) * n * nrhs);
     c->kw.reserve(static_cast<size_t>(n) * nrhs);
     c->kr.reserve(static_cast<size_t>(n) * nrhs);
     c->kz.reserve(static_cast<size_t>(n) * nrhs);
@@ -562,6 +564,7 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
     a.m = m;
     a.nrhs = nrhs;
     a.V = c->kV.p;
+    a.Z = flex ? c->kZ.p : nullptr;
     a.w = c->kw.p;
     a.H = c->kH.p;
     a.cs = c->kcs.p;
@@ -637,8 +640,9 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
             const int j1 = std::min(m, j0 + batch);
             for (int j = j0; j < j1; ++j) {
                 const double* vj = c->kV.p + static_cast<int64_t>(j) * nrhs * n;
-                applyImpl(c, vj, c->kz.p, nrhs, live, s);               // z = M^{-1} v_j
-                spmvImpl(c, c->kz.p, c->kw.p, nullptr, nrhs, live, s);  // w = A z
+                double* zj = flex ? c->kZ.p + static_cast<int64_t>(j) * nrhs * n : c->kz.p;
+                applyImpl(c, vj, zj, nrhs, live, s);                    // z = M^{-1} v_j (kept: flexible)
+                spmvImpl(c, zj, c->kw.p, nullptr, nrhs, live, s);       // w = A z
                 gmDots(a, j, s);                                        // CGS2 pass 1
                 gmUpdateDots(a, j, s);                                  // w -= V h; pass 2
                 gmUpdateNorm(a, j, s);                                  // w -= V h2; ||w||; Givens
@@ -653,8 +657,12 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
         }
         // x += M^{-1} V_k y_k; true residual (gmres.cpp:210-231)
         gmLsq(a, s);
-        gmCombine(a, c->ku.p, s);
-        applyImpl(c, c->ku.p, c->kz.p, nrhs, idl, s);
+        if (flex) {
+            gmCombine(a, c->kZ.p, c->kz.p, s);                          // x += Z y
+        } else {
+            gmCombine(a, c->kV.p, c->ku.p, s);                          // x += M^{-1}(V y)
+            applyImpl(c, c->ku.p, c->kz.p, nrhs, idl, s);
+        }
         launchAxpby(n, 1.0, c->kz.p, 1.0, d_x, nrhs, idl, s);
         spmvImpl(c, d_x, c->kr.p, d_b, nrhs, idl, s);
         HIPCHECK(hipMemcpyAsync(c->pst.p, c->kst.p, sizeof(GmState) * nrhs, hipMemcpyDeviceToHost, s));
@@ -716,6 +724,7 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
 {
     require(o->maxit >= 0, S2L_EINVAL, "maxit must be >= 0");
     require(o->stop_norm == 0 || o->stop_norm == 1, S2L_EINVAL, "stop_norm must be 0 (l2) or 1 (openfoam)");
+    require(o->flexible == 0, S2L_EINVAL, "flexible applies to GMRES only");
     const bool of = o->stop_norm == 1;
     const int64_t n = c->n;
     const int maxit = o->maxit;

@@ -130,6 +130,7 @@ struct GmArgs {
     int32_t m;          ///< restart
     int32_t nrhs;
     double* V;          ///< basis, V[(k * nrhs + r) * n + i]
+    double* Z;          ///< flexible GMRES: M^{-1} v_k in the same layout (null otherwise)
     double* w;          ///< n per rhs
     double* H;          ///< (m + 1) x m per rhs, column-major
     double* cs;
@@ -152,7 +153,8 @@ void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s);
 void gmNormalize(const GmArgs& a, int j, hipStream_t s);
 void gmStartCycle(const GmArgs& a, const double* r, hipStream_t s);
 void gmLsq(const GmArgs& a, hipStream_t s);
-void gmCombine(const GmArgs& a, double* u, hipStream_t s);
+/// u = B_k y for the basis B = V (or Z for flexible GMRES)
+void gmCombine(const GmArgs& a, const double* B, double* u, hipStream_t s);
 
 } // namespace s2l
 

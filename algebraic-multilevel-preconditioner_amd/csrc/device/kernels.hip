@@ -843,8 +843,8 @@ __global__ void k_gm_lsq(GmArgs a, const int32_t* idle)
     }
 }
 
-// u = V_k y
-__global__ void k_gm_combine(GmArgs a, double* u, const int32_t* idle)
+// u = B_k y (B = V, or Z = M^{-1} V for flexible GMRES)
+__global__ void k_gm_combine(GmArgs a, const double* B, double* u, const int32_t* idle)
 {
     const int r = blockIdx.y;
     if (idle[r]) return;
@@ -854,7 +854,7 @@ __global__ void k_gm_combine(GmArgs a, double* u, const int32_t* idle)
     const double* y = a.y + r * a.m;
     double s = 0.0;
 #pragma unroll 8
-    for (int l = 0; l < k; ++l) s += basis(a, r, l)[i] * y[l];
+    for (int l = 0; l < k; ++l) s += B[(static_cast<int64_t>(l) * a.nrhs + r) * a.n + i] * y[l];
     u[static_cast<int64_t>(r) * a.n + i] = s;
 }
 
@@ -1001,9 +1001,9 @@ void gmLsq(const GmArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_lsq, dim3(nblk(a.nrhs, 64)), dim3(64), 0, s, a, a.flags + a.nrhs);
 }
-void gmCombine(const GmArgs& a, double* u, hipStream_t s)
+void gmCombine(const GmArgs& a, const double* B, double* u, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gm_combine, dim3(nblk(a.n, 256), a.nrhs), dim3(256), 0, s, a, u, a.flags + a.nrhs);
+    hipLaunchKernelGGL(k_gm_combine, dim3(nblk(a.n, 256), a.nrhs), dim3(256), 0, s, a, B, u, a.flags + a.nrhs);
 }
 
 } // namespace s2l

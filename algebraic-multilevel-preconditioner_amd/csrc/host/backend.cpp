@@ -135,7 +135,7 @@ const prec::HipTwoLevelSchwarz& deviceSide(const HipOperator& A, const prec::Pre
 }
 
 KrylovResult run(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x, int method,
-                 int restart, double rtol, double atol, int maxit, StopNorm stop)
+                 int restart, double rtol, double atol, int maxit, StopNorm stop, bool flexible)
 {
     const prec::HipTwoLevelSchwarz& hm = deviceSide(A, M);
     const size_t n = static_cast<size_t>(hm.n());
@@ -148,6 +148,7 @@ KrylovResult run(const HipOperator& A, const prec::Preconditioner& M, const core
     o.atol = atol;
     o.maxit = maxit;
     o.stop_norm = stop == StopNorm::OpenFOAM ? 1 : 0;
+    o.flexible = flexible ? 1 : 0;
     s2l_krylov_result r{};
     std::vector<double> hist(static_cast<size_t>(maxit) + 1);
     check(hm.context(), s2l_krylov_solve_host(hm.context(), &o, b.data(), x.data(), 1, &r, hist.data()));
@@ -166,8 +167,8 @@ KrylovResult run(const HipOperator& A, const prec::Preconditioner& M, const core
 
 } // namespace
 
-HipGmres::HipGmres(int restart, double rtol, int maxit, StopNorm stop, double atol)
-    : m_(restart), maxit_(maxit), rtol_(rtol), atol_(atol), stop_(stop)
+HipGmres::HipGmres(int restart, double rtol, int maxit, StopNorm stop, double atol, bool flexible)
+    : m_(restart), maxit_(maxit), rtol_(rtol), atol_(atol), stop_(stop), flexible_(flexible)
 {
     if (restart < 1) throw std::invalid_argument("GMRES restart must be >= 1");
     if (stop != StopNorm::L2) throw std::invalid_argument("the device GMRES implements the l2 stopping norm");
@@ -175,7 +176,7 @@ HipGmres::HipGmres(int restart, double rtol, int maxit, StopNorm stop, double at
 
 KrylovResult HipGmres::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x) const
 {
-    return run(A, M, b, x, 0, m_, rtol_, atol_, maxit_, stop_);
+    return run(A, M, b, x, 0, m_, rtol_, atol_, maxit_, stop_, flexible_);
 }
 
 HipBiCgStab::HipBiCgStab(double rtol, int maxit, StopNorm stop, double atol)
@@ -186,7 +187,7 @@ HipBiCgStab::HipBiCgStab(double rtol, int maxit, StopNorm stop, double atol)
 KrylovResult HipBiCgStab::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
                                 core::Vec& x) const
 {
-    return run(A, M, b, x, 1, 0, rtol_, atol_, maxit_, stop_);
+    return run(A, M, b, x, 1, 0, rtol_, atol_, maxit_, stop_, false);
 }
 
 std::unique_ptr<KrylovSolver> makeKrylovSolver(const std::string& krylov, int restart, double rtol, double atol, int maxit,

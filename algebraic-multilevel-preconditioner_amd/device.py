@@ -27,7 +27,8 @@ _i32p, _i64p, _dp = P(ctypes.c_int32), P(ctypes.c_int64), P(ctypes.c_double)
 
 class KrylovOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("rtol", ctypes.c_double),
-                ("atol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_norm", ctypes.c_int)]
+                ("atol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_norm", ctypes.c_int),
+                ("flexible", ctypes.c_int)]
 
 
 class KrylovResultC(ctypes.Structure):
@@ -289,12 +290,12 @@ class HipContext:
         return y
 
     def krylov(self, b: np.ndarray, x: np.ndarray | None = None, method: str = "gmres", restart=30, rtol=1e-8,
-               maxit=1000, stop: str = "l2", atol: float = 0.0):
+               maxit=1000, stop: str = "l2", atol: float = 0.0, flexible: bool = False):
         """KrylovSolver::solve on the device through s2l_krylov_solve_host (host arrays in and out)."""
         b = np.ascontiguousarray(b, np.float64)
         nrhs = 1 if b.ndim == 1 else b.shape[0]
         x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, np.float64).copy()
-        o = KrylovOpts(_METHODS[method], restart, rtol, atol, maxit, _STOPS[stop])
+        o = KrylovOpts(_METHODS[method], restart, rtol, atol, maxit, _STOPS[stop], 1 if flexible else 0)
         res = (KrylovResultC * nrhs)()
         hist = np.zeros(nrhs * (maxit + 1))
         self._check(self.L.s2l_krylov_solve_host(self.h, ctypes.byref(o), _p(b, ctypes.c_double),
@@ -307,8 +308,9 @@ class HipContext:
                                     list(hist[r * (maxit + 1): r * (maxit + 1) + R.history_len])))
         return x, (out[0] if nrhs == 1 else out)
 
-    def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000):
-        return self.krylov(b, x, "gmres", restart, rtol, maxit)
+    def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000,
+              flexible: bool = False):
+        return self.krylov(b, x, "gmres", restart, rtol, maxit, flexible=flexible)
 
     def bicgstab(self, b: np.ndarray, x: np.ndarray | None = None, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
         return self.krylov(b, x, "bicgstab", 0, rtol, maxit, stop, atol)
@@ -323,8 +325,9 @@ class HipContext:
     def spmv_device(self, x: DeviceArray, y: DeviceArray, nrhs: int = 1):
         self._check(self.L.s2l_spmv(self.h, x.ptr, y.ptr, nrhs, None))
 
-    def gmres_device(self, b: DeviceArray, x: DeviceArray, nrhs=1, restart=30, rtol=1e-8, maxit=1000, method="gmres"):
-        o = KrylovOpts(_METHODS[method], restart, rtol, 0.0, maxit, 0)
+    def gmres_device(self, b: DeviceArray, x: DeviceArray, nrhs=1, restart=30, rtol=1e-8, maxit=1000, method="gmres",
+                     flexible: bool = False):
+        o = KrylovOpts(_METHODS[method], restart, rtol, 0.0, maxit, 0, 1 if flexible else 0)
         res = (KrylovResultC * nrhs)()
         self._check(self.L.s2l_krylov_solve(self.h, ctypes.byref(o), b.ptr, x.ptr, nrhs, res, None, None))
         return [res[r] for r in range(nrhs)]
@@ -399,18 +402,19 @@ class HipTwoLevelSchwarz:
 
 
 class HipGmres:
-    """solv::Gmres (gmres.hpp:39-71) with the loop on the device."""
+    """solv::Gmres (gmres.hpp:39-71) with the loop on the device; flexible=True stores M^{-1} v_j
+    (flexible GMRES, an extension without reference counterpart)."""
 
-    def __init__(self, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
+    def __init__(self, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0, flexible=False):
         if stop != "l2":
             raise ValueError("the device GMRES implements the l2 stopping norm")
-        self.m, self.rtol, self.maxit = restart, rtol, maxit
+        self.m, self.rtol, self.maxit, self.flexible = restart, rtol, maxit, flexible
 
     def name(self) -> str:
         return f"GMRES({self.m})"
 
     def solve(self, M: HipTwoLevelSchwarz, b, x=None):
-        return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit)
+        return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit, flexible=self.flexible)
 
 
 class HipBiCgStab:

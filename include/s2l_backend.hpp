@@ -158,9 +158,10 @@ public:
 };
 
 /// Right-preconditioned restarted GMRES(m) with the whole loop on the device (gmres.cpp:18-161).
+/// @p flexible: store M^{-1} v_j (flexible GMRES; no reference counterpart, see s2l_krylov_opts).
 class HipGmres final : public KrylovSolver {
 public:
-    HipGmres(int restart, double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0);
+    HipGmres(int restart, double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0, bool flexible = false);
     KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
                        core::Vec& x) const override;
     std::string name() const override { return "GMRES(" + std::to_string(m_) + ")"; }
@@ -169,6 +170,7 @@ private:
     int m_, maxit_;
     double rtol_, atol_;
     StopNorm stop_;
+    bool flexible_;
 };
 
 /// Right-preconditioned BiCGStab, OpenFOAM's PBiCGStab variant (bicgstab.cpp:15-154).

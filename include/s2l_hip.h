@@ -113,6 +113,11 @@ typedef struct s2l_krylov_opts {
     double atol;      /* OpenFOAM norm only: absolute tolerance (0 = off) */
     int maxit;        /* maximum applications of M^{-1} */
     int stop_norm;    /* 0 l2, 1 openfoam (include/solver/krylov.hpp:29-45) */
+    int flexible;     /* GMRES only: 0 the reference's x = x0 + M^{-1}(V y) at the end of a cycle
+                         (gmres.cpp:143-156); 1 flexible GMRES (Saad, SIAM J. Sci. Comput. 14,
+                         1993): the basis Z_j = M^{-1} v_j is stored and x = x0 + Z y, which keeps
+                         the Arnoldi relation exact for an inexact (FP32) preconditioner and saves
+                         the apply at the end of a cycle. No reference counterpart. */
 } s2l_krylov_opts;
 
 typedef struct s2l_krylov_result {

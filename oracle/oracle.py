@@ -285,10 +285,12 @@ def openfoam_norm_factor(A, b, x0, Ax0):
     return norm1(Ax0 - xm * rowsum) + norm1(b - xm * rowsum) + 1e-20
 
 
-def gmres(A, M, b, x=None, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0) -> tuple:
+def gmres(A, M, b, x=None, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0, flexible=False) -> tuple:
     """Right-preconditioned restarted GMRES(m) with CGS2 and Givens rotations (gmres.cpp:18-161).
 
-    A, M: callables (vector -> vector). Returns (x, KrylovResult)."""
+    A, M: callables (vector -> vector). Returns (x, KrylovResult). flexible=True keeps
+    Z_j = M(v_j) and forms x = x0 + Z y (flexible GMRES, Saad 1993) instead of x0 + M(V y); no
+    reference counterpart."""
     n = b.size
     of = stop == "openfoam"
     res = KrylovResult()
@@ -300,6 +302,7 @@ def gmres(A, M, b, x=None, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.
         return np.zeros(n), res
     m = restart
     V = np.zeros((m + 1, n))
+    Z = np.zeros((m, n)) if flexible else None
     H = np.zeros((m + 1, m))
     cs = np.zeros(m)
     sn = np.zeros(m)
@@ -333,6 +336,8 @@ def gmres(A, M, b, x=None, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.
             if res.iterations >= maxit:
                 break
             z = M(V[j])
+            if flexible:
+                Z[j] = z
             w = A(z)
             h = V[:j + 1] @ w
             w = w - V[:j + 1].T @ h
@@ -383,8 +388,11 @@ def gmres(A, M, b, x=None, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.
         y = np.zeros(k)
         for i in range(k - 1, -1, -1):   # upper-triangular solve H_k y = g_k
             y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
-        u = V[:k].T @ y
-        z = M(u)
+        if flexible:
+            z = Z[:k].T @ y
+        else:
+            u = V[:k].T @ y
+            z = M(u)
         x = x + z
         r = A(x)
         r = b - r

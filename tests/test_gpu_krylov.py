@@ -106,3 +106,20 @@ def test_several_rhs_equal_single_solves(method):
         x, r = M.ctx.krylov(B[k], None, method, 30, 1e-8, 1000)
         assert R[k].iterations == r.iterations and R[k].converged == r.converged
         assert np.array_equal(X[k], x)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_flexible_gmres_matches_oracle(precision):
+    st = _st(*CASES[1])
+    M = _ctx(st, precision)
+    O = orc.OracleSchwarz(st, precision)
+    b = st.dec.b
+    xo, ro = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=500, flexible=True)
+    x, r = M.ctx.gmres(b, None, 30, 1e-8, 500, flexible=True)
+    assert r.converged and r.true_rel_residual <= 1.1e-8
+    if precision == "double":
+        assert r.iterations == ro.iterations
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    else:
+        assert abs(r.iterations - ro.iterations) <= 1
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6

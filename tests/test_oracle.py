@@ -427,3 +427,19 @@ def test_zero_rhs():
     for solve in (orc.gmres, orc.bicgstab):
         x, r = solve(O.spmv, O.apply, np.zeros(st.n), np.ones(st.n))
         assert r.converged and r.iterations == 0 and len(r.history) == 1 and not x.any()
+
+
+def test_flexible_gmres_equals_gmres_for_a_linear_preconditioner():
+    """Flexible GMRES (Z_j = M v_j kept) is GMRES in exact arithmetic when M is linear: same
+    iterations, same solution to rounding; with FP32 local factors it removes the second cycle."""
+    st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=6)))
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    x, r = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=500)
+    xf, rf = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=1e-8, maxit=500, flexible=True)
+    assert rf.converged and rf.iterations == r.iterations
+    assert np.linalg.norm(xf - x) / np.linalg.norm(x) < 1e-10
+    O32 = orc.OracleSchwarz(st, "single")
+    xs, rs = orc.gmres(O32.spmv, O32.apply, b, restart=30, rtol=1e-8, maxit=500, flexible=True)
+    t2 = np.linalg.norm(b - st.dec.A @ xs) / np.linalg.norm(b)
+    assert rs.converged and t2 <= 1.1e-8 and rs.restarts == 0 and rs.iterations <= r.iterations + 1
