@@ -41,22 +41,34 @@ template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const S
 }
 
 // ------------------------------------------------------------------ wide supernodes (w > 64): whole workgroup
-// The diagonal block is applied as its inverse (row-major, computed at upload): y = X x_C with a
-// wavefront per row of X, then the rows below push their products (forward), or the columns pull
-// the products with the rows below first (backward).
+// The diagonal block is applied as its inverse (column-major, computed at upload): y = X x_C with
+// G groups of w threads, thread (g, j) summing row j over the g-th of G column ranges (consecutive
+// threads read consecutive rows of a column), the partial sums combined in g order. Then the rows
+// below push their products (forward), or the columns pull the products with the rows below first
+// (backward).
 template <typename T>
-__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw)
+__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int nt)
 {
-    for (int j = wave; j < w; j += nw) {
-        const T* row = X + static_cast<int64_t>(j) * w;
-        const int lo = lower ? 0 : j, hi = lower ? j + 1 : w;
+    const int G = max(1, min(nt / w, kChunkRows / w));
+    const int cw = (w + G - 1) / G;
+    for (int u = tid; u < G * w; u += nt) {
+        const int g = u / w, j = u - g * w;
+        int lo = g * cw, hi = min(w, lo + cw);
+        if (lower)
+            hi = min(hi, j + 1);
+        else
+            lo = max(lo, j);
         T v = T(0);
-        for (int c = lo + lane; c < hi; c += kWave) v += row[c] * x[c0 + c];
-        v = waveSum(v);
-        if (lane == 0) ybuf[j] = v;
+#pragma unroll 8
+        for (int c = lo; c < hi; ++c) v += X[static_cast<int64_t>(c) * w + j] * x[c0 + c];
+        ybuf[u] = v;
     }
     __syncthreads();
-    for (int j = tid; j < w; j += nw * kWave) x[c0 + j] = ybuf[j];
+    for (int j = tid; j < w; j += nt) {
+        T v = T(0);
+        for (int g = 0; g < G; ++g) v += ybuf[g * w + j];
+        x[c0 + j] = v;
+    }
     __syncthreads();
 }
 
@@ -67,7 +79,7 @@ __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
     for (int j = tid; j < w; j += nt) x[c0 + j] = pullRow(a, sd, tb, x, c0 + j);
     __syncthreads();
-    wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, lane, wave, nw);
+    wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, nt);
     const T* B = a.below + s.below;
     const int32_t* dst = a.cidx + sd.dst_off + s.toff;
     for (int k = wave * 4; k < h; k += nw * 4) {
@@ -88,7 +100,7 @@ __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s
 
 template <typename T> __device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
-    const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
+    const int w = s.w, h = s.h, c0 = s.c0;
     if (h > 0) {
         // G groups of w threads, group g sums the rows k = g, g + G, ...; partial sums in ybuf,
         // combined in group order
@@ -110,7 +122,7 @@ template <typename T> __device__ void bwdWide(const LocalArgs<T>& a, const SnDes
         }
         __syncthreads();
     }
-    wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, lane, wave, nw);
+    wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, nt);
 }
 
 // ------------------------------------------------------------------ small supernodes of one level, all at once

@@ -124,18 +124,14 @@ void invertUpper(double* B, int ld, int m)
 
 namespace {
 
-// Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64;
-// column-major for a small supernode (a thread per row walks a column-major block: coalesced over
-// the rows), row-major for a wide one (a wavefront per row reads it contiguously).
+// Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64,
+// column-major (a thread per row walks a column-major block: coalesced over the rows).
 void invertDiagonal(double* B, int w, bool lower)
 {
     if (lower)
         invertLower(B, w, w);
     else
         invertUpper(B, w, w);
-    if (w > kSmallWidth)
-        for (int r = 0; r < w; ++r)
-            for (int c = r + 1; c < w; ++c) std::swap(B[static_cast<std::size_t>(c) * w + r], B[static_cast<std::size_t>(r) * w + c]);
 }
 
 } // namespace
