@@ -23,18 +23,62 @@ template <typename T> __device__ inline T waveSum(T v)
     return v;
 }
 
-// x[row] minus the products pushed to this row by its descendants: one contiguous slot range.
+// sum over k < n of p[k * ps] * q[k * qs], in order k = 0, 1, ..., with the loads issued eight at
+// a time ahead of their products: the scheduling barrier keeps the compiler from pulling a
+// batch's loads down next to their uses, which leaves one memory round trip per element.
+template <typename T> __device__ inline T dot8(const T* p, int ps, const T* q, int qs, int n, T acc)
+{
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        T a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = p[(k + u) * ps];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b[u] = q[(k + u) * qs];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += a[u] * b[u];
+    }
+    for (; k < n; ++k) acc += p[k * ps] * q[k * qs];
+    return acc;
+}
+
+// sum over k < n of p[k * ps] * x[r[k * rs]] (x in LDS), in order, with the loads of p and r issued
+// eight at a time ahead of the products.
+template <typename T> __device__ inline T gatherDot8(const T* p, int ps, const int32_t* r, int rs, const T* x, int n)
+{
+    T acc = T(0);
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        T a[8];
+        int32_t i[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = p[(k + u) * ps];
+            i[u] = r[(k + u) * rs];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += a[u] * x[i[u]];
+    }
+    for (; k < n; ++k) acc += p[k * ps] * x[r[k * rs]];
+    return acc;
+}
+
+// x[row] minus the products pushed to this row by its descendants: one contiguous slot range,
+// subtracted in slot order, eight loads per batch.
 template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const SubDesc& sd, const T* tb, const T* x, int row)
 {
     T v = x[row];
     const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
     int q = cb;
-    for (; q + 4 <= ce; q += 4) {
-        const T t0 = tb[q], t1 = tb[q + 1], t2 = tb[q + 2], t3 = tb[q + 3];
-        v -= t0;
-        v -= t1;
-        v -= t2;
-        v -= t3;
+    for (; q + 8 <= ce; q += 8) {
+        T t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = tb[q + u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v -= t[u];
     }
     for (; q < ce; ++q) v -= tb[q];
     return v;
@@ -58,10 +102,7 @@ __device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, i
             hi = min(hi, j + 1);
         else
             lo = max(lo, j);
-        T v = T(0);
-#pragma unroll 8
-        for (int c = lo; c < hi; ++c) v += X[static_cast<int64_t>(c) * w + j] * x[c0 + c];
-        ybuf[u] = v;
+        ybuf[u] = hi > lo ? dot8(X + lo * w + j, w, x + c0 + lo, 1, hi - lo, T(0)) : T(0);
     }
     __syncthreads();
     for (int j = tid; j < w; j += nt) {
@@ -109,10 +150,7 @@ template <typename T> __device__ void bwdWide(const LocalArgs<T>& a, const SnDes
         const int32_t* R = a.ridx + s.ridx;
         for (int u = tid; u < G * w; u += nt) {
             const int g = u / w, c = u - g * w;
-            T sum = T(0);
-#pragma unroll 4
-            for (int k = g; k < h; k += G) sum += B[static_cast<int64_t>(k) * w + c] * x[R[k]];
-            ybuf[u] = sum;
+            ybuf[u] = g < h ? gatherDot8(B + g * w + c, G * w, R + g, G, x, (h - g + G - 1) / G) : T(0);
         }
         __syncthreads();
         for (int c = tid; c < w; c += nt) {
@@ -174,14 +212,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
                 const int i = findItem(wp, cnt, t);
                 const SnDesc& s = st[i];
                 const int c = t - wp[i];
-                if (s.h > 0) {
-                    const T* B = a.below + s.below + c;
-                    const int32_t* R = a.ridx + s.ridx;
-                    T sum = T(0);
-#pragma unroll 8
-                    for (int k = 0; k < s.h; ++k) sum += B[static_cast<int64_t>(k) * s.w] * x[R[k]];
-                    x[s.c0 + c] -= sum;
-                }
+                if (s.h > 0) x[s.c0 + c] -= gatherDot8(a.below + s.below + c, s.w, a.ridx + s.ridx, 1, x, s.h);
             }
         }
         __syncthreads();
@@ -190,15 +221,8 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             const SnDesc& s = st[i];
             const int j = t - wp[i];
             const T* D = a.diag + s.diag;
-            T y = T(0);
-            if (forward) {
-#pragma unroll 8
-                for (int c = 0; c <= j; ++c) y += D[c * s.w + j] * x[s.c0 + c];
-            } else {
-#pragma unroll 8
-                for (int c = j; c < s.w; ++c) y += D[c * s.w + j] * x[s.c0 + c];
-            }
-            ybuf[t] = y;
+            const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
+            ybuf[t] = dot8(D + lo * s.w + j, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
         }
         __syncthreads();
         for (int t = tid; t < W; t += nt) {
@@ -212,10 +236,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
                 const SnDesc& s = st[i];
                 const int k = t - hp[i];
                 const T* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
-                T v = T(0);
-#pragma unroll 8
-                for (int c = 0; c < s.w; ++c) v += B[c] * x[s.c0 + c];
-                tb[a.cidx[sd.dst_off + s.toff + k]] = v;
+                tb[a.cidx[sd.dst_off + s.toff + k]] = dot8(B, 1, x + s.c0, 1, s.w, T(0));
             }
             __syncthreads();
         }
