@@ -85,34 +85,112 @@ template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const S
 }
 
 // ------------------------------------------------------------------ wide supernodes (w > 64): whole workgroup
-// The diagonal block is applied as its inverse (column-major, computed at upload): y = X x_C with
-// G groups of w threads, thread (g, j) summing row j over the g-th of G column ranges (consecutive
-// threads read consecutive rows of a column), the partial sums combined in g order. Then the rows
-// below push their products (forward), or the columns pull the products with the rows below first
-// (backward).
-template <typename T>
-__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int nt)
+// The blocks of a wide supernode are row-major with a leading dimension padded to four entries
+// and start 16-byte aligned, so a lane reads four consecutive columns with one vector load.
+
+// Four consecutive entries from a 16-byte aligned address.
+template <typename T> __device__ inline void load4(const T* p, T (&v)[4]);
+template <> __device__ inline void load4<float>(const float* p, float (&v)[4])
 {
-    const int G = max(1, min(nt / w, kChunkRows / w));
-    const int cw = (w + G - 1) / G;
-    for (int u = tid; u < G * w; u += nt) {
-        const int g = u / w, j = u - g * w;
-        int lo = g * cw, hi = min(w, lo + cw);
-        if (lower)
-            hi = min(hi, j + 1);
-        else
-            lo = max(lo, j);
-        ybuf[u] = hi > lo ? dot8(X + lo * w + j, w, x + c0 + lo, 1, hi - lo, T(0)) : T(0);
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+}
+template <> __device__ inline void load4<double>(const double* p, double (&v)[4])
+{
+    const double2 q0 = reinterpret_cast<const double2*>(p)[0], q1 = reinterpret_cast<const double2*>(p)[1];
+    v[0] = q0.x;
+    v[1] = q0.y;
+    v[2] = q1.x;
+    v[3] = q1.y;
+}
+
+// Rows of a dense block taken by one wavefront at a time: 4 in FP32, 2 in FP64 (registers).
+template <typename T> constexpr int blockRows() { return sizeof(T) == 4 ? 4 : 2; }
+
+// Sums of R rows spread over the 64 lanes (v[r]: this lane's part of row r), by a transposing
+// butterfly: at the step over lane bit b every lane keeps half of its rows and adds the partner's
+// part of them, so after log2(R) steps lane l holds row l / (64 / R) summed over R lanes, and
+// plain steps over the remaining lane bits finish it (7 shuffles for 4 rows). The order is fixed.
+template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lane)
+{
+#pragma unroll
+    for (int half = R / 2, bit = kWave / 2; half >= 1; half >>= 1, bit >>= 1) {
+        const bool hi = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < half; ++j) {
+            const T keep = hi ? v[j + half] : v[j];
+            const T send = hi ? v[j] : v[j + half];
+            v[j] = keep + __shfl_xor(send, bit, kWave);
+        }
     }
+    T s = v[0];
+#pragma unroll
+    for (int bit = kWave / R / 2; bit >= 1; bit >>= 1) s += __shfl_xor(s, bit, kWave);
+    return s;
+}
+
+// y_r = sum over c < cols of M[r * ld + c] * xs[c] for the rows r < rows of a wide supernode's
+// block (xs in LDS): a wavefront takes blockRows rows at a time with each lane on four consecutive
+// columns (one vector load per row and 256 columns), and out(r, y_r) is called by one lane per
+// row. kTri 1 skips the column quads right of the rows (lower triangle), 2 those left of them
+// (upper); the stored block is zero there. All lanes of a wavefront run the same rows.
+template <typename T, int kTri, typename Out>
+__device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
+{
+    constexpr int R = blockRows<T>();
+    for (int r0 = wave * R; r0 < rows; r0 += nw * R) {
+        const int rn = min(R, rows - r0);
+        T acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = T(0);
+        for (int cb = 0; cb < cols; cb += 4 * kWave) {
+            const int c = cb + 4 * lane;
+            bool need = c < cols;
+            if (kTri == 1) need = need && c <= r0 + rn - 1;
+            if (kTri == 2) need = need && c + 3 >= r0;
+            T xv[4], m[R][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xv[i] = need && c + i < cols ? xs[c + i] : T(0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (need && r < rn) {
+                    load4(M + static_cast<int64_t>(r0 + r) * ld + c, m[r]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m[r][i] = T(0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[r] += m[r][i] * xv[i];
+        }
+        const T y = transposeSum<T, R>(acc, lane);
+        const int r = r0 + lane / (kWave / R);
+        if (lane % (kWave / R) == 0 && r < rows) out(r, y);
+    }
+}
+
+// y = X x_C for the inverse X of a wide supernode's diagonal block (lower or upper triangle),
+// written over x_C.
+template <typename T>
+__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw, int nt)
+{
+    const int ld = wideLd(w);
+    if (lower)
+        denseRows<T, 1>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
+    else
+        denseRows<T, 2>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
     __syncthreads();
-    for (int j = tid; j < w; j += nt) {
-        T v = T(0);
-        for (int g = 0; g < G; ++g) v += ybuf[g * w + j];
-        x[c0 + j] = v;
-    }
+    for (int j = tid; j < w; j += nt) x[c0 + j] = ybuf[j];
     __syncthreads();
 }
 
+// Forward: every row pulls its products, y = X x_C, then the rows below push B y to their slots.
 template <typename T>
 __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, T* tb, int tid, int lane,
                         int wave, int nt)
@@ -120,47 +198,59 @@ __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
     for (int j = tid; j < w; j += nt) x[c0 + j] = pullRow(a, sd, tb, x, c0 + j);
     __syncthreads();
-    wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, nt);
-    const T* B = a.below + s.below;
+    wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, lane, wave, nw, nt);
     const int32_t* dst = a.cidx + sd.dst_off + s.toff;
-    for (int k = wave * 4; k < h; k += nw * 4) {
-        T v[4] = {T(0), T(0), T(0), T(0)};
-        for (int c = lane; c < w; c += kWave) {
-            const T xv = x[c0 + c];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (k + q < h) v[q] += B[static_cast<int64_t>(k + q) * w + c] * xv;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const T r = waveSum(v[q]);
-            if (lane == 0 && k + q < h) tb[dst[k + q]] = r;
-        }
-    }
+    denseRows<T, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { tb[dst[k]] = v; });
 }
 
-template <typename T> __device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
+// Backward: x_C -= B^T x_R with each lane on four consecutive columns and each wavefront on a
+// range of the rows below (blockRows rows per batch of loads); the wavefronts' partial sums are
+// combined in wavefront order through LDS, 256 columns at a time. Then x_C = X x_C.
+template <typename T>
+__device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
-    const int w = s.w, h = s.h, c0 = s.c0;
+    const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave, ld = wideLd(w);
+    constexpr int K = blockRows<T>();
     if (h > 0) {
-        // G groups of w threads, group g sums the rows k = g, g + G, ...; partial sums in ybuf,
-        // combined in group order
-        const int G = max(1, min(nt / w, kChunkRows / w));
         const T* B = a.below + s.below;
         const int32_t* R = a.ridx + s.ridx;
-        for (int u = tid; u < G * w; u += nt) {
-            const int g = u / w, c = u - g * w;
-            ybuf[u] = g < h ? gatherDot8(B + g * w + c, G * w, R + g, G, x, (h - g + G - 1) / G) : T(0);
+        const int per = (h + nw - 1) / nw;
+        const int k0 = min(h, wave * per), k1 = min(h, k0 + per);
+        for (int cb = 0; cb < w; cb += 4 * kWave) {
+            const int c = cb + 4 * lane;
+            const bool need = c < w;
+            T acc[4] = {T(0), T(0), T(0), T(0)};
+            for (int k = k0; k < k1; k += K) {
+                T m[K][4], xk[K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const bool ok = k + j < k1;
+                    xk[j] = ok ? x[R[k + j]] : T(0);
+                    if (need && ok) {
+                        load4(B + static_cast<int64_t>(k + j) * ld + c, m[j]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) m[j][i] = T(0);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < K; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i] += m[j][i] * xk[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ybuf[wave * 4 * kWave + 4 * lane + i] = acc[i];
+            __syncthreads();
+            for (int t = tid; t < min(4 * kWave, w - cb); t += nt) {
+                T sum = T(0);
+                for (int v = 0; v < nw; ++v) sum += ybuf[v * 4 * kWave + t];
+                x[c0 + cb + t] -= sum;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int c = tid; c < w; c += nt) {
-            T sum = T(0);
-            for (int g = 0; g < G; ++g) sum += ybuf[g * w + c];
-            x[c0 + c] -= sum;
-        }
-        __syncthreads();
     }
-    wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, nt);
+    wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, lane, wave, nw, nt);
 }
 
 // ------------------------------------------------------------------ small supernodes of one level, all at once

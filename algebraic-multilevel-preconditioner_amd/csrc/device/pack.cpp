@@ -4,13 +4,15 @@
 // Input: the factors of all subdomains in compressed-column form (L, and U^T for LU), the form
 // the reference's LocalSolver backends expose. Output: one table of supernodes per sweep; dense
 // blocks (diagonal block column-major, rows below row-major); for the forward sweep the lists of
-// products each row pulls from its descendants; and the tasks of the work queue: every subtree
-// below a split depth of the supernodal elimination tree is one task, every supernode above it
-// is one task. The queue lists the forward entries by decreasing task depth, then the backward
-// entries by increasing depth, all subdomains interleaved (a level-ordered work queue).
+// products each row pulls from its descendants; and the tasks of the work queue: the supernodal
+// elimination tree is cut into subtree tasks of balanced estimated time and a skeleton of
+// single-supernode tasks above them. The queue lists the forward entries of all subtree tasks
+// first, then the skeleton's by decreasing depth, then the backward entries in the reverse order,
+// all subdomains interleaved (a level-ordered work queue).
 #include "pack.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -124,14 +126,31 @@ void invertUpper(double* B, int ld, int m)
 
 namespace {
 
-// Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64,
-// column-major (a thread per row walks a column-major block: coalesced over the rows).
-void invertDiagonal(double* B, int w, bool lower)
+// Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64.
+// A small supernode's stays column-major (a thread per row walks it: coalesced over the rows); a
+// wide one's is stored row-major with the padded leading dimension wideLd(w), starting at a
+// multiple of four entries (a wavefront reads a row in 16-byte pieces).
+void appendDiagonal(std::vector<double>& out, std::vector<double>& B, int w, bool lower)
 {
     if (lower)
-        invertLower(B, w, w);
+        invertLower(B.data(), w, w);
     else
-        invertUpper(B, w, w);
+        invertUpper(B.data(), w, w);
+    if (w <= kSmallWidth) {
+        out.insert(out.end(), B.begin(), B.end());
+        return;
+    }
+    const int ld = wideLd(w);
+    for (int r = 0; r < w; ++r) {
+        for (int c = 0; c < w; ++c) out.push_back(B[static_cast<std::size_t>(c) * w + r]);
+        out.insert(out.end(), static_cast<std::size_t>(ld - w), 0.0);
+    }
+}
+
+// Pads a value array to a multiple of four entries (the start of a wide supernode's blocks).
+void alignFour(std::vector<double>& v)
+{
+    while (v.size() % 4 != 0) v.push_back(0.0);
 }
 
 } // namespace
@@ -162,6 +181,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         for (int32_t s = 0; s < nsf; ++s)
             for (int32_t j = fst[s]; j < fst[s + 1]; ++j) snodeOf[static_cast<std::size_t>(j)] = s;
         const int32_t f_base = static_cast<int32_t>(P.sn.size());
+        std::vector<double> blk;  // one diagonal block, column-major, before its inversion
         std::vector<int32_t> parent(static_cast<std::size_t>(nsf), -1);
         int32_t toff = 0;
         std::vector<int64_t> belowOff(static_cast<std::size_t>(nsf)), ridxOff(static_cast<std::size_t>(nsf));
@@ -178,13 +198,20 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             d.h = h;
             d.toff = toff;
             toff += h;
+            const bool wide = w > kSmallWidth;
+            if (wide) alignFour(P.diag);
             d.diag = static_cast<int64_t>(P.diag.size());
+            blk.assign(static_cast<std::size_t>(w) * w, 0.0);
             for (int32_t c = 0; c < w; ++c)
-                for (int32_t r = 0; r < w; ++r) P.diag.push_back(r >= c ? F.lv[F.lcp[c0 + c] + (r - c)] : 0.0);
-            invertDiagonal(P.diag.data() + d.diag, w, true);
+                for (int32_t r = c; r < w; ++r) blk[static_cast<std::size_t>(c) * w + r] = F.lv[F.lcp[c0 + c] + (r - c)];
+            appendDiagonal(P.diag, blk, w, true);
+            if (wide) alignFour(P.below);
             d.below = static_cast<int64_t>(P.below.size());
-            for (int32_t k = 0; k < h; ++k)
+            const int32_t ldb = wide ? wideLd(w) : w;
+            for (int32_t k = 0; k < h; ++k) {
                 for (int32_t c = 0; c < w; ++c) P.below.push_back(F.lv[F.lcp[c0 + c] + (w - c) + k]);
+                P.below.insert(P.below.end(), static_cast<std::size_t>(ldb - w), 0.0);
+            }
             d.ridx = static_cast<int64_t>(P.ridx.size());
             for (int32_t k = 0; k < h; ++k) P.ridx.push_back(F.lr[p0 + w + k]);
             belowOff[static_cast<std::size_t>(s)] = d.below;
@@ -237,18 +264,25 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             d.c0 = c0;
             d.w = w;
             d.h = h;
+            const bool wide = w > kSmallWidth;
+            if (wide) alignFour(P.diag);
             d.diag = static_cast<int64_t>(P.diag.size());
             // G[c * w + r] = U(c0 + r, c0 + c) = U^T(c0 + c, c0 + r), r <= c
+            blk.assign(static_cast<std::size_t>(w) * w, 0.0);
             for (int32_t c = 0; c < w; ++c)
-                for (int32_t r = 0; r < w; ++r) P.diag.push_back(r <= c ? uv[ucp[c0 + r] + (c - r)] : 0.0);
-            invertDiagonal(P.diag.data() + d.diag, w, false);
+                for (int32_t r = 0; r <= c; ++r) blk[static_cast<std::size_t>(c) * w + r] = uv[ucp[c0 + r] + (c - r)];
+            appendDiagonal(P.diag, blk, w, false);
             if (chol) {
                 d.below = belowOff[static_cast<std::size_t>(s)];
                 d.ridx = ridxOff[static_cast<std::size_t>(s)];
             } else {
+                if (wide) alignFour(P.below);
                 d.below = static_cast<int64_t>(P.below.size());
-                for (int32_t k = 0; k < h; ++k)
+                const int32_t ldb = wide ? wideLd(w) : w;
+                for (int32_t k = 0; k < h; ++k) {
                     for (int32_t c = 0; c < w; ++c) P.below.push_back(uv[ucp[c0 + c] + (w - c) + k]);
+                    P.below.insert(P.below.end(), static_cast<std::size_t>(ldb - w), 0.0);
+                }
                 d.ridx = static_cast<int64_t>(P.ridx.size());
                 for (int32_t k = 0; k < h; ++k) P.ridx.push_back(ur[p0 + w + k]);
             }
@@ -261,38 +295,100 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
 
         // ---------------- tasks
         std::vector<int32_t> depth(static_cast<std::size_t>(nsf), 0);
-        int32_t maxDepth = 0;
         for (int32_t s = nsf - 1; s >= 0; --s) {
             const int32_t p = parent[static_cast<std::size_t>(s)];
             depth[static_cast<std::size_t>(s)] = (p >= 0) ? depth[static_cast<std::size_t>(p)] + 1 : 0;
-            maxDepth = std::max(maxDepth, depth[static_cast<std::size_t>(s)]);
         }
-        std::vector<int32_t> cntDepth(static_cast<std::size_t>(maxDepth) + 1, 0);
-        for (int32_t s = 0; s < nsf; ++s) ++cntDepth[static_cast<std::size_t>(depth[static_cast<std::size_t>(s)])];
-        int32_t dsplit = 0;
+        // Task split: the elimination tree is cut into a skeleton of single-supernode tasks near
+        // the top and subtree tasks below it. Starting from the whole tree, the subtree with the
+        // largest estimated time is cut repeatedly (its root joins the skeleton, its children
+        // become subtrees) while it is estimated above 1 / tasks_per_sub of the whole, for at most
+        // kMaxSkeleton cuts. The subtrees hanging from one skeleton supernode are then packed
+        // into as few tasks as the same bound allows (first fit by decreasing estimate): a task
+        // may hold several sibling subtrees. The estimate of a subtree is its dense entries
+        // (w (w + 1) / 2 + h w per supernode) plus kLevelEntries per level of its height: the
+        // sweeps pay a fixed latency per level.
+        std::vector<char> skel(static_cast<std::size_t>(nsf), 0);
+        std::vector<std::vector<int32_t>> kids(static_cast<std::size_t>(nsf));
+        std::vector<int64_t> entries(static_cast<std::size_t>(nsf), 0);
+        std::vector<int32_t> height(static_cast<std::size_t>(nsf), 0);
+        for (int32_t s = 0; s < nsf; ++s) {
+            const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
+            entries[static_cast<std::size_t>(s)] += static_cast<int64_t>(d.w) * (d.w + 1) / 2 + static_cast<int64_t>(d.h) * d.w;
+            const int32_t p = parent[static_cast<std::size_t>(s)];
+            if (p >= 0) {
+                kids[static_cast<std::size_t>(p)].push_back(s);
+                entries[static_cast<std::size_t>(p)] += entries[static_cast<std::size_t>(s)];
+                height[static_cast<std::size_t>(p)] = std::max(height[static_cast<std::size_t>(p)], height[static_cast<std::size_t>(s)] + 1);
+            }
+        }
+        auto estimate = [&](int32_t s) {
+            return entries[static_cast<std::size_t>(s)] + kLevelEntries * (height[static_cast<std::size_t>(s)] + 1);
+        };
+        int64_t limit = std::numeric_limits<int64_t>::max();
         if (samePartition && tasks_per_sub > 1) {
-            for (int32_t d = 0; d <= maxDepth; ++d)
-                if (cntDepth[static_cast<std::size_t>(d)] >= tasks_per_sub) {
-                    dsplit = d;
-                    break;
+            std::vector<std::pair<int64_t, int32_t>> heap;
+            int64_t total = 0;
+            for (int32_t s = 0; s < nsf; ++s)
+                if (parent[static_cast<std::size_t>(s)] < 0) {
+                    heap.emplace_back(estimate(s), s);
+                    total += estimate(s);
                 }
+            std::make_heap(heap.begin(), heap.end());
+            limit = total / tasks_per_sub;
+            for (int32_t cuts = 0; !heap.empty() && cuts < kMaxSkeleton; ++cuts) {
+                std::pop_heap(heap.begin(), heap.end());
+                const auto top = heap.back();
+                const auto& ch = kids[static_cast<std::size_t>(top.second)];
+                if (top.first <= limit || ch.empty()) break;
+                heap.pop_back();
+                skel[static_cast<std::size_t>(top.second)] = 1;
+                for (int32_t c : ch) {
+                    heap.emplace_back(estimate(c), c);
+                    std::push_heap(heap.begin(), heap.end());
+                }
+            }
         }
-        // the backward sweep may use another supernode partition (LU with another structure):
-        // then every subdomain is a single task
-        if (!samePartition) dsplit = 0;
         const int32_t t_base = static_cast<int32_t>(P.task.size());
         std::vector<int32_t> taskOf(static_cast<std::size_t>(nsf), -1);
         std::vector<int32_t> taskRoot;
         for (int32_t s = nsf - 1; s >= 0; --s) {
-            const int32_t dpt = depth[static_cast<std::size_t>(s)];
+            const int32_t p = parent[static_cast<std::size_t>(s)];
             if (!samePartition) {
+                // the backward sweep uses another supernode partition (LU with another
+                // structure): every subdomain is a single task
                 if (taskRoot.empty()) taskRoot.push_back(nsf - 1);
                 taskOf[static_cast<std::size_t>(s)] = 0;
-            } else if (dpt <= dsplit) {
-                taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
-                taskRoot.push_back(s);
-            } else {
-                taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(parent[static_cast<std::size_t>(s)])];
+                continue;
+            }
+            if (p < 0 || skel[static_cast<std::size_t>(s)]) {
+                if (taskOf[static_cast<std::size_t>(s)] < 0) {
+                    taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
+                    taskRoot.push_back(s);
+                }
+            } else if (!skel[static_cast<std::size_t>(p)]) {
+                taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(p)];
+            }
+            if (!skel[static_cast<std::size_t>(s)]) continue;
+            // pack the subtrees below skeleton supernode s into tasks; a skeleton child gets its
+            // own task when it is visited
+            std::vector<int32_t> sub;
+            for (int32_t c : kids[static_cast<std::size_t>(s)])
+                if (!skel[static_cast<std::size_t>(c)]) sub.push_back(c);
+            std::stable_sort(sub.begin(), sub.end(), [&](int32_t x, int32_t y) { return estimate(x) > estimate(y); });
+            const int64_t cap = sub.empty() ? 0 : std::max(limit, estimate(sub.front()));
+            std::vector<int64_t> load;
+            std::vector<int32_t> binTask;
+            for (int32_t c : sub) {
+                std::size_t b = 0;
+                while (b < load.size() && load[b] + estimate(c) > cap) ++b;
+                if (b == load.size()) {
+                    load.push_back(0);
+                    binTask.push_back(static_cast<int32_t>(taskRoot.size()));
+                    taskRoot.push_back(c);
+                }
+                load[b] += estimate(c);
+                taskOf[static_cast<std::size_t>(c)] = binTask[b];
             }
         }
         const int32_t nt = static_cast<int32_t>(taskRoot.size());
@@ -370,11 +466,15 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             td.nchild = static_cast<int32_t>(children[static_cast<std::size_t>(t)].size());
             P.task_child.insert(P.task_child.end(), children[static_cast<std::size_t>(t)].begin(), children[static_cast<std::size_t>(t)].end());
         }
+        // tasks without child tasks (the subtrees) depend on no other task in the forward sweep:
+        // they all go to the first forward group, and to the last backward group; the skeleton
+        // tasks follow by decreasing depth (forward) and precede by increasing depth (backward)
         for (int32_t t = 0; t < nt; ++t) {
             const TaskDesc& td = P.task[static_cast<std::size_t>(t_base + t)];
             const int32_t id = t_base + t;
-            groups[{0, -td.depth}].push_back(id << 1);
-            if (td.parent >= 0) groups[{1, td.depth}].push_back((id << 1) | 1);
+            const bool leaf = td.nchild == 0;
+            groups[{0, leaf ? std::numeric_limits<int32_t>::min() : -td.depth}].push_back(id << 1);
+            if (td.parent >= 0) groups[{1, leaf ? std::numeric_limits<int32_t>::max() : td.depth}].push_back((id << 1) | 1);
         }
         P.max_levels_fwd = std::max(P.max_levels_fwd, 0);
         P.sub.push_back(sd);

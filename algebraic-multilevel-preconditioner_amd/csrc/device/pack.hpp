@@ -11,12 +11,20 @@ namespace s2l {
 constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide are "small"
 constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
 constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
+constexpr int32_t kMaxSkeleton = 16;       ///< most skeleton supernodes of one subdomain's task split
+constexpr int64_t kLevelEntries = 16384;   ///< dense entries that cost about as much as one level of a sweep
+
+/// Leading dimension of a wide supernode's blocks: w rounded up to a multiple of four entries.
+constexpr int32_t wideLd(int32_t w) { return (w + 3) & ~3; }
 
 /// One supernode of a sweep: columns [c0, c0 + w) of the factor, h rows below the diagonal block.
+/// Blocks of a small supernode (w <= kSmallWidth): the inverse of the diagonal block column-major
+/// w x w, the block below row-major h x w. Blocks of a wide one start at a multiple of four
+/// entries and are row-major with leading dimension wideLd(w) (zero padding).
 struct SnDesc {
     int32_t c0, w, h, toff; ///< toff: forward only, offset of its h products in the T buffer
-    int64_t diag;           ///< offset of the w x w diagonal block (column-major)
-    int64_t below;          ///< offset of the h x w block below (row-major)
+    int64_t diag;           ///< offset of the inverse of the diagonal block
+    int64_t below;          ///< offset of the h x w block below
     int64_t ridx;           ///< offset of the h row indices (factor positions)
 };
 
@@ -31,9 +39,9 @@ struct SubDesc {
     int32_t dst_off;  ///< offset of the slot of every pushed product (indexed by toff + k)
 };
 
-/// A task of the work queue: a subtree of the supernodal elimination tree (every supernode at or
-/// below a split depth), or one supernode above it. Forward entries run bottom-up, backward
-/// entries top-down; a task without parent runs both sweeps in one entry.
+/// A task of the work queue: one or more sibling subtrees of the supernodal elimination tree, or
+/// one supernode of the skeleton above the subtrees. Forward entries run bottom-up, backward entries top-down; a
+/// task without parent runs both sweeps in one entry.
 struct TaskDesc {
     int32_t sub;
     int32_t f_lev, f_nlev;          ///< forward level table of the task's supernodes
