@@ -203,52 +203,28 @@ __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s
     denseRows<T, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { tb[dst[k]] = v; });
 }
 
-// Backward: x_C -= B^T x_R with each lane on four consecutive columns and each wavefront on a
-// range of the rows below (blockRows rows per batch of loads); the wavefronts' partial sums are
-// combined in wavefront order through LDS, 256 columns at a time. Then x_C = X x_C.
+// Backward: x_C -= B^T x_R with G groups of w threads, group g summing the rows k = g, g + G, ...
+// (consecutive threads on consecutive columns of a row), the partial sums combined in group
+// order through LDS. Then x_C = X x_C.
 template <typename T>
 __device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave, ld = wideLd(w);
-    constexpr int K = blockRows<T>();
     if (h > 0) {
+        const int G = max(1, min(nt / w, kChunkRows / w));
         const T* B = a.below + s.below;
         const int32_t* R = a.ridx + s.ridx;
-        const int per = (h + nw - 1) / nw;
-        const int k0 = min(h, wave * per), k1 = min(h, k0 + per);
-        for (int cb = 0; cb < w; cb += 4 * kWave) {
-            const int c = cb + 4 * lane;
-            const bool need = c < w;
-            T acc[4] = {T(0), T(0), T(0), T(0)};
-            for (int k = k0; k < k1; k += K) {
-                T m[K][4], xk[K];
-#pragma unroll
-                for (int j = 0; j < K; ++j) {
-                    const bool ok = k + j < k1;
-                    xk[j] = ok ? x[R[k + j]] : T(0);
-                    if (need && ok) {
-                        load4(B + static_cast<int64_t>(k + j) * ld + c, m[j]);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) m[j][i] = T(0);
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int j = 0; j < K; ++j)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[i] += m[j][i] * xk[j];
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) ybuf[wave * 4 * kWave + 4 * lane + i] = acc[i];
-            __syncthreads();
-            for (int t = tid; t < min(4 * kWave, w - cb); t += nt) {
-                T sum = T(0);
-                for (int v = 0; v < nw; ++v) sum += ybuf[v * 4 * kWave + t];
-                x[c0 + cb + t] -= sum;
-            }
-            __syncthreads();
+        for (int u = tid; u < G * w; u += nt) {
+            const int g = u / w, c = u - g * w;
+            ybuf[u] = g < h ? gatherDot8(B + g * ld + c, G * ld, R + g, G, x, (h - g + G - 1) / G) : T(0);
         }
+        __syncthreads();
+        for (int c = tid; c < w; c += nt) {
+            T sum = T(0);
+            for (int g = 0; g < G; ++g) sum += ybuf[g * w + c];
+            x[c0 + c] -= sum;
+        }
+        __syncthreads();
     }
     wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, lane, wave, nw, nt);
 }
