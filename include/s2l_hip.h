@@ -182,9 +182,11 @@ int s2l_timing_read(s2l_ctx* ctx, int64_t* launches, double* ms_total, int reset
 int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
 
 /* Work split of the batched local solve: the supernodal elimination tree of every subdomain is cut
- * at the first depth holding at least @p tasks supernodes; each subtree below the cut and each
- * supernode above it is one task of the level-ordered queue (1 = one task per subdomain). Takes
- * effect at the next s2l_finalize_local(). Default 1. */
+ * into subtrees of at most about 1/@p tasks of its estimated time (dense entries plus a fixed cost
+ * per level) below a skeleton of single supernodes; sibling subtrees are packed into shared tasks.
+ * Subtree tasks run in the first forward and the last backward launch, the skeleton in between
+ * (1 = one task per subdomain, one launch). Results do not depend on the split. Takes effect at the
+ * next s2l_finalize_local(). Default 2. */
 int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
 
 #ifdef __cplusplus
