@@ -16,6 +16,7 @@ namespace s2l {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kPullSplit = 4;  // a backward chunk of at most blockDim / kPullSplit columns splits its rows below
 
 template <typename T> __device__ inline T waveSum(T v)
 {
@@ -273,12 +274,31 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
                 const int row = st[i].c0 + (t - wp[i]);
                 x[row] = pullRow(a, sd, tb, x, row);
             }
-        } else {
+        } else if (W * kPullSplit > nt) {
             for (int t = tid; t < W; t += nt) {
                 const int i = findItem(wp, cnt, t);
                 const SnDesc& s = st[i];
                 const int c = t - wp[i];
                 if (s.h > 0) x[s.c0 + c] -= gatherDot8(a.below + s.below + c, s.w, a.ridx + s.ridx, 1, x, s.h);
+            }
+        } else {
+            // a narrow chunk (few columns, long columns below): G groups of W threads, thread
+            // (g, t) sums the rows g, g + G, ... below column t; partial sums combined in group order
+            const int G = min(nt / W, kChunkRows / W);
+            for (int u = tid; u < G * W; u += nt) {
+                const int g = u / W, t = u - g * W;
+                const int i = findItem(wp, cnt, t);
+                const SnDesc& s = st[i];
+                const int c = t - wp[i];
+                ybuf[u] = g < s.h ? gatherDot8(a.below + s.below + g * s.w + c, G * s.w, a.ridx + s.ridx + g, G, x, (s.h - g + G - 1) / G)
+                                  : T(0);
+            }
+            __syncthreads();
+            for (int t = tid; t < W; t += nt) {
+                const int i = findItem(wp, cnt, t);
+                T sum = T(0);
+                for (int g = 0; g < G; ++g) sum += ybuf[g * W + t];
+                x[st[i].c0 + (t - wp[i])] -= sum;
             }
         }
         __syncthreads();

@@ -112,22 +112,28 @@ def test_apply_bitwise_repeatable_and_launch_modes_agree(precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-def test_subtree_tasks_give_the_same_bits(precision):
+def test_subtree_tasks_agree(precision):
     """Splitting every subdomain's elimination tree into subtree tasks (several workgroups per
-    subdomain, one launch per depth group) changes the schedule, not the arithmetic: the apply is
-    bitwise the same for 1 to 4 tasks per subdomain, on Cholesky (Poisson) and LU (convection)
-    factors, and it matches the oracle."""
+    subdomain, several launches) changes the schedule, and with it only the rounding of the sums
+    below narrow chunks: 1 to 4 tasks per subdomain agree to rounding and with the oracle, on
+    Cholesky (Poisson) and LU (convection) factors, and every split is bitwise repeatable."""
     from amp_amd import device
+    tol = 1e-13 if precision == "double" else 2e-6
     for name, kw, N in (("poisson3d", dict(nx=16), 8), ("convdiff2d", dict(nx=24), 4)):
         st = _setup(name, kw, N, nev=8, overlap=2)
         print(f"case {name} {kw} N={N} {precision}", flush=True)
         r = np.cos(np.linspace(0.0, 7.0, st.n))
-        out = [device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t).apply(r) for t in (1, 2, 3, 4)]
+        out = []
+        for t in (1, 2, 3, 4):
+            M = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t)
+            z = M.apply(r)
+            assert np.array_equal(z, M.apply(r)), t
+            out.append(z)
         for z in out[1:]:
-            assert np.array_equal(z, out[0])
-        if precision == "double":
-            zo = orc.OracleSchwarz(st).apply(r)
-            assert np.linalg.norm(out[0] - zo) / np.linalg.norm(zo) < 1e-12
+            assert np.linalg.norm(z - out[0]) / np.linalg.norm(out[0]) < tol
+        zo = orc.OracleSchwarz(st).apply(r)
+        err = np.linalg.norm(out[0] - zo) / np.linalg.norm(zo)
+        assert err < (1e-12 if precision == "double" else 1e-5)
 
 
 @pytest.mark.parametrize("mode", [2, 0])
