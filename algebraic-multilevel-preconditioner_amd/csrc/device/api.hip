@@ -211,7 +211,7 @@ struct s2l_ctx {
     DevBuf<TaskDesc> task_d;
     DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
     std::vector<int32_t> qsub_h;
-    int tasks_per_sub = 2;
+    int tasks_per_sub = 4;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
 

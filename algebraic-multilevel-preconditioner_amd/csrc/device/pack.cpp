@@ -306,12 +306,10 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         // kMaxSkeleton cuts. The subtrees hanging from one skeleton supernode are then packed
         // into as few tasks as the same bound allows (first fit by decreasing estimate): a task
         // may hold several sibling subtrees. The estimate of a subtree is its dense entries
-        // (w (w + 1) / 2 + h w per supernode) plus kLevelEntries per level of its height: the
-        // sweeps pay a fixed latency per level.
+        // (w (w + 1) / 2 + h w per supernode).
         std::vector<char> skel(static_cast<std::size_t>(nsf), 0);
         std::vector<std::vector<int32_t>> kids(static_cast<std::size_t>(nsf));
         std::vector<int64_t> entries(static_cast<std::size_t>(nsf), 0);
-        std::vector<int32_t> height(static_cast<std::size_t>(nsf), 0);
         for (int32_t s = 0; s < nsf; ++s) {
             const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
             entries[static_cast<std::size_t>(s)] += static_cast<int64_t>(d.w) * (d.w + 1) / 2 + static_cast<int64_t>(d.h) * d.w;
@@ -319,12 +317,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             if (p >= 0) {
                 kids[static_cast<std::size_t>(p)].push_back(s);
                 entries[static_cast<std::size_t>(p)] += entries[static_cast<std::size_t>(s)];
-                height[static_cast<std::size_t>(p)] = std::max(height[static_cast<std::size_t>(p)], height[static_cast<std::size_t>(s)] + 1);
             }
         }
-        auto estimate = [&](int32_t s) {
-            return entries[static_cast<std::size_t>(s)] + kLevelEntries * (height[static_cast<std::size_t>(s)] + 1);
-        };
+        auto estimate = [&](int32_t s) { return entries[static_cast<std::size_t>(s)]; };
         int64_t limit = std::numeric_limits<int64_t>::max();
         if (samePartition && tasks_per_sub > 1) {
             std::vector<std::pair<int64_t, int32_t>> heap;

@@ -12,7 +12,6 @@ constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide a
 constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
 constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
 constexpr int32_t kMaxSkeleton = 16;       ///< most skeleton supernodes of one subdomain's task split
-constexpr int64_t kLevelEntries = 16384;   ///< dense entries that cost about as much as one level of a sweep
 
 /// Leading dimension of a wide supernode's blocks: w rounded up to a multiple of four entries.
 constexpr int32_t wideLd(int32_t w) { return (w + 3) & ~3; }
