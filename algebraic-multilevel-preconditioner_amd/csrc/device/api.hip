@@ -211,7 +211,8 @@ struct s2l_ctx {
     DevBuf<TaskDesc> task_d;
     DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
     std::vector<int32_t> qsub_h;
-    int tasks_per_sub = 4;
+    int tasks_per_sub = 0;  // 0: one subtree task per compute unit (compute units / subdomains)
+    int cus = 0;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
 
@@ -1038,9 +1039,11 @@ int s2l_ctx_create(int device, s2l_ctx** out)
             HIPCHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
             HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
             HIPCHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-            int lds = 0;
+            int lds = 0, cus = 0;
             HIPCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+            HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
             c->max_lds = static_cast<size_t>(lds);
+            c->cus = cus;
         } catch (...) {
             delete c;
             throw;
@@ -1253,7 +1256,9 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
             values(c->ls_bb, L.belowB);
         }
         // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
-        c->pk = packFactors(c->hf, moff, c->tasks_per_sub);
+        // the subtree tasks of all subdomains run in one launch, one workgroup per compute unit
+        const int tasks = c->tasks_per_sub > 0 ? c->tasks_per_sub : std::max(1, c->cus / std::max(1, c->nsub));
+        c->pk = packFactors(c->hf, moff, tasks);
         Packed& P = c->pk;
         for (int i = 0; i < c->nsub; ++i) {
             P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
@@ -1673,7 +1678,7 @@ int s2l_timing_read(s2l_ctx* c, int64_t* launches, double* ms_total, int reset)
 int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
 {
     return guarded(c, [&] {
-        require(c && tasks >= 1, S2L_EINVAL, "tasks per subdomain must be >= 1");
+        require(c && tasks >= 0, S2L_EINVAL, "tasks per subdomain must be >= 0 (0 = automatic)");
         c->tasks_per_sub = tasks;
         c->local_ready = false;
     });

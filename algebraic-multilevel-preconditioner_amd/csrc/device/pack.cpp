@@ -5,10 +5,10 @@
 // the reference's LocalSolver backends expose. Output: one table of supernodes per sweep; dense
 // blocks (diagonal block column-major, rows below row-major); for the forward sweep the lists of
 // products each row pulls from its descendants; and the tasks of the work queue: the supernodal
-// elimination tree is cut into subtree tasks of balanced estimated time and a skeleton of
-// single-supernode tasks above them. The queue lists the forward entries of all subtree tasks
-// first, then the skeleton's by decreasing depth, then the backward entries in the reverse order,
-// all subdomains interleaved (a level-ordered work queue).
+// elimination tree is cut into subtree tasks of balanced dense work and one task above them (the
+// skeleton with the root). The queue lists the forward entries of all subtree tasks first, then
+// the top tasks (both sweeps in one entry), then the subtrees' backward entries, all subdomains
+// interleaved (a level-ordered work queue).
 #include "pack.hpp"
 
 #include <algorithm>
@@ -299,10 +299,10 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             const int32_t p = parent[static_cast<std::size_t>(s)];
             depth[static_cast<std::size_t>(s)] = (p >= 0) ? depth[static_cast<std::size_t>(p)] + 1 : 0;
         }
-        // Task split: the elimination tree is cut into a skeleton of single-supernode tasks near
-        // the top and subtree tasks below it. Starting from the whole tree, the subtree with the
-        // largest estimated time is cut repeatedly (its root joins the skeleton, its children
-        // become subtrees) while it is estimated above 1 / tasks_per_sub of the whole, for at most
+        // Task split: the elimination tree is cut into a skeleton near the top, one task with the
+        // root, and subtree tasks below it. Starting from the whole tree, the subtree with the
+        // largest estimate is cut repeatedly (its root joins the skeleton, its children become
+        // subtrees) while it is estimated above 1 / tasks_per_sub of the whole, for at most
         // kMaxSkeleton cuts. The subtrees hanging from one skeleton supernode are then packed
         // into as few tasks as the same bound allows (first fit by decreasing estimate): a task
         // may hold several sibling subtrees. The estimate of a subtree is its dense entries
@@ -356,13 +356,13 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 taskOf[static_cast<std::size_t>(s)] = 0;
                 continue;
             }
-            if (p < 0 || skel[static_cast<std::size_t>(s)]) {
-                if (taskOf[static_cast<std::size_t>(s)] < 0) {
-                    taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
-                    taskRoot.push_back(s);
-                }
-            } else if (!skel[static_cast<std::size_t>(p)]) {
-                taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(p)];
+            if (p < 0) {
+                taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
+                taskRoot.push_back(s);
+            } else if (skel[static_cast<std::size_t>(s)] || !skel[static_cast<std::size_t>(p)]) {
+                // the skeleton is one task with the root; a subtree below a skeleton supernode
+                // was assigned when its parent was visited
+                if (taskOf[static_cast<std::size_t>(s)] < 0) taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(p)];
             }
             if (!skel[static_cast<std::size_t>(s)]) continue;
             // pack the subtrees below skeleton supernode s into tasks; a skeleton child gets its

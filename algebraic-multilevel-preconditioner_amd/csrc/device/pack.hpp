@@ -11,7 +11,7 @@ namespace s2l {
 constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide are "small"
 constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
 constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
-constexpr int32_t kMaxSkeleton = 16;       ///< most skeleton supernodes of one subdomain's task split
+constexpr int32_t kMaxSkeleton = 16;       ///< most supernodes cut from the subtrees into the top task
 
 /// Leading dimension of a wide supernode's blocks: w rounded up to a multiple of four entries.
 constexpr int32_t wideLd(int32_t w) { return (w + 3) & ~3; }
@@ -39,7 +39,7 @@ struct SubDesc {
 };
 
 /// A task of the work queue: one or more sibling subtrees of the supernodal elimination tree, or
-/// one supernode of the skeleton above the subtrees. Forward entries run bottom-up, backward entries top-down; a
+/// the skeleton above the subtrees (with the root). Forward entries run bottom-up, backward entries top-down; a
 /// task without parent runs both sweeps in one entry.
 struct TaskDesc {
     int32_t sub;

@@ -148,7 +148,8 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--tasks", type=int, default=4, help="tasks per subdomain of the batched local solve")
+    ap.add_argument("--tasks", type=int, default=0,
+                    help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
     ap.add_argument("--local-mode", type=int, default=None, choices=(0, 1, 2),
                     help="schedule of the local solves (s2l_set_local_launch_mode; default: the library's)")
     ap.add_argument("--profile-applies", type=int, default=0,

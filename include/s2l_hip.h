@@ -183,10 +183,11 @@ int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
 
 /* Work split of the batched local solve: the supernodal elimination tree of every subdomain is cut
  * into subtrees of at most about 1/@p tasks of its dense entries below a skeleton of at most 16
- * single supernodes; sibling subtrees are packed into shared tasks.
- * Subtree tasks run in the first forward and the last backward launch, the skeleton in between
- * (1 = one task per subdomain, one launch). Splits agree to rounding; each is bitwise repeatable.
- * Takes effect at the next s2l_finalize_local(). Default 4. */
+ * supernodes (one task with the root); sibling subtrees are packed into shared tasks. The subtree
+ * tasks of all subdomains run in one forward and one backward launch, the top tasks (both sweeps)
+ * in between (1 = one task per subdomain, one launch). Splits agree to rounding; each is bitwise
+ * repeatable. 0 (default) = compute units / subdomains, at least 1, so that the subtree launch
+ * holds about one workgroup per compute unit. Takes effect at the next s2l_finalize_local(). */
 int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
 
 #ifdef __cplusplus
