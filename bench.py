@@ -67,13 +67,17 @@ def time_applies(ctx, r, z, count, warm=3):
     return ms / max(nl, 1), wall * 1e3
 
 
-def roofline(bytes_per_launch, avg_ms, launches):
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+def roofline(bytes_per_apply, avg_ms, applies):
+    """The dominant kernel's roofline: k_local_solve<float>, whose launches of one apply (the subtree
+    forward launch, the top tasks, the subtree backward launch) are timed together with HIP events
+    on their stream; avg_us is their time per apply, bytes the algorithmic bytes of one apply's
+    local solves."""
+    achieved = bytes_per_apply / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_local_solve<float> (task queue, the launches of one apply)", "launches": int(launches),
+            "kernel": "k_local_solve<float> (task queue, the launches of one apply)", "applies_timed": int(applies),
             "avg_us": round(avg_ms * 1e3, 2),
-            "bytes_per_launch": int(bytes_per_launch)}
+            "bytes_per_apply": int(bytes_per_apply)}
 
 
 def measure_traffic(args):

@@ -169,10 +169,12 @@ typedef struct s2l_stats {
 } s2l_stats;
 int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
 
-/* Event timing of the local-solve kernel: when enabled, every launch of it is bracketed by HIP
- * events on its stream; s2l_timing_read returns the launch count and the summed milliseconds. */
+/* Event timing of the batched local solve: when enabled, the launches of every local solve (one
+ * apply's: all depth groups of the task queue, or all levels, or all subdomains) are bracketed by
+ * HIP events on their stream; s2l_timing_read returns the number of local solves and their summed
+ * milliseconds. */
 int s2l_timing_enable(s2l_ctx* ctx, int on);
-int s2l_timing_read(s2l_ctx* ctx, int64_t* launches, double* ms_total, int reset);
+int s2l_timing_read(s2l_ctx* ctx, int64_t* solves, double* ms_total, int reset);
 
 /* Schedule of the batched local solve: 0 = task queue, one launch per depth group of all
  * subdomains' task trees, a subtree per workgroup with its vector in LDS (default); 1 = the same
