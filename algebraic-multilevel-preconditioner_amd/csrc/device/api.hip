@@ -158,6 +158,8 @@ bool denseLu(int n, std::vector<double>& A, std::vector<int32_t>& p)
 
 } // namespace
 
+constexpr size_t kMaxGraphs = 160;  // cached apply graphs per context (GMRES(m): m inputs, m outputs if flexible)
+
 struct s2l_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -231,6 +233,29 @@ struct s2l_ctx {
     DevBuf<double> x0, w, loc, cvec, tvec, yvec, cpart;
     int work_nrhs = 0;
 
+    // HIP graphs of one apply + SpMV in the Krylov loops, one per (input, output) pair, captured on
+    // first use; every upload, schedule change or work-buffer growth invalidates them (gen)
+    struct ApplyGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t gen = 0;
+        const double* in = nullptr;
+        double* z = nullptr;
+        double* w = nullptr;
+        const int32_t* skip = nullptr;
+        int nrhs = 0;
+        hipStream_t s = nullptr;
+    };
+    std::vector<ApplyGraph> graphs;
+    uint64_t gen = 1;
+    bool use_graphs = false;  // measured: replays end within noise of eager launches (see s2l_set_graphs)
+    void invalidateGraphs()
+    {
+        ++gen;
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -254,6 +279,7 @@ struct s2l_ctx {
     void ensureWork(int nrhs)
     {
         if (nrhs <= work_nrhs) return;
+        invalidateGraphs();
         x0.reserve(static_cast<size_t>(n) * nrhs);
         w.reserve(static_cast<size_t>(n) * nrhs);
         loc.reserve(static_cast<size_t>(local_total) * nrhs);
@@ -526,6 +552,49 @@ void spmvImpl(s2l_ctx* c, const double* x, double* y, const double* b, int nrhs,
     HIPCHECK(hipGetLastError());
 }
 
+// z = M^{-1} in, w = A z: one launch of a HIP graph of all their kernels (about a dozen launches
+// otherwise), captured on the first call for these buffers and replayed afterwards. Without graphs
+// (or while the local solve is timed with events) the kernels are launched one by one.
+void applySpmv(s2l_ctx* c, const double* in, double* z, double* w, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    if (!c->use_graphs || c->timing) {
+        applyImpl(c, in, z, nrhs, skip, s);
+        spmvImpl(c, z, w, nullptr, nrhs, skip, s);
+        return;
+    }
+    c->ensureWork(nrhs);  // no allocation while capturing
+    for (const auto& g : c->graphs)
+        if (g.gen == c->gen && g.in == in && g.z == z && g.w == w && g.skip == skip && g.nrhs == nrhs && g.s == s) {
+            HIPCHECK(hipGraphLaunch(g.exec, s));
+            return;
+        }
+    if (c->graphs.size() >= kMaxGraphs) c->invalidateGraphs();
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        applyImpl(c, in, z, nrhs, skip, s);
+        spmvImpl(c, z, w, nullptr, nrhs, skip, s);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(s, &graph));
+    s2l_ctx::ApplyGraph g;
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(e);
+    g.gen = c->gen;
+    g.in = in;
+    g.z = z;
+    g.w = w;
+    g.skip = skip;
+    g.nrhs = nrhs;
+    g.s = s;
+    c->graphs.push_back(g);
+    HIPCHECK(hipGraphLaunch(g.exec, s));
+}
+
 // ---- device GMRES (gmres.cpp:18-161), nrhs independent solves sharing every launch
 int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int nrhs, s2l_krylov_result* res,
               double* history, hipStream_t s)
@@ -642,8 +711,7 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
             for (int j = j0; j < j1; ++j) {
                 const double* vj = c->kV.p + static_cast<int64_t>(j) * nrhs * n;
                 double* zj = flex ? c->kZ.p + static_cast<int64_t>(j) * nrhs * n : c->kz.p;
-                applyImpl(c, vj, zj, nrhs, live, s);                    // z = M^{-1} v_j (kept: flexible)
-                spmvImpl(c, zj, c->kw.p, nullptr, nrhs, live, s);       // w = A z
+                applySpmv(c, vj, zj, c->kw.p, nrhs, live, s);           // z = M^{-1} v_j (kept: flexible); w = A z
                 gmDots(a, j, s);                                        // CGS2 pass 1
                 gmUpdateDots(a, j, s);                                  // w -= V h; pass 2
                 gmUpdateNorm(a, j, s);                                  // w -= V h2; ||w||; Givens
@@ -1062,6 +1130,7 @@ int s2l_ctx_destroy(s2l_ctx* ctx)
         (void)hipEventDestroy(pr.second);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    ctx->invalidateGraphs();
     if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
@@ -1075,6 +1144,7 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
                       const int32_t* ownership)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && n > 0 && rowptr && col && val && nsub >= 1 && ownership, S2L_EINVAL, "upload_system: bad arguments");
         require(ownership[0] == 0 && ownership[nsub] == n, S2L_EINVAL, "ownership must cover [0, n)");
         for (int p = 0; p < nsub; ++p) require(ownership[p + 1] > ownership[p], S2L_EINVAL, "empty subdomain in ownership");
@@ -1125,6 +1195,7 @@ int s2l_upload_subdomains(s2l_ctx* c, int32_t nsub, const int32_t* local_ptr, co
                           int one_level)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && c->system_ready, S2L_ESTATE, "upload_subdomains: upload the system first");
         require(nsub == c->nsub && local_ptr && gid, S2L_EINVAL, "upload_subdomains: bad arguments");
         require(one_level == 0 || one_level == 1, S2L_EINVAL, "one_level must be 0 (RAS) or 1 (ASM)");
@@ -1183,6 +1254,7 @@ int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* per
                             const double* l_val, const int64_t* ut_colptr, const int32_t* ut_row, const double* ut_val)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "upload_local_factor: upload the subdomains first");
         require(i >= 0 && i < c->nsub, S2L_EINVAL, "subdomain index out of range");
         require(n == c->local_ptr[i + 1] - c->local_ptr[i], S2L_EINVAL, "factor size differs from n_i");
@@ -1216,6 +1288,7 @@ int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* per
 int s2l_finalize_local(s2l_ctx* c, int precision)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
         require(precision == 0 || precision == 1, S2L_EINVAL, "precision must be 0 (FP64) or 1 (FP32)");
         for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
@@ -1336,6 +1409,7 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
                       int correction)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "upload_coarse: upload the subdomains first");
         require(n0 >= 0 && offsets && (correction == 0 || correction == 1), S2L_EINVAL, "upload_coarse: bad arguments");
         require(offsets[0] == 0 && offsets[c->nsub] == n0, S2L_EINVAL, "coarse offsets must cover [0, n0)");
@@ -1656,6 +1730,7 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
 int s2l_timing_enable(s2l_ctx* c, int on)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c != nullptr, S2L_EINVAL, "timing_enable: null context");
         c->timing = (on != 0);
     });
@@ -1678,15 +1753,26 @@ int s2l_timing_read(s2l_ctx* c, int64_t* launches, double* ms_total, int reset)
 int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && tasks >= 0, S2L_EINVAL, "tasks per subdomain must be >= 0 (0 = automatic)");
         c->tasks_per_sub = tasks;
         c->local_ready = false;
     });
 }
 
+int s2l_set_graphs(s2l_ctx* c, int on)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "set_graphs: null context");
+        c->invalidateGraphs();
+        c->use_graphs = on != 0;
+    });
+}
+
 int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
 {
     return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
         require(c && mode >= 0 && mode <= 2, S2L_EINVAL, "launch mode must be 0, 1 or 2");
         require(mode == 2 || !c->local_ready || c->queue_ok, S2L_ELIMIT,
                 "launch modes 0 and 1 need a subdomain vector in the LDS of one workgroup");

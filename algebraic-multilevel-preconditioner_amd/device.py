@@ -58,7 +58,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
            "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_copy_d2d", "s2l_synchronize",
            "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
-           "s2l_set_tasks_per_subdomain"]
+           "s2l_set_tasks_per_subdomain", "s2l_set_graphs"]
 
 
 def hip_lib():
@@ -94,6 +94,7 @@ def hip_lib():
         "s2l_timing_read": [vp, P(ctypes.c_int64), P(ctypes.c_double), ctypes.c_int],
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
         "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
+        "s2l_set_graphs": [vp, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -352,6 +353,9 @@ class HipContext:
         ms = ctypes.c_double()
         self._check(self.L.s2l_timing_read(self.h, ctypes.byref(n), ctypes.byref(ms), 1 if reset else 0))
         return n.value, ms.value
+
+    def set_graphs(self, on: bool):
+        self._check(self.L.s2l_set_graphs(self.h, 1 if on else 0))
 
     def set_launch_mode(self, mode: int):
         self._check(self.L.s2l_set_local_launch_mode(self.h, mode))

@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--graphs", type=int, default=0, choices=(0, 1),
+                    help="replay each GMRES iteration's apply + SpMV from a captured HIP graph (s2l_set_graphs)")
     ap.add_argument("--tasks", type=int, default=0,
                     help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
     ap.add_argument("--local-mode", type=int, default=None, choices=(0, 1, 2),
@@ -193,6 +195,7 @@ def main():
 
     M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks)
     ctx = M32.ctx
+    ctx.set_graphs(args.graphs != 0)
     if args.local_mode is not None:
         ctx.set_launch_mode(args.local_mode)
     stats = ctx.stats()
@@ -216,6 +219,7 @@ def main():
 
     M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks)
     ctx64 = M64.ctx
+    ctx64.set_graphs(args.graphs != 0)
     nsteps = args.steps + args.warmup
     # right-hand sides in the renumbered order, resident on the device before timing
     B = np.stack([hs.random_rhs(n, args.seed + k)[st.dec.new_to_old] for k in range(nsteps)])

@@ -192,6 +192,14 @@ int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
  * holds about one workgroup per compute unit. Takes effect at the next s2l_finalize_local(). */
 int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
 
+/* HIP graphs in the device Krylov loops (no counterpart in the reference, whose apply is a host
+ * call): on, the kernels of one preconditioner apply and the following SpMV of a GMRES iteration
+ * are captured once per input/output buffer pair and replayed as one graph launch; every upload,
+ * schedule change or work-buffer growth drops the captured graphs. Off (default) launches every
+ * kernel by itself: on MI355X the replays measured within noise of it (the launch gaps left are
+ * the host polls of the convergence flags). Results are identical either way. */
+int s2l_set_graphs(s2l_ctx* ctx, int on);
+
 #ifdef __cplusplus
 }
 #endif
