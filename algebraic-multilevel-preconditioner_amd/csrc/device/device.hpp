@@ -91,6 +91,8 @@ void launchLs(const LsArgs<T>& a, int kind, int32_t first, int32_t count, int32_
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s);
+/// Right-hand sides from which the coarse products run on the matrix cores (16 per group).
+constexpr int kMfmaMinRhs = 4;
 /// c = R_0 r; part: nrhs x nsub x rchunks x mmax partials of the first stage
 void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
                           const int32_t* skip, hipStream_t s);

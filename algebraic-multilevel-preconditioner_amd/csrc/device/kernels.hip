@@ -486,6 +486,138 @@ __global__ __launch_bounds__(kCrows) void k_coarse_restrict(CoarseArgs c, const 
     }
 }
 
+// ------------------------------------------------------------------ coarse products of several
+// right-hand sides on the matrix cores (v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16) in
+// FP64). Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4 j][l % 16]
+// for j = 0..3. A group of up to 16 right-hand sides shares every read of V and of A00's inverted
+// factors, which the one-right-hand-side kernels read once per right-hand side.
+typedef double mfmaAcc __attribute__((ext_vector_type(4)));
+constexpr int kMfmaTiles = 4;  // coarse columns of one subdomain covered by the restriction: 64
+
+__device__ inline mfmaAcc mfma4(double a, double b, mfmaAcc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// First stage of c_i = V_i^T r for the right-hand sides 16 g .. 16 g + 15: workgroup (i, chunk, g),
+// wavefront w on rows [64 w, 64 w + 64) of the chunk's 256, the four wavefronts' tiles summed in
+// wavefront order; partial sums in the layout of k_coarse_restrict.
+__global__ __launch_bounds__(kCrows) void k_coarse_restrict_mfma(CoarseArgs c, const double* r, int64_t n, double* part,
+                                                                  int nrhs, const int32_t* skip)
+{
+    const int i = blockIdx.x, ch = blockIdx.y, g0 = blockIdx.z * 16;
+    const int rows = c.vrows[i];
+    const int nch = (rows + kCrows - 1) / kCrows;
+    if (ch >= nch) return;
+    const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
+    const int ntile = (m + 15) / 16;
+    const double* V = c.V + c.voff[i];
+    const int32_t* gr = c.vrow_g + c.vrow_ptr[i];
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int rhs = g0 + (lane & 15);
+    const bool rok = rhs < nrhs;
+    const double* rv = r + static_cast<int64_t>(rok ? rhs : 0) * n;
+    mfmaAcc acc[kMfmaTiles];
+#pragma unroll
+    for (int t = 0; t < kMfmaTiles; ++t) acc[t] = mfmaAcc{0.0, 0.0, 0.0, 0.0};
+    const int k0 = ch * kCrows + wave * 64, k1 = min(rows, k0 + 64);
+    for (int kb = k0; kb < k1; kb += 4) {
+        const int k = kb + (lane >> 4);
+        const bool kok = k < k1;
+        const double b = (kok && rok) ? rv[gr[k]] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kMfmaTiles; ++t) {
+            if (t < ntile) {
+                const int q = 16 * t + (lane & 15);
+                const double a = (kok && q < m) ? V[static_cast<int64_t>(q) * rows + k] : 0.0;
+                acc[t] = mfma4(a, b, acc[t]);
+            }
+        }
+    }
+    __shared__ double red[kCrows / kWave][kMfmaTiles][256];
+#pragma unroll
+    for (int t = 0; t < kMfmaTiles; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[wave][t][((lane >> 4) + 4 * j) * 16 + (lane & 15)] = acc[t][j];
+    __syncthreads();
+    // thread u: coarse column q = 16 t + u / 16, right-hand side g0 + u % 16
+    for (int t = 0; t < ntile; ++t) {
+        const int u = threadIdx.x, q = 16 * t + u / 16, rr = g0 + (u & 15);
+        if (q < m && rr < nrhs && !(skip && skip[rr])) {
+            double sum = 0.0;
+            for (int w = 0; w < kCrows / kWave; ++w) sum += red[w][t][u];
+            part[((static_cast<int64_t>(rr) * c.nsub + i) * c.rchunks + ch) * c.mmax + q] = sum;
+        }
+    }
+}
+
+// y = T x for 16 right-hand sides at a time (T an inverted triangular factor of A00, row-major
+// n0 x n0 with explicit zeros; x rows permuted by perm when given): wavefront w of workgroup
+// (b, g) computes rows [64 b + 16 w, +16) over the columns the triangle holds.
+__global__ __launch_bounds__(256) void k_tri_gemm_mfma(int32_t n0, const double* T, int lower, const int32_t* perm,
+                                                        const double* x, double* y, int nrhs, const int32_t* skip)
+{
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int r0 = blockIdx.x * 64 + wave * 16, g0 = blockIdx.y * 16;
+    if (r0 >= n0) return;
+    const int rhs = g0 + (lane & 15);
+    const bool rok = rhs < nrhs;
+    const double* xv = x + static_cast<int64_t>(rok ? rhs : 0) * n0;
+    const int row = r0 + (lane & 15);
+    const bool wok = row < n0;
+    const double* Trow = T + static_cast<int64_t>(wok ? row : 0) * n0;
+    const int kbeg = lower ? 0 : (r0 & ~3), kend = lower ? min(n0, r0 + 16) : n0;
+    mfmaAcc acc = {0.0, 0.0, 0.0, 0.0};
+    for (int kb = kbeg; kb < kend; kb += 4) {
+        const int k = kb + (lane >> 4);
+        const bool kok = k < kend;
+        const double a = (kok && wok) ? Trow[k] : 0.0;
+        const double b = (kok && rok) ? xv[perm ? perm[k] : k] : 0.0;
+        acc = mfma4(a, b, acc);
+    }
+    if (!rok || (skip && skip[rhs])) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int rw = r0 + (lane >> 4) + 4 * j;
+        if (rw < n0) y[static_cast<int64_t>(rhs) * n0 + rw] = acc[j];
+    }
+}
+
+// x0 rows of V_i = V_i y_i for 16 right-hand sides at a time: workgroup (prolongation block b, g),
+// wavefront w on rows [64 w, 64 w + 64) of the block's 256, 16 rows per tile, K = m_i.
+__global__ __launch_bounds__(256) void k_coarse_prolong_mfma(CoarseArgs c, const double* y, double* out, int64_t ld_out,
+                                                              int boolean, int nrhs, const int32_t* skip)
+{
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int b = blockIdx.x, g0 = blockIdx.y * 16;
+    const int i = c.blk_sub[b];
+    const int rows = c.vrows[i];
+    const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
+    const double* V = c.V + c.voff[i];
+    const int rhs = g0 + (lane & 15);
+    const bool rok = rhs < nrhs;
+    const double* yv = y + static_cast<int64_t>(rok ? rhs : 0) * c.n0 + c.off[i];
+    const bool write = rok && !(skip && skip[rhs]);
+    for (int t = 0; t < 4; ++t) {
+        const int l0 = c.blk_row[b] + wave * 64 + t * 16;
+        if (l0 >= rows) break;
+        const int row = l0 + (lane & 15);
+        mfmaAcc acc = {0.0, 0.0, 0.0, 0.0};
+        for (int kb = 0; kb < m; kb += 4) {
+            const int q = kb + (lane >> 4);
+            const double a = (q < m && row < rows) ? V[static_cast<int64_t>(q) * rows + row] : 0.0;
+            const double bb = (q < m && rok) ? yv[q] : 0.0;
+            acc = mfma4(a, bb, acc);
+        }
+        if (!write) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int l = l0 + (lane >> 4) + 4 * j;
+            if (l < rows) {
+                const int64_t dst = boolean ? c.vrow_g[c.vrow_ptr[i] + l] : c.vrow_ptr[i] + l;
+                out[static_cast<int64_t>(rhs) * ld_out + dst] = acc[j];
+            }
+        }
+    }
+}
+
 // c_i[q] = sum over the chunks of subdomain i, in chunk order.
 __global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, const int32_t* skip)
 {
@@ -1002,20 +1134,32 @@ void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* 
 void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
                           const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_coarse_restrict, dim3(c.nsub, c.rchunks, nrhs), dim3(kCrows), 0, s, c, r, n, part, skip);
+    if (nrhs >= kMfmaMinRhs && c.mmax <= 16 * kMfmaTiles)
+        hipLaunchKernelGGL(k_coarse_restrict_mfma, dim3(c.nsub, c.rchunks, (nrhs + 15) / 16), dim3(kCrows), 0, s, c, r, n, part,
+                           nrhs, skip);
+    else
+        hipLaunchKernelGGL(k_coarse_restrict, dim3(c.nsub, c.rchunks, nrhs), dim3(kCrows), 0, s, c, r, n, part, skip);
     hipLaunchKernelGGL(k_coarse_final, dim3(c.nsub, nrhs), dim3(64), 0, s, c, part, cvec, skip);
 }
 
 void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
                    const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tri_gemv, dim3(nblk(n0, 4), nrhs), dim3(256), 0, s, n0, T, lower, perm, x, y, skip);
+    if (nrhs >= kMfmaMinRhs)
+        hipLaunchKernelGGL(k_tri_gemm_mfma, dim3(nblk(n0, 64), (nrhs + 15) / 16), dim3(256), 0, s, n0, T, lower, perm, x, y, nrhs,
+                           skip);
+    else
+        hipLaunchKernelGGL(k_tri_gemv, dim3(nblk(n0, 4), nrhs), dim3(256), 0, s, n0, T, lower, perm, x, y, skip);
 }
 
 void launchCoarseProlong(const CoarseArgs& c, const double* y, double* out, int64_t ld_out, int boolean, int nrhs,
                          const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_coarse_prolong, dim3(c.nblk, nrhs), dim3(256), 0, s, c, y, out, ld_out, boolean, skip);
+    if (nrhs >= kMfmaMinRhs)
+        hipLaunchKernelGGL(k_coarse_prolong_mfma, dim3(c.nblk, (nrhs + 15) / 16), dim3(256), 0, s, c, y, out, ld_out, boolean, nrhs,
+                           skip);
+    else
+        hipLaunchKernelGGL(k_coarse_prolong, dim3(c.nblk, nrhs), dim3(256), 0, s, c, y, out, ld_out, boolean, skip);
 }
 
 void launchRevGather(int32_t n, const int32_t* ptr, const int32_t* idx, const double* src, int64_t src_ld,

@@ -173,3 +173,26 @@ def test_gmres_fp32_local_within_target():
     assert r32.converged and r32.true_rel_residual <= 1.1e-8
     assert r32.iterations <= int(1.1 * r64.iterations) + 1
     assert np.linalg.norm(x32 - x64) / np.linalg.norm(x64) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_many_rhs_coarse_products_on_matrix_cores(precision):
+    """16 right-hand sides (coarse restriction, A00 products and prolongation on the FP64 matrix
+    cores, one read of V and of the inverted factors for the group) agree with 16 single applies to
+    rounding (FP64 local factors: 1e-12; FP32 ones: the float rounding of the local right-hand
+    sides, 1e-6), and with the oracle. High-contrast channels, 8 subdomains, 12 coarse vectors each."""
+    from amp_amd import device
+    st = _setup("highcontrast3d", dict(nx=14, seed=5, contrast=1e6, channel_fraction=0.1), 8, nev=12, overlap=1)
+    print(f"case highcontrast3d 14^3 N=8 16 rhs {precision}", flush=True)
+    M = device.HipTwoLevelSchwarz(st, precision)
+    rng = np.random.default_rng(7)
+    R = rng.standard_normal((16, st.n))
+    Z = M.ctx.apply(R)
+    Zs = np.stack([M.ctx.apply(R[k]) for k in range(16)])
+    diff = np.linalg.norm(Z - Zs, axis=1) / np.linalg.norm(Zs, axis=1)
+    assert diff.max() < (1e-12 if precision == "double" else 1e-6), diff.max()
+    if precision == "double":
+        O = orc.OracleSchwarz(st)
+        for k in (0, 7, 15):
+            zo = O.apply(R[k])
+            assert np.linalg.norm(Z[k] - zo) / np.linalg.norm(zo) < 1e-12
