@@ -226,7 +226,8 @@ struct s2l_ctx {
     std::vector<int64_t> off_h;
     DevBuf<int64_t> off_d, voff_d;
     DevBuf<int32_t> vrows_d, vrowptr_d, vrowg_d, blksub_d, blkrow_d, cperm_d;
-    DevBuf<double> V_d, T1_d, T2_d;
+    DevBuf<double> V_d, T1_d, T2_d;  // inverted factors of A00, row-major with leading dimension tri_ld
+    int32_t tri_ld = 0;
     bool coarse_perm = false;
 
     // work
@@ -528,8 +529,8 @@ void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* 
     }
     // x0 = R_0^T A_00^{-1} R_0 r
     launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, c->cpart.p, nrhs, skip, s);
-    launchTriGemv(c->n0, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
-    launchTriGemv(c->n0, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
+    launchTriGemv(c->n0, c->tri_ld, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
+    launchTriGemv(c->n0, c->tri_ld, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
     if (c->boolean) {
         launchCoarseProlong(c->ca, c->yvec.p, c->x0.p, c->n, 1, nrhs, skip, s);
     } else {
@@ -1475,8 +1476,18 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
                 c->coarse_perm = true;
             }
             for (double v : T1) require(std::isfinite(v), S2L_ENUMERIC, "A_00 factor is not finite");
-            c->T1_d.upload(T1, c->stream);
-            c->T2_d.upload(T2, c->stream);
+            // rows padded with zeros to a multiple of eight entries (64-byte aligned rows)
+            c->tri_ld = triLd(n0);
+            std::vector<double> Tp(static_cast<size_t>(n0) * c->tri_ld, 0.0);
+            for (int i = 0; i < n0; ++i)
+                std::copy(T1.begin() + static_cast<size_t>(i) * n0, T1.begin() + static_cast<size_t>(i + 1) * n0,
+                          Tp.begin() + static_cast<size_t>(i) * c->tri_ld);
+            c->T1_d.upload(Tp, c->stream);
+            HIPCHECK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < n0; ++i)
+                std::copy(T2.begin() + static_cast<size_t>(i) * n0, T2.begin() + static_cast<size_t>(i + 1) * n0,
+                          Tp.begin() + static_cast<size_t>(i) * c->tri_ld);
+            c->T2_d.upload(Tp, c->stream);
             HIPCHECK(hipStreamSynchronize(c->stream));
             CoarseArgs& a = c->ca;
             a.nsub = c->nsub;

@@ -96,7 +96,10 @@ constexpr int kMfmaMinRhs = 4;
 /// c = R_0 r; part: nrhs x nsub x rchunks x mmax partials of the first stage
 void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
                           const int32_t* skip, hipStream_t s);
-void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
+/// Leading dimension of the inverted A00 factors: n0 rounded up to a multiple of 8 (zero padding).
+inline int32_t triLd(int32_t n0) { return (n0 + 7) & ~7; }
+/// y = T x (perm: x[perm[k]]) for the lower or upper triangle of T (n0 x n0, leading dimension ld)
+void launchTriGemv(int32_t n0, int32_t ld, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
                    const int32_t* skip, hipStream_t s);
 void launchCoarseProlong(const CoarseArgs& c, const double* y, double* out, int64_t ld_out, int boolean, int nrhs,
                          const int32_t* skip, hipStream_t s);

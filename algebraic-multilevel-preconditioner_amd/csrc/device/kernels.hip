@@ -443,6 +443,46 @@ __global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const d
     y[off + i] = b ? b[off + i] - s : s;
 }
 
+// The same products for a group of up to kSpmvGroup right-hand sides per thread: the row's column
+// indices and values are read once for the group, and every right-hand side's sum keeps the order
+// and the fma of k_spmv (bitwise the same results).
+constexpr int kSpmvGroup = 8;
+__global__ __launch_bounds__(256) void k_spmv_group(int32_t n, const int32_t* rp, const int32_t* col, const double* val,
+                                                     const double* x, double* y, const double* b, int nrhs,
+                                                     const int32_t* skip)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r0 = blockIdx.y * kSpmvGroup;
+    bool on[kSpmvGroup];
+    double s[kSpmvGroup];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < kSpmvGroup; ++u) {
+        on[u] = r0 + u < nrhs && !(skip && skip[r0 + u]);
+        any = any || on[u];
+        s[u] = 0.0;
+    }
+    if (!any) return;
+    const double* xr = x + static_cast<int64_t>(r0) * n;
+    const int ke = rp[i + 1];
+    for (int k = rp[i]; k < ke; ++k) {
+        const double v = val[k];
+        const int c = col[k];
+        double xv[kSpmvGroup];
+#pragma unroll
+        for (int u = 0; u < kSpmvGroup; ++u) xv[u] = on[u] ? xr[static_cast<int64_t>(u) * n + c] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kSpmvGroup; ++u) s[u] = fma(v, xv[u], s[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kSpmvGroup; ++u) {
+        if (!on[u]) continue;
+        const int64_t o = static_cast<int64_t>(r0 + u) * n + i;
+        y[o] = b ? b[o] - s[u] : s[u];
+    }
+}
+
 // ------------------------------------------------------------------ coarse space
 constexpr int kCq = 8;       // coarse columns per pass of the restriction
 constexpr int kCrows = 256;  // rows of V_i per workgroup of the restriction (one per thread)
@@ -549,34 +589,65 @@ __global__ __launch_bounds__(kCrows) void k_coarse_restrict_mfma(CoarseArgs c, c
 }
 
 // y = T x for 16 right-hand sides at a time (T an inverted triangular factor of A00, row-major
-// n0 x n0 with explicit zeros; x rows permuted by perm when given): wavefront w of workgroup
-// (b, g) computes rows [64 b + 16 w, +16) over the columns the triangle holds.
-__global__ __launch_bounds__(256) void k_tri_gemm_mfma(int32_t n0, const double* T, int lower, const int32_t* perm,
-                                                        const double* x, double* y, int nrhs, const int32_t* skip)
+// n0 x n0 with explicit zeros and rows padded to ld, a multiple of eight; x rows permuted by perm
+// when given). Workgroup (t, g) computes rows [16 t, 16 t + 16) with its kTriWaves wavefronts
+// splitting the columns the triangle holds into blocks of eight (wavefront w takes blocks w,
+// w + kTriWaves, ...). A lane reads two consecutive entries of its row with one 16-byte load; they
+// feed the two products over the columns k = 8 b + 2 q and 8 b + 2 q + 1 (q = lane / 16), and the
+// matching entries of x go to the same lanes. The wavefronts' 16 x 16 results are summed in
+// wavefront order.
+constexpr int kTriWaves = 16;
+constexpr int kTriBatch = 4;  // blocks of eight columns per batch of loads
+template <bool kPerm>
+__global__ __launch_bounds__(kTriWaves * kWave) void k_tri_gemm_mfma(int32_t n0, int32_t ld, const double* T, int lower,
+                                                                      const int32_t* perm, const double* x, double* y,
+                                                                      int nrhs, const int32_t* skip)
 {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int r0 = blockIdx.x * 64 + wave * 16, g0 = blockIdx.y * 16;
-    if (r0 >= n0) return;
-    const int rhs = g0 + (lane & 15);
+    const int r0 = blockIdx.x * 16, g0 = blockIdx.y * 16;
+    const int rhs = g0 + (lane & 15), q = lane >> 4;
     const bool rok = rhs < nrhs;
     const double* xv = x + static_cast<int64_t>(rok ? rhs : 0) * n0;
-    const int row = r0 + (lane & 15);
-    const bool wok = row < n0;
-    const double* Trow = T + static_cast<int64_t>(wok ? row : 0) * n0;
-    const int kbeg = lower ? 0 : (r0 & ~3), kend = lower ? min(n0, r0 + 16) : n0;
+    const int row = min(r0 + (lane & 15), n0 - 1);  // rows past n0 repeat the last one; their sums are dropped
+    const double* Trow = T + static_cast<int64_t>(row) * ld;
+    const int b0 = lower ? 0 : (r0 >> 3);
+    const int b1 = lower ? ((min(n0, r0 + 16) + 7) >> 3) : (ld >> 3);
     mfmaAcc acc = {0.0, 0.0, 0.0, 0.0};
-    for (int kb = kbeg; kb < kend; kb += 4) {
-        const int k = kb + (lane >> 4);
-        const bool kok = k < kend;
-        const double a = (kok && wok) ? Trow[k] : 0.0;
-        const double b = (kok && rok) ? xv[perm ? perm[k] : k] : 0.0;
-        acc = mfma4(a, b, acc);
-    }
-    if (!rok || (skip && skip[rhs])) return;
+    for (int bb = b0 + wave; bb < b1; bb += kTriWaves * kTriBatch) {
+        double2 a[kTriBatch];
+        double xa[kTriBatch], xb[kTriBatch];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int rw = r0 + (lane >> 4) + 4 * j;
-        if (rw < n0) y[static_cast<int64_t>(rhs) * n0 + rw] = acc[j];
+        for (int u = 0; u < kTriBatch; ++u) {
+            const int b = bb + u * kTriWaves;
+            const int k = 8 * b + 2 * q;
+            const bool ok = b < b1;
+            a[u] = ok ? *reinterpret_cast<const double2*>(Trow + k) : double2{0.0, 0.0};
+            const bool oka = ok && rok && k < n0, okb = ok && rok && k + 1 < n0;
+            if (kPerm) {
+                xa[u] = oka ? xv[perm[k]] : 0.0;
+                xb[u] = okb ? xv[perm[k + 1]] : 0.0;
+            } else {
+                xa[u] = oka ? xv[k] : 0.0;
+                xb[u] = okb ? xv[k + 1] : 0.0;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kTriBatch; ++u) {
+            acc = mfma4(a[u].x, xa[u], acc);
+            acc = mfma4(a[u].y, xb[u], acc);
+        }
+    }
+    __shared__ double red[kTriWaves][256];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[wave][(q + 4 * j) * 16 + (lane & 15)] = acc[j];
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        const int u = threadIdx.x, rw = r0 + u / 16, rr = g0 + (u & 15);
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTriWaves; ++w) sum += red[w][u];
+        if (rw < n0 && rr < nrhs && !(skip && skip[rr])) y[static_cast<int64_t>(rr) * n0 + rw] = sum;
     }
 }
 
@@ -595,16 +666,31 @@ __global__ __launch_bounds__(256) void k_coarse_prolong_mfma(CoarseArgs c, const
     const bool rok = rhs < nrhs;
     const double* yv = y + static_cast<int64_t>(rok ? rhs : 0) * c.n0 + c.off[i];
     const bool write = rok && !(skip && skip[rhs]);
+    // K = m <= 64 in at most 16 steps of four: the entries of y first, then V eight steps at a time
+    double bv[16];
+#pragma unroll
+    for (int st = 0; st < 16; ++st) {
+        const int q = 4 * st + (lane >> 4);
+        bv[st] = (q < m && rok) ? yv[q] : 0.0;
+    }
+    const int nst = (m + 3) / 4;
     for (int t = 0; t < 4; ++t) {
         const int l0 = c.blk_row[b] + wave * 64 + t * 16;
         if (l0 >= rows) break;
         const int row = l0 + (lane & 15);
         mfmaAcc acc = {0.0, 0.0, 0.0, 0.0};
-        for (int kb = 0; kb < m; kb += 4) {
-            const int q = kb + (lane >> 4);
-            const double a = (q < m && row < rows) ? V[static_cast<int64_t>(q) * rows + row] : 0.0;
-            const double bb = (q < m && rok) ? yv[q] : 0.0;
-            acc = mfma4(a, bb, acc);
+#pragma unroll
+        for (int s0 = 0; s0 < 16; s0 += 8) {
+            if (s0 >= nst) break;
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = 4 * (s0 + u) + (lane >> 4);
+                av[u] = (q < m && row < rows) ? V[static_cast<int64_t>(q) * rows + row] : 0.0;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = mfma4(av[u], bv[s0 + u], acc);
         }
         if (!write) continue;
 #pragma unroll
@@ -634,7 +720,7 @@ __global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, c
     }
 }
 
-__global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, const double* Tm, int lower, const int32_t* perm,
+__global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const double* Tm, int lower, const int32_t* perm,
                                                    const double* x, double* y, const int32_t* skip)
 {
     const int rr = blockIdx.y;
@@ -644,7 +730,7 @@ __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, const double* Tm, 
     if (i >= n0) return;
     const double* xv = x + static_cast<int64_t>(rr) * n0;
     const int lo = lower ? 0 : i, hi = lower ? i + 1 : n0;
-    const double* row = Tm + static_cast<int64_t>(i) * n0;
+    const double* row = Tm + static_cast<int64_t>(i) * ld;
     double s = 0.0;
     for (int j = lo + lane; j < hi; j += kWave) s += row[j] * xv[perm ? perm[j] : j];
     s = waveSum(s);
@@ -1128,7 +1214,11 @@ template const void* localSolveKernel<double>();
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spmv, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, rp, col, val, x, y, b, skip);
+    if (nrhs >= 2)
+        hipLaunchKernelGGL(k_spmv_group, dim3(nblk(n, 256), (nrhs + kSpmvGroup - 1) / kSpmvGroup), dim3(256), 0, s, n, rp, col,
+                           val, x, y, b, nrhs, skip);
+    else
+        hipLaunchKernelGGL(k_spmv, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, rp, col, val, x, y, b, skip);
 }
 
 void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, double* cvec, double* part, int nrhs,
@@ -1142,14 +1232,18 @@ void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, doubl
     hipLaunchKernelGGL(k_coarse_final, dim3(c.nsub, nrhs), dim3(64), 0, s, c, part, cvec, skip);
 }
 
-void launchTriGemv(int32_t n0, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
+void launchTriGemv(int32_t n0, int32_t ld, const double* T, int lower, const int32_t* perm, const double* x, double* y, int nrhs,
                    const int32_t* skip, hipStream_t s)
 {
-    if (nrhs >= kMfmaMinRhs)
-        hipLaunchKernelGGL(k_tri_gemm_mfma, dim3(nblk(n0, 64), (nrhs + 15) / 16), dim3(256), 0, s, n0, T, lower, perm, x, y, nrhs,
-                           skip);
-    else
-        hipLaunchKernelGGL(k_tri_gemv, dim3(nblk(n0, 4), nrhs), dim3(256), 0, s, n0, T, lower, perm, x, y, skip);
+    if (nrhs >= kMfmaMinRhs) {
+        const dim3 grid(nblk(n0, 16), (nrhs + 15) / 16), block(kTriWaves * kWave);
+        if (perm)
+            hipLaunchKernelGGL(k_tri_gemm_mfma<true>, grid, block, 0, s, n0, ld, T, lower, perm, x, y, nrhs, skip);
+        else
+            hipLaunchKernelGGL(k_tri_gemm_mfma<false>, grid, block, 0, s, n0, ld, T, lower, perm, x, y, nrhs, skip);
+    } else {
+        hipLaunchKernelGGL(k_tri_gemv, dim3(nblk(n0, 4), nrhs), dim3(256), 0, s, n0, ld, T, lower, perm, x, y, skip);
+    }
 }
 
 void launchCoarseProlong(const CoarseArgs& c, const double* y, double* out, int64_t ld_out, int boolean, int nrhs,

@@ -196,3 +196,42 @@ def test_many_rhs_coarse_products_on_matrix_cores(precision):
         for k in (0, 7, 15):
             zo = O.apply(R[k])
             assert np.linalg.norm(Z[k] - zo) / np.linalg.norm(zo) < 1e-12
+
+
+@pytest.mark.parametrize("name,kw,N,nev,nrhs", [("convdiff2d", dict(nx=32), 8, 9, 16), ("convdiff2d", dict(nx=32), 8, 9, 5),
+                                                ("poisson3d", dict(nx=12), 6, 13, 16)])
+def test_many_rhs_coarse_odd_sizes(name, kw, N, nev, nrhs):
+    """Coarse dimensions that are not multiples of 8 or 16 (zero-padded rows of the inverted A00
+    factors, a last partial tile of rows), a partial group of right-hand sides, and a nonsymmetric
+    A00 (LU with row pivoting: the first product gathers x in pivot order): the batched FP64 apply
+    against single applies (1e-12) and the oracle (1e-12)."""
+    from amp_amd import device
+    st = _setup(name, kw, N, nev=nev)
+    print(f"case {name} {kw} N={N} nev={nev} nrhs={nrhs}", flush=True)
+    M = device.HipTwoLevelSchwarz(st, "double")
+    assert M.ctx.stats().n0 % 8 != 0
+    rng = np.random.default_rng(11)
+    R = rng.standard_normal((nrhs, st.n))
+    Z = M.ctx.apply(R)
+    Zs = np.stack([M.ctx.apply(R[k]) for k in range(nrhs)])
+    diff = np.linalg.norm(Z - Zs, axis=1) / np.linalg.norm(Zs, axis=1)
+    assert diff.max() < 1e-12, diff.max()
+    O = orc.OracleSchwarz(st)
+    for k in (0, nrhs - 1):
+        zo = O.apply(R[k])
+        assert np.linalg.norm(Z[k] - zo) / np.linalg.norm(zo) < 1e-12
+
+
+@pytest.mark.parametrize("nrhs", [2, 8, 13])
+def test_spmv_groups_bitwise(nrhs):
+    """Several right-hand sides share each row's reads of A (groups of eight per thread) and give the
+    same bits as one product at a time and as the oracle."""
+    from amp_amd import device
+    st = _setup("convdiff2d", dict(nx=24), 4, nev=4)
+    M = device.HipTwoLevelSchwarz(st, "double")
+    X = np.stack([np.sin(np.linspace(-1.0, 2.0 + k, st.n)) for k in range(nrhs)])
+    Y = M.ctx.spmv(X)
+    O = orc.OracleSchwarz(st)
+    for k in range(nrhs):
+        assert np.array_equal(Y[k], M.ctx.spmv(X[k]))
+        assert np.array_equal(Y[k], O.spmv(X[k]))
