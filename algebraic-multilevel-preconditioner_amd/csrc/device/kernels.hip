@@ -17,6 +17,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kPullSplit = 4;  // a backward chunk of at most blockDim / kPullSplit columns splits its rows below
+constexpr int kMaxGroups = 32; // row groups of such a split
 
 template <typename T> __device__ inline T waveSum(T v)
 {
@@ -24,45 +25,72 @@ template <typename T> __device__ inline T waveSum(T v)
     return v;
 }
 
-// sum over k < n of p[k * ps] * q[k * qs], in order k = 0, 1, ..., with the loads issued eight at
-// a time ahead of their products: the scheduling barrier keeps the compiler from pulling a
-// batch's loads down next to their uses, which leaves one memory round trip per element.
+// Global loads issued per batch by the dot products of the local solve: each batch is one memory
+// round trip, so a thread's dot product of length n costs ceil(n / 8) dependent round trips. The
+// remainder is one predicated batch rather than a loop with a round trip per element.
+constexpr int kB = 8;
+
+// sum over k < n of p[k * ps] * q[k * qs], in order k = 0, 1, ..., with the loads of a batch issued
+// together ahead of their products: the scheduling barrier keeps the compiler from pulling the
+// loads down next to their uses.
 template <typename T> __device__ inline T dot8(const T* p, int ps, const T* q, int qs, int n, T acc)
 {
     int k = 0;
-    for (; k + 8 <= n; k += 8) {
-        T a[8], b[8];
+    for (; k + kB <= n; k += kB) {
+        T a[kB], b[kB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = p[(k + u) * ps];
+        for (int u = 0; u < kB; ++u) a[u] = p[(k + u) * ps];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) b[u] = q[(k + u) * qs];
+        for (int u = 0; u < kB; ++u) b[u] = q[(k + u) * qs];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += a[u] * b[u];
+        for (int u = 0; u < kB; ++u) acc += a[u] * b[u];
     }
-    for (; k < n; ++k) acc += p[k * ps] * q[k * qs];
+    if (k < n) {
+        T a[kB - 1], b[kB - 1];
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u) a[u] = k + u < n ? p[(k + u) * ps] : T(0);
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u) b[u] = k + u < n ? q[(k + u) * qs] : T(0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u)
+            if (k + u < n) acc += a[u] * b[u];
+    }
     return acc;
 }
 
-// sum over k < n of p[k * ps] * x[r[k * rs]] (x in LDS), in order, with the loads of p and r issued
-// eight at a time ahead of the products.
+// sum over k < n of p[k * ps] * x[r[k * rs]] (x in LDS), in order, with the loads of p and r of a
+// batch issued together ahead of the products.
 template <typename T> __device__ inline T gatherDot8(const T* p, int ps, const int32_t* r, int rs, const T* x, int n)
 {
     T acc = T(0);
     int k = 0;
-    for (; k + 8 <= n; k += 8) {
-        T a[8];
-        int32_t i[8];
+    for (; k + kB <= n; k += kB) {
+        T a[kB];
+        int32_t i[kB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kB; ++u) {
             a[u] = p[(k + u) * ps];
             i[u] = r[(k + u) * rs];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += a[u] * x[i[u]];
+        for (int u = 0; u < kB; ++u) acc += a[u] * x[i[u]];
     }
-    for (; k < n; ++k) acc += p[k * ps] * x[r[k * rs]];
+    if (k < n) {
+        T a[kB - 1];
+        int32_t i[kB - 1];
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u) {
+            a[u] = k + u < n ? p[(k + u) * ps] : T(0);
+            i[u] = k + u < n ? r[(k + u) * rs] : 0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u)
+            if (k + u < n) acc += a[u] * x[i[u]];
+    }
     return acc;
 }
 
@@ -73,15 +101,23 @@ template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const S
     T v = x[row];
     const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
     int q = cb;
-    for (; q + 8 <= ce; q += 8) {
-        T t[8];
+    for (; q + kB <= ce; q += kB) {
+        T t[kB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) t[u] = tb[q + u];
+        for (int u = 0; u < kB; ++u) t[u] = tb[q + u];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v -= t[u];
+        for (int u = 0; u < kB; ++u) v -= t[u];
     }
-    for (; q < ce; ++q) v -= tb[q];
+    if (q < ce) {
+        T t[kB - 1];
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u) t[u] = q + u < ce ? tb[q + u] : T(0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u)
+            if (q + u < ce) v -= t[u];
+    }
     return v;
 }
 
@@ -212,7 +248,7 @@ __device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, in
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave, ld = wideLd(w);
     if (h > 0) {
-        const int G = max(1, min(nt / w, kChunkRows / w));
+        const int G = max(1, min(min(nt / w, kChunkRows / w), min(h, kMaxGroups)));
         const T* B = a.below + s.below;
         const int32_t* R = a.ridx + s.ridx;
         for (int u = tid; u < G * w; u += nt) {
@@ -268,6 +304,8 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             hp[i] = a.lev_hpre[q0 + i] - hbase;
         }
         __syncthreads();
+        // The first entries of each thread's first row (t = tid) of the inverted diagonal blocks
+        // are loaded before the pull phase, so that their round trip overlaps it.
         if (forward) {
             for (int t = tid; t < W; t += nt) {
                 const int i = findItem(wp, cnt, t);
@@ -284,7 +322,8 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
         } else {
             // a narrow chunk (few columns, long columns below): G groups of W threads, thread
             // (g, t) sums the rows g, g + G, ... below column t; partial sums combined in group order
-            const int G = min(nt / W, kChunkRows / W);
+            // (G at most the rows below and kMaxGroups, which bound the serial combine)
+            const int G = max(1, min(min(nt / W, kChunkRows / W), min(H, kMaxGroups)));
             for (int u = tid; u < G * W; u += nt) {
                 const int g = u / W, t = u - g * W;
                 const int i = findItem(wp, cnt, t);
@@ -306,9 +345,9 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             const int i = findItem(wp, cnt, t);
             const SnDesc& s = st[i];
             const int j = t - wp[i];
-            const T* D = a.diag + s.diag;
             const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
-            ybuf[t] = dot8(D + lo * s.w + j, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
+            const T* D = a.diag + s.diag + lo * s.w + j;
+            ybuf[t] = dot8(D, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
         }
         __syncthreads();
         for (int t = tid; t < W; t += nt) {
