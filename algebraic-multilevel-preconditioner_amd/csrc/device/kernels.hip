@@ -304,8 +304,6 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             hp[i] = a.lev_hpre[q0 + i] - hbase;
         }
         __syncthreads();
-        // The first entries of each thread's first row (t = tid) of the inverted diagonal blocks
-        // are loaded before the pull phase, so that their round trip overlaps it.
         if (forward) {
             for (int t = tid; t < W; t += nt) {
                 const int i = findItem(wp, cnt, t);
