@@ -304,15 +304,18 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             hp[i] = a.lev_hpre[q0 + i] - hbase;
         }
         __syncthreads();
+        // the supernode of each thread's first row (t = tid) and first row below, found once
+        const int iw = tid < W ? findItem(wp, cnt, tid) : 0;
+        const int ih = forward && tid < H ? findItem(hp, cnt, tid) : 0;
         if (forward) {
             for (int t = tid; t < W; t += nt) {
-                const int i = findItem(wp, cnt, t);
+                const int i = t == tid ? iw : findItem(wp, cnt, t);
                 const int row = st[i].c0 + (t - wp[i]);
                 x[row] = pullRow(a, sd, tb, x, row);
             }
         } else if (W * kPullSplit > nt) {
             for (int t = tid; t < W; t += nt) {
-                const int i = findItem(wp, cnt, t);
+                const int i = t == tid ? iw : findItem(wp, cnt, t);
                 const SnDesc& s = st[i];
                 const int c = t - wp[i];
                 if (s.h > 0) x[s.c0 + c] -= gatherDot8(a.below + s.below + c, s.w, a.ridx + s.ridx, 1, x, s.h);
@@ -332,7 +335,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             }
             __syncthreads();
             for (int t = tid; t < W; t += nt) {
-                const int i = findItem(wp, cnt, t);
+                const int i = t == tid ? iw : findItem(wp, cnt, t);
                 T sum = T(0);
                 for (int g = 0; g < G; ++g) sum += ybuf[g * W + t];
                 x[st[i].c0 + (t - wp[i])] -= sum;
@@ -340,7 +343,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
         }
         __syncthreads();
         for (int t = tid; t < W; t += nt) {
-            const int i = findItem(wp, cnt, t);
+            const int i = t == tid ? iw : findItem(wp, cnt, t);
             const SnDesc& s = st[i];
             const int j = t - wp[i];
             const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
@@ -349,13 +352,13 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
         }
         __syncthreads();
         for (int t = tid; t < W; t += nt) {
-            const int i = findItem(wp, cnt, t);
+            const int i = t == tid ? iw : findItem(wp, cnt, t);
             x[st[i].c0 + (t - wp[i])] = ybuf[t];
         }
         __syncthreads();
         if (forward) {
             for (int t = tid; t < H; t += nt) {
-                const int i = findItem(hp, cnt, t);
+                const int i = t == tid ? ih : findItem(hp, cnt, t);
                 const SnDesc& s = st[i];
                 const int k = t - hp[i];
                 const T* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
