@@ -177,38 +177,61 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
 template <typename T, int kTri, typename Out>
 __device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
 {
+    // FP64: each pass takes two groups of R rows, nw * R rows apart, whose loads are issued
+    // together (FP32 has no registers left for the second group)
     constexpr int R = blockRows<T>();
-    for (int r0 = wave * R; r0 < rows; r0 += nw * R) {
-        const int rn = min(R, rows - r0);
-        T acc[R];
+    constexpr int P = sizeof(T) == 8 ? 2 : 1;
+    for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
+        const int rA = r0, rB = r0 + nw * R;
+        const int nA = min(R, rows - rA), nB = P == 2 ? max(0, min(R, rows - rB)) : 0;
+        T accA[R], accB[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] = T(0);
+        for (int r = 0; r < R; ++r) accA[r] = accB[r] = T(0);
         for (int cb = 0; cb < cols; cb += 4 * kWave) {
             const int c = cb + 4 * lane;
-            bool need = c < cols;
-            if (kTri == 1) need = need && c <= r0 + rn - 1;
-            if (kTri == 2) need = need && c + 3 >= r0;
-            T xv[4], m[R][4];
+            bool needA = c < cols, needB = c < cols && nB > 0;
+            if (kTri == 1) {
+                needA = needA && c <= rA + nA - 1;
+                needB = needB && c <= rB + nB - 1;
+            }
+            if (kTri == 2) {
+                needA = needA && c + 3 >= rA;
+                needB = needB && c + 3 >= rB;
+            }
+            T xv[4], mA[R][4], mB[R][4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) xv[i] = need && c + i < cols ? xs[c + i] : T(0);
+            for (int i = 0; i < 4; ++i) xv[i] = c + i < cols ? xs[c + i] : T(0);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if (need && r < rn) {
-                    load4(M + static_cast<int64_t>(r0 + r) * ld + c, m[r]);
+                if (needA && r < nA) {
+                    load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[r]);
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) m[r][i] = T(0);
+                    for (int i = 0; i < 4; ++i) mA[r][i] = T(0);
+                }
+                if (needB && r < nB) {
+                    load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[r]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) mB[r][i] = T(0);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[r] += m[r][i] * xv[i];
+                for (int i = 0; i < 4; ++i) {
+                    accA[r] += mA[r][i] * xv[i];
+                    accB[r] += mB[r][i] * xv[i];
+                }
         }
-        const T y = transposeSum<T, R>(acc, lane);
-        const int r = r0 + lane / (kWave / R);
-        if (lane % (kWave / R) == 0 && r < rows) out(r, y);
+        const T yA = transposeSum<T, R>(accA, lane);
+        const T yB = transposeSum<T, R>(accB, lane);
+        const int rr = lane / (kWave / R);
+        if (lane % (kWave / R) == 0) {
+            if (rA + rr < rows) out(rA + rr, yA);
+            if (rB + rr < rows && nB > 0) out(rB + rr, yB);
+        }
     }
 }
 
