@@ -502,7 +502,24 @@ __global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const d
     const int64_t off = static_cast<int64_t>(r) * n;
     const double* xr = x + off;
     double s = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) s = fma(val[k], xr[col[k]], s);
+    // the row's entries eight at a time: columns and values in one round trip, x in the next
+    const int ke = rp[i + 1];
+    for (int k0 = rp[i]; k0 < ke; k0 += 8) {
+        int c[8];
+        double v[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            c[u] = k0 + u < ke ? col[k0 + u] : 0;
+            v[u] = k0 + u < ke ? val[k0 + u] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = k0 + u < ke ? xr[c[u]] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + u < ke) s = fma(v[u], xv[u], s);
+    }
     y[off + i] = b ? b[off + i] - s : s;
 }
 
@@ -547,45 +564,61 @@ __global__ __launch_bounds__(256) void k_spmv_group(int32_t n, const int32_t* rp
 }
 
 // ------------------------------------------------------------------ coarse space
-constexpr int kCq = 8;       // coarse columns per pass of the restriction
 constexpr int kCrows = 256;  // rows of V_i per workgroup of the restriction (one per thread)
 
 
-// c_i = V_i^T r(rows of V_i), first stage: workgroup (i, chunk) sums its 256 rows for every column.
-__global__ __launch_bounds__(kCrows) void k_coarse_restrict(CoarseArgs c, const double* r, int64_t n, double* part,
-                                                             const int32_t* skip)
+// c_i = V_i^T r for fewer right-hand sides than the matrix-core path takes, in one stage:
+// workgroup (i, group, rhs) owns kCg coarse columns of subdomain i over all its rows (no partial
+// sums across workgroups). Each thread sums its rows (tid, tid + blockDim, ...) in order, the
+// wavefronts' sums are combined in wavefront order.
+constexpr int kCg = 8;           // coarse columns per workgroup
+constexpr int kCgThreads = 512;
+__global__ __launch_bounds__(kCgThreads) void k_coarse_restrict_cols(CoarseArgs c, const double* r, int64_t n, double* cvec,
+                                                                   const int32_t* skip)
 {
     const int rr = blockIdx.z;
     if (skip && skip[rr]) return;
-    const int i = blockIdx.x, ch = blockIdx.y;
-    const int rows = c.vrows[i];
-    const int nch = (rows + kCrows - 1) / kCrows;
-    if (ch >= nch) return;
+    const int i = blockIdx.x, q0 = blockIdx.y * kCg;
     const int m = static_cast<int>(c.off[i + 1] - c.off[i]);
-    const double* V = c.V + c.voff[i];
+    if (q0 >= m) return;
+    const int rows = c.vrows[i];
+    const double* V = c.V + c.voff[i] + static_cast<int64_t>(q0) * rows;
     const int32_t* g = c.vrow_g + c.vrow_ptr[i];
     const double* rv = r + static_cast<int64_t>(rr) * n;
-    const int l = ch * kCrows + threadIdx.x;
-    const double xv = l < rows ? rv[g[l]] : 0.0;
-    __shared__ double red[kCrows / kWave][kCq];
+    const int nq = min(kCg, m - q0);
+    double acc[kCg];
+#pragma unroll
+    for (int q = 0; q < kCg; ++q) acc[q] = 0.0;
+    // two rows per batch: their map entries, then r and the V entries
+    for (int l0 = threadIdx.x; l0 < rows; l0 += 2 * kCgThreads) {
+        const int l1 = l0 + kCgThreads;
+        const int g0 = g[l0], g1 = l1 < rows ? g[l1] : 0;
+        double va[kCg], vb[kCg];
+#pragma unroll
+        for (int q = 0; q < kCg; ++q) {
+            va[q] = q < nq ? V[static_cast<int64_t>(q) * rows + l0] : 0.0;
+            vb[q] = q < nq && l1 < rows ? V[static_cast<int64_t>(q) * rows + l1] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const double x0 = rv[g0], x1 = l1 < rows ? rv[g1] : 0.0;
+#pragma unroll
+        for (int q = 0; q < kCg; ++q) {
+            acc[q] += va[q] * x0;
+            if (l1 < rows) acc[q] += vb[q] * x1;
+        }
+    }
+    __shared__ double red[kCgThreads / kWave][kCg];
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    double* pr = part + ((static_cast<int64_t>(rr) * c.nsub + i) * c.rchunks + ch) * c.mmax;
-    for (int cq = 0; cq < m; cq += kCq) {
-        double acc[kCq];
 #pragma unroll
-        for (int q = 0; q < kCq; ++q) acc[q] = (l < rows && cq + q < m) ? V[static_cast<int64_t>(cq + q) * rows + l] * xv : 0.0;
-#pragma unroll
-        for (int q = 0; q < kCq; ++q) {
-            const double v = waveSum(acc[q]);
-            if (lane == 0) red[wave][q] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x < kCq && cq + threadIdx.x < m) {
-            double t = 0.0;
-            for (int wv = 0; wv < kCrows / kWave; ++wv) t += red[wv][threadIdx.x];
-            pr[cq + threadIdx.x] = t;
-        }
-        __syncthreads();
+    for (int q = 0; q < kCg; ++q) {
+        const double v = waveSum(acc[q]);
+        if (lane == 0) red[wave][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        double t = 0.0;
+        for (int w = 0; w < kCgThreads / kWave; ++w) t += red[w][threadIdx.x];
+        cvec[static_cast<int64_t>(rr) * c.n0 + c.off[i] + q0 + threadIdx.x] = t;
     }
 }
 
@@ -783,6 +816,9 @@ __global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, c
     }
 }
 
+// y = T x for one right-hand side per grid row: one wavefront per row of the triangle, a lane
+// reading two consecutive entries with one 16-byte load (rows padded to ld, a multiple of eight,
+// with zeros), four such loads per batch; lane partial sums, then the wavefront sum.
 __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const double* Tm, int lower, const int32_t* perm,
                                                    const double* x, double* y, const int32_t* skip)
 {
@@ -792,10 +828,24 @@ __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const 
     const int i = blockIdx.x * (blockDim.x / kWave) + wave;
     if (i >= n0) return;
     const double* xv = x + static_cast<int64_t>(rr) * n0;
-    const int lo = lower ? 0 : i, hi = lower ? i + 1 : n0;
+    const int lo = lower ? 0 : (i & ~1), hi = lower ? i + 1 : n0;  // even start: 16-byte aligned
     const double* row = Tm + static_cast<int64_t>(i) * ld;
     double s = 0.0;
-    for (int j = lo + lane; j < hi; j += kWave) s += row[j] * xv[perm ? perm[j] : j];
+    for (int j0 = lo; j0 < hi; j0 += 8 * kWave) {
+        double2 t[4];
+        double xa[4], xb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + 2 * lane + 2 * kWave * u;
+            const bool ok = j < hi;
+            t[u] = ok ? *reinterpret_cast<const double2*>(row + j) : double2{0.0, 0.0};
+            xa[u] = ok ? xv[perm ? perm[j] : j] : 0.0;
+            xb[u] = ok && j + 1 < n0 ? xv[perm ? perm[j + 1] : j + 1] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += t[u].x * xa[u] + t[u].y * xb[u];
+    }
     s = waveSum(s);
     if (lane == 0) y[static_cast<int64_t>(rr) * n0 + i] = s;
 }
@@ -814,7 +864,17 @@ __global__ __launch_bounds__(256) void k_coarse_prolong(CoarseArgs c, const doub
     const double* V = c.V + c.voff[i];
     const double* yv = y + static_cast<int64_t>(rr) * c.n0 + c.off[i];
     double s = 0.0;
-    for (int k = 0; k < m; ++k) s += V[static_cast<int64_t>(k) * rows + l] * yv[k];
+    for (int k0 = 0; k0 < m; k0 += 8) {
+        double v[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = k0 + u < m ? V[static_cast<int64_t>(k0 + u) * rows + l] : 0.0;
+            w[u] = k0 + u < m ? yv[k0 + u] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u] * w[u];
+    }
     if (boolean)
         out[static_cast<int64_t>(rr) * ld_out + c.vrow_g[c.vrow_ptr[i] + l]] = s;
     else
@@ -1290,8 +1350,11 @@ void launchCoarseRestrict(const CoarseArgs& c, const double* r, int64_t n, doubl
     if (nrhs >= kMfmaMinRhs && c.mmax <= 16 * kMfmaTiles)
         hipLaunchKernelGGL(k_coarse_restrict_mfma, dim3(c.nsub, c.rchunks, (nrhs + 15) / 16), dim3(kCrows), 0, s, c, r, n, part,
                            nrhs, skip);
-    else
-        hipLaunchKernelGGL(k_coarse_restrict, dim3(c.nsub, c.rchunks, nrhs), dim3(kCrows), 0, s, c, r, n, part, skip);
+    else {
+        hipLaunchKernelGGL(k_coarse_restrict_cols, dim3(c.nsub, (c.mmax + kCg - 1) / kCg, nrhs), dim3(kCgThreads), 0, s, c, r,
+                           n, cvec, skip);
+        return;
+    }
     hipLaunchKernelGGL(k_coarse_final, dim3(c.nsub, nrhs), dim3(64), 0, s, c, part, cvec, skip);
 }
 
