@@ -16,7 +16,7 @@
 #include "device.hpp"
 
 namespace s2l {
-template <typename T> const void* localSolveKernel();
+template <typename T, typename TF> const void* localSolveKernel();
 }
 
 namespace {
@@ -288,7 +288,7 @@ struct s2l_ctx {
         tvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         yvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         cpart.reserve(static_cast<size_t>(std::max(nsub, 1)) * std::max(ca.rchunks, 1) * std::max(ca.mmax, 1) * nrhs);
-        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
+        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type
         tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nrhs * esz);
         xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         ls_slots.reserve(static_cast<size_t>(std::max<int64_t>(ls.slots, 1)) * nrhs * esz);
@@ -330,10 +330,10 @@ hipStream_t pick(s2l_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s)
 
 // ---- local solve launch: the persistent work-queue kernel over every subdomain (mode 0) or one
 //      launch per subdomain over its slice of the queue (mode 1)
-template <typename T> LocalArgs<T> localArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs,
-                                             const int32_t* skip)
+template <typename T, typename TF>
+LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip)
 {
-    LocalArgs<T> a{};
+    LocalArgs<T, TF> a{};
     a.sub = c->sub_d.p;
     a.sn = c->sn_d.p;
     a.lev = c->lev_d.p;
@@ -343,7 +343,7 @@ template <typename T> LocalArgs<T> localArgs(s2l_ctx* c, const double* rhs, cons
     a.lev_chunk = c->levc_d.p;
     a.y_off = static_cast<int32_t>(c->y_off);
     a.st_off = static_cast<int32_t>(c->st_off);
-    if constexpr (std::is_same_v<T, float>) {
+    if constexpr (std::is_same_v<TF, float>) {
         a.diag = c->diag_f.p;
         a.below = c->below_f.p;
     } else {
@@ -376,15 +376,15 @@ template <typename T> LocalArgs<T> localArgs(s2l_ctx* c, const double* rhs, cons
 
 // One launch over @p count queue entries (x nrhs): a grid of one workgroup per entry, or a
 // single workgroup running them in order.
-template <typename T>
-void launchQueue(s2l_ctx* c, LocalArgs<T>& a, const int32_t* q, int32_t count, bool serial, hipStream_t s)
+template <typename T, typename TF>
+void launchQueue(s2l_ctx* c, LocalArgs<T, TF>& a, const int32_t* q, int32_t count, bool serial, hipStream_t s)
 {
     if (count <= 0) return;
     a.queue = q;
     a.qcount = count;
     a.serial = serial ? 1 : 0;
     const int grid = serial ? 1 : count * a.nrhs;
-    launchLocalSolve<T>(a, grid, c->lds_bytes, s);
+    launchLocalSolve<T, TF>(a, grid, c->lds_bytes, s);
 }
 
 template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode,
@@ -451,11 +451,11 @@ template <typename T> void lsLaunches(s2l_ctx* c, const LsArgs<T>& a, int nrhs, 
     }
 }
 
-template <typename T>
+template <typename T, typename TF>
 void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip,
                 hipStream_t s)
 {
-    LocalArgs<T> a = localArgs<T>(c, rhs, x0, out, mode, nrhs, skip);
+    LocalArgs<T, TF> a = localArgs<T, TF>(c, rhs, x0, out, mode, nrhs, skip);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         if (c->ev_pool.size() < 2) {
@@ -472,16 +472,20 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
         HIPCHECK(hipEventRecord(e0, s));
     }
     if (c->launch_mode == 2) {
-        // level-synchronous: one launch per level of all subdomains' trees
-        lsLaunches<T>(c, lsArgs<T>(c, rhs, x0, out, mode, skip), nrhs, s);
+        // level-synchronous: one launch per level of all subdomains' trees (its factors are stored
+        // in the arithmetic type)
+        if constexpr (std::is_same_v<T, TF>)
+            lsLaunches<T>(c, lsArgs<T>(c, rhs, x0, out, mode, skip), nrhs, s);
+        else
+            require(false, S2L_ESTATE, "the level-synchronous schedule has no FP32-storage / FP64-arithmetic mode");
     } else if (c->launch_mode == 0) {
         // level-ordered queue: one launch per depth group, every subdomain batched
         for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g)
-            launchQueue<T>(c, a, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
+            launchQueue<T, TF>(c, a, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
     } else {
         for (int i = 0; i < c->nsub; ++i)
-            launchQueue<T>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
-                           c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, s);
+            launchQueue<T, TF>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
+                               c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, s);
     }
     HIPCHECK(hipGetLastError());
     if (c->timing) {
@@ -512,9 +516,11 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
     const int mode = direct ? 0 : 1;
     double* out = direct ? z : c->loc.p;
     if (c->precision == 1)
-        localSolve<float>(c, rhs, x0, out, mode, nrhs, skip, s);
+        localSolve<float, float>(c, rhs, x0, out, mode, nrhs, skip, s);
+    else if (c->precision == 2)
+        localSolve<double, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else
-        localSolve<double>(c, rhs, x0, out, mode, nrhs, skip, s);
+        localSolve<double, double>(c, rhs, x0, out, mode, nrhs, skip, s);
     if (!direct) launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, x0, z, nrhs, skip, s);
 }
 
@@ -1291,12 +1297,13 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
     return guarded(c, [&] {
         if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
-        require(precision == 0 || precision == 1, S2L_EINVAL, "precision must be 0 (FP64) or 1 (FP32)");
+        require(precision >= 0 && precision <= 2, S2L_EINVAL,
+                "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 factors, FP64 arithmetic)");
         for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
         HIPCHECK(hipSetDevice(c->device));
         std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
-        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);
-        // level-synchronous schedule (default launch mode)
+        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type (LDS)
+        // level-synchronous schedule (launch mode 2)
         c->ls = buildLsPlan(c->hf, c->local_ptr);
         {
             LsPlan& L = c->ls;
@@ -1315,7 +1322,7 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
             c->ls_owned.upload(owned, c->stream);
             c->ls_begin.upload(begin, c->stream);
             auto values = [&](DevBuf<unsigned char>& dst, std::vector<double>& src) {
-                if (precision == 1) {
+                if (precision != 0) {
                     std::vector<float> f(src.begin(), src.end());
                     dst.upload(reinterpret_cast<const unsigned char*>(f.data()), f.size() * sizeof(float), c->stream);
                     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1344,9 +1351,13 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->lds_bytes = c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
         c->queue_ok = c->lds_bytes <= c->max_lds;
         if (c->queue_ok) {
-            const void* kern = precision == 1 ? localSolveKernel<float>() : localSolveKernel<double>();
+            const void* kern = precision == 1   ? localSolveKernel<float, float>()
+                               : precision == 2 ? localSolveKernel<double, float>()
+                                                : localSolveKernel<double, double>();
             HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
         } else if (c->launch_mode != 2) {
+            require(precision != 2, S2L_ELIMIT,
+                    "FP32 factors with FP64 arithmetic need the task queue, whose subdomain vector does not fit the LDS");
             c->launch_mode = 2;  // the task queue cannot hold these subdomains in LDS
         }
         // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
@@ -1383,7 +1394,7 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
             c->below_f.release();
             c->diag_dd.release();
             c->below_dd.release();
-        } else if (precision == 1) {
+        } else if (precision != 0) {
             std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
             c->diag_f.upload(d, c->stream);
             c->below_f.upload(b, c->stream);
@@ -1612,10 +1623,17 @@ int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
                 lsLaunches<T>(c, a, 1, c->stream);
                 return;
             }
-            LocalArgs<T> a = localArgs<T>(c, dg.p, nullptr, c->loc.p, 1, 1, nullptr);
-            a.pou = nullptr; // y by local index, no partition of unity
-            launchQueue<T>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
-                           c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, c->stream);
+            auto queue = [&](auto ftag) {
+                using F = decltype(ftag);
+                LocalArgs<T, F> a = localArgs<T, F>(c, dg.p, nullptr, c->loc.p, 1, 1, nullptr);
+                a.pou = nullptr; // y by local index, no partition of unity
+                launchQueue<T, F>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
+                                  c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, c->stream);
+            };
+            if (c->precision == 2)
+                queue(float{});
+            else
+                queue(T{});
         };
         if (c->precision == 1)
             run(float{});
@@ -1705,7 +1723,8 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
         o->levels_fwd_max = lsm ? c->ls.levels_fwd : c->pk.max_levels_fwd;
         o->levels_bwd_max = lsm ? c->ls.levels_bwd : c->pk.max_levels_bwd;
         o->supernodes = static_cast<int32_t>(lsm ? c->ls.sn.size() : c->pk.sn.size());
-        const double sv = c->precision == 1 ? 4.0 : 8.0;
+        const double sv = c->precision == 0 ? 8.0 : 4.0;    // factor entries
+        const double st = c->precision == 1 ? 4.0 : 8.0;    // arithmetic type: slots, partials, LDS
         double local = 0.0;
         if (lsm) {
             // level-synchronous: factor values of both sweeps (triangles of the inverted diagonal
@@ -1716,14 +1735,14 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
             // (x0 read 8 B + z write 8 B)
             const double lt = static_cast<double>(c->local_total);
             local = static_cast<double>(c->ls.entries) * sv + static_cast<double>(c->ls.bytes_index) + lt * (8.0 + 4.0 + 4.0) +
-                    static_cast<double>(c->n) * 16.0 + static_cast<double>(c->ls.slots + c->ls.partials) * sv * 2.0 + lt * sv * 3.0 +
-                    static_cast<double>(c->ls.bwd_below_rows) * sv;
+                    static_cast<double>(c->n) * 16.0 + static_cast<double>(c->ls.slots + c->ls.partials) * st * 2.0 + lt * st * 3.0 +
+                    static_cast<double>(c->ls.bwd_below_rows) * st;
         } else {
             // task queue: factor values of both sweeps, index data, rhs gather (8 B value + 4 B map),
             // permutation and owned-row output (read x0 8 B + write z 8 B)
             local = (c->pk.entries_fwd + c->pk.entries_bwd) * sv + static_cast<double>(c->pk.bytes_index) +
                     static_cast<double>(c->local_total) * (8.0 + 4.0 + 4.0) + static_cast<double>(c->n) * 16.0 +
-                    static_cast<double>(c->pk.t_total) * sv * 2.0;
+                    static_cast<double>(c->pk.t_total) * st * 2.0;
         }
         o->bytes_local_solve = local;
         const double spmv = static_cast<double>(c->nnz) * 12.0 + (static_cast<double>(c->n) + 1) * 4.0 + static_cast<double>(c->n) * 16.0;
@@ -1785,6 +1804,8 @@ int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
     return guarded(c, [&] {
         if (c) c->invalidateGraphs();
         require(c && mode >= 0 && mode <= 2, S2L_EINVAL, "launch mode must be 0, 1 or 2");
+        require(mode != 2 || !c->local_ready || c->precision != 2, S2L_EINVAL,
+                "the level-synchronous schedule has no FP32-storage / FP64-arithmetic mode");
         require(mode == 2 || !c->local_ready || c->queue_ok, S2L_ELIMIT,
                 "launch modes 0 and 1 need a subdomain vector in the LDS of one workgroup");
         c->launch_mode = mode;

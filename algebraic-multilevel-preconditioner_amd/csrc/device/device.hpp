@@ -15,7 +15,9 @@ constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per queue
 constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
 
 /// Arguments of the batched local solve (all pointers are device pointers).
-template <typename T> struct LocalArgs {
+/// T: arithmetic type of the local solve (LDS vector, pushed products); TF: storage type of the
+/// factor entries (TF = float with T = double: FP32 factors applied in FP64 arithmetic).
+template <typename T, typename TF = T> struct LocalArgs {
     const SubDesc* sub;
     const SnDesc* sn;
     const int32_t* lev;
@@ -25,8 +27,8 @@ template <typename T> struct LocalArgs {
     const int32_t* lev_chunk; ///< chunk starts of every level
     int32_t y_off;            ///< byte offset of the result buffer in the dynamic LDS
     int32_t st_off;           ///< byte offset of the staged supernode descriptors
-    const T* diag;
-    const T* below;
+    const TF* diag;
+    const TF* below;
     const int32_t* ridx;
     const int32_t* cptr;
     const int32_t* cidx;  ///< slot of every pushed product
@@ -81,8 +83,8 @@ struct GmState {
     int32_t pad[2];
 };
 
-template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int grid, size_t lds, hipStream_t s);
-template <typename T> const void* localSolveKernel();
+template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, hipStream_t s);
+template <typename T, typename TF> const void* localSolveKernel();
 
 /// One launch of the level-synchronous local solve: items [first, first + count) of kind LsKind
 /// (widest supernode maxw) for nrhs right-hand sides.

@@ -32,42 +32,44 @@ constexpr int kB = 8;
 
 // sum over k < n of p[k * ps] * q[k * qs], in order k = 0, 1, ..., with the loads of a batch issued
 // together ahead of their products: the scheduling barrier keeps the compiler from pulling the
-// loads down next to their uses.
-template <typename T> __device__ inline T dot8(const T* p, int ps, const T* q, int qs, int n, T acc)
+// loads down next to their uses. p (factor storage, type P) is converted to the arithmetic type T.
+template <typename T, typename P> __device__ inline T dot8(const P* p, int ps, const T* q, int qs, int n, T acc)
 {
     int k = 0;
     for (; k + kB <= n; k += kB) {
-        T a[kB], b[kB];
+        P a[kB];
+        T b[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) a[u] = p[(k + u) * ps];
 #pragma unroll
         for (int u = 0; u < kB; ++u) b[u] = q[(k + u) * qs];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kB; ++u) acc += a[u] * b[u];
+        for (int u = 0; u < kB; ++u) acc += static_cast<T>(a[u]) * b[u];
     }
     if (k < n) {
-        T a[kB - 1], b[kB - 1];
+        P a[kB - 1];
+        T b[kB - 1];
 #pragma unroll
-        for (int u = 0; u < kB - 1; ++u) a[u] = k + u < n ? p[(k + u) * ps] : T(0);
+        for (int u = 0; u < kB - 1; ++u) a[u] = k + u < n ? p[(k + u) * ps] : P(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u) b[u] = k + u < n ? q[(k + u) * qs] : T(0);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
-            if (k + u < n) acc += a[u] * b[u];
+            if (k + u < n) acc += static_cast<T>(a[u]) * b[u];
     }
     return acc;
 }
 
 // sum over k < n of p[k * ps] * x[r[k * rs]] (x in LDS), in order, with the loads of p and r of a
 // batch issued together ahead of the products.
-template <typename T> __device__ inline T gatherDot8(const T* p, int ps, const int32_t* r, int rs, const T* x, int n)
+template <typename T, typename P> __device__ inline T gatherDot8(const P* p, int ps, const int32_t* r, int rs, const T* x, int n)
 {
     T acc = T(0);
     int k = 0;
     for (; k + kB <= n; k += kB) {
-        T a[kB];
+        P a[kB];
         int32_t i[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
@@ -76,27 +78,28 @@ template <typename T> __device__ inline T gatherDot8(const T* p, int ps, const i
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kB; ++u) acc += a[u] * x[i[u]];
+        for (int u = 0; u < kB; ++u) acc += static_cast<T>(a[u]) * x[i[u]];
     }
     if (k < n) {
-        T a[kB - 1];
+        P a[kB - 1];
         int32_t i[kB - 1];
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u) {
-            a[u] = k + u < n ? p[(k + u) * ps] : T(0);
+            a[u] = k + u < n ? p[(k + u) * ps] : P(0);
             i[u] = k + u < n ? r[(k + u) * rs] : 0;
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
-            if (k + u < n) acc += a[u] * x[i[u]];
+            if (k + u < n) acc += static_cast<T>(a[u]) * x[i[u]];
     }
     return acc;
 }
 
 // x[row] minus the products pushed to this row by its descendants: one contiguous slot range,
 // subtracted in slot order, eight loads per batch.
-template <typename T> __device__ inline T pullRow(const LocalArgs<T>& a, const SubDesc& sd, const T* tb, const T* x, int row)
+template <typename T, typename TF>
+__device__ inline T pullRow(const LocalArgs<T, TF>& a, const SubDesc& sd, const T* tb, const T* x, int row)
 {
     T v = x[row];
     const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
@@ -174,13 +177,14 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
 // columns (one vector load per row and 256 columns), and out(r, y_r) is called by one lane per
 // row. kTri 1 skips the column quads right of the rows (lower triangle), 2 those left of them
 // (upper); the stored block is zero there. All lanes of a wavefront run the same rows.
-template <typename T, int kTri, typename Out>
-__device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
+template <typename T, typename TF, int kTri, typename Out>
+__device__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
 {
-    // FP64: each pass takes two groups of R rows, nw * R rows apart, whose loads are issued
-    // together (FP32 has no registers left for the second group)
-    constexpr int R = blockRows<T>();
-    constexpr int P = sizeof(T) == 8 ? 2 : 1;
+    // FP64 factors: each pass takes two groups of R rows, nw * R rows apart, whose loads are
+    // issued together (FP32 factors leave no registers for the second group). The block's
+    // entries (TF) are converted to the arithmetic type T.
+    constexpr int R = blockRows<TF>();
+    constexpr int P = sizeof(TF) == 8 ? 2 : 1;
     for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
         const int rA = r0, rB = r0 + nw * R;
         const int nA = min(R, rows - rA), nB = P == 2 ? max(0, min(R, rows - rB)) : 0;
@@ -198,7 +202,8 @@ __device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, i
                 needA = needA && c + 3 >= rA;
                 needB = needB && c + 3 >= rB;
             }
-            T xv[4], mA[R][4], mB[R][4];
+            T xv[4];
+            TF mA[R][4], mB[R][4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) xv[i] = c + i < cols ? xs[c + i] : T(0);
 #pragma unroll
@@ -207,13 +212,13 @@ __device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, i
                     load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[r]);
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) mA[r][i] = T(0);
+                    for (int i = 0; i < 4; ++i) mA[r][i] = TF(0);
                 }
                 if (needB && r < nB) {
                     load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[r]);
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) mB[r][i] = T(0);
+                    for (int i = 0; i < 4; ++i) mB[r][i] = TF(0);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -221,8 +226,8 @@ __device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, i
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    accA[r] += mA[r][i] * xv[i];
-                    accB[r] += mB[r][i] * xv[i];
+                    accA[r] += static_cast<T>(mA[r][i]) * xv[i];
+                    accB[r] += static_cast<T>(mB[r][i]) * xv[i];
                 }
         }
         const T yA = transposeSum<T, R>(accA, lane);
@@ -237,22 +242,22 @@ __device__ void denseRows(const T* M, int ld, int rows, int cols, const T* xs, i
 
 // y = X x_C for the inverse X of a wide supernode's diagonal block (lower or upper triangle),
 // written over x_C.
-template <typename T>
-__device__ void wideDiag(const T* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw, int nt)
+template <typename T, typename TF>
+__device__ void wideDiag(const TF* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw, int nt)
 {
     const int ld = wideLd(w);
     if (lower)
-        denseRows<T, 1>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
+        denseRows<T, TF, 1>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
     else
-        denseRows<T, 2>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
+        denseRows<T, TF, 2>(X, ld, w, w, x + c0, lane, wave, nw, [&](int r, T v) { ybuf[r] = v; });
     __syncthreads();
     for (int j = tid; j < w; j += nt) x[c0 + j] = ybuf[j];
     __syncthreads();
 }
 
 // Forward: every row pulls its products, y = X x_C, then the rows below push B y to their slots.
-template <typename T>
-__device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, T* tb, int tid, int lane,
+template <typename T, typename TF>
+__device__ void fwdWide(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, T* tb, int tid, int lane,
                         int wave, int nt)
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
@@ -260,19 +265,19 @@ __device__ void fwdWide(const LocalArgs<T>& a, const SubDesc& sd, const SnDesc s
     __syncthreads();
     wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, lane, wave, nw, nt);
     const int32_t* dst = a.cidx + sd.dst_off + s.toff;
-    denseRows<T, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { tb[dst[k]] = v; });
+    denseRows<T, TF, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { tb[dst[k]] = v; });
 }
 
 // Backward: x_C -= B^T x_R with G groups of w threads, group g summing the rows k = g, g + G, ...
 // (consecutive threads on consecutive columns of a row), the partial sums combined in group
 // order through LDS. Then x_C = X x_C.
-template <typename T>
-__device__ void bwdWide(const LocalArgs<T>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
+template <typename T, typename TF>
+__device__ void bwdWide(const LocalArgs<T, TF>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave, ld = wideLd(w);
     if (h > 0) {
         const int G = max(1, min(min(nt / w, kChunkRows / w), min(h, kMaxGroups)));
-        const T* B = a.below + s.below;
+        const TF* B = a.below + s.below;
         const int32_t* R = a.ridx + s.ridx;
         for (int u = tid; u < G * w; u += nt) {
             const int g = u / w, c = u - g * w;
@@ -310,8 +315,8 @@ __device__ inline int findItem(const int* prefix, int count, int t)
 //             products of the rows below to their slots;
 //   backward: every column subtracts the products with the rows below; y = inverse(diagonal) * x.
 // Then the level's wide supernodes, one after the other.
-template <typename T>
-__device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv, bool forward, T* x, T* ybuf, SnDesc* st,
+template <typename T, typename TF>
+__device__ void runLevel(const LocalArgs<T, TF>& a, const SubDesc& sd, const int* lv, bool forward, T* x, T* ybuf, SnDesc* st,
                          int* wp, int* hp, T* tb, int tid, int lane, int wave, int nt)
 {
     const int end = lv[1], split = lv[2], wsum = lv[3], hsum = lv[4], coff = lv[5], nchunk = lv[6];
@@ -370,7 +375,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
             const SnDesc& s = st[i];
             const int j = t - wp[i];
             const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
-            const T* D = a.diag + s.diag + lo * s.w + j;
+            const TF* D = a.diag + s.diag + lo * s.w + j;
             ybuf[t] = dot8(D, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
         }
         __syncthreads();
@@ -384,7 +389,7 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
                 const int i = t == tid ? ih : findItem(hp, cnt, t);
                 const SnDesc& s = st[i];
                 const int k = t - hp[i];
-                const T* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
+                const TF* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
                 tb[a.cidx[sd.dst_off + s.toff + k]] = dot8(B, 1, x + s.c0, 1, s.w, T(0));
             }
             __syncthreads();
@@ -399,8 +404,8 @@ __device__ void runLevel(const LocalArgs<T>& a, const SubDesc& sd, const int* lv
     __syncthreads();
 }
 
-template <typename T>
-__device__ void runLevels(const LocalArgs<T>& a, const SubDesc& sd, int lev0, int nlev, bool forward, T* x, unsigned char* smem,
+template <typename T, typename TF>
+__device__ void runLevels(const LocalArgs<T, TF>& a, const SubDesc& sd, int lev0, int nlev, bool forward, T* x, unsigned char* smem,
                           T* tb, int tid, int lane, int wave, int nt)
 {
     T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
@@ -414,8 +419,8 @@ __device__ void runLevels(const LocalArgs<T>& a, const SubDesc& sd, int lev0, in
 // task without parent. Entries of one launch are independent (one depth group of the queue, or
 // one subdomain's entries run in order by a single workgroup); a later launch on the same stream
 // sees the T buffer and the shared results of earlier launches.
-template <typename T>
-__device__ void runEntry(const LocalArgs<T>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
+template <typename T, typename TF>
+__device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
 {
     const int r = static_cast<int>(e / a.qcount);
     if (a.skip && a.skip[r]) return;
@@ -477,7 +482,7 @@ __device__ void runEntry(const LocalArgs<T>& a, long long e, T* x, unsigned char
 // Batched local solve. Grid mode: workgroup b runs entry b of the launch (all subdomains' tasks
 // of one depth group, for every right-hand side). Serial mode: one workgroup runs the entries of
 // one subdomain in queue order (the one-launch-per-subdomain comparison and single solves).
-template <typename T> __global__ __launch_bounds__(kLocalThreads) void k_local_solve(LocalArgs<T> a)
+template <typename T, typename TF> __global__ __launch_bounds__(kLocalThreads) void k_local_solve(LocalArgs<T, TF> a)
 {
     extern __shared__ unsigned char smem_raw[];
     T* x = reinterpret_cast<T*>(smem_raw);
@@ -1323,16 +1328,21 @@ inline unsigned nblk(int64_t n, int t) { return static_cast<unsigned>((n + t - 1
 } // namespace
 
 // ------------------------------------------------------------------ launchers
-template <typename T> void launchLocalSolve(const LocalArgs<T>& a, int grid, size_t lds, hipStream_t s)
+template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_local_solve<T>, dim3(grid), dim3(kLocalThreads), lds, s, a);
+    hipLaunchKernelGGL((k_local_solve<T, TF>), dim3(grid), dim3(kLocalThreads), lds, s, a);
 }
-template void launchLocalSolve<float>(const LocalArgs<float>&, int, size_t, hipStream_t);
-template void launchLocalSolve<double>(const LocalArgs<double>&, int, size_t, hipStream_t);
+template void launchLocalSolve<float, float>(const LocalArgs<float, float>&, int, size_t, hipStream_t);
+template void launchLocalSolve<double, double>(const LocalArgs<double, double>&, int, size_t, hipStream_t);
+template void launchLocalSolve<double, float>(const LocalArgs<double, float>&, int, size_t, hipStream_t);
 
-template <typename T> const void* localSolveKernel() { return reinterpret_cast<const void*>(&k_local_solve<T>); }
-template const void* localSolveKernel<float>();
-template const void* localSolveKernel<double>();
+template <typename T, typename TF> const void* localSolveKernel()
+{
+    return reinterpret_cast<const void*>(&k_local_solve<T, TF>);
+}
+template const void* localSolveKernel<float, float>();
+template const void* localSolveKernel<double, double>();
+template const void* localSolveKernel<double, float>();
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)

@@ -34,7 +34,8 @@ HipTwoLevelSchwarz::HipTwoLevelSchwarz(const SetupData& s, int device)
     if (s.one_level != "ras" && s.one_level != "asm") throw std::invalid_argument("one-level variant must be ras or asm");
     if (s.correction != "deflated" && s.correction != "additive")
         throw std::invalid_argument("coarse correction must be deflated or additive");
-    if (s.precision != "double" && s.precision != "single") throw std::invalid_argument("precision must be double or single");
+    if (s.precision != "double" && s.precision != "single" && s.precision != "mixed")
+        throw std::invalid_argument("precision must be double, single or mixed");
     check(nullptr, s2l_ctx_create(device, &ctx_));
     try {
         upload(s);
@@ -67,7 +68,7 @@ void HipTwoLevelSchwarz::upload(const SetupData& s)
                                             lu ? f.ut_colptr.data() : nullptr, lu ? f.ut_row.data() : nullptr,
                                             lu ? f.ut_val.data() : nullptr));
     }
-    check(ctx_, s2l_finalize_local(ctx_, s.precision == "single" ? 1 : 0));
+    check(ctx_, s2l_finalize_local(ctx_, s.precision == "single" ? 1 : (s.precision == "mixed" ? 2 : 0)));
     check(ctx_, s2l_upload_coarse(ctx_, s.n0, s.n0 > 0 ? s.coarse_offsets.data() : nullptr, s.n0 > 0 ? s.V.data() : nullptr,
                                   s.n0 > 0 ? s.E.data() : nullptr, s.symmetric ? 1 : 0, s.correction == "additive" ? 1 : 0));
     n_ = s.n;

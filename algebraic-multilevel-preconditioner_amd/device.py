@@ -110,6 +110,11 @@ def _p(a, t):
     return a.ctypes.data_as(P(t))
 
 
+# local factor precision: "double" (FP64), "single" (FP32, the reference's solveIn<float>), "mixed"
+# (FP32 factor entries applied in FP64 arithmetic, no reference counterpart)
+_PRECISIONS = {"double": 0, "single": 1, "mixed": 2}
+
+
 def setup_arrays(setup, precision: str = "double", one_level: str | None = None,
                  correction: str | None = None) -> dict:
     """The arrays of the C-ABI upload (include/s2l_hip.h) from a hostsetup.SchwarzSetup: the
@@ -248,7 +253,7 @@ class HipContext:
                                                    _p(lr, ctypes.c_int32), _p(lv, ctypes.c_double), *args))
 
     def finalize(self, precision: str = "double"):
-        self._check(self.L.s2l_finalize_local(self.h, 1 if precision == "single" else 0))
+        self._check(self.L.s2l_finalize_local(self.h, _PRECISIONS[precision]))
 
     def upload_coarse(self, setup, correction: str = "deflated"):
         cs = setup.coarse

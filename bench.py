@@ -52,7 +52,7 @@ DEFAULT_NRHS = {"highcontrast3d-96": 16}
 COPY_COUNT = 1 << 25   # doubles of the calibration copy (256 MiB read + 256 MiB written)
 PMC_APPLIES = 3        # applies of a counter pass
 PROFILE_WARM = 2       # untimed applies before the timed ones in --profile-applies runs
-LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float>")
+LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float>")
 
 
 def time_applies(ctx, r, z, count, warm=3):
@@ -74,14 +74,14 @@ def time_applies(ctx, r, z, count, warm=3):
 
 
 def roofline(bytes_per_apply, avg_ms, applies):
-    """The dominant kernel's roofline: k_local_solve<float>, whose launches of one apply (the subtree
+    """The dominant kernel's roofline: k_local_solve<float, float>, whose launches of one apply (the subtree
     forward launch, the top tasks, the subtree backward launch) are timed together with HIP events
     on their stream; avg_us is their time per apply, bytes the algorithmic bytes of one apply's
     local solves."""
     achieved = bytes_per_apply / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_local_solve<float> (task queue, the launches of one apply)", "applies_timed": int(applies),
+            "kernel": "k_local_solve<float, float> (task queue, the launches of one apply)", "applies_timed": int(applies),
             "avg_us": round(avg_ms * 1e3, 2),
             "bytes_per_apply": int(bytes_per_apply)}
 
@@ -230,6 +230,10 @@ def main():
     M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks)
     ctx64 = M64.ctx
     ctx64.set_graphs(args.graphs != 0)
+    # FP32 factor entries applied in FP64 arithmetic (no reference counterpart)
+    Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks)
+    ctxm = Mmx.ctx
+    ctxm.set_graphs(args.graphs != 0)
     nsteps = args.steps + args.warmup
     # right-hand sides in the renumbered order, resident on the device before timing; step k solves
     # the nrhs right-hand sides k * nrhs .. k * nrhs + nrhs - 1 together
@@ -292,11 +296,25 @@ def main():
     tf = time.perf_counter() - tfs
     xf = dX[args.warmup].download()[:n]
 
+    # ---- FP32 factor entries in FP64 arithmetic, the reference's GMRES
+    for k in range(args.warmup):
+        run(ctxm, k)
+    latm, itsm, convm = [], [], []
+    tms = time.perf_counter()
+    for k in range(args.warmup, nsteps):
+        dt, res = run(ctxm, k)
+        latm.append(dt)
+        itsm.append(res.iterations)
+        convm.append(bool(res.converged))
+    tm = time.perf_counter() - tms
+    xm = dX[args.warmup].download()[:n]
+
     # ---- solution error against a tight FP64 solve
     xr, rr = ctx64.gmres(B[args.warmup * nrhs], None, restart, 1e-12, 4000)
     err32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
     err64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
     errf = float(np.linalg.norm(xf - xr) / np.linalg.norm(xr))
+    errm = float(np.linalg.norm(xm - xr) / np.linalg.norm(xr))
 
     # ---- local-solve schedules (FP32 applies, dedicated loop: every timed local-solve launch does
     # the full work, unlike the GMRES loop where a launch after convergence is skipped by its mask):
@@ -311,6 +329,7 @@ def main():
         ctx.set_launch_mode(mode)
         mode_ms[mode] = time_applies(ctx, r, z, 20, warm=3)
     ctx.set_launch_mode(0)
+    mixed_ms = time_applies(ctxm, r, z, 20, warm=3)
 
     # ---- roofline of the dominant kernel (batched local solve, FP32 factors)
     roof = roofline(stats.bytes_local_solve, mode_ms[0][0], 20)
@@ -365,6 +384,14 @@ def main():
                         "speedup_vs_fp64": round(t64 / tf, 3),
                         "note": "flexible GMRES(m): the basis M^{-1} v_j is stored (an option of s2l_krylov_opts "
                                 "without reference counterpart); the headline value uses the reference's GMRES"},
+        "mixed_fp32_factors": {"value": round(args.steps * nrhs / tm, 3), "unit": "solves/s",
+                               "p50_ms": round(1e3 * float(np.percentile(latm, 50)), 3),
+                               "p99_ms": round(1e3 * float(np.percentile(latm, 99)), 3),
+                               "iterations": float(np.mean(itsm)), "converged": all(convm), "solution_rel_err": errm,
+                               "speedup_vs_fp64": round(t64 / tm, 3), "local_solve_us": _us(mixed_ms[0]),
+                               "note": "FP32 factor entries (the bytes of the FP32 path) applied in FP64 arithmetic, "
+                                       "the reference's GMRES (precision 2 of s2l_finalize_local, no reference "
+                                       "counterpart); the headline value uses the FP32 arithmetic of solveIn<float>"},
         "solution_rel_err_fp32": err32,
         "solution_rel_err_fp64": err64,
         "local_solve_us": {"level_synchronous": _us(mode_ms[2][0]),

@@ -56,7 +56,7 @@ struct SetupData {
     bool symmetric = true;
     std::string one_level = "ras";       ///< "ras" | "asm"
     std::string correction = "deflated"; ///< "deflated" | "additive"
-    std::string precision = "double";    ///< local factors: "double" | "single"
+    std::string precision = "double";    ///< local factors: "double" | "single" | "mixed" (FP32 entries, FP64 arithmetic)
     int nsub() const { return static_cast<int>(local_ptr.size()) - 1; }
 };
 

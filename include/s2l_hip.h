@@ -77,7 +77,10 @@ int s2l_upload_local_factor(s2l_ctx* ctx, int32_t i, int32_t n, const int32_t* p
 /* Packs every uploaded factor into the device layout (supernodes, level schedule) with values in
  * @p precision: 0 FP64, 1 FP32 (--local-precision single, include/core/config.hpp:72: the
  * right-hand side is rounded to float, both sweeps run in float, the result is widened, as
- * solveIn<float> in src/linalg/local_solver.cpp:77-84). */
+ * solveIn<float> in src/linalg/local_solver.cpp:77-84), 2 FP32 factor entries applied in FP64
+ * arithmetic (no reference counterpart: the bytes of FP32 factors, the sums and the vector of
+ * FP64; the local solve is then a linear operator up to FP64 rounding; task-queue launch modes 0
+ * and 1 only). */
 int s2l_finalize_local(s2l_ctx* ctx, int precision);
 
 /* Coarse space: V_i blocks (rows x m_i, column-major, concatenated; rows = owned rows with the

@@ -75,19 +75,28 @@ def _p(a, t):
 
 
 class OracleLocalFactor:
-    """LocalSolver::solve restated on one factor, FP64 or FP32 values (local_solver.cpp:77-126)."""
+    """LocalSolver::solve restated on one factor, FP64 or FP32 values (local_solver.cpp:77-126).
+
+    precision "mixed" (no reference counterpart) keeps the factor entries rounded to FP32 and
+    solves in FP64: the error it leaves is the rounding of the factors alone."""
 
     def __init__(self, f, precision: str = "double"):
         self.n = f.n
         self.single = precision == "single"
         vt = np.float32 if self.single else np.float64
+
+        def values(v):
+            if v is None:
+                return None
+            return np.asarray(v, np.float32).astype(np.float64) if precision == "mixed" else v
+
         self.perm = np.ascontiguousarray(f.perm, np.int32)
         self.lcp = np.ascontiguousarray(f.l_colptr, np.int64)
         self.lr = np.ascontiguousarray(f.l_row, np.int32)
-        self.lv = np.ascontiguousarray(f.l_val, vt)
+        self.lv = np.ascontiguousarray(values(f.l_val), vt)
         self.ucp = np.ascontiguousarray(f.ut_colptr, np.int64)
         self.ur = np.ascontiguousarray(f.ut_row, np.int32)
-        self.uv = np.ascontiguousarray(f.ut_val, vt)
+        self.uv = np.ascontiguousarray(values(f.ut_val), vt)
 
     def solve(self, b: np.ndarray) -> np.ndarray:
         L = lib()

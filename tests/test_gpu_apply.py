@@ -89,6 +89,49 @@ def test_apply_fp32_as_accurate_as_oracle_float_path(name, kw, N):
     assert e_dev <= 3.0 * e_orc + 1e-7
 
 
+@pytest.mark.parametrize("name,kw,N", [CASES[0], CASES[1], CASES[4]])
+def test_apply_mixed_precision_linear_and_accurate(name, kw, N):
+    """FP32 factor entries in FP64 arithmetic (precision "mixed", no reference counterpart): the
+    device apply is linear to FP64 rounding (1e-13), its error against the FP64 oracle is that of
+    rounding the factors (within 3x the oracle's FP32 path, like the FP32 mode), it agrees with the
+    oracle's mixed restatement to the difference of the two factor roundings, and the local solves
+    agree with the oracle's."""
+    from amp_amd import device
+    st = _setup(name, kw, N, nev=8)
+    print(f"case {name} {kw} N={N} mixed", flush=True)
+    M = device.HipTwoLevelSchwarz(st, "mixed")
+    r1 = np.cos(np.linspace(0.0, 7.0, st.n))
+    r2 = np.sin(np.linspace(0.0, 3.0, st.n))
+    z1, z2, z12 = M.apply(r1), M.apply(r2), M.apply(2.0 * r1 - 3.0 * r2)
+    assert np.linalg.norm(z12 - (2.0 * z1 - 3.0 * z2)) / np.linalg.norm(z12) < 1e-13
+    assert np.array_equal(z1, M.apply(r1))
+    z64 = orc.OracleSchwarz(st, "double").apply(r1)
+    z32o = orc.OracleSchwarz(st, "single").apply(r1)
+    zmo = orc.OracleSchwarz(st, "mixed").apply(r1)
+    e_dev = np.linalg.norm(z1 - z64) / np.linalg.norm(z64)
+    assert e_dev <= 3.0 * np.linalg.norm(z32o - z64) / np.linalg.norm(z64) + 1e-9
+    assert np.linalg.norm(z1 - zmo) / np.linalg.norm(zmo) < 1e-5
+    rng = np.random.default_rng(3)
+    for i in range(N):
+        f = st.factors[i]
+        b = rng.standard_normal(f.n)
+        y = M.ctx.local_solve(i, b)
+        yo = orc.OracleLocalFactor(f, "double").solve(b)
+        assert np.linalg.norm(y - yo) / np.linalg.norm(yo) < 1e-5, i
+
+
+def test_mixed_precision_refuses_level_synchronous_mode():
+    """The level-synchronous schedule stores factors in its arithmetic type: mode 2 is refused for
+    the mixed precision, and the context keeps working in the task queue."""
+    from amp_amd import device
+    st = _setup("laplace2d", dict(nx=20), 4, nev=4)
+    M = device.HipTwoLevelSchwarz(st, "mixed")
+    with pytest.raises(Exception):
+        M.ctx.set_launch_mode(2)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    assert np.all(np.isfinite(M.apply(r)))
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_apply_bitwise_repeatable_and_launch_modes_agree(precision):
     """Unchanged matrix -> bitwise identical apply (test/test_reuse.cpp:309-316) in every launch
@@ -111,14 +154,14 @@ def test_apply_bitwise_repeatable_and_launch_modes_agree(precision):
     assert np.linalg.norm(out[2] - out[0]) / np.linalg.norm(out[0]) < tol
 
 
-@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("precision", ["double", "single", "mixed"])
 def test_subtree_tasks_agree(precision):
     """Splitting every subdomain's elimination tree into subtree tasks (several workgroups per
     subdomain, several launches) changes the schedule, and with it only the rounding of the sums
     below narrow chunks: 1 to 4 tasks per subdomain agree to rounding and with the oracle, on
     Cholesky (Poisson) and LU (convection) factors, and every split is bitwise repeatable."""
     from amp_amd import device
-    tol = 1e-13 if precision == "double" else 2e-6
+    tol = 2e-6 if precision == "single" else 1e-13
     for name, kw, N in (("poisson3d", dict(nx=16), 8), ("convdiff2d", dict(nx=24), 4)):
         st = _setup(name, kw, N, nev=8, overlap=2)
         print(f"case {name} {kw} N={N} {precision}", flush=True)
@@ -161,6 +204,20 @@ def test_wide_supernodes_and_partials(mode, precision):
         y = M.ctx.local_solve(i, b)
         yo = orc.OracleLocalFactor(f, "double").solve(b)
         assert np.linalg.norm(y - yo) / np.linalg.norm(yo) < (1e-12 if precision == "double" else 1e-5), i
+
+
+def test_gmres_mixed_precision_matches_fp64_iterations():
+    """FP32 factor entries in FP64 arithmetic with the reference's GMRES: the preconditioner is a
+    linear operator to FP64 rounding, so the solution formed at the end of a cycle agrees with the
+    Arnoldi estimate and the solve takes the FP64 iteration count (+1 at most), with a true residual
+    <= 1.1e-8 and a solution within 1e-6 of the FP64 one."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=20), 8, nev=12, overlap=2)
+    x64, r64 = device.HipGmres(50, 1e-8, 500).solve(device.HipTwoLevelSchwarz(st, "double"), st.dec.b)
+    xm, rm = device.HipGmres(50, 1e-8, 500).solve(device.HipTwoLevelSchwarz(st, "mixed"), st.dec.b)
+    assert rm.converged and rm.true_rel_residual <= 1.1e-8
+    assert rm.iterations <= r64.iterations + 1
+    assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-6
 
 
 def test_gmres_fp32_local_within_target():

@@ -143,6 +143,28 @@ def test_local_solve_single_rounds_rhs_to_float():
     assert np.all(orc.OracleLocalFactor(f, "double").solve(b) > 0.0)
 
 
+@pytest.mark.parametrize("name,kw", [("laplace2d", dict(nx=12)), ("convdiff2d", dict(nx=14))])
+def test_mixed_precision_is_linear_and_only_rounds_the_factors(name, kw):
+    """FP32 factor entries applied in FP64 arithmetic (precision "mixed"): the apply is linear to
+    FP64 rounding (a r1 + b r2 maps to a z1 + b z2 within 1e-14), which the FP32 arithmetic of
+    solveIn<float> is not (1e-8 and above); its error against the FP64 apply is that of rounding
+    the factors and stays below the FP32 path's."""
+    st = setup(name, items(kw), 4, items(dict(nev=4)))
+    r1 = np.cos(np.linspace(0.0, 7.0, st.n))
+    r2 = np.sin(np.linspace(0.0, 3.0, st.n))
+    lin = {}
+    for prec in ("mixed", "single"):
+        O = orc.OracleSchwarz(st, prec)
+        z = O.apply(2.0 * r1 - 3.0 * r2)
+        lin[prec] = np.linalg.norm(z - (2.0 * O.apply(r1) - 3.0 * O.apply(r2))) / np.linalg.norm(z)
+    assert lin["mixed"] < 1e-14
+    assert lin["single"] > 1e-10
+    z64 = orc.OracleSchwarz(st, "double").apply(r1)
+    em = np.linalg.norm(orc.OracleSchwarz(st, "mixed").apply(r1) - z64) / np.linalg.norm(z64)
+    es = np.linalg.norm(orc.OracleSchwarz(st, "single").apply(r1) - z64) / np.linalg.norm(z64)
+    assert 0.0 < em < es
+
+
 def test_planted_transposed_factor_is_caught():
     """An LU solve with L used in place of U^T (a transposed operand) fails the backward-error check."""
     st = setup("convdiff2d", items(dict(nx=14)), 4, items(dict(nev=2)))
