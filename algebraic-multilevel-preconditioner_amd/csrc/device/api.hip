@@ -721,8 +721,8 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
                 applySpmv(c, vj, zj, c->kw.p, nrhs, live, s);           // z = M^{-1} v_j (kept: flexible); w = A z
                 gmDots(a, j, s);                                        // CGS2 pass 1
                 gmUpdateDots(a, j, s);                                  // w -= V h; pass 2
-                gmUpdateNorm(a, j, s);                                  // w -= V h2; ||w||; Givens
-                gmNormalize(a, j, s);                                   // v_{j+1} = w / ||w||
+                gmUpdateNorm(a, j, s);                                  // w -= V h2; partials of ||w||^2
+                gmNormalize(a, j, s);                                   // ||w||, Givens, v_{j+1} = w / ||w||
             }
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(c->pflags.p, c->kflags.p, sizeof(int32_t) * nrhs, hipMemcpyDeviceToHost, s));

@@ -1211,18 +1211,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_final(GmArgs a, int j, int
     if (threadIdx.x == 0) (second ? a.h2 : a.h)[r * (a.m + 1) + k] = t;
 }
 
-// hn = ||w|| from the partials of k_gm_update_norm, then the end of the Arnoldi step: Hessenberg
-// column, Givens rotations, residual estimate and the stopping test (gmres.cpp:149-208); grid nrhs.
-__global__ __launch_bounds__(kBlockThreads) void k_gm_givens(GmArgs a, int j)
+// The end of the Arnoldi step for right-hand side r with hn = ||w||: Hessenberg column, Givens
+// rotations, residual estimate and the stopping test (gmres.cpp:149-208); one thread.
+__device__ void gmGivensStep(const GmArgs& a, int j, int r, double hn)
 {
-    const int r = blockIdx.x;
-    if (a.flags[r]) return;
-    const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
-    double v = 0.0;
-    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
-    const double ss = blockSum(v);
-    if (threadIdx.x != 0) return;
-    const double hn = sqrt(ss);
     const int m = a.m;
     double* H = a.H + static_cast<int64_t>(r) * (m + 1) * m;
     double* Hc = H + static_cast<int64_t>(j) * (m + 1);
@@ -1264,11 +1256,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_givens(GmArgs a, int j)
     a.flags[r] = done;
 }
 
-__global__ void k_gm_normalize(GmArgs a, int j)
+// hn = ||w|| from the partials of k_gm_update_norm (every workgroup sums them in the same order,
+// so all hold the same bits), the Givens step (workgroup 0) and v_{j+1} = w / hn. Workgroup 0 may
+// set the right-hand side's flag while others start: one that reads it set skips its rows of
+// v_{j+1}, which the finished cycle does not read.
+__global__ __launch_bounds__(kBlockThreads) void k_gm_givens_normalize(GmArgs a, int j)
 {
     const int r = blockIdx.y;
     if (a.flags[r]) return;
-    const double hn = a.st[r].hn;
+    const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+    const double ss = blockSum(v);
+    __shared__ double hs;
+    if (threadIdx.x == 0) hs = sqrt(ss);
+    __syncthreads();
+    const double hn = hs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) gmGivensStep(a, j, r, hn);
     if (!(hn > 0.0)) return;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -1480,11 +1484,10 @@ void gmUpdateDots(const GmArgs& a, int j, hipStream_t s)
 void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_update_norm, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);
-    hipLaunchKernelGGL(k_gm_givens, dim3(a.nrhs), dim3(kBlockThreads), 0, s, a, j);
 }
 void gmNormalize(const GmArgs& a, int j, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gm_normalize, dim3(nblk(a.n, 256), a.nrhs), dim3(256), 0, s, a, j);
+    hipLaunchKernelGGL(k_gm_givens_normalize, dim3(nblk(a.n, kBlockThreads), a.nrhs), dim3(kBlockThreads), 0, s, a, j);
 }
 void gmStartCycle(const GmArgs& a, const double* r, hipStream_t s)
 {
