@@ -207,6 +207,9 @@ struct s2l_ctx {
     DevBuf<SnDesc> sn_d;
     DevBuf<int32_t> lev_d, levsn_d, levw_d, levh_d, levc_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
     size_t y_off = 0, st_off = 0;
+    // FP32 factors, two right-hand sides per workgroup (a float2 vector in LDS): its LDS layout
+    size_t y_off2 = 0, st_off2 = 0, lds_bytes2 = 0;
+    bool pair_ok = false;
     DevBuf<float> diag_f, below_f;
     DevBuf<double> diag_dd, below_dd;
     DevBuf<unsigned char> tbuf, xgbuf;
@@ -289,8 +292,9 @@ struct s2l_ctx {
         yvec.reserve(static_cast<size_t>(std::max(n0, 1)) * nrhs);
         cpart.reserve(static_cast<size_t>(std::max(nsub, 1)) * std::max(ca.rchunks, 1) * std::max(ca.mmax, 1) * nrhs);
         const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type
-        tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nrhs * esz);
-        xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
+        const int nr = (nrhs + 1) / 2 * 2;  // FP32 pairs of right-hand sides: an even count
+        tbuf.reserve(static_cast<size_t>(std::max<int64_t>(pk.t_total, 1)) * nr * esz);
+        xgbuf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nr * esz);
         ls_slots.reserve(static_cast<size_t>(std::max<int64_t>(ls.slots, 1)) * nrhs * esz);
         ls_part.reserve(static_cast<size_t>(std::max<int64_t>(ls.partials, 1)) * nrhs * esz);
         ls_b.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
@@ -341,8 +345,9 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     a.lev_wpre = c->levw_d.p;
     a.lev_hpre = c->levh_d.p;
     a.lev_chunk = c->levc_d.p;
-    a.y_off = static_cast<int32_t>(c->y_off);
-    a.st_off = static_cast<int32_t>(c->st_off);
+    const bool pair = std::is_same_v<T, float2>;
+    a.y_off = static_cast<int32_t>(pair ? c->y_off2 : c->y_off);
+    a.st_off = static_cast<int32_t>(pair ? c->st_off2 : c->st_off);
     if constexpr (std::is_same_v<TF, float>) {
         a.diag = c->diag_f.p;
         a.below = c->below_f.p;
@@ -383,8 +388,9 @@ void launchQueue(s2l_ctx* c, LocalArgs<T, TF>& a, const int32_t* q, int32_t coun
     a.queue = q;
     a.qcount = count;
     a.serial = serial ? 1 : 0;
-    const int grid = serial ? 1 : count * a.nrhs;
-    launchLocalSolve<T, TF>(a, grid, c->lds_bytes, s);
+    const bool pair = std::is_same_v<T, float2>;  // a workgroup per entry and pair of right-hand sides
+    const int grid = serial ? 1 : count * (pair ? (a.nrhs + 1) / 2 : a.nrhs);
+    launchLocalSolve<T, TF>(a, grid, pair ? c->lds_bytes2 : c->lds_bytes, s);
 }
 
 template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode,
@@ -515,7 +521,9 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
     const bool direct = (c->one_level == 0 && c->boolean);
     const int mode = direct ? 0 : 1;
     double* out = direct ? z : c->loc.p;
-    if (c->precision == 1)
+    if (c->precision == 1 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
+        localSolve<float2, float>(c, rhs, x0, out, mode, nrhs, skip, s);
+    else if (c->precision == 1)
         localSolve<float, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else if (c->precision == 2)
         localSolve<double, float>(c, rhs, x0, out, mode, nrhs, skip, s);
@@ -1350,6 +1358,13 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
         c->lds_bytes = c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
         c->queue_ok = c->lds_bytes <= c->max_lds;
+        c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
+        c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
+        c->lds_bytes2 = c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
+        c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
+        if (c->pair_ok)
+            HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         static_cast<int>(c->lds_bytes2)));
         if (c->queue_ok) {
             const void* kern = precision == 1   ? localSolveKernel<float, float>()
                                : precision == 2 ? localSolveKernel<double, float>()

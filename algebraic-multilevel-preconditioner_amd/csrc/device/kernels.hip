@@ -19,9 +19,21 @@ constexpr int kWave = 64;
 constexpr int kPullSplit = 4;  // a backward chunk of at most blockDim / kPullSplit columns splits its rows below
 constexpr int kMaxGroups = 32; // row groups of such a split
 
+// Lane exchange of a value; a pair of floats (two right-hand sides solved together) moves as two.
+template <typename T> __device__ inline T shflXor(T v, int m) { return __shfl_xor(v, m, kWave); }
+template <> __device__ inline float2 shflXor<float2>(float2 v, int m)
+{
+    return float2(__shfl_xor(v.x, m, kWave), __shfl_xor(v.y, m, kWave));
+}
+
+// A factor entry in the arithmetic type; for a pair of right-hand sides the same entry multiplies
+// both (float2's one-argument constructor would leave the second component zero).
+template <typename T, typename P> __device__ inline T widen(P v) { return static_cast<T>(v); }
+template <> __device__ inline float2 widen<float2, float>(float v) { return float2(v, v); }
+
 template <typename T> __device__ inline T waveSum(T v)
 {
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    for (int off = kWave / 2; off > 0; off >>= 1) v += shflXor(v, off);
     return v;
 }
 
@@ -45,7 +57,7 @@ template <typename T, typename P> __device__ inline T dot8(const P* p, int ps, c
         for (int u = 0; u < kB; ++u) b[u] = q[(k + u) * qs];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kB; ++u) acc += static_cast<T>(a[u]) * b[u];
+        for (int u = 0; u < kB; ++u) acc += widen<T>(a[u]) * b[u];
     }
     if (k < n) {
         P a[kB - 1];
@@ -57,7 +69,7 @@ template <typename T, typename P> __device__ inline T dot8(const P* p, int ps, c
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
-            if (k + u < n) acc += static_cast<T>(a[u]) * b[u];
+            if (k + u < n) acc += widen<T>(a[u]) * b[u];
     }
     return acc;
 }
@@ -78,7 +90,7 @@ template <typename T, typename P> __device__ inline T gatherDot8(const P* p, int
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < kB; ++u) acc += static_cast<T>(a[u]) * x[i[u]];
+        for (int u = 0; u < kB; ++u) acc += widen<T>(a[u]) * x[i[u]];
     }
     if (k < n) {
         P a[kB - 1];
@@ -91,7 +103,7 @@ template <typename T, typename P> __device__ inline T gatherDot8(const P* p, int
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
-            if (k + u < n) acc += static_cast<T>(a[u]) * x[i[u]];
+            if (k + u < n) acc += widen<T>(a[u]) * x[i[u]];
     }
     return acc;
 }
@@ -163,12 +175,12 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
         for (int j = 0; j < half; ++j) {
             const T keep = hi ? v[j + half] : v[j];
             const T send = hi ? v[j] : v[j + half];
-            v[j] = keep + __shfl_xor(send, bit, kWave);
+            v[j] = keep + shflXor(send, bit);
         }
     }
     T s = v[0];
 #pragma unroll
-    for (int bit = kWave / R / 2; bit >= 1; bit >>= 1) s += __shfl_xor(s, bit, kWave);
+    for (int bit = kWave / R / 2; bit >= 1; bit >>= 1) s += shflXor(s, bit);
     return s;
 }
 
@@ -226,8 +238,8 @@ __device__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, 
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    accA[r] += static_cast<T>(mA[r][i]) * xv[i];
-                    accB[r] += static_cast<T>(mB[r][i]) * xv[i];
+                    accA[r] += widen<T>(mA[r][i]) * xv[i];
+                    accB[r] += widen<T>(mB[r][i]) * xv[i];
                 }
         }
         const T yA = transposeSum<T, R>(accA, lane);
@@ -422,8 +434,13 @@ __device__ void runLevels(const LocalArgs<T, TF>& a, const SubDesc& sd, int lev0
 template <typename T, typename TF>
 __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
 {
+    // kPair: T = float2 holds right-hand sides 2 r and 2 r + 1 (the second absent when nrhs is odd)
+    constexpr bool kPair = std::is_same_v<T, float2>;
     const int r = static_cast<int>(e / a.qcount);
-    if (a.skip && a.skip[r]) return;
+    const int rA = kPair ? 2 * r : r, rB = rA + 1;
+    const bool onA = !(a.skip && a.skip[rA]);
+    const bool onB = kPair && rB < a.nrhs && !(a.skip && a.skip[rB]);
+    if (!onA && !onB) return;
     const int code = a.queue[e % a.qcount];
     const int tk = code >> 1;
     const bool backward = (code & 1) != 0;
@@ -434,10 +451,14 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
     const int32_t* rows = a.task_rows + td.rows_first;
     if (!backward) {
-        const double* rhs = a.rhs + static_cast<int64_t>(r) * a.n_global;
+        const double* rhs = a.rhs + static_cast<int64_t>(rA) * a.n_global;
         for (int q = tid; q < td.rows_count; q += blockDim.x) {
             const int row = rows[q];
-            x[row] = static_cast<T>(rhs[a.gmap[sd.map_off + row]]);
+            const int g = a.gmap[sd.map_off + row];
+            if constexpr (kPair)
+                x[row] = float2(static_cast<float>(rhs[g]), onB ? static_cast<float>(rhs[a.n_global + g]) : 0.0f);
+            else
+                x[row] = static_cast<T>(rhs[g]);
         }
         __syncthreads();
         runLevels(a, sd, td.f_lev, td.f_nlev, true, x, smem, tb, tid, lane, wave, nt);
@@ -455,25 +476,25 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     runLevels(a, sd, td.b_lev, td.b_nlev, false, x, smem, tb, tid, lane, wave, nt);
     if (td.nchild > 0)
         for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
-    if (a.mode == 0) {
-        double* out = a.out + static_cast<int64_t>(r) * a.n_global;
-        const double* x0 = a.x0 ? a.x0 + static_cast<int64_t>(r) * a.n_global : nullptr;
-        for (int q = tid; q < td.rows_count; q += blockDim.x) {
-            const int row = rows[q];
-            const int l = a.lperm[sd.map_off + row];
+    // the owned rows of the result (z = x0 + y), or the local buffer (scaled by D_i) for a reverse add
+    auto store = [&](int rr, int row, double y) {
+        const int l = a.lperm[sd.map_off + row];
+        if (a.mode == 0) {
             if (l < sd.n_owned) {
-                const int64_t gi = sd.begin + l;
-                const double y = static_cast<double>(x[row]);
-                out[gi] = x0 ? x0[gi] + y : y;
+                const int64_t gi = static_cast<int64_t>(rr) * a.n_global + sd.begin + l;
+                a.out[gi] = a.x0 ? a.x0[gi] + y : y;
             }
+        } else {
+            a.out[static_cast<int64_t>(rr) * a.n_local_total + sd.map_off + l] = a.pou ? a.pou[sd.map_off + l] * y : y;
         }
-    } else {
-        double* out = a.out + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
-        for (int q = tid; q < td.rows_count; q += blockDim.x) {
-            const int row = rows[q];
-            const int l = a.lperm[sd.map_off + row];
-            const double y = static_cast<double>(x[row]);
-            out[l] = a.pou ? a.pou[sd.map_off + l] * y : y;
+    };
+    for (int q = tid; q < td.rows_count; q += blockDim.x) {
+        const int row = rows[q];
+        if constexpr (kPair) {
+            if (onA) store(rA, row, static_cast<double>(x[row].x));
+            if (onB) store(rB, row, static_cast<double>(x[row].y));
+        } else {
+            store(rA, row, static_cast<double>(x[row]));
         }
     }
     __syncthreads();
@@ -1339,6 +1360,7 @@ template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>&
 template void launchLocalSolve<float, float>(const LocalArgs<float, float>&, int, size_t, hipStream_t);
 template void launchLocalSolve<double, double>(const LocalArgs<double, double>&, int, size_t, hipStream_t);
 template void launchLocalSolve<double, float>(const LocalArgs<double, float>&, int, size_t, hipStream_t);
+template void launchLocalSolve<float2, float>(const LocalArgs<float2, float>&, int, size_t, hipStream_t);
 
 template <typename T, typename TF> const void* localSolveKernel()
 {
@@ -1347,6 +1369,7 @@ template <typename T, typename TF> const void* localSolveKernel()
 template const void* localSolveKernel<float, float>();
 template const void* localSolveKernel<double, double>();
 template const void* localSolveKernel<double, float>();
+template const void* localSolveKernel<float2, float>();
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)

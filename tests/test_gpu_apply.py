@@ -255,6 +255,27 @@ def test_many_rhs_coarse_products_on_matrix_cores(precision):
             assert np.linalg.norm(Z[k] - zo) / np.linalg.norm(zo) < 1e-12
 
 
+@pytest.mark.parametrize("nrhs", [2, 3])
+@pytest.mark.parametrize("name,kw,N", [CASES[1], CASES[4]])
+def test_fp32_pairs_of_rhs_equal_single_applies(name, kw, N, nrhs):
+    """FP32 local solves of two right-hand sides in one workgroup (a float pair per row in LDS, one
+    read of the factors for both; an odd count leaves the last pair half empty) give the results of
+    one right-hand side at a time (the same float operations per component: bitwise, or to 1e-6
+    should the compiler contract them differently), in every launch of the task queue."""
+    from amp_amd import device
+    st = _setup(name, kw, N, nev=8, overlap=2)
+    print(f"case {name} {kw} N={N} fp32 pairs nrhs={nrhs}", flush=True)
+    M = device.HipTwoLevelSchwarz(st, "single")
+    rng = np.random.default_rng(13)
+    R = rng.standard_normal((nrhs, st.n))
+    Z = M.ctx.apply(R)
+    Zs = np.stack([M.ctx.apply(R[k]) for k in range(nrhs)])
+    diff = np.linalg.norm(Z - Zs, axis=1) / np.linalg.norm(Zs, axis=1)
+    print(f"pairs bitwise equal to single applies: {np.array_equal(Z, Zs)}, max rel diff {diff.max():.2e}", flush=True)
+    assert diff.max() < 1e-6, diff.max()
+    assert np.array_equal(Z, M.ctx.apply(R))
+
+
 @pytest.mark.parametrize("name,kw,N,nev,nrhs", [("convdiff2d", dict(nx=32), 8, 9, 16), ("convdiff2d", dict(nx=32), 8, 9, 5),
                                                 ("poisson3d", dict(nx=12), 6, 13, 16)])
 def test_many_rhs_coarse_odd_sizes(name, kw, N, nev, nrhs):

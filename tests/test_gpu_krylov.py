@@ -151,14 +151,16 @@ def test_graph_replays_equal_eager_launches(precision):
             assert its == its_e
 
 
-def test_many_rhs_gmres_matches_single_solves():
-    """GMRES with 8 right-hand sides (coarse products on the matrix cores) converges every one of
-    them in the iterations it takes alone (to one) and to the same solution within the tolerance;
-    a right-hand side that converges early is masked while the others continue."""
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_many_rhs_gmres_matches_single_solves(precision):
+    """GMRES with 8 right-hand sides (coarse products on the matrix cores; in FP32 the local solves
+    of two right-hand sides share a workgroup) converges every one of them in the iterations it
+    takes alone (to one) and to the same solution within the tolerance; a right-hand side that
+    converges early is masked while the others continue."""
     from amp_amd import hostsetup as hs
     st = setup("highcontrast3d", items(dict(nx=14, seed=5, contrast=1e6, channel_fraction=0.1)), 8, items(dict(nev=12)))
-    print("case highcontrast3d 14^3 N=8 8 rhs", flush=True)
-    M = _ctx(st)
+    print(f"case highcontrast3d 14^3 N=8 8 rhs {precision}", flush=True)
+    M = _ctx(st, precision)
     B = np.stack([hs.random_rhs(st.n, 500 + k) for k in range(8)])
     B[3] *= 1e-6
     X, R = M.ctx.krylov(B, None, "gmres", 30, 1e-8, 500)
@@ -166,6 +168,6 @@ def test_many_rhs_gmres_matches_single_solves():
         x, r = M.ctx.krylov(B[k], None, "gmres", 30, 1e-8, 500)
         assert R[k].converged and r.converged
         assert abs(R[k].iterations - r.iterations) <= 1
-        assert np.linalg.norm(X[k] - x) / np.linalg.norm(x) < 1e-7
+        assert np.linalg.norm(X[k] - x) / np.linalg.norm(x) < (1e-7 if precision == "double" else 1e-6)
         t = np.linalg.norm(B[k] - st.dec.A @ X[k]) / np.linalg.norm(B[k])
         assert t <= 1.1e-8
