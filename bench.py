@@ -181,8 +181,10 @@ def main():
                          "rocprofv3 --pmc, which initialises the GPU before the program starts)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the path would shard by subdomain across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the apply path runs on one GPU; --gpus {args.gpus} is not supported (it would "
+              f"shard by subdomain: a range of subdomains per GPU, a halo exchange per apply, an "
+              f"all-gather of the coarse right-hand side and three all-reduces per GMRES iteration)",
+              file=sys.stderr)
         sys.exit(2)
     _load()
     from amp_amd import device, hostsetup as hs
