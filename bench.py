@@ -40,6 +40,9 @@ WORKLOADS = {
                      "3D Poisson 7-point 64^3, 64 subdomains, overlap 2, nev 24, GMRES(50), rtol 1e-8"),
     "poisson2d-256": ("poisson2d", dict(nx=256), 8, 1, 20, 30,
                       "2D Poisson 5-point 256^2, 8 subdomains, overlap 1, nev 20, additive, GMRES(30), rtol 1e-8"),
+    "bubbles3d-128": ("bubbles3d", dict(nx=128, nbubbles=200, rho_ratio=1000.0, seed=2024), 256, 1, 8, 50,
+                      "bubbly pressure 7-point 128^3 (200 seeded bubbles, density ratio 1000), 256 subdomains, "
+                      "overlap 1, nev 8, GMRES(50), rtol 1e-8"),
     "highcontrast3d-96": ("highcontrast3d", dict(nx=96, seed=2024, contrast=1e6, channel_fraction=0.1), 128, 1, 32, 50,
                           "3D diffusion 7-point 96^3 with seeded axis-aligned channels of coefficient 1e6 (10 % of "
                           "the volume), 128 subdomains, overlap 1, nev 32, 16 right-hand sides solved together, "
