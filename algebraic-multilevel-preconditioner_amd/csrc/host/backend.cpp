@@ -1,5 +1,7 @@
 // backend.cpp -- C++17 host classes over the C-ABI (include/s2l_backend.hpp).
 #include "s2l_backend.hpp"
+#include <cmath>
+#include <algorithm>
 
 #include <stdexcept>
 
@@ -110,6 +112,90 @@ UpdateLevel HipTwoLevelSchwarz::update(UpdateLevel requested)
     if (up_to_date_) return UpdateLevel::Reuse;
     if (requested == UpdateLevel::Reuse) return UpdateLevel::Reuse;
     throw std::logic_error("the preconditioner is stale: refresh() it with the refactorized values");
+}
+
+// ---- ReusePolicy (src/preconditioner/reuse_policy.cpp:14-84): the same rules, in this file's terms
+ReusePolicy::Mode ReusePolicy::parseMode(const std::string& s)
+{
+    if (s == "off") return Mode::Off;
+    if (s == "refactor") return Mode::Refactor;
+    if (s == "auto") return Mode::Auto;
+    throw std::invalid_argument("unknown reuse mode '" + s + "' (off|refactor|auto)");
+}
+
+double ReusePolicy::perDecade(int iterations, double reduction)
+{
+    const bool defined = iterations > 0 && reduction > 0.0 && reduction < 1.0;
+    return defined ? iterations / std::log10(1.0 / reduction) : 0.0;
+}
+
+double ReusePolicy::lowerMedian(std::vector<double> v)
+{
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    return v[(v.size() - 1) / 2];
+}
+
+double ReusePolicy::rebuildPremium() const
+{
+    const double rebuild = lowerMedian(t_rebuild_);
+    if (!(rebuild > 0.0)) return 0.0;
+    return std::max(0.0, rebuild - lowerMedian(t_refactor_));
+}
+
+double ReusePolicy::reference(int kind) const
+{
+    return (kind >= 0 && static_cast<std::size_t>(kind) < ref_.size()) ? ref_[static_cast<std::size_t>(kind)] : 0.0;
+}
+
+UpdateLevel ReusePolicy::decide(bool matrix_changed) const
+{
+    switch (mode_) {
+    case Mode::Off:
+        return UpdateLevel::Rebuild;
+    case Mode::Refactor:
+        return matrix_changed ? UpdateLevel::Refactorize : UpdateLevel::Reuse;
+    case Mode::Auto:
+        break;
+    }
+    if (!matrix_changed) return UpdateLevel::Reuse;
+    const bool not_converged = !converged_;
+    const bool too_old = max_age_ > 0 && age_ >= max_age_;
+    const bool too_slow = growth_ > 0.0 && last_ref_ > 0.0 && last_ > growth_ * last_ref_;
+    const double premium = rebuildPremium();
+    const bool break_even = premium > 0.0 && wasted_ >= premium;
+    return (not_converged || too_old || too_slow || break_even) ? UpdateLevel::Rebuild : UpdateLevel::Refactorize;
+}
+
+void ReusePolicy::record(UpdateLevel done, int iterations, double reduction, bool converged, double t_update,
+                         double t_solve, int kind)
+{
+    if (done == UpdateLevel::Rebuild) {
+        // a fresh basis: every kind measures a new reference, nothing is wasted yet
+        age_ = 0;
+        wasted_ = 0.0;
+        last_ref_ = 0.0;
+        std::fill(ref_.begin(), ref_.end(), 0.0);
+        if (t_update > 0.0) t_rebuild_.push_back(t_update);
+    } else if (done == UpdateLevel::Refactorize) {
+        age_ += 1;
+        if (t_update > 0.0) t_refactor_.push_back(t_update);
+    }
+    converged_ = converged;
+    const double q = perDecade(iterations, reduction);
+    if (!(q > 0.0)) return;
+    last_ = q;
+    const std::size_t k = static_cast<std::size_t>(std::max(kind, 0));
+    if (ref_.size() <= k) ref_.resize(k + 1, 0.0);
+    if (!(ref_[k] > 0.0)) {
+        ref_[k] = q;
+        last_ref_ = q;
+        return;
+    }
+    last_ref_ = ref_[k];
+    // iterations beyond those the reference rate needs for this reduction, at this solve's cost each
+    const double extra = iterations - ref_[k] * std::log10(1.0 / reduction);
+    if (extra > 0.0) wasted_ += extra * (t_solve / iterations);
 }
 
 } // namespace prec

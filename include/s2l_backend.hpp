@@ -116,6 +116,49 @@ private:
     bool up_to_date_ = true;
 };
 
+/// Chooses the update level before each solve of a sequence of matrices with one sparsity pattern
+/// (prec::ReusePolicy, include/preconditioner/reuse_policy.hpp:20-136). Off rebuilds before every
+/// solve; Refactor never rebuilds after the first setup; Auto reuses an unchanged matrix,
+/// refactorizes a changed one and rebuilds once the time lost to the aging coarse basis reaches
+/// the measured extra cost of a rebuild (the lower medians of the rebuild and refactorization
+/// times), or when a safety net fires: no convergence, q above growth times its reference, more
+/// than max_age refactorizations. q is iterations per decade of residual reduction; every kind of
+/// solve (e.g. a corrector index) has its own reference, the q of its first solve after a rebuild.
+class ReusePolicy {
+public:
+    enum class Mode { Off, Refactor, Auto };
+    ReusePolicy(Mode mode, double growth, int max_age) : mode_(mode), growth_(growth), max_age_(max_age) {}
+    /// "off" | "refactor" | "auto"; std::invalid_argument otherwise.
+    static Mode parseMode(const std::string& s);
+    /// Level to perform before the next solve.
+    UpdateLevel decide(bool matrix_changed) const;
+    /// Outcome of a solve performed after @p done (t_update: its time, t_solve: the solve's) [s].
+    void record(UpdateLevel done, int iterations, double reduction, bool converged, double t_update, double t_solve,
+                int kind = 0);
+    /// iterations / log10(1 / reduction); 0 when undefined (no iteration, reduction outside (0, 1)).
+    static double perDecade(int iterations, double reduction);
+    /// Element of rank floor((n - 1) / 2) in increasing order; 0 for none.
+    static double lowerMedian(std::vector<double> v);
+    /// Lower median of the rebuild times minus that of the refactorization times (>= 0; 0 = unknown).
+    double rebuildPremium() const;
+    double reference(int kind = 0) const;
+    double last() const { return last_; }
+    int age() const { return age_; }
+    double wasted() const { return wasted_; }
+    Mode mode() const { return mode_; }
+
+private:
+    Mode mode_;
+    double growth_;
+    int max_age_;
+    std::vector<double> ref_;       ///< per kind: q of the first solve after the last rebuild (0 = none yet)
+    double last_ = 0.0, last_ref_ = 0.0;
+    int age_ = 0;
+    bool converged_ = true;
+    double wasted_ = 0.0;           ///< seconds lost to the old basis since the last rebuild
+    std::vector<double> t_rebuild_, t_refactor_;
+};
+
 } // namespace prec
 
 namespace solv {

@@ -1,0 +1,120 @@
+"""The reuse policy (prec::ReusePolicy, include/preconditioner/reuse_policy.hpp:20-136) against the
+cases of the reference's own test (test/test_reuse.cpp:50-110), in its Python form and in the C++
+backend (a test program built with g++ and run on the CPU). A planted mistake (the break-even
+comparison without the safety nets' reference) must fail one of the cases."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from amp_amd.reuse import REBUILD, REFACTORIZE, REUSE, ReusePolicy as P
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "algebraic-multilevel-preconditioner_amd")
+
+
+def _near(a, b):
+    return abs(a - b) < 1e-12
+
+
+def policy_cases(P):
+    """The reference's policy checks; returns the names of the failing ones."""
+    bad = []
+    if not (_near(P.per_decade(10, 1e-5), 2.0) and P.per_decade(0, 1e-5) == 0.0 and P.per_decade(5, 1.0) == 0.0):
+        bad.append("per decade")
+    a = P("auto", 0.0, 0)
+    a.record(REBUILD, 10, 1e-8, True, 2.0, 1.0)
+    ok = a.decide(False) == REUSE and a.decide(True) == REFACTORIZE and a.wasted() == 0.0
+    a.record(REFACTORIZE, 15, 1e-8, True, 0.2, 1.5)
+    ok = ok and _near(a.wasted(), 0.5) and a.decide(True) == REFACTORIZE
+    a.record(REUSE, 15, 1e-8, True, 0.0, 1.5)
+    a.record(REFACTORIZE, 19, 1e-8, True, 0.2, 1.9)
+    ok = ok and _near(a.wasted(), 1.9) and _near(a.rebuild_premium(), 1.8)
+    ok = ok and a.decide(True) == REBUILD and a.decide(False) == REUSE
+    a.record(REBUILD, 12, 1e-8, True, 2.5, 1.2)
+    ok = ok and a.decide(True) == REFACTORIZE and a.wasted() == 0.0 and _near(a.reference(), 1.5)
+    if not ok:
+        bad.append("break-even")
+    g = P("auto", 1.5, 0)
+    g.record(REBUILD, 10, 1e-8, True, 1e9, 1.0)
+    g.record(REFACTORIZE, 16, 1e-8, True, 0.1, 1.6)
+    ok = g.decide(True) == REBUILD
+    n = P("auto", 0.0, 0)
+    n.record(REBUILD, 10, 1e-8, True, 1e9, 1.0)
+    n.record(REFACTORIZE, 500, 1e-3, False, 0.1, 50.0)
+    ok = ok and n.decide(True) == REBUILD
+    b = P("auto", 0.0, 2)
+    b.record(REBUILD, 10, 1e-8, True, 1e9, 1.0)
+    ok = ok and b.decide(True) == REFACTORIZE
+    b.record(REFACTORIZE, 10, 1e-8, True, 0.1, 1.0)
+    ok = ok and b.decide(True) == REFACTORIZE
+    b.record(REFACTORIZE, 10, 1e-8, True, 0.1, 1.0)
+    ok = ok and b.decide(True) == REBUILD
+    if not ok:
+        bad.append("safety nets")
+    ok = (P.lower_median([]) == 0.0 and P.lower_median([3.0]) == 3.0 and P.lower_median([5.0, 1.0]) == 1.0
+          and P.lower_median([4.0, 1.0, 2.0]) == 2.0 and P.lower_median([4.0, 1.0, 2.0, 3.0]) == 2.0)
+    s = P("auto", 0.0, 0)
+    s.record(REBUILD, 10, 1e-8, True, 2.0, 1.0)
+    s.record(REFACTORIZE, 10, 1e-8, True, 0.2, 1.0)
+    s.record(REBUILD, 10, 1e-8, True, 30.0, 1.0)
+    ok = ok and _near(s.rebuild_premium(), 1.8)
+    s.record(REFACTORIZE, 10, 1e-8, True, 9.0, 1.0)
+    ok = ok and _near(s.rebuild_premium(), 1.8)
+    if not ok:
+        bad.append("lower median")
+    off = P("off", 0.0, 0)
+    if not (off.decide(False) == REBUILD and off.decide(True) == REBUILD):
+        bad.append("off")
+    r = P("refactor", 0.0, 0)
+    r.record(REBUILD, 10, 1e-8, True, 2.0, 1.0)
+    r.record(REFACTORIZE, 900, 1e-3, False, 0.1, 90.0)
+    if not (r.decide(False) == REUSE and r.decide(True) == REFACTORIZE):
+        bad.append("refactor")
+    return bad
+
+
+def test_python_policy_matches_reference_cases():
+    assert policy_cases(P) == []
+    with pytest.raises(ValueError):
+        P("sometimes")
+
+
+def test_per_kind_references():
+    """Solves of another kind (e.g. the second corrector) are measured against their own first solve
+    after a rebuild, so a kind that is slower by nature does not count as aging."""
+    p = P("auto", 0.0, 0)
+    p.record(REBUILD, 10, 1e-8, True, 2.0, 1.0, kind=0)
+    p.record(REUSE, 20, 1e-8, True, 0.0, 2.0, kind=1)   # first solve of kind 1: its reference
+    assert p.wasted() == 0.0 and _near(p.reference(1), 2.5)
+    p.record(REFACTORIZE, 20, 1e-8, True, 0.2, 2.0, kind=1)
+    assert p.wasted() == 0.0
+    p.record(REFACTORIZE, 12, 1e-8, True, 0.2, 1.2, kind=0)   # 2 iterations over kind 0's reference
+    assert _near(p.wasted(), 0.2)
+
+
+def test_planted_break_even_mistake_is_caught(monkeypatch):
+    """Rebuilding as soon as anything is wasted (the premium left out) fails the reference's cases."""
+    def decide(self, matrix_changed):
+        if self.mode == "off":
+            return REBUILD
+        if not matrix_changed:
+            return REUSE
+        return REBUILD if self._wasted > 0.0 else REFACTORIZE
+    monkeypatch.setattr(P, "decide", decide)
+    assert "break-even" in policy_cases(P)
+
+
+def test_cpp_policy_matches_reference_cases(tmp_path):
+    """The C++ ReusePolicy of the backend (include/s2l_backend.hpp) on the same cases, on the CPU."""
+    lib = os.path.join(PKG, "libs2l_backend.so")
+    if not os.path.exists(lib) or shutil.which("g++") is None:
+        pytest.skip("the backend library or g++ is missing (build with build.py all)")
+    exe = str(tmp_path / "test_reuse_policy")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(REPO, "include"), "-o", exe,
+                           os.path.join(REPO, "tests", "cpp", "test_reuse_policy.cpp"), "-L", PKG, "-ls2l_backend",
+                           "-ls2l_hip", f"-Wl,-rpath,{PKG}"])
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "0 failure(s)" in p.stdout
