@@ -390,6 +390,7 @@ class HipTwoLevelSchwarz:
         self.precision = precision
         self.one_level = one_level or setup.one_level
         self.correction = correction or setup.correction
+        self.tasks_per_subdomain = tasks_per_subdomain
         self.ctx = HipContext(device)
         self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain)
 
@@ -400,11 +401,25 @@ class HipTwoLevelSchwarz:
         """OneLevelSchwarz::name + ",deflated"/",additive" (two_level_schwarz.cpp:129-132)."""
         return self.one_level.upper() + ("," + self.correction)
 
-    def update(self, level: str = "reuse") -> str:
-        """UpdateLevel semantics (preconditioner.hpp:30): nothing is recomputed on the device."""
+    def refresh(self, setup) -> str:
+        """Refactorize: uploads the operator, the local factors and A_00 of ``setup`` (new values of
+        A with the same pattern and coarse basis, hostsetup.refactorize) into this context."""
+        self.setup = setup
+        self.ctx.upload_setup(setup, self.precision, self.one_level, self.correction, self.tasks_per_subdomain)
+        return "refactorize"
+
+    def update(self, level: str = "reuse", setup=None) -> str:
+        """UpdateLevel semantics (preconditioner.hpp:30): reuse keeps everything; refactorize
+        uploads the refactorized host setup given; rebuild needs a new object (the host setup of a
+        new coarse basis is not part of this class): ValueError."""
         if level not in ("reuse", "refactorize", "rebuild"):
             raise ValueError(f"unknown update level '{level}'")
-        return "reuse"
+        if level == "reuse":
+            return "reuse"
+        if level == "refactorize" and setup is not None:
+            return self.refresh(setup)
+        raise ValueError(f"update level '{level}' needs " + ("the refactorized setup" if level == "refactorize"
+                                                              else "a new preconditioner object"))
 
     def up_to_date(self) -> bool:
         return True

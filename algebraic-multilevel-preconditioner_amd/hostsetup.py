@@ -498,3 +498,36 @@ def build_setup(A, b, nparts, *, xyz=None, symmetric=None, block_size=1, overlap
         CoarseSpace([np.zeros((s.n_local, 0)) for s in subs], np.zeros(len(subs) + 1, np.int64), np.zeros((0, 0)))
     t["coarse"] = time.perf_counter() - t0
     return SchwarzSetup(dec, subs, factors, coarse, one_level, correction if correction != "none" else "additive", t)
+
+
+def refactorize(setup: SchwarzSetup, A_new, b_new=None, *, overlap: int = 1, pou: str = "boolean",
+                leaf: int = 0) -> SchwarzSetup:
+    """Refactorize level for new values of A with the same pattern (two_level_schwarz.cpp:35-41,
+    coarse_space.cpp:95-105,186-206, subdomain.cpp:107-148): the partition, the subdomains and the
+    coarse basis V are kept; the local matrices, their factors and A_00 = W^T A W are recomputed.
+    overlap and pou must be those of the setup. ValueError if the overlap sets changed (a new
+    pattern needs a rebuild)."""
+    t = {}
+    t0 = time.perf_counter()
+    dec0 = setup.dec
+    A = sp.csr_matrix(A_new)
+    A.sort_indices()
+    b = np.asarray(b_new, np.float64) if b_new is not None else dec0.b[dec0.old_to_new]
+    dec = decompose(A, b, dec0.part, dec0.symmetric, dec0.block_size)
+    subs = build_subdomains(dec, overlap, pou)
+    for s0, s1 in zip(setup.subs, subs):
+        if s0.n_owned != s1.n_owned or not np.array_equal(s0.ghosts, s1.ghosts):
+            raise ValueError(f"subdomain {s0.index}: the overlap changed with the new matrix; rebuild instead")
+    t["subdomains"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    kind = 0 if setup.factors[0].ut_colptr is None else 1
+    factors = factorize_all(subs, kind, leaf=leaf)
+    t["factorization"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    V = setup.coarse.V
+    W = assemble_W(dec, subs, V)
+    E = (W.T @ (dec.A @ W)).toarray() if W.shape[1] else np.zeros((0, 0))
+    t["coarse"] = time.perf_counter() - t0
+    return SchwarzSetup(dec, subs, factors, CoarseSpace(V, setup.coarse.offsets.copy(), E), setup.one_level,
+                        setup.correction, t)
+

@@ -1,0 +1,49 @@
+"""A sequence of solves with a reuse policy (src/solver/time_loop.cpp:57-195, restated for one GPU):
+every step generates the matrix of a moving field (the bubbles rise with the step), every
+corrector of a step solves it for a new right-hand side (time_loop.cpp:31-43), and the reuse
+policy decides before each solve whether the preconditioner is reused, refactorized (new local
+factors and A_00 with the same coarse basis, uploaded into the same device context) or rebuilt (a
+new host setup and context)."""
+import time
+
+from . import device, hostsetup as hs
+from .reuse import REBUILD, REFACTORIZE, REUSE, ReusePolicy
+
+
+def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts: int = 4, steps: int = 6,
+                  correctors: int = 2, reuse: str = "auto", growth: float = 4.0, max_age: int = 0,
+                  precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
+                  seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
+                  log=None) -> list:
+    """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
+    performed before it, iterations, convergence, true relative residual, update and solve times)."""
+    gen_kw = dict(gen_kw or {})
+    policy = ReusePolicy(reuse, growth, max_age)
+    setup = M = None
+    records = []
+    for step in range(steps):
+        A, _, xyz, sym, bs = hs.generate(family, **{**gen_kw, "time": step * dt})
+        for c in range(correctors):
+            b = hs.random_rhs_seq(A.shape[0], seed, step, c)
+            level = REBUILD if M is None else policy.decide(c == 0)
+            t0 = time.perf_counter()
+            if level == REBUILD:
+                setup = hs.build_setup(A, b, nparts, xyz=xyz, symmetric=sym, block_size=bs, overlap=overlap, pou=pou,
+                                       nev=nev)
+                M = device.HipTwoLevelSchwarz(setup, precision)
+            elif level == REFACTORIZE:
+                setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou)
+                M.refresh(setup)
+            t_update = time.perf_counter() - t0 if level != REUSE else 0.0
+            t0 = time.perf_counter()
+            x, res = device.HipGmres(restart, rtol, maxit).solve(M, b[setup.dec.new_to_old])
+            t_solve = time.perf_counter() - t0
+            # the solves start from 0: the reduction is the final relative residual
+            policy.record(level, res.iterations, res.true_rel_residual, bool(res.converged), t_update, t_solve, kind=c)
+            rec = dict(step=step, corrector=c, level=level, iterations=res.iterations, converged=bool(res.converged),
+                       true_rel_residual=res.true_rel_residual, t_update=t_update, t_solve=t_solve,
+                       wasted=policy.wasted())
+            records.append(rec)
+            if log:
+                log(rec)
+    return records

@@ -171,3 +171,52 @@ def test_many_rhs_gmres_matches_single_solves(precision):
         assert np.linalg.norm(X[k] - x) / np.linalg.norm(x) < (1e-7 if precision == "double" else 1e-6)
         t = np.linalg.norm(B[k] - st.dec.A @ X[k]) / np.linalg.norm(B[k])
         assert t <= 1.1e-8
+
+
+def test_refactorize_equals_fresh_setup_and_oracle():
+    """Refactorize (new values of A, same pattern and coarse basis: test/test_reuse.cpp:113-140):
+    uploading the refactorized host setup into the existing context gives bitwise the apply of a
+    new context built from it, and matches the oracle on the new values (1e-12)."""
+    from amp_amd import device, hostsetup as hs
+    kw = dict(nx=16, nbubbles=6, radius=0.12)
+    A0, _, xyz, sym, bs = hs.generate("bubbles3d", **kw, time=0.0)
+    st0 = hs.build_setup(A0, hs.random_rhs(A0.shape[0], 1), 4, xyz=xyz, symmetric=sym, block_size=bs, nev=6)
+    M = device.HipTwoLevelSchwarz(st0, "double")
+    A1, _, _, _, _ = hs.generate("bubbles3d", **kw, time=3.0)
+    assert (A1 != A0).nnz > 0
+    st1 = hs.refactorize(st0, A1, hs.random_rhs(A1.shape[0], 2))
+    print("case bubbles3d 16^3 N=4 refactorize", flush=True)
+    assert M.update("refactorize", st1) == "refactorize"
+    r = np.cos(np.linspace(0.0, 7.0, st1.n))
+    z = M.apply(r)
+    assert np.array_equal(z, device.HipTwoLevelSchwarz(st1, "double").apply(r))
+    zo = orc.OracleSchwarz(st1).apply(r)
+    assert np.linalg.norm(z - zo) / np.linalg.norm(zo) < 1e-12
+    assert M.update("reuse") == "reuse"
+    with pytest.raises(ValueError):
+        M.update("rebuild")
+
+
+@pytest.mark.parametrize("reuse", ["auto", "refactor", "off"])
+def test_time_loop_with_reuse_policy(reuse):
+    """A sequence of 4 steps x 2 correctors on rising bubbles (src/solver/time_loop.cpp:57-195): the
+    first solve rebuilds; the second corrector of a step reuses (the matrix did not change) except
+    with the Off policy; the first corrector of a later step refactorizes or rebuilds as the policy
+    says; every solve converges with a true residual <= 1.1e-8."""
+    from amp_amd.timeloop import run_time_loop
+    print(f"case time loop bubbles3d 16^3 {reuse}", flush=True)
+    recs = run_time_loop("bubbles3d", dict(nx=16, nbubbles=6, radius=0.12), nparts=4, steps=4, correctors=2,
+                         reuse=reuse, precision="single", restart=30, nev=6)
+    assert len(recs) == 8 and recs[0]["level"] == "rebuild"
+    for rec in recs:
+        assert rec["converged"] and rec["true_rel_residual"] <= 1.1e-8, rec
+    for rec in recs[1:]:
+        if reuse == "off":
+            assert rec["level"] == "rebuild"
+        elif rec["corrector"] == 1:
+            assert rec["level"] == "reuse"
+        elif reuse == "refactor":
+            assert rec["level"] == "refactorize"
+        else:
+            assert rec["level"] in ("refactorize", "rebuild")
+

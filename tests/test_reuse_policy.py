@@ -118,3 +118,39 @@ def test_cpp_policy_matches_reference_cases(tmp_path):
     p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "0 failure(s)" in p.stdout
+
+
+def _bubbles(time):
+    from amp_amd import hostsetup as hs
+    return hs.generate("bubbles3d", nx=12, nbubbles=5, radius=0.14, time=time)
+
+
+def test_host_refactorize_keeps_basis_and_refreshes_a00():
+    """Refactorize on the host (same partition, subdomains and coarse basis; new local factors and
+    A_00 = W^T A W): the deflation consistency R_0 (r - A x0) = 0 of the reference's reuse test
+    (test/test_reuse.cpp:229,336-346) holds with the new values, which it does not with the stale
+    A_00 (planted), and the new factors solve the new local matrices (backward error < 1e-12)."""
+    import numpy as np
+    from amp_amd import hostsetup as hs
+    from oracle import oracle as orc
+    A0, _, xyz, sym, bs = _bubbles(0.0)
+    st0 = hs.build_setup(A0, hs.random_rhs(A0.shape[0], 1), 4, xyz=xyz, symmetric=sym, block_size=bs, nev=5)
+    A1 = _bubbles(4.0)[0]
+    assert (A1 != A0).nnz > 0
+    st1 = hs.refactorize(st0, A1, hs.random_rhs(A1.shape[0], 2))
+    assert all(np.array_equal(v0, v1) for v0, v1 in zip(st0.coarse.V, st1.coarse.V))
+    r = np.cos(np.linspace(0.0, 7.0, st1.n))
+
+    def residual_of_deflation(st_values, st_coarse):
+        O = orc.OracleSchwarz(st_coarse)
+        w = r - st_values.dec.A @ O.coarse(r)
+        return max(np.abs(V.T @ w[sd.global_ids()]).max() for sd, V in zip(st_values.subs, st_values.coarse.V))
+
+    assert residual_of_deflation(st1, st1) < 1e-10 * np.abs(r).max()
+    assert residual_of_deflation(st1, st0) > 1e-6 * np.abs(r).max()   # stale A_00
+    rng = np.random.default_rng(5)
+    for sd, f in zip(st1.subs, st1.factors):
+        b = rng.standard_normal(sd.n_local)
+        x = orc.OracleLocalFactor(f, "double").solve(b)
+        err = np.abs(sd.Aii @ x - b).max() / (abs(sd.Aii).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+        assert err < 1e-12
