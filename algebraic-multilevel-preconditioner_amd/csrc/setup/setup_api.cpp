@@ -96,6 +96,70 @@ void s2l_random_rhs_seq(int64_t n, uint64_t seed, int step, int corrector, doubl
     for (int64_t i = 0; i < n; ++i) out[i] = dist(gen);
 }
 
+s2l_problem* s2l_read_matrix(const char* path)
+{
+    s2l_problem* out = nullptr;
+    if (guarded([&] {
+            if (!path) throw std::invalid_argument("read_matrix: null path");
+            auto* p = new s2l_problem;
+            try {
+                p->p = s2l::readMatrixMarket(path);
+            } catch (...) {
+                delete p;
+                throw;
+            }
+            out = p;
+        }) != 0)
+        return nullptr;
+    return out;
+}
+
+int s2l_read_vector(const char* path, double* out, int64_t cap, int64_t* n)
+{
+    return guarded([&] {
+        if (!path || !n) throw std::invalid_argument("read_vector: null argument");
+        const std::vector<double> v = s2l::readMatrixMarketVector(path);
+        *n = static_cast<int64_t>(v.size());
+        if (out && cap > 0) std::memcpy(out, v.data(), sizeof(double) * static_cast<std::size_t>(std::min<int64_t>(cap, *n)));
+    });
+}
+
+int s2l_write_matrix(const char* path, int64_t n, const int64_t* rowptr, const int64_t* col, const double* val)
+{
+    return guarded([&] {
+        if (!path || n < 0 || !rowptr || (rowptr[n] > 0 && (!col || !val))) throw std::invalid_argument("write_matrix: bad arguments");
+        s2l::writeMatrixMarket(path, n, rowptr, col, val);
+    });
+}
+
+int s2l_write_vector(const char* path, int64_t n, const double* v)
+{
+    return guarded([&] {
+        if (!path || n < 0 || (n > 0 && !v)) throw std::invalid_argument("write_vector: bad arguments");
+        s2l::writeMatrixMarketVector(path, n, v);
+    });
+}
+
+int64_t s2l_step_file_name(const char* pattern, int step, char* out, int64_t cap)
+{
+    int64_t len = -1;
+    if (guarded([&] {
+            if (!pattern) throw std::invalid_argument("step_file_name: null pattern");
+            const std::string name = s2l::stepFileName(pattern, step);
+            if (!out || cap < static_cast<int64_t>(name.size()) + 1)
+                throw std::invalid_argument("step_file_name: buffer too short");
+            std::memcpy(out, name.c_str(), name.size() + 1);
+            len = static_cast<int64_t>(name.size());
+        }) != 0)
+        return -1;
+    return len;
+}
+
+uint64_t s2l_hash_pattern(int64_t n, const int64_t* rowptr, const int64_t* col)
+{
+    return s2l::hashPattern(n, rowptr, col);
+}
+
 int s2l_partition(int64_t n, const int64_t* rowptr, const int64_t* col, int nparts, const double* xyz, int block_size,
                   int32_t* part)
 {

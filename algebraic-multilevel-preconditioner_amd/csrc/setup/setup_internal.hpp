@@ -31,6 +31,14 @@ GlobalCsr fromCoo(int64_t n, const std::vector<int64_t>& rows, const std::vector
                   const std::vector<double>& vals);
 Problem generate(const std::string& name, const s2l_gen_opts& o);
 
+/// Matrix Market input/output (mmio.cpp).
+Problem readMatrixMarket(const std::string& path);
+std::vector<double> readMatrixMarketVector(const std::string& path);
+void writeMatrixMarket(const std::string& path, int64_t n, const int64_t* rp, const int64_t* col, const double* val);
+void writeMatrixMarketVector(const std::string& path, int64_t n, const double* v);
+std::string stepFileName(const std::string& pattern, int step);
+uint64_t hashPattern(int64_t n, const int64_t* rp, const int64_t* col);
+
 /// Symmetric adjacency (no self loops) of an n-vertex graph, CSR with int32 indices.
 struct Graph {
     int32_t n = 0;

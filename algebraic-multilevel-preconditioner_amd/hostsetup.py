@@ -77,6 +77,146 @@ def random_rhs_seq(n: int, seed: int, step: int, corrector: int) -> np.ndarray:
     return out
 
 
+# --------------------------------------------------------------------------- Matrix Market files
+def _path(p) -> bytes:
+    return os.fsencode(os.fspath(p))
+
+
+def _problem_csr(L, h) -> sp.csr_matrix:
+    n = L.s2l_problem_n(h)
+    nnz = L.s2l_problem_nnz(h)
+    rp = np.empty(n + 1, np.int64)
+    col = np.empty(nnz, np.int64)
+    val = np.empty(nnz, np.float64)
+    L.s2l_problem_copy(h, ptr(rp, ctypes.c_int64), ptr(col, ctypes.c_int64), ptr(val, ctypes.c_double), None, None)
+    return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+
+def read_matrix(path) -> tuple[sp.csr_matrix, bool]:
+    """Sparse matrix of a Matrix Market file and its header's symmetric flag
+    (MatrixMarketIO::readMatrix, src/partition/matrix_market_io.cpp:92-140): coordinate real,
+    integer or pattern; general, symmetric (expanded) or skew-symmetric (expanded, mirror negated);
+    explicit zeros dropped, duplicates summed, rows sorted by column. RuntimeError on complex,
+    Hermitian or array files, a non-square matrix, indices out of range or a short file."""
+    L = _lib.setup_lib()
+    h = L.s2l_read_matrix(_path(path))
+    if not h:
+        raise RuntimeError(L.s2l_setup_last_error().decode())
+    try:
+        return _problem_csr(L, h), bool(L.s2l_problem_symmetric(h))
+    finally:
+        L.s2l_problem_free(h)
+
+
+def read_vector(path) -> np.ndarray:
+    """Dense vector of a Matrix Market file: array with one column, or coordinate
+    (MatrixMarketIO::readVector, matrix_market_io.cpp:142-172)."""
+    L = _lib.setup_lib()
+    n = ctypes.c_int64(0)
+    if L.s2l_read_vector(_path(path), None, 0, ctypes.byref(n)) != 0:
+        raise RuntimeError(L.s2l_setup_last_error().decode())
+    out = np.empty(n.value, np.float64)
+    if L.s2l_read_vector(_path(path), ptr(out, ctypes.c_double), out.size, ctypes.byref(n)) != 0:
+        raise RuntimeError(L.s2l_setup_last_error().decode())
+    return out
+
+
+def write_matrix(path, A) -> None:
+    """Writes A as `coordinate real general`, 17 significant digits (matrix_market_io.cpp:174-186)."""
+    L = _lib.setup_lib()
+    A = sp.csr_matrix(A)
+    if A.shape[0] != A.shape[1]:
+        raise ValueError("write_matrix: the matrix must be square")
+    rp = np.ascontiguousarray(A.indptr, np.int64)
+    col = np.ascontiguousarray(A.indices, np.int64)
+    val = np.ascontiguousarray(A.data, np.float64)
+    if L.s2l_write_matrix(_path(path), A.shape[0], ptr(rp, ctypes.c_int64), ptr(col, ctypes.c_int64),
+                          ptr(val, ctypes.c_double)) != 0:
+        raise RuntimeError(L.s2l_setup_last_error().decode())
+
+
+def write_vector(path, v) -> None:
+    """Writes v as `array real general` with one column (matrix_market_io.cpp:188-194)."""
+    L = _lib.setup_lib()
+    v = np.ascontiguousarray(v, np.float64).reshape(-1)
+    if L.s2l_write_vector(_path(path), v.size, ptr(v, ctypes.c_double)) != 0:
+        raise RuntimeError(L.s2l_setup_last_error().decode())
+
+
+def step_file_name(pattern: str, step: int) -> str:
+    """Name of step ``step`` of a file sequence: one %d / %Nd / %0Nd conversion takes the step, a
+    name without '%' is returned as is (domain_decomposer.cpp:80-97). ValueError on other patterns."""
+    L = _lib.setup_lib()
+    pat = pattern.encode()
+    buf = ctypes.create_string_buffer(len(pat) + 32)
+    n = L.s2l_step_file_name(pat, step, buf, len(buf))
+    if n < 0:
+        raise ValueError(L.s2l_setup_last_error().decode())
+    return buf.value.decode()
+
+
+def hash_pattern(A) -> int:
+    """Hash of the sparsity pattern (size, row pointers, columns) of a CSR matrix
+    (domain_decomposer.cpp:116-130); two steps of a sequence must have the same one."""
+    A = sp.csr_matrix(A)
+    rp = np.ascontiguousarray(A.indptr, np.int64)
+    col = np.ascontiguousarray(A.indices, np.int64)
+    return int(_lib.setup_lib().s2l_hash_pattern(A.shape[0], ptr(rp, ctypes.c_int64), ptr(col, ctypes.c_int64)))
+
+
+@dataclass
+class Problem:
+    """A read or generated global problem (GlobalProblem, domain_decomposer.cpp:66-74)."""
+
+    A: sp.csr_matrix
+    b: np.ndarray
+    symmetric: bool
+    block_size: int = 1
+    xyz: np.ndarray | None = None
+    description: str = ""
+
+
+def load_problem(*, matrix: str | None = None, generate_family: str | None = None, rhs: str = "random",
+                 seed: int = 12345, step: int = 0, block_size: int = 0, gen_kw: dict | None = None) -> Problem:
+    """The matrix of step ``step`` and its right-hand side (loadProblem, domain_decomposer.cpp:133-175):
+    ``matrix`` is a Matrix Market file or a step pattern (``p_%04d.mtx``), ``generate_family`` a
+    generator family (time = step); rhs is "random" (mt19937_64(seed), uniform[-1, 1] in the
+    original order), "ones", "generated" (the generator's) or a Matrix Market vector file. The
+    symmetric flag is the header's or the generator's, else detected numerically."""
+    if (matrix is None) == (generate_family is None):
+        raise ValueError("either matrix or generate_family must be given")
+    xyz = None
+    gen_rhs = None
+    if generate_family is not None:
+        A, gen_rhs, xyz, sym, bs = generate(generate_family, **{**(gen_kw or {}), "time": float(step)})
+        desc = generate_family
+    else:
+        name = step_file_name(matrix, step)
+        A, sym = read_matrix(name)
+        bs = 1
+        desc = f"{name} (n={A.shape[0]}, nnz={A.nnz})"
+    if block_size > 0:
+        bs = block_size
+    n = A.shape[0]
+    if n % bs != 0:
+        raise RuntimeError("n is not a multiple of the block size")
+    if not sym:
+        sym = is_numerically_symmetric(A)
+    if rhs == "random":
+        b = random_rhs(n, seed)
+    elif rhs == "ones":
+        b = np.ones(n)
+    elif rhs == "generated":
+        if gen_rhs is None:
+            raise RuntimeError("--rhs generated requires --generate")
+        b = gen_rhs
+    else:
+        b = read_vector(rhs)
+        if b.size != n:
+            raise RuntimeError("right-hand side size does not match the matrix")
+    return Problem(A, b, sym, bs, xyz, desc)
+
+
 def is_numerically_symmetric(A: sp.csr_matrix) -> bool:
     """|a_ij - a_ji| <= 1e-14 max|a| on both triangles (domain_decomposer.cpp:49-66)."""
     amax = np.abs(A.data).max() if A.nnz else 0.0
@@ -364,7 +504,11 @@ def local_eigenvectors(sd: Subdomain, nev: int, tau: float = 0.6, symmetric: boo
                 lu = spla.splu(As)
                 if symmetric:
                     Minv = spla.LinearOperator((n, n), matvec=lu.solve, dtype=np.float64)
-                    lam, U = spla.eigsh(B, k=nev, M=As, Minv=Minv, which="LA", tol=tol, ncv=min(ncv, n))
+                    # a fixed start vector per subdomain makes the setup reproducible (ARPACK draws
+                    # a random one otherwise); pseudo-random, so no symmetry of the subdomain hides
+                    # an eigenvector from it as a constant vector could
+                    v0 = np.random.default_rng(0x5EED + sd.index).uniform(-1.0, 1.0, n)
+                    lam, U = spla.eigsh(B, k=nev, M=As, Minv=Minv, which="LA", tol=tol, ncv=min(ncv, n), v0=v0)
                 else:
                     T = spla.LinearOperator((n, n), matvec=lambda x: lu.solve(B @ x), dtype=np.float64)
                     v0 = np.ones(n)
@@ -501,19 +645,27 @@ def build_setup(A, b, nparts, *, xyz=None, symmetric=None, block_size=1, overlap
 
 
 def refactorize(setup: SchwarzSetup, A_new, b_new=None, *, overlap: int = 1, pou: str = "boolean",
-                leaf: int = 0) -> SchwarzSetup:
+                leaf: int = 0, symmetric: bool | None = None) -> SchwarzSetup:
     """Refactorize level for new values of A with the same pattern (two_level_schwarz.cpp:35-41,
     coarse_space.cpp:95-105,186-206, subdomain.cpp:107-148): the partition, the subdomains and the
     coarse basis V are kept; the local matrices, their factors and A_00 = W^T A W are recomputed.
-    overlap and pou must be those of the setup. ValueError if the overlap sets changed (a new
-    pattern needs a rebuild)."""
+    overlap and pou must be those of the setup. ValueError, as the reference's updateValues, when
+    the size, the sparsity pattern or (if ``symmetric`` is given) the symmetry changed, or when
+    the overlap sets changed: all need a rebuild."""
     t = {}
     t0 = time.perf_counter()
     dec0 = setup.dec
     A = sp.csr_matrix(A_new)
     A.sort_indices()
+    # the checks of DomainDecomposer::updateValues (domain_decomposer.cpp:345-352)
+    if A.shape != dec0.A.shape:
+        raise ValueError("refactorize: the size changed; rebuild instead")
+    if symmetric is not None and bool(symmetric) != dec0.symmetric:
+        raise ValueError("refactorize: the symmetry of the matrix changed; rebuild instead")
     b = np.asarray(b_new, np.float64) if b_new is not None else dec0.b[dec0.old_to_new]
     dec = decompose(A, b, dec0.part, dec0.symmetric, dec0.block_size)
+    if hash_pattern(dec.A) != hash_pattern(dec0.A):
+        raise ValueError("refactorize: the sparsity pattern changed; rebuild instead")
     subs = build_subdomains(dec, overlap, pou)
     for s0, s1 in zip(setup.subs, subs):
         if s0.n_owned != s1.n_owned or not np.array_equal(s0.ghosts, s1.ghosts):

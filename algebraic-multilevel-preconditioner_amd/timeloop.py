@@ -1,5 +1,6 @@
 """A sequence of solves with a reuse policy (src/solver/time_loop.cpp:57-195, restated for one GPU):
-every step generates the matrix of a moving field (the bubbles rise with the step), every
+every step generates the matrix of a moving field (the bubbles rise with the step) or reads it from
+the step's Matrix Market file (a ``p_%04d.mtx`` pattern, domain_decomposer.cpp:80-97,143-150), every
 corrector of a step solves it for a new right-hand side (time_loop.cpp:31-43), and the reuse
 policy decides before each solve whether the preconditioner is reused, refactorized (new local
 factors and A_00 with the same coarse basis, uploaded into the same device context) or rebuilt (a
@@ -14,15 +15,24 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                   correctors: int = 2, reuse: str = "auto", growth: float = 4.0, max_age: int = 0,
                   precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
                   seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
-                  log=None) -> list:
+                  log=None, matrix: str | None = None, coordinates: bool = True) -> list:
     """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
-    performed before it, iterations, convergence, true relative residual, update and solve times)."""
+    performed before it, iterations, convergence, true relative residual, update and solve times).
+    With ``matrix`` (a file name or a step pattern) the matrices are read, not generated; a step
+    whose size, pattern or symmetry differs from the first one raises ValueError on refactorize.
+    ``coordinates=False`` partitions generated matrices by their graph only, as read ones are."""
     gen_kw = dict(gen_kw or {})
     policy = ReusePolicy(reuse, growth, max_age)
     setup = M = None
     records = []
     for step in range(steps):
-        A, _, xyz, sym, bs = hs.generate(family, **{**gen_kw, "time": step * dt})
+        if matrix is not None:
+            A, sym = hs.read_matrix(hs.step_file_name(matrix, step))
+            xyz, bs = None, 1
+            sym = sym or hs.is_numerically_symmetric(A)
+        else:
+            A, _, xyz, sym, bs = hs.generate(family, **{**gen_kw, "time": step * dt})
+            xyz = xyz if coordinates else None
         for c in range(correctors):
             b = hs.random_rhs_seq(A.shape[0], seed, step, c)
             level = REBUILD if M is None else policy.decide(c == 0)
@@ -32,7 +42,7 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                                        nev=nev)
                 M = device.HipTwoLevelSchwarz(setup, precision)
             elif level == REFACTORIZE:
-                setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou)
+                setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou, symmetric=sym)
                 M.refresh(setup)
             t_update = time.perf_counter() - t0 if level != REUSE else 0.0
             t0 = time.perf_counter()

@@ -82,6 +82,33 @@ void s2l_random_rhs(int64_t n, uint64_t seed, double* out);
  * (src/solver/time_loop.cpp:31-43). */
 void s2l_random_rhs_seq(int64_t n, uint64_t seed, int step, int corrector, double* out);
 
+/* ---------------------------------------------------------------- Matrix Market files */
+
+/* Reads a sparse matrix (MatrixMarketIO::readMatrix, src/partition/matrix_market_io.cpp:92-140):
+ * `matrix coordinate real|integer|pattern general|symmetric|skew-symmetric`; symmetric storage is
+ * expanded (the mirror entry negated for skew-symmetric), explicit zeros dropped and duplicates
+ * summed. The matrix must be square (src/partition/domain_decomposer.cpp:153). The problem's
+ * symmetric flag is the header's (symmetric), its rhs and coordinates are empty and its block size
+ * 1. Returns NULL on error (complex / Hermitian / array files, bad sizes or indices, short file). */
+s2l_problem* s2l_read_matrix(const char* path);
+/* Reads a dense vector (MatrixMarketIO::readVector, matrix_market_io.cpp:142-172): array format
+ * with one column, or coordinate format (entries added into their rows). Stores its length in *n
+ * and, when out is not NULL, its first min(n, cap) entries into out. Returns 0 or -1. */
+int s2l_read_vector(const char* path, double* out, int64_t cap, int64_t* n);
+/* Writes A (CSR, int64 indices, n x n) as `coordinate real general` with 17 significant digits
+ * (MatrixMarketIO::writeMatrix, matrix_market_io.cpp:174-186). Returns 0 or -1. */
+int s2l_write_matrix(const char* path, int64_t n, const int64_t* rowptr, const int64_t* col, const double* val);
+/* Writes v as `array real general` with one column (MatrixMarketIO::writeVector, :188-194). */
+int s2l_write_vector(const char* path, int64_t n, const double* v);
+/* File name of step @p step of a sequence (stepFileName, src/partition/domain_decomposer.cpp:80-97):
+ * a pattern with one %d / %Nd / %0Nd conversion is formatted with the step; a name without '%' is
+ * returned unchanged. Writes at most cap bytes including the NUL. Returns the length, or -1 on a
+ * bad pattern or a buffer that is too short. */
+int64_t s2l_step_file_name(const char* pattern, int step, char* out, int64_t cap);
+/* Hash of a sparsity pattern (hashPattern, domain_decomposer.cpp:116-130): FNV-1a steps over the
+ * 8 little-endian bytes of n, of every row pointer and of every column index. */
+uint64_t s2l_hash_pattern(int64_t n, const int64_t* rowptr, const int64_t* col);
+
 /* ---------------------------------------------------------------- partition */
 
 /* Partition of G(A + A^T) into @p nparts parts. With @p xyz (n*3 coordinates) a recursive

@@ -220,3 +220,23 @@ def test_time_loop_with_reuse_policy(reuse):
         else:
             assert rec["level"] in ("refactorize", "rebuild")
 
+
+
+def test_time_loop_from_matrix_files_equals_generator(tmp_path):
+    """The same sequence read from Matrix Market files through a %02d step pattern gives the same
+    levels and iterations as the generator (test/test_reuse.cpp:447-476); both partition by the
+    graph alone, and the host setup is reproducible, so the residuals are the same too."""
+    from amp_amd import hostsetup as hs
+    from amp_amd.timeloop import run_time_loop
+    print("case time loop from files bubbles3d 16^3 refactor", flush=True)
+    kw = dict(nx=16, nbubbles=6, radius=0.12)
+    for s in range(3):
+        hs.write_matrix(tmp_path / f"seq_{s:02d}.mtx", hs.generate("bubbles3d", **kw, time=float(s))[0])
+    common = dict(nparts=4, steps=3, correctors=2, reuse="refactor", precision="single", restart=30, nev=6,
+                  coordinates=False)
+    ra = run_time_loop("bubbles3d", kw, **common)
+    rf = run_time_loop(matrix=str(tmp_path / "seq_%02d.mtx"), **common)
+    assert [r["level"] for r in rf] == [r["level"] for r in ra]
+    assert [r["iterations"] for r in rf] == [r["iterations"] for r in ra]
+    for a, f in zip(ra, rf):
+        assert f["converged"] and f["true_rel_residual"] == a["true_rel_residual"]
