@@ -76,6 +76,10 @@ def setup_lib():
     L.s2l_step_file_name.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     L.s2l_hash_pattern.restype = ctypes.c_uint64
     L.s2l_hash_pattern.argtypes = [ctypes.c_int64, _c_i64p, _c_i64p]
+    L.s2l_from_ldu.restype = ctypes.c_void_p
+    L.s2l_from_ldu.argtypes = [ctypes.c_int64, ctypes.c_int64, _c_i64p, _c_i64p, _c_dp, _c_dp, _c_dp, _c_i64p, _c_dp]
+    L.s2l_problem_rhs_size.restype = ctypes.c_int64
+    L.s2l_problem_rhs_size.argtypes = [ctypes.c_void_p]
     L.s2l_partition.restype = ctypes.c_int
     L.s2l_partition.argtypes = [ctypes.c_int64, _c_i64p, _c_i64p, ctypes.c_int, _c_dp, ctypes.c_int, _c_i32p]
     L.s2l_factorize.restype = ctypes.c_void_p

@@ -25,7 +25,7 @@ def _run(cmd):
 
 def build_setup():
     src = os.path.join(HERE, "csrc", "setup")
-    files = [os.path.join(src, f) for f in ("generators.cpp", "partition.cpp", "factor.cpp", "mmio.cpp", "setup_api.cpp")]
+    files = [os.path.join(src, f) for f in ("generators.cpp", "partition.cpp", "factor.cpp", "mmio.cpp", "ldu.cpp", "setup_api.cpp")]
     out = os.path.join(HERE, "libs2l_setup.so")
     _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v3", "-fPIC", "-shared", "-Wall", "-Wextra",
           "-I", INCLUDE, "-o", out] + files)

@@ -160,6 +160,27 @@ uint64_t s2l_hash_pattern(int64_t n, const int64_t* rowptr, const int64_t* col)
     return s2l::hashPattern(n, rowptr, col);
 }
 
+s2l_problem* s2l_from_ldu(int64_t n, int64_t nfaces, const int64_t* lower_addr, const int64_t* upper_addr,
+                          const double* diag, const double* upper, const double* lower, const int64_t* gid,
+                          const double* source)
+{
+    s2l_problem* out = nullptr;
+    if (guarded([&] {
+            auto* p = new s2l_problem;
+            try {
+                p->p = s2l::fromLdu(n, nfaces, lower_addr, upper_addr, diag, upper, lower, gid, source);
+            } catch (...) {
+                delete p;
+                throw;
+            }
+            out = p;
+        }) != 0)
+        return nullptr;
+    return out;
+}
+
+int64_t s2l_problem_rhs_size(const s2l_problem* p) { return static_cast<int64_t>(p->p.rhs.size()); }
+
 int s2l_partition(int64_t n, const int64_t* rowptr, const int64_t* col, int nparts, const double* xyz, int block_size,
                   int32_t* part)
 {

@@ -38,6 +38,9 @@ void writeMatrixMarket(const std::string& path, int64_t n, const int64_t* rp, co
 void writeMatrixMarketVector(const std::string& path, int64_t n, const double* v);
 std::string stepFileName(const std::string& pattern, int step);
 uint64_t hashPattern(int64_t n, const int64_t* rp, const int64_t* col);
+/// OpenFOAM LDU face storage to the global CSR matrix (ldu.cpp).
+Problem fromLdu(int64_t n, int64_t nfaces, const int64_t* l, const int64_t* u, const double* diag, const double* upper,
+                const double* lower, const int64_t* gid, const double* source);
 
 /// Symmetric adjacency (no self loops) of an n-vertex graph, CSR with int32 indices.
 struct Graph {

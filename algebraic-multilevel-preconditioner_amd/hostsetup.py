@@ -164,6 +164,40 @@ def hash_pattern(A) -> int:
     return int(_lib.setup_lib().s2l_hash_pattern(A.shape[0], ptr(rp, ctypes.c_int64), ptr(col, ctypes.c_int64)))
 
 
+def from_ldu(diag, lower_addr, upper_addr, upper, lower=None, gid=None, source=None):
+    """Global matrix (and right-hand side) of an OpenFOAM lduMatrix, as the reference's OpenFOAM
+    solver writes it to Matrix Market (openfoam/bubblyPressureFoam/bubblyPressureFoam.C:120-170):
+    diag[c] at (gid[c], gid[c]); face f gives upper[f] at (gid[l_f], gid[u_f]) and lower[f] at
+    (gid[u_f], gid[l_f]); lower None = symmetric (lower = upper); b[gid[c]] = source[c].
+    Returns (A csr, b or None, symmetric)."""
+    L = _lib.setup_lib()
+    d = np.ascontiguousarray(diag, np.float64)
+    la = np.ascontiguousarray(lower_addr, np.int64)
+    ua = np.ascontiguousarray(upper_addr, np.int64)
+    up = np.ascontiguousarray(upper, np.float64)
+    lo = None if lower is None else np.ascontiguousarray(lower, np.float64)
+    g = None if gid is None else np.ascontiguousarray(gid, np.int64)
+    src = None if source is None else np.ascontiguousarray(source, np.float64)
+    n, nf = d.size, la.size
+    if ua.size != nf or up.size != nf or (lo is not None and lo.size != nf) or (g is not None and g.size != n) \
+            or (src is not None and src.size != n):
+        raise ValueError("from_ldu: array sizes do not match the cells and faces")
+    h = L.s2l_from_ldu(n, nf, ptr(la, ctypes.c_int64), ptr(ua, ctypes.c_int64), ptr(d, ctypes.c_double),
+                       ptr(up, ctypes.c_double), ptr(lo, ctypes.c_double), ptr(g, ctypes.c_int64),
+                       ptr(src, ctypes.c_double))
+    if not h:
+        raise ValueError(L.s2l_setup_last_error().decode())
+    try:
+        A = _problem_csr(L, h)
+        b = None
+        if L.s2l_problem_rhs_size(h):
+            b = np.empty(n, np.float64)
+            L.s2l_problem_copy(h, None, None, None, ptr(b, ctypes.c_double), None)
+        return A, b, bool(L.s2l_problem_symmetric(h))
+    finally:
+        L.s2l_problem_free(h)
+
+
 @dataclass
 class Problem:
     """A read or generated global problem (GlobalProblem, domain_decomposer.cpp:66-74)."""

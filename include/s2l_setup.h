@@ -109,6 +109,23 @@ int64_t s2l_step_file_name(const char* pattern, int step, char* out, int64_t cap
  * 8 little-endian bytes of n, of every row pointer and of every column index. */
 uint64_t s2l_hash_pattern(int64_t n, const int64_t* rowptr, const int64_t* col);
 
+/* ---------------------------------------------------------------- OpenFOAM LDU matrices */
+
+/* The global problem of an OpenFOAM lduMatrix, built in memory as the reference's OpenFOAM solver
+ * writes it to a Matrix Market file (openfoam/bubblyPressureFoam/bubblyPressureFoam.C:120-170):
+ * A(gid[c], gid[c]) = diag[c] (D(), the diagonal with the boundary contributions), for every face
+ * f: A(gid[l[f]], gid[u[f]]) = upper[f] and A(gid[u[f]], gid[l[f]]) = lower[f] (lower NULL: a
+ * symmetric matrix, lower = upper), b[gid[c]] = source[c]. gid NULL is the identity; source NULL
+ * leaves the problem's rhs empty. Exact zeros are dropped and rows sorted, so the result equals
+ * the file route bitwise. The problem's symmetric flag is (lower == NULL). Returns NULL on error
+ * (a face with equal or out-of-range cells, gid not a permutation of 0..n-1). */
+s2l_problem* s2l_from_ldu(int64_t n, int64_t nfaces, const int64_t* lower_addr, const int64_t* upper_addr,
+                          const double* diag, const double* upper, const double* lower, const int64_t* gid,
+                          const double* source);
+/* Length of the problem's right-hand side (n for a generated problem or an LDU one with a source,
+ * 0 for a read matrix). */
+int64_t s2l_problem_rhs_size(const s2l_problem* p);
+
 /* ---------------------------------------------------------------- partition */
 
 /* Partition of G(A + A^T) into @p nparts parts. With @p xyz (n*3 coordinates) a recursive

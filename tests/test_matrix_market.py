@@ -5,7 +5,9 @@ What pins them: the reference's committed generator fixture (test/gamg/run_gamg_
 check_s2l*.mtx) read with this reader matches the digest in tests/golden (when the reference is
 present), and a write -> read round trip of the generated matrix matches it too; scipy.io.mmread,
 an independent parser, on files of every supported header; Python's printf formatting for the
-step names; the symmetry-detection cases of the reference's test/test_subdomain.cpp:105-147."""
+step names; the symmetry-detection cases of the reference's test/test_subdomain.cpp:105-147. The
+OpenFOAM LDU adapter is pinned by lduMatrix::Amul's face loop, by the OpenFOAM solver's own export
+format read back, and by the reference's check_of.mtx split into cells and faces."""
 import os
 
 import numpy as np
@@ -254,3 +256,81 @@ def test_refactorize_refuses_a_new_pattern():
         hs.refactorize(setup, hs.generate("poisson2d", nx=13)[0])
     with pytest.raises(ValueError, match="symmetry"):
         hs.refactorize(setup, A, rhs, symmetric=False)
+
+
+def _random_ldu(n=300, nfaces=900, seed=5):
+    rng = np.random.default_rng(seed)
+    faces = set()
+    while len(faces) < nfaces:
+        a, b = (int(x) for x in rng.integers(0, n, 2))
+        if a != b:
+            faces.add((min(a, b), max(a, b)))
+    faces = sorted(faces, key=lambda f: (f[0], f[1]))
+    la = np.array([f[0] for f in faces], np.int64)
+    ua = np.array([f[1] for f in faces], np.int64)
+    return (rng.uniform(1.0, 2.0, n), la, ua, rng.normal(size=nfaces), rng.normal(size=nfaces),
+            rng.permutation(n).astype(np.int64), rng.normal(size=n))
+
+
+def test_ldu_product_is_openfoams_amul():
+    """A x of the built matrix equals lduMatrix::Amul's face loop: y_c = D_c x_c, then for every
+    face y_l += upper_f x_u and y_u += lower_f x_l (with the cells renumbered by gid)."""
+    diag, la, ua, up, lo, gid, src = _random_ldu()
+    A, b, sym = hs.from_ldu(diag, la, ua, up, lo, gid, src)
+    assert not sym
+    x_cells = np.random.default_rng(9).normal(size=diag.size)
+    y = diag * x_cells
+    np.add.at(y, la, up * x_cells[ua])
+    np.add.at(y, ua, lo * x_cells[la])
+    x = np.empty_like(x_cells)
+    x[gid] = x_cells  # global order
+    assert np.allclose((A @ x)[gid], y, rtol=1e-14, atol=1e-14)
+    bc = np.empty_like(src)
+    bc[gid] = src
+    assert np.array_equal(b, bc)
+    # the orientation matters: with lower and upper swapped the product differs
+    A2, _, _ = hs.from_ldu(diag, la, ua, lo, up, gid, src)
+    assert not np.allclose((A2 @ x)[gid], y)
+
+
+def test_ldu_equals_the_file_route(tmp_path):
+    """The OpenFOAM solver's Matrix Market export (bubblyPressureFoam.C:120-170: the diagonal lines,
+    then each face's upper and lower lines, 1-based global cells, 17 digits), read back, is
+    bitwise the in-memory matrix; exact zeros are dropped on both routes."""
+    diag, la, ua, up, lo, gid, src = _random_ldu(seed=11)
+    up[::17] = 0.0
+    lines = ["%%MatrixMarket matrix coordinate real general", f"{diag.size} {diag.size} {diag.size + 2 * la.size}"]
+    lines += [f"{gid[c] + 1} {gid[c] + 1} {diag[c]:.17g}" for c in range(diag.size)]
+    for f in range(la.size):
+        lines.append(f"{gid[la[f]] + 1} {gid[ua[f]] + 1} {up[f]:.17g}")
+        lines.append(f"{gid[ua[f]] + 1} {gid[la[f]] + 1} {lo[f]:.17g}")
+    path = _write(tmp_path / "of.mtx", "\n".join(lines) + "\n")
+    A, _, _ = hs.from_ldu(diag, la, ua, up, lo, gid, src)
+    B, _ = hs.read_matrix(path)
+    assert _same_matrix(A, B)
+    S, _, sym = hs.from_ldu(diag, la, ua, up)  # symmetric lduMatrix: no lower coefficients
+    assert sym and (S != S.T).nnz == 0
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF_DIR, "check_of.mtx")), reason="reference fixture absent")
+def test_ldu_rebuilds_the_reference_openfoam_matrix():
+    """check_of.mtx (written by the OpenFOAM solver from its lduMatrix) split back into cells and
+    faces gives the same matrix through the adapter."""
+    A, _ = hs.read_matrix(os.path.join(REF_DIR, "check_of.mtx"))
+    U = sp.triu(A, k=1).tocoo()
+    Lt = sp.csr_matrix(sp.tril(A, k=-1).T)
+    lower = np.asarray(Lt[U.row, U.col]).ravel()
+    B, _, _ = hs.from_ldu(A.diagonal(), U.row, U.col, U.data, lower)
+    assert _same_matrix(A, B)
+
+
+def test_ldu_errors():
+    d = np.ones(3)
+    with pytest.raises(ValueError, match="invalid cells"):
+        hs.from_ldu(d, [0], [0], [1.0])
+    with pytest.raises(ValueError, match="invalid cells"):
+        hs.from_ldu(d, [0], [3], [1.0])
+    with pytest.raises(ValueError, match="permutation"):
+        hs.from_ldu(d, [0], [1], [1.0], gid=[0, 0, 1])
+    with pytest.raises(ValueError, match="sizes"):
+        hs.from_ldu(d, [0, 1], [1], [1.0])
