@@ -37,19 +37,28 @@ struct ApiError : std::runtime_error {
         if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));               \
     } while (0)
 
-/// Device buffer (hipMalloc), grown on demand.
+/// Byte counter of the context whose API call runs on this thread (set by guarded()).
+thread_local int64_t* t_device_bytes = nullptr;
+
+/// Device buffer (hipMalloc), grown on demand. Its bytes count toward the device footprint of
+/// the context that allocated it.
 template <typename T> struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    int64_t* acct = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (acct) *acct -= static_cast<int64_t>(n * sizeof(T));
+        }
         p = nullptr;
         n = 0;
+        acct = nullptr;
     }
     void reserve(size_t cnt)
     {
@@ -58,6 +67,8 @@ template <typename T> struct DevBuf {
         const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
         HIPCHECK(hipMalloc(reinterpret_cast<void**>(&p), bytes));
         n = std::max<size_t>(cnt, 1);
+        acct = t_device_bytes;
+        if (acct) *acct += static_cast<int64_t>(bytes);
     }
     void upload(const T* h, size_t cnt, hipStream_t s)
     {
@@ -162,6 +173,7 @@ constexpr size_t kMaxGraphs = 160;  // cached apply graphs per context (GMRES(m)
 
 struct s2l_ctx {
     int device = 0;
+    int64_t device_bytes = 0;   ///< bytes of the context's device buffers (DevBuf)
     hipStream_t stream = nullptr;
     // second stream of the level-synchronous local solve: the large supernodes of a level run on
     // it beside the small ones (fork and join by events on the caller's stream)
@@ -310,6 +322,11 @@ thread_local std::string g_err;
 
 template <typename F> int guarded(s2l_ctx* ctx, F&& f)
 {
+    struct Scope {
+        int64_t* prev;
+        ~Scope() { t_device_bytes = prev; }
+    } scope{t_device_bytes};
+    t_device_bytes = ctx ? &ctx->device_bytes : nullptr;
     try {
         f();
         return S2L_OK;
@@ -1711,6 +1728,14 @@ int s2l_synchronize(s2l_ctx* c)
     return guarded(c, [&] {
         require(c != nullptr, S2L_EINVAL, "synchronize: null context");
         HIPCHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int s2l_device_footprint(s2l_ctx* c, int64_t* bytes)
+{
+    return guarded(c, [&] {
+        require(c && bytes, S2L_EINVAL, "device_footprint: bad arguments");
+        *bytes = c->device_bytes;
     });
 }
 

@@ -57,7 +57,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_upload_local_factor", "s2l_finalize_local", "s2l_upload_coarse", "s2l_apply", "s2l_spmv",
            "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
            "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_copy_d2d", "s2l_synchronize",
-           "s2l_get_stats", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
+           "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs"]
 
 
@@ -90,6 +90,7 @@ def hip_lib():
         "s2l_copy_d2d": [vp, vp, vp, ctypes.c_int64, vp],
         "s2l_synchronize": [vp],
         "s2l_get_stats": [vp, P(Stats)],
+        "s2l_device_footprint": [vp, P(ctypes.c_int64)],
         "s2l_timing_enable": [vp, ctypes.c_int],
         "s2l_timing_read": [vp, P(ctypes.c_int64), P(ctypes.c_double), ctypes.c_int],
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
@@ -349,6 +350,12 @@ class HipContext:
         s = Stats()
         self._check(self.L.s2l_get_stats(self.h, ctypes.byref(s)))
         return s
+
+    def footprint(self) -> int:
+        """Bytes of device memory the context holds (s2l_device_footprint)."""
+        v = ctypes.c_int64(0)
+        self._check(self.L.s2l_device_footprint(self.h, ctypes.byref(v)))
+        return int(v.value)
 
     def timing(self, on: bool):
         self._check(self.L.s2l_timing_enable(self.h, 1 if on else 0))

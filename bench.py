@@ -151,6 +151,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def serving_main(args):
+    """The serving mix: a seeded Poisson-arrival stream of requests over 6 resident families
+    (2D 5-point, 3D 7-point, 2D anisotropic 1e-3; n log-uniform in [1e4, 2e6]), 1-8 right-hand
+    sides each, GMRES(50) to rtol with FP32 local factors, admitted by device footprint. Service
+    times are measured back to back on the GPU; the FIFO queue is replayed at 70 % offered load."""
+    from amp_amd import serving as sv
+    fams, reqs = sv.draw_mix(args.requests, 6, (1e4, 2e6), 8, seed=args.seed)
+    log("families:", ", ".join(f"{f.kind} nx={f.nx} (n={f.n})" for f in fams))
+    budget = int(args.serving_budget_gb * 2**30)
+    resident = sv.ResidentSet(budget, lambda k: sv.build_family(fams[k], precision="single"))
+    t0 = time.perf_counter()
+    for k in range(len(fams)):  # a server admits its resident families before taking requests
+        resident.get(k)
+        log(f"admitted family {k}: {resident.footprint(resident.live[k]) / 2**20:.1f} MiB on the device")
+    admit_s = time.perf_counter() - t0
+    recs = sv.serve(fams, reqs, resident, restart=50, rtol=args.rtol,
+                    log=lambda r: log(f"request {r['id']}: family {r['family']} nrhs {r['nrhs']} "
+                                      f"{1e3 * r['service']:.2f} ms {r['iterations']} its")
+                    if r["id"] % 100 == 0 else None)
+    t = sv.timeline(recs, 0.7)
+    out = {"metric": "serving mix: solves/s + p50/p99 request latency (FIFO, 70 % offered load)",
+           "value": round(t["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1, "steps": len(recs), "warmup": 0,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 local factors / f64 Krylov",
+           "data": "synthetic (generated matrices, seeded right-hand sides, seeded arrivals)",
+           "config": {"workload": f"serving mix: {len(recs)} requests, 6 resident families "
+                                  f"(2D 5-point, 3D 7-point, 2D anisotropic 1e-3; n log-uniform in [1e4, 2e6]), "
+                                  f"1-8 right-hand sides each, GMRES(50) to {args.rtol}",
+                      "families": [f"{f.kind}:{f.nx}" for f in fams], "parallelism": "single GPU, FIFO"},
+           "p50_ms": round(t["p50_ms"], 3), "p99_ms": round(t["p99_ms"], 3), "max_ms": round(t["max_ms"], 3),
+           "mean_service_ms": round(t["mean_service_ms"], 3), "arrival_rate_per_s": round(t["rate_per_s"], 2),
+           "capacity_solves_per_s": round(t["capacity_solves_per_s"], 2),
+           "all_converged": all(r["converged"] for r in recs),
+           "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
+           "admission": {"budget_bytes": budget, "builds": resident.builds, "evictions": resident.evictions,
+                         "admit_s": round(admit_s, 1),
+                         "resident_bytes": int(resident.used())}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +221,11 @@ def main():
     ap.add_argument("--setup-workers", type=int, default=0,
                     help="processes of the host eigensolves (0 = min(16, cores); 1 = no fork, needed under "
                          "rocprofv3 --pmc, which initialises the GPU before the program starts)")
+    ap.add_argument("--serving-mix", action="store_true",
+                    help="run the serving mix (Poisson arrivals over 6 resident families) instead of a workload")
+    ap.add_argument("--requests", type=int, default=2000, help="requests of the serving mix")
+    ap.add_argument("--serving-budget-gb", type=float, default=64.0,
+                    help="device memory the serving mix may keep resident (GiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the apply path runs on one GPU; --gpus {args.gpus} is not supported (it would "
@@ -192,6 +236,9 @@ def main():
     _load()
     from amp_amd import device, hostsetup as hs
 
+    if args.serving_mix:
+        serving_main(args)
+        return
     gen, kw, N, overlap, nev, restart, desc = WORKLOADS[args.workload]
     nrhs = args.nrhs or DEFAULT_NRHS.get(args.workload, 1)
     if args.profile_applies > 0 and args.profile_one_level:

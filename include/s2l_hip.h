@@ -171,6 +171,11 @@ typedef struct s2l_stats {
     double bytes_spmv;            /* algorithmic HBM bytes of one SpMV (1 rhs) */
 } s2l_stats;
 int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
+/* Bytes of device memory the context holds (its uploaded system, subdomains, factors, coarse
+ * space, graphs' buffers and the Krylov workspace grown so far); buffers of s2l_device_alloc are the
+ * caller's and not counted. No reference counterpart: the footprint a server admits resident
+ * matrices by. */
+int s2l_device_footprint(s2l_ctx* ctx, int64_t* bytes);
 
 /* Event timing of the batched local solve: when enabled, the launches of every local solve (one
  * apply's: all depth groups of the task queue, or all levels, or all subdomains) are bracketed by

@@ -240,3 +240,25 @@ def test_time_loop_from_matrix_files_equals_generator(tmp_path):
     assert [r["iterations"] for r in rf] == [r["iterations"] for r in ra]
     for a, f in zip(ra, rf):
         assert f["converged"] and f["true_rel_residual"] == a["true_rel_residual"]
+
+
+def test_serving_mix_small():
+    """A small serving mix end to end: every request converges, the families' measured device
+    footprints are positive, and a budget below their sum forces evictions (and rebuilds) by the
+    least-recently-used rule without breaking any request."""
+    from amp_amd import serving as sv
+    print("case serving mix 3 families n in [2e3, 2e4], 24 requests", flush=True)
+    fams, reqs = sv.draw_mix(24, 3, (2e3, 2e4), 4, seed=11)
+    build = lambda k: sv.build_family(fams[k], precision="single", rows_per_subdomain=1500, nev=6)
+    big = sv.ResidentSet(1 << 40, build)
+    recs = sv.serve(fams, reqs, big, restart=30)
+    assert all(r["converged"] and r["true_rel_residual"] <= 1.1e-8 for r in recs)
+    assert big.builds == len({q.family for q in reqs}) and big.evictions == 0
+    fp = {k: big.footprint(M) for k, M in big.live.items()}
+    assert all(v > 0 for v in fp.values())
+    small = sv.ResidentSet(max(fp.values()) + min(fp.values()) // 2, build)
+    recs2 = sv.serve(fams, reqs, small, restart=30)
+    assert small.evictions > 0 and small.builds > big.builds
+    assert [r["iterations"] for r in recs2] == [r["iterations"] for r in recs]
+    t = sv.timeline(recs2, 0.7)
+    assert t["p99_ms"] >= t["p50_ms"] > 0 and t["solves_per_s"] > 0
