@@ -15,12 +15,13 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                   correctors: int = 2, reuse: str = "auto", growth: float = 4.0, max_age: int = 0,
                   precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
                   seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
-                  log=None, matrix: str | None = None, coordinates: bool = True) -> list:
+                  log=None, matrix: str | None = None, coordinates: bool = True, krylov: str = "gmres") -> list:
     """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
     performed before it, iterations, convergence, true relative residual, update and solve times).
     With ``matrix`` (a file name or a step pattern) the matrices are read, not generated; a step
     whose size, pattern or symmetry differs from the first one raises ValueError on refactorize.
-    ``coordinates=False`` partitions generated matrices by their graph only, as read ones are."""
+    ``coordinates=False`` partitions generated matrices by their graph only, as read ones are.
+    ``krylov``: "gmres" (GMRES(restart)) or "bicgstab"."""
     gen_kw = dict(gen_kw or {})
     policy = ReusePolicy(reuse, growth, max_age)
     setup = M = None
@@ -46,7 +47,8 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                 M.refresh(setup)
             t_update = time.perf_counter() - t0 if level != REUSE else 0.0
             t0 = time.perf_counter()
-            x, res = device.HipGmres(restart, rtol, maxit).solve(M, b[setup.dec.new_to_old])
+            solver = device.HipGmres(restart, rtol, maxit) if krylov == "gmres" else device.HipBiCgStab(rtol, maxit)
+            x, res = solver.solve(M, b[setup.dec.new_to_old])
             t_solve = time.perf_counter() - t0
             # the solves start from 0: the reduction is the final relative residual
             policy.record(level, res.iterations, res.true_rel_residual, bool(res.converged), t_update, t_solve, kind=c)

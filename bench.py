@@ -190,6 +190,44 @@ def serving_main(args):
     print(json.dumps(out), flush=True)
 
 
+def reuse_main(args):
+    """The setup-reuse sequence: bubbly pressure matrices 128^3 (200 seeded bubbles, density ratio
+    1000) at pseudo-times 0..steps-1, 256 subdomains, overlap 1, nev 8, deflated; the first solve
+    builds the preconditioner, every later step refactorizes (new local factors and A_00, the same
+    coarse basis Z) and its second corrector reuses; GMRES(50) and BiCGStab, FP32 local factors."""
+    from amp_amd.timeloop import run_time_loop
+    kw = dict(nx=args.seq_nx, nbubbles=200, rho_ratio=1000.0, seed=2024)
+    nparts = max(4, (args.seq_nx ** 3) // 8192)
+    out = {"metric": "setup-reuse sequence: solves/s over the sequence (updates + solves)", "unit": "solves/s",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 local factors / f64 Krylov", "data": "synthetic (generated matrices, seeded right-hand sides)",
+           "config": {"workload": f"bubbly pressure 7-point {args.seq_nx}^3, 200 bubbles drifting with the step, "
+                                  f"{args.seq_steps} steps x 2 correctors, {nparts} subdomains, overlap 1, nev 8, "
+                                  f"deflated, reuse policy 'refactor' (factors refreshed, coarse basis kept)",
+                      "parallelism": "single GPU, all subdomains resident"}}
+    for krylov in ("gmres", "bicgstab"):
+        recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse="refactor",
+                             precision="single", restart=50, rtol=args.rtol, nev=8, krylov=krylov,
+                             log=lambda r: log(f"{krylov} step {r['step']} corrector {r['corrector']}: {r['level']} "
+                                               f"{r['t_update']:.2f} s + {1e3 * r['t_solve']:.1f} ms, "
+                                               f"{r['iterations']} its"))
+        upd = sum(r["t_update"] for r in recs)
+        sol = sum(r["t_solve"] for r in recs)
+        refac = [r["t_update"] for r in recs if r["level"] == "refactorize"]
+        out[krylov] = {"solves_per_s": round(len(recs) / (upd + sol), 3), "solves": len(recs),
+                       "solve_only_per_s": round(len(recs) / sol, 2),
+                       "mean_iterations": round(float(np.mean([r["iterations"] for r in recs])), 2),
+                       "first_iterations": recs[0]["iterations"], "last_iterations": recs[-1]["iterations"],
+                       "rebuild_s": round(recs[0]["t_update"], 2),
+                       "refactorize_s_mean": round(float(np.mean(refac)), 2) if refac else None,
+                       "solve_ms_mean": round(1e3 * sol / len(recs), 2),
+                       "all_converged": all(r["converged"] for r in recs),
+                       "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
+                       "levels": "".join({"rebuild": "2", "refactorize": "1", "reuse": "0"}[r["level"]] for r in recs)}
+    out["value"] = out["gmres"]["solves_per_s"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +264,10 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="requests of the serving mix")
     ap.add_argument("--serving-budget-gb", type=float, default=64.0,
                     help="device memory the serving mix may keep resident (GiB)")
+    ap.add_argument("--reuse-sequence", action="store_true",
+                    help="run the setup-reuse sequence (bubbly pressure, refactorized every step) instead")
+    ap.add_argument("--seq-steps", type=int, default=20, help="steps of the reuse sequence")
+    ap.add_argument("--seq-nx", type=int, default=128, help="grid of the reuse sequence (nx^3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the apply path runs on one GPU; --gpus {args.gpus} is not supported (it would "
@@ -238,6 +280,9 @@ def main():
 
     if args.serving_mix:
         serving_main(args)
+        return
+    if args.reuse_sequence:
+        reuse_main(args)
         return
     gen, kw, N, overlap, nev, restart, desc = WORKLOADS[args.workload]
     nrhs = args.nrhs or DEFAULT_NRHS.get(args.workload, 1)
