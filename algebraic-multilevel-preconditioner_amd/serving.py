@@ -9,7 +9,8 @@ process; the request loop is this library's: it reuses the reference's per-solve
 every launch). Service times are measured on the device request by request, back to back; the
 queue is then replayed on a virtual clock for the offered load (start = max(arrival, previous
 finish)), which is the timeline of a single FIFO server with those service times, without idle
-waits on the GPU."""
+waits on the GPU. serve_batched is the continuous-batching server: requests waiting for the same
+family join one solve that shares every launch, on the same virtual clock with measured times."""
 from __future__ import annotations
 
 import math
@@ -171,3 +172,65 @@ def timeline(records: list, load: float, *, exclude_builds: bool = True) -> dict
     return dict(load=load, rate_per_s=1.0 / scale, mean_service_ms=1e3 * mean, p50_ms=1e3 * float(np.percentile(lat, 50)),
                 p99_ms=1e3 * float(np.percentile(lat, 99)), max_ms=1e3 * float(lat.max()), solves_per_s=nrhs / span,
                 requests_per_s=len(records) / span, capacity_solves_per_s=nrhs / float(svc.sum()))
+
+
+def serve_batched(fams: list, reqs: list, resident: ResidentSet, *, scale: float, max_rhs: int = 16,
+                  krylov: str = "gmres", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
+                  solve=None) -> list:
+    """Continuous batching on a virtual clock: arrivals at ``arrival * scale`` seconds; whenever the
+    server is free it takes the oldest waiting request and every other waiting request of the same
+    family (up to ``max_rhs`` right-hand sides), solves them together (each right-hand side's
+    result is that of its solve alone: the launches are shared, the arithmetic per right-hand side
+    is not) and advances the clock by the measured time. Returns per request its arrival, finish,
+    latency, batch size, iterations, convergence and true residual. ``solve(M, B) -> list`` replaces
+    the device solve in tests."""
+    def device_solve(M, B):
+        solver = device.HipGmres(restart, rtol, maxit) if krylov == "gmres" else device.HipBiCgStab(rtol, maxit)
+        _, res = solver.solve(M, B if B.shape[0] > 1 else B[0])
+        return res if isinstance(res, list) else [res]
+    solve = solve or device_solve
+    pending = sorted(reqs, key=lambda q: q.arrival)
+    waiting: list = []
+    out = []
+    t = 0.0
+    i = 0
+    while i < len(pending) or waiting:
+        while i < len(pending) and pending[i].arrival * scale <= t:
+            waiting.append(pending[i])
+            i += 1
+        if not waiting:
+            t = pending[i].arrival * scale
+            continue
+        fam = waiting[0].family
+        batch, rhs = [], 0
+        for q in waiting:
+            if q.family == fam and rhs + q.nrhs <= max(max_rhs, waiting[0].nrhs):
+                batch.append(q)
+                rhs += q.nrhs
+        waiting = [q for q in waiting if q not in batch]
+        t0 = time.perf_counter()
+        M = resident.get(fam)
+        n = M.setup.n
+        B = np.stack([hs.random_rhs(n, q.seed + r) for q in batch for r in range(q.nrhs)])
+        if hasattr(M, "setup") and hasattr(M.setup, "dec"):
+            B = B[:, M.setup.dec.new_to_old]
+        res = solve(M, B)
+        resident.fit(fam)
+        t += time.perf_counter() - t0
+        k = 0
+        for q in batch:
+            rr = res[k:k + q.nrhs]
+            k += q.nrhs
+            out.append(dict(id=q.id, family=fam, nrhs=q.nrhs, arrival=q.arrival * scale, finish=t,
+                            latency=t - q.arrival * scale, batch=rhs, iterations=max(r.iterations for r in rr),
+                            converged=all(r.converged for r in rr),
+                            true_rel_residual=max(r.true_rel_residual for r in rr)))
+    return sorted(out, key=lambda r: r["id"])
+
+
+def batched_summary(records: list) -> dict:
+    lat = np.array([r["latency"] for r in records])
+    span = max(r["finish"] for r in records) - min(r["arrival"] for r in records)
+    return dict(p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                max_ms=1e3 * float(lat.max()), solves_per_s=sum(r["nrhs"] for r in records) / span,
+                mean_batch_rhs=float(np.mean([r["batch"] for r in records])))

@@ -262,3 +262,24 @@ def test_serving_mix_small():
     assert [r["iterations"] for r in recs2] == [r["iterations"] for r in recs]
     t = sv.timeline(recs2, 0.7)
     assert t["p99_ms"] >= t["p50_ms"] > 0 and t["solves_per_s"] > 0
+
+
+def test_serving_results_same_alone_and_batched():
+    """Continuous batching: with every request waiting (arrivals compressed to 0) the requests of a
+    family are solved together, and each one's result is that of its solve alone within the stated
+    tolerance: the same iteration count, both converged to a true residual <= 1.1e-8, true
+    residuals within 5 % of each other (two or more FP32 right-hand sides run the float-pair
+    kernel, whose rounding may differ from the one-RHS kernel's)."""
+    from amp_amd import serving as sv
+    print("case serving batched vs alone, 2 families, 10 requests", flush=True)
+    fams, reqs = sv.draw_mix(10, 2, (2e3, 1e4), 4, seed=5)
+    rs = sv.ResidentSet(1 << 40, lambda k: sv.build_family(fams[k], precision="single", rows_per_subdomain=1500,
+                                                          nev=6))
+    alone = sv.serve(fams, reqs, rs, restart=30)
+    mixed = sv.serve_batched(fams, reqs, rs, scale=0.0, max_rhs=16, restart=30)
+    assert max(r["batch"] for r in mixed) > max(q.nrhs for q in reqs)
+    for a, b in zip(alone, mixed):
+        assert a["id"] == b["id"] and b["converged"]
+        assert a["iterations"] == b["iterations"]
+        assert max(a["true_rel_residual"], b["true_rel_residual"]) <= 1.1e-8
+        assert abs(a["true_rel_residual"] - b["true_rel_residual"]) <= 0.05 * a["true_rel_residual"]
