@@ -36,6 +36,20 @@ class GenOpts(ctypes.Structure):
 _setup = None
 _hip = None
 
+#: set once this process has created a device context (the HIP runtime is initialised)
+gpu_initialised = False
+
+
+def fork_is_safe() -> bool:
+    """Whether the host setup may fork worker processes: not after this process initialised the HIP
+    runtime, and not under rocprofv3, whose preloaded library initialises the GPU before the program
+    starts and aborts in forked children at their exit."""
+    if gpu_initialised:
+        return False
+    if "rocprof" in os.environ.get("LD_PRELOAD", "") or any(k.startswith("ROCPROF") for k in os.environ):
+        return False
+    return True
+
 
 def _load(name):
     path = os.path.join(HERE, name)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -169,6 +170,10 @@ bool denseLu(int n, std::vector<double>& A, std::vector<int32_t>& p)
 
 } // namespace
 
+namespace {
+struct GmresRun;
+}
+
 constexpr size_t kMaxGraphs = 160;  // cached apply graphs per context (GMRES(m): m inputs, m outputs if flexible)
 
 struct s2l_ctx {
@@ -187,6 +192,7 @@ struct s2l_ctx {
     int32_t n = 0, nsub = 0;
     int64_t nnz = 0;
     std::vector<int32_t> ownership;
+    std::vector<int32_t> rp_h, col_h;  ///< uploaded pattern (columns in the owners' local order)
     DevBuf<int32_t> rp, col;
     DevBuf<double> val;
     bool system_ready = false;
@@ -279,12 +285,10 @@ struct s2l_ctx {
     int64_t launches = 0;
     double ms_total = 0.0;
 
-    // Krylov
-    DevBuf<double> kV, kZ, kw, kH, kcs, ksn, kg, kh, kh2, ky, khist, kpart, kr, kz, ku, kred;
-    DevBuf<GmState> kst;
-    DevBuf<int32_t> kflags;
-    PinBuf<GmState> pst;
-    PinBuf<int32_t> pflags;
+    // Krylov: the GMRES run of s2l_krylov_solve (kept for the next solve of the same shape) and
+    // the reduction buffers of BiCGStab
+    std::shared_ptr<GmresRun> gm_cache;
+    DevBuf<double> kpart, kred;
     PinBuf<double> pred;
     // BiCGStab
     DevBuf<double> bR, bRh, bP, bV, bY, bS, bZ, bT, bcoef, bones;
@@ -627,184 +631,342 @@ void applySpmv(s2l_ctx* c, const double* in, double* z, double* w, int nrhs, con
     HIPCHECK(hipGraphLaunch(g.exec, s));
 }
 
-// ---- device GMRES (gmres.cpp:18-161), nrhs independent solves sharing every launch
+// ---- device GMRES (gmres.cpp:18-161) over `slots` right-hand sides sharing every launch. Each
+// slot is an independent solve: its sums, its masks and its decisions are its own, so its results
+// do not depend on which other slots run beside it. Slots enter (begin) and leave between restart
+// cycles, which is what a serving loop needs to admit requests into a running solve.
+struct GmresRun {
+    s2l_ctx* c = nullptr;
+    s2l_krylov_opts o{};
+    int cap = 0;          ///< slots = right-hand sides of every launch
+    int64_t n = 0;
+    bool of = false, flex = false;
+    GmArgs a{};
+    DevBuf<double> V, Z, w, r, z, u, H, cs, sn, g, h, h2, y, hist, part, red, ones, coef;
+    DevBuf<GmState> st;
+    DevBuf<int32_t> flags, mask;
+    PinBuf<GmState> pst;
+    PinBuf<int32_t> pflags, pmask;
+    PinBuf<double> pred, pcoef;
+    struct Slot {
+        bool live = false;      ///< holds a solve
+        bool conv = false, breakdown = false;
+        int iters = 0;
+        double bnorm = 0, beta = 0, rel = 0;
+        double normF = 1, l1_0 = 0, l1 = 0, thr = 0;  ///< OpenFOAM test
+        std::vector<double> hist;
+        bool active(int maxit) const { return live && !conv && iters < maxit; }
+    };
+    std::vector<Slot> sl;
+    std::vector<char> fresh;  ///< admitted since the last cycle (begin() pending)
+
+    GmresRun(s2l_ctx* ctx, const s2l_krylov_opts& opts, int slots) : c(ctx) { configure(opts, slots); }
+    /// options and slot count of the next solves; the device buffers only grow
+    void configure(const s2l_krylov_opts& opts, int slots);
+    double sum(int q, int k) const { return pred.p[3 * q + k]; }
+    /// per-slot sums (kind of launchReduce) of the slots whose mask entry is 0, on the host
+    void reduce(int kind, const double* x, const double* yv, hipStream_t s)
+    {
+        launchReduce(kind, n, x, yv, part.p, red.p, cap, mask.p, s);
+        HIPCHECK(hipMemcpyAsync(pred.p, red.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    void setMask(const std::vector<char>& on, hipStream_t s)
+    {
+        HIPCHECK(hipStreamSynchronize(s));  // earlier copies have read the staging buffer
+        for (int q = 0; q < cap; ++q) pmask.p[q] = on[static_cast<size_t>(q)] ? 0 : 1;
+        HIPCHECK(hipMemcpyAsync(mask.p, pmask.p, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
+    }
+    void begin(const std::vector<char>& on, const double* d_b, double* d_x, hipStream_t s);
+    bool cycle(const double* d_b, double* d_x, hipStream_t s);
+    void result(int q, s2l_krylov_result* R, double* history) const;
+};
+
+void GmresRun::configure(const s2l_krylov_opts& opts, int slots)
+{
+    o = opts;
+    cap = slots;
+    n = c->n;
+    const int m = o.restart;
+    require(m >= 1 && m + 1 <= kGmMaxK, S2L_ELIMIT, "GMRES restart must be in [1, 64] on the device");
+    require(o.stop_norm == 0 || o.stop_norm == 1, S2L_EINVAL, "stop_norm must be 0 (l2) or 1 (openfoam)");
+    require(o.maxit >= 0, S2L_EINVAL, "maxit must be >= 0");
+    require(o.atol == 0.0 || o.stop_norm == 1, S2L_EINVAL, "an absolute tolerance needs the OpenFOAM stopping norm");
+    require(slots >= 1, S2L_EINVAL, "at least one right-hand side");
+    of = o.stop_norm == 1;
+    flex = o.flexible != 0;
+    const int nb = reduceBlocks(n);
+    const size_t N = static_cast<size_t>(n) * cap;
+    V.reserve(static_cast<size_t>(m + 1) * N);
+    if (flex) Z.reserve(static_cast<size_t>(m) * N);
+    w.reserve(N);
+    r.reserve(N);
+    z.reserve(N);
+    u.reserve(N);
+    H.reserve(static_cast<size_t>(m + 1) * m * cap);
+    cs.reserve(static_cast<size_t>(m) * cap);
+    sn.reserve(static_cast<size_t>(m) * cap);
+    g.reserve(static_cast<size_t>(m + 1) * cap);
+    h.reserve(static_cast<size_t>(m + 1) * cap);
+    h2.reserve(static_cast<size_t>(m + 1) * cap);
+    y.reserve(static_cast<size_t>(m) * cap);
+    hist.reserve(static_cast<size_t>(o.maxit + 1) * cap);
+    part.reserve(static_cast<size_t>(nb) * kGmMaxK * cap);
+    red.reserve(static_cast<size_t>(3) * cap);
+    coef.reserve(static_cast<size_t>(4) * cap);
+    st.reserve(static_cast<size_t>(cap));
+    flags.reserve(static_cast<size_t>(2) * cap);
+    mask.reserve(static_cast<size_t>(cap));
+    pst.reserve(static_cast<size_t>(cap));
+    pflags.reserve(static_cast<size_t>(2) * cap);
+    pmask.reserve(static_cast<size_t>(cap));
+    pred.reserve(static_cast<size_t>(3) * cap);
+    pcoef.reserve(static_cast<size_t>(4) * cap);
+    sl.assign(static_cast<size_t>(cap), Slot{});
+    fresh.assign(static_cast<size_t>(cap), 0);
+    a.n = n;
+    a.m = m;
+    a.nrhs = cap;
+    a.V = V.p;
+    a.Z = flex ? Z.p : nullptr;
+    a.w = w.p;
+    a.H = H.p;
+    a.cs = cs.p;
+    a.sn = sn.p;
+    a.g = g.p;
+    a.h = h.p;
+    a.h2 = h2.p;
+    a.y = y.p;
+    a.st = st.p;
+    a.flags = flags.p;
+    a.hist = hist.p;
+    a.hist_ld = o.maxit + 1;
+    a.rtol = o.rtol;
+    a.partial = part.p;
+    a.nblocks = nb;
+    a.of = of ? 1 : 0;
+}
+
+// Start of the solves in the slots marked in @p on (gmres.cpp:26-55): ||b||, r = b - A x0, the
+// OpenFOAM normalisation and threshold, and the test of x0. Other slots are untouched.
+void GmresRun::begin(const std::vector<char>& on, const double* d_b, double* d_x, hipStream_t s)
+{
+    bool any = false;
+    for (int q = 0; q < cap; ++q) {
+        if (!on[static_cast<size_t>(q)]) continue;
+        any = true;
+        sl[static_cast<size_t>(q)] = Slot{};
+        sl[static_cast<size_t>(q)].live = true;
+    }
+    if (!any) return;
+    setMask(on, s);
+    reduce(0, d_b, nullptr, s);
+    std::vector<char> nz(static_cast<size_t>(cap), 0);
+    for (int q = 0; q < cap; ++q) {
+        if (!on[static_cast<size_t>(q)]) continue;
+        Slot& S = sl[static_cast<size_t>(q)];
+        S.bnorm = std::sqrt(sum(q, 0));
+        if (S.bnorm == 0.0) {  // x = 0, converged (gmres.cpp:28-33)
+            HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(q) * n, 0, sizeof(double) * n, s));
+            S.conv = true;
+            S.hist.push_back(0.0);
+        } else {
+            nz[static_cast<size_t>(q)] = 1;
+        }
+    }
+    bool anynz = false;
+    for (char v : nz) anynz = anynz || v;
+    if (!anynz) return;
+    setMask(nz, s);
+    if (!of) {
+        spmvImpl(c, d_x, r.p, d_b, cap, mask.p, s);  // r = b - A x
+        reduce(0, r.p, nullptr, s);
+    } else {
+        // F = sum|A x0 - xm A 1| + sum|b - xm A 1| + 1e-20, xm = mean(x0) (krylov.cpp:47-57)
+        spmvImpl(c, d_x, r.p, nullptr, cap, mask.p, s);  // r = A x0
+        reduce(5, d_x, nullptr, s);
+        std::vector<double> xm(static_cast<size_t>(cap), 0.0);
+        std::vector<char> zero_m(static_cast<size_t>(cap), 0), with_m(static_cast<size_t>(cap), 0);
+        for (int q = 0; q < cap; ++q) {
+            if (!nz[static_cast<size_t>(q)]) continue;
+            xm[static_cast<size_t>(q)] = n ? sum(q, 1) / static_cast<double>(n) : 0.0;
+            (xm[static_cast<size_t>(q)] == 0.0 ? zero_m : with_m)[static_cast<size_t>(q)] = 1;
+        }
+        bool az = false, aw = false;
+        for (int q = 0; q < cap; ++q) {
+            az = az || zero_m[static_cast<size_t>(q)];
+            aw = aw || with_m[static_cast<size_t>(q)];
+        }
+        if (az) {
+            setMask(zero_m, s);
+            reduce(1, r.p, nullptr, s);
+            for (int q = 0; q < cap; ++q)
+                if (zero_m[static_cast<size_t>(q)]) sl[static_cast<size_t>(q)].normF = sum(q, 1);
+            reduce(1, d_b, nullptr, s);
+            for (int q = 0; q < cap; ++q)
+                if (zero_m[static_cast<size_t>(q)]) sl[static_cast<size_t>(q)].normF += sum(q, 1) + 1e-20;
+        }
+        if (aw) {
+            setMask(with_m, s);
+            ones.reserve(static_cast<size_t>(n) * 2);  // A 1 of the current values
+            launchFill(n, 1.0, ones.p, s);
+            spmvImpl(c, ones.p, ones.p + n, nullptr, 1, nullptr, s);
+            for (int q = 0; q < cap; ++q) {
+                pcoef.p[4 * q] = xm[static_cast<size_t>(q)];
+                pcoef.p[4 * q + 1] = pcoef.p[4 * q + 2] = pcoef.p[4 * q + 3] = 0.0;
+            }
+            HIPCHECK(hipMemcpyAsync(coef.p, pcoef.p, sizeof(double) * 4 * cap, hipMemcpyHostToDevice, s));
+            ofNorm(n, r.p, d_b, ones.p + n, coef.p, part.p, red.p, cap, mask.p, s);
+            HIPCHECK(hipMemcpyAsync(pred.p, red.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            for (int q = 0; q < cap; ++q)
+                if (with_m[static_cast<size_t>(q)]) sl[static_cast<size_t>(q)].normF = sum(q, 0) + sum(q, 1) + 1e-20;
+        }
+        setMask(nz, s);
+        launchAxpby(n, 1.0, d_b, -1.0, r.p, cap, mask.p, s);  // r = b - A x0
+        reduce(1, r.p, nullptr, s);
+    }
+    for (int q = 0; q < cap; ++q) {
+        if (!nz[static_cast<size_t>(q)]) continue;
+        Slot& S = sl[static_cast<size_t>(q)];
+        S.beta = std::sqrt(sum(q, 0));
+        S.rel = S.beta / S.bnorm;
+        S.hist.push_back(S.rel);
+        if (of) {
+            S.l1_0 = S.l1 = sum(q, 1);
+            S.thr = std::max(o.rtol * S.l1_0, o.atol * S.normF);
+            if (S.l1_0 <= S.thr) S.conv = true;  // no iteration if x0 satisfies the test
+        } else if (S.beta <= o.rtol * S.bnorm) {
+            S.conv = true;
+        }
+    }
+}
+
+// One restart cycle (gmres.cpp:57-157) of every slot that holds an unfinished solve; slots that
+// finish early are masked while the others continue. Returns false when no slot had work.
+bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
+{
+    const int m = o.restart, maxit = o.maxit;
+    bool any = false;
+    std::vector<char> act(static_cast<size_t>(cap), 0);
+    for (int q = 0; q < cap; ++q) {
+        act[static_cast<size_t>(q)] = sl[static_cast<size_t>(q)].active(maxit) ? 1 : 0;
+        any = any || act[static_cast<size_t>(q)];
+    }
+    if (!any) return false;
+    HIPCHECK(hipStreamSynchronize(s));  // earlier copies have read the staging buffers
+    for (int q = 0; q < cap; ++q) {
+        const Slot& S = sl[static_cast<size_t>(q)];
+        GmState& G = pst.p[q];
+        std::memset(&G, 0, sizeof(G));
+        G.bnorm = S.bnorm;
+        G.beta = S.beta;
+        G.iters = S.iters;
+        G.maxit = maxit;
+        G.thr = S.thr;
+        G.done = act[static_cast<size_t>(q)] ? 0 : 1;
+        pflags.p[q] = act[static_cast<size_t>(q)] ? 0 : 1;
+        pflags.p[cap + q] = act[static_cast<size_t>(q)] ? 0 : 1;
+    }
+    HIPCHECK(hipMemcpyAsync(st.p, pst.p, sizeof(GmState) * cap, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(flags.p, pflags.p, sizeof(int32_t) * 2 * cap, hipMemcpyHostToDevice, s));
+    gmStartCycle(a, r.p, s);
+    const int32_t* live = flags.p;
+    const int32_t* idl = flags.p + cap;
+    const int64_t N = static_cast<int64_t>(n) * cap;
+    const int batch = 4;  // iterations between host polls of the flags
+    for (int j0 = 0; j0 < m; j0 += batch) {
+        const int j1 = std::min(m, j0 + batch);
+        for (int j = j0; j < j1; ++j) {
+            const double* vj = V.p + static_cast<int64_t>(j) * N;
+            double* zj = flex ? Z.p + static_cast<int64_t>(j) * N : z.p;
+            applySpmv(c, vj, zj, w.p, cap, live, s);  // z = M^{-1} v_j (kept: flexible); w = A z
+            gmDots(a, j, s);                          // CGS2 pass 1
+            gmUpdateDots(a, j, s);                    // w -= V h; pass 2
+            gmUpdateNorm(a, j, s);                    // w -= V h2; partials of ||w||^2
+            gmNormalize(a, j, s);                     // ||w||, Givens, v_{j+1} = w / ||w||
+            if (of) gmOpenFoamTest(a, j, s);          // sum|r_j| against the threshold
+        }
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(pflags.p, flags.p, sizeof(int32_t) * cap, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        bool all = true;
+        for (int q = 0; q < cap; ++q) all = all && pflags.p[q];
+        if (all) break;
+    }
+    // x += M^{-1} V_k y_k; true residual (gmres.cpp:135-156)
+    gmLsq(a, s);
+    if (flex) {
+        gmCombine(a, Z.p, z.p, s);      // x += Z y
+    } else {
+        gmCombine(a, V.p, u.p, s);      // x += M^{-1}(V y)
+        applyImpl(c, u.p, z.p, cap, idl, s);
+    }
+    launchAxpby(n, 1.0, z.p, 1.0, d_x, cap, idl, s);
+    spmvImpl(c, d_x, r.p, d_b, cap, idl, s);
+    HIPCHECK(hipMemcpyAsync(pst.p, st.p, sizeof(GmState) * cap, hipMemcpyDeviceToHost, s));
+    launchReduce(of ? 1 : 0, n, r.p, nullptr, part.p, red.p, cap, idl, s);
+    HIPCHECK(hipMemcpyAsync(pred.p, red.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, s));
+    std::vector<double> hc(static_cast<size_t>(maxit + 1) * cap);
+    HIPCHECK(hipMemcpyAsync(hc.data(), hist.p, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < cap; ++q) {
+        if (!act[static_cast<size_t>(q)]) continue;
+        Slot& S = sl[static_cast<size_t>(q)];
+        const GmState& G = pst.p[q];
+        for (int it = S.iters + 1; it <= G.iters; ++it) S.hist.push_back(hc[static_cast<size_t>(q) * (maxit + 1) + it]);
+        S.iters = G.iters;
+        S.rel = G.rel;
+        const double b2 = std::sqrt(sum(q, 0));
+        S.beta = b2;
+        if (of) S.l1 = sum(q, 1);
+        bool cv = G.conv != 0;
+        const bool within = of ? (S.l1 <= S.thr) : (b2 <= o.rtol * S.bnorm);
+        const bool within_slack = of ? (S.l1 <= 1.1 * S.thr) : (b2 <= 1.1 * o.rtol * S.bnorm);
+        if (G.breakdown) cv = within || cv;
+        // the estimate can be optimistic for a preconditioner that is not exactly linear (FP32
+        // factors): accept only a true residual within 10 % (gmres.cpp:151-155)
+        if (cv && !within_slack) cv = false;
+        if (b2 == 0.0) cv = true;
+        S.conv = cv;
+        S.breakdown = S.breakdown || G.breakdown;
+    }
+    return true;
+}
+
+void GmresRun::result(int q, s2l_krylov_result* R, double* history) const
+{
+    const Slot& S = sl[static_cast<size_t>(q)];
+    std::memset(R, 0, sizeof(*R));
+    R->converged = S.conv;
+    R->breakdown = 0;
+    R->iterations = S.iters;
+    R->rel_residual = S.rel;
+    R->true_rel_residual = S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0;
+    if (of && S.bnorm > 0.0) {
+        R->initial_residual_of = S.l1_0 / S.normF;
+        R->final_residual_of = S.l1 / S.normF;
+    }
+    R->history_len = static_cast<int>(S.hist.size());
+    if (history)
+        for (size_t k = 0; k < S.hist.size() && k < static_cast<size_t>(o.maxit + 1); ++k) history[k] = S.hist[k];
+}
+
+// s2l_krylov_solve with GMRES: every right-hand side in its own slot, cycles until all finish. The
+// run (and its workspace) is kept by the context for the next solve of the same shape.
 int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int nrhs, s2l_krylov_result* res,
               double* history, hipStream_t s)
 {
-    const int m = o->restart;
-    require(m >= 1 && m + 1 <= kGmMaxK, S2L_ELIMIT, "GMRES restart must be in [1, 64] on the device");
-    require(o->stop_norm == 0, S2L_EINVAL, "the device Krylov loops implement the l2 stopping norm");
-    require(o->maxit >= 0, S2L_EINVAL, "maxit must be >= 0");
-    const int64_t n = c->n;
-    const int maxit = o->maxit;
-    const int nb = reduceBlocks(n);
-    c->kV.reserve(static_cast<size_t>(m + 1) * n * nrhs);
-    const bool flex = o->flexible != 0;
-    if (flex) c->kZ.reserve(static_cast<size_t>(m) * n * nrhs);
-    c->kw.reserve(static_cast<size_t>(n) * nrhs);
-    c->kr.reserve(static_cast<size_t>(n) * nrhs);
-    c->kz.reserve(static_cast<size_t>(n) * nrhs);
-    c->ku.reserve(static_cast<size_t>(n) * nrhs);
-    c->kH.reserve(static_cast<size_t>(m + 1) * m * nrhs);
-    c->kcs.reserve(static_cast<size_t>(m) * nrhs);
-    c->ksn.reserve(static_cast<size_t>(m) * nrhs);
-    c->kg.reserve(static_cast<size_t>(m + 1) * nrhs);
-    c->kh.reserve(static_cast<size_t>(m + 1) * nrhs);
-    c->kh2.reserve(static_cast<size_t>(m + 1) * nrhs);
-    c->ky.reserve(static_cast<size_t>(m) * nrhs);
-    c->khist.reserve(static_cast<size_t>(maxit + 1) * nrhs);
-    c->kpart.reserve(static_cast<size_t>(nb) * kGmMaxK * nrhs);
-    c->kred.reserve(static_cast<size_t>(3) * nrhs);
-    c->kst.reserve(static_cast<size_t>(nrhs));
-    c->kflags.reserve(static_cast<size_t>(2) * nrhs);
-    c->pst.reserve(static_cast<size_t>(nrhs));
-    c->pflags.reserve(static_cast<size_t>(2) * nrhs);
-    c->pred.reserve(static_cast<size_t>(3) * nrhs);
-
-    GmArgs a{};
-    a.n = n;
-    a.m = m;
-    a.nrhs = nrhs;
-    a.V = c->kV.p;
-    a.Z = flex ? c->kZ.p : nullptr;
-    a.w = c->kw.p;
-    a.H = c->kH.p;
-    a.cs = c->kcs.p;
-    a.sn = c->ksn.p;
-    a.g = c->kg.p;
-    a.h = c->kh.p;
-    a.h2 = c->kh2.p;
-    a.y = c->ky.p;
-    a.st = c->kst.p;
-    a.flags = c->kflags.p;
-    a.hist = c->khist.p;
-    a.hist_ld = maxit + 1;
-    a.rtol = o->rtol;
-    a.partial = c->kpart.p;
-    a.nblocks = nb;
-
-    auto reduceToHost = [&](int kind, const double* x, const double* y, const int32_t* skip) {
-        launchReduce(kind, n, x, y, c->kpart.p, c->kred.p, nrhs, skip, s);
-        HIPCHECK(hipMemcpyAsync(c->pred.p, c->kred.p, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-    };
-
-    std::vector<double> bnorm(static_cast<size_t>(nrhs)), beta(static_cast<size_t>(nrhs));
-    std::vector<int> conv(static_cast<size_t>(nrhs), 0), iters(static_cast<size_t>(nrhs), 0);
-    std::vector<int> breakdown(static_cast<size_t>(nrhs), 0);
-    std::vector<double> rel(static_cast<size_t>(nrhs), 0.0);
-    std::vector<int32_t> idle(static_cast<size_t>(nrhs), 0);
-    std::vector<std::vector<double>> hist(static_cast<size_t>(nrhs));
-
-    reduceToHost(0, d_b, nullptr, nullptr);
-    for (int r = 0; r < nrhs; ++r) bnorm[static_cast<size_t>(r)] = std::sqrt(c->pred.p[3 * r]);
-    // bnorm == 0: x = 0, converged (gmres.cpp:101-108)
-    for (int r = 0; r < nrhs; ++r)
-        if (bnorm[static_cast<size_t>(r)] == 0.0) {
-            HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(r) * n, 0, sizeof(double) * n, s));
-            conv[static_cast<size_t>(r)] = 1;
-            hist[static_cast<size_t>(r)].push_back(0.0);
-        }
-    // r = b - A x
-    spmvImpl(c, d_x, c->kr.p, d_b, nrhs, nullptr, s);
-    reduceToHost(0, c->kr.p, nullptr, nullptr);
-    for (int r = 0; r < nrhs; ++r) {
-        if (bnorm[static_cast<size_t>(r)] == 0.0) continue;
-        beta[static_cast<size_t>(r)] = std::sqrt(c->pred.p[3 * r]);
-        rel[static_cast<size_t>(r)] = beta[static_cast<size_t>(r)] / bnorm[static_cast<size_t>(r)];
-        hist[static_cast<size_t>(r)].push_back(rel[static_cast<size_t>(r)]);
-        if (beta[static_cast<size_t>(r)] <= o->rtol * bnorm[static_cast<size_t>(r)]) conv[static_cast<size_t>(r)] = 1;
+    std::shared_ptr<GmresRun>& run = c->gm_cache;
+    if (!run)
+        run = std::make_shared<GmresRun>(c, *o, nrhs);
+    else
+        run->configure(*o, nrhs);
+    std::vector<char> all(static_cast<size_t>(nrhs), 1);
+    run->begin(all, d_b, d_x, s);
+    while (run->cycle(d_b, d_x, s)) {
     }
-    const int batch = 4;
-    for (;;) {
-        bool any = false;
-        for (int r = 0; r < nrhs; ++r) {
-            const bool act = !conv[static_cast<size_t>(r)] && iters[static_cast<size_t>(r)] < maxit;
-            idle[static_cast<size_t>(r)] = act ? 0 : 1;
-            any = any || act;
-            GmState& st = c->pst.p[r];
-            std::memset(&st, 0, sizeof(st));
-            st.bnorm = bnorm[static_cast<size_t>(r)];
-            st.beta = beta[static_cast<size_t>(r)];
-            st.iters = iters[static_cast<size_t>(r)];
-            st.maxit = maxit;
-            st.done = act ? 0 : 1;
-            c->pflags.p[r] = act ? 0 : 1;
-            c->pflags.p[nrhs + r] = act ? 0 : 1;
-        }
-        if (!any) break;
-        HIPCHECK(hipMemcpyAsync(c->kst.p, c->pst.p, sizeof(GmState) * nrhs, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(c->kflags.p, c->pflags.p, sizeof(int32_t) * 2 * nrhs, hipMemcpyHostToDevice, s));
-        gmStartCycle(a, c->kr.p, s);
-        const int32_t* live = c->kflags.p;
-        const int32_t* idl = c->kflags.p + nrhs;
-        for (int j0 = 0; j0 < m; j0 += batch) {
-            const int j1 = std::min(m, j0 + batch);
-            for (int j = j0; j < j1; ++j) {
-                const double* vj = c->kV.p + static_cast<int64_t>(j) * nrhs * n;
-                double* zj = flex ? c->kZ.p + static_cast<int64_t>(j) * nrhs * n : c->kz.p;
-                applySpmv(c, vj, zj, c->kw.p, nrhs, live, s);           // z = M^{-1} v_j (kept: flexible); w = A z
-                gmDots(a, j, s);                                        // CGS2 pass 1
-                gmUpdateDots(a, j, s);                                  // w -= V h; pass 2
-                gmUpdateNorm(a, j, s);                                  // w -= V h2; partials of ||w||^2
-                gmNormalize(a, j, s);                                   // ||w||, Givens, v_{j+1} = w / ||w||
-            }
-            HIPCHECK(hipGetLastError());
-            HIPCHECK(hipMemcpyAsync(c->pflags.p, c->kflags.p, sizeof(int32_t) * nrhs, hipMemcpyDeviceToHost, s));
-            HIPCHECK(hipStreamSynchronize(s));
-            bool all = true;
-            for (int r = 0; r < nrhs; ++r) all = all && c->pflags.p[r];
-            if (all) break;
-        }
-        // x += M^{-1} V_k y_k; true residual (gmres.cpp:210-231)
-        gmLsq(a, s);
-        if (flex) {
-            gmCombine(a, c->kZ.p, c->kz.p, s);                          // x += Z y
-        } else {
-            gmCombine(a, c->kV.p, c->ku.p, s);                          // x += M^{-1}(V y)
-            applyImpl(c, c->ku.p, c->kz.p, nrhs, idl, s);
-        }
-        launchAxpby(n, 1.0, c->kz.p, 1.0, d_x, nrhs, idl, s);
-        spmvImpl(c, d_x, c->kr.p, d_b, nrhs, idl, s);
-        HIPCHECK(hipMemcpyAsync(c->pst.p, c->kst.p, sizeof(GmState) * nrhs, hipMemcpyDeviceToHost, s));
-        reduceToHost(0, c->kr.p, nullptr, idl);
-        // history of this cycle
-        std::vector<double> hc(static_cast<size_t>(maxit + 1) * nrhs);
-        HIPCHECK(hipMemcpyAsync(hc.data(), c->khist.p, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        for (int r = 0; r < nrhs; ++r) {
-            if (idle[static_cast<size_t>(r)]) continue;
-            const GmState& st = c->pst.p[r];
-            for (int it = iters[static_cast<size_t>(r)] + 1; it <= st.iters; ++it)
-                hist[static_cast<size_t>(r)].push_back(hc[static_cast<size_t>(r) * (maxit + 1) + it]);
-            iters[static_cast<size_t>(r)] = st.iters;
-            rel[static_cast<size_t>(r)] = st.rel;
-            const double b2 = std::sqrt(c->pred.p[3 * r]);
-            beta[static_cast<size_t>(r)] = b2;
-            const double bn = bnorm[static_cast<size_t>(r)];
-            bool cv = st.conv != 0;
-            const bool within = b2 <= o->rtol * bn;
-            const bool within_slack = b2 <= 1.1 * o->rtol * bn;
-            if (st.breakdown) cv = within || cv;
-            if (cv && !within_slack) cv = false;
-            if (b2 == 0.0) cv = true;
-            conv[static_cast<size_t>(r)] = cv ? 1 : 0;
-            breakdown[static_cast<size_t>(r)] = breakdown[static_cast<size_t>(r)] || st.breakdown;
-        }
-    }
-    for (int r = 0; r < nrhs; ++r) {
-        s2l_krylov_result& R = res[r];
-        std::memset(&R, 0, sizeof(R));
-        R.converged = conv[static_cast<size_t>(r)];
-        R.breakdown = 0;
-        R.iterations = iters[static_cast<size_t>(r)];
-        R.rel_residual = rel[static_cast<size_t>(r)];
-        R.true_rel_residual = bnorm[static_cast<size_t>(r)] > 0.0 ? beta[static_cast<size_t>(r)] / bnorm[static_cast<size_t>(r)] : 0.0;
-        const auto& H = hist[static_cast<size_t>(r)];
-        R.history_len = static_cast<int>(H.size());
-        if (history)
-            for (size_t q = 0; q < H.size() && q < static_cast<size_t>(maxit + 1); ++q) history[static_cast<size_t>(r) * (maxit + 1) + q] = H[q];
-    }
+    for (int q = 0; q < nrhs; ++q) run->result(q, res + q, history ? history + static_cast<size_t>(q) * (o->maxit + 1) : nullptr);
     return S2L_OK;
 }
 
@@ -1119,6 +1281,16 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
 }
 } // namespace
 
+// A GMRES run over a fixed number of right-hand-side slots with its own right-hand sides and
+// solutions on the device: requests enter free slots between restart cycles (continuous batching).
+struct s2l_solver {
+    s2l_ctx* c = nullptr;
+    std::unique_ptr<GmresRun> run;
+    DevBuf<double> B, X;
+    std::vector<char> pending;  ///< admitted, started at the next step
+    std::vector<char> used;     ///< holds a solve (pending, running, or finished and not taken)
+};
+
 // ============================================================================ C-ABI
 extern "C" {
 
@@ -1204,19 +1376,25 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
                 ++q;
             }
         }
+        // the same pattern (row pointers and every column) and ownership refresh the values only;
+        // anything else is a new system whose subdomains, factors and coarse space follow
         if (c->system_ready && c->n == n && c->nnz == nnz && c->nsub == nsub &&
-            std::equal(c->ownership.begin(), c->ownership.end(), ownership)) {
-            c->val.upload(vl, c->stream); // same pattern: refresh the values
+            std::equal(c->ownership.begin(), c->ownership.end(), ownership) &&
+            std::equal(c->rp_h.begin(), c->rp_h.end(), rowptr) && c->col_h == cl) {
+            c->val.upload(vl, c->stream);
         } else {
             c->n = n;
             c->nnz = nnz;
             c->nsub = nsub;
             c->ownership.assign(ownership, ownership + nsub + 1);
+            c->rp_h.assign(rowptr, rowptr + n + 1);
+            c->col_h = cl;
             c->rp.upload(rowptr, static_cast<size_t>(n) + 1, c->stream);
             c->col.upload(cl, c->stream);
             c->val.upload(vl, c->stream);
             c->subs_ready = c->local_ready = c->coarse_ready = false;
             c->work_nrhs = 0;
+            c->gm_cache.reset();
         }
         HIPCHECK(hipStreamSynchronize(c->stream));
         c->system_ready = true;
@@ -1584,6 +1762,97 @@ int s2l_krylov_solve(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, do
                             : bicgImpl(c, o, d_b, d_x, nrhs, results, history, pick(c, stream));
     });
     return g != S2L_OK ? g : rc;
+}
+
+int s2l_solver_create(s2l_ctx* c, const s2l_krylov_opts* o, int32_t slots, s2l_solver** out)
+{
+    return guarded(c, [&] {
+        require(c && o && out && slots >= 1, S2L_EINVAL, "solver_create: bad arguments");
+        require(o->method == 0, S2L_EINVAL, "solver_create: the slot solver runs GMRES (method 0)");
+        require(c->system_ready && c->subs_ready && c->local_ready && c->coarse_ready, S2L_ESTATE,
+                "solver_create: upload the system, subdomains, factors and coarse space first");
+        HIPCHECK(hipSetDevice(c->device));
+        auto sv = std::make_unique<s2l_solver>();
+        sv->c = c;
+        sv->run = std::make_unique<GmresRun>(c, *o, slots);
+        sv->B.reserve(static_cast<size_t>(c->n) * slots);
+        sv->X.reserve(static_cast<size_t>(c->n) * slots);
+        sv->pending.assign(static_cast<size_t>(slots), 0);
+        sv->used.assign(static_cast<size_t>(slots), 0);
+        c->ensureWork(slots);
+        *out = sv.release();
+    });
+}
+
+int s2l_solver_destroy(s2l_solver* sv)
+{
+    if (!sv) return S2L_OK;
+    s2l_ctx* c = sv->c;
+    return guarded(c, [&] {
+        HIPCHECK(hipSetDevice(c->device));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        delete sv;
+    });
+}
+
+int s2l_solver_admit(s2l_solver* sv, int32_t slot, const double* b, const double* x0, int host)
+{
+    s2l_ctx* c = sv ? sv->c : nullptr;
+    return guarded(c, [&] {
+        require(sv && b && slot >= 0 && slot < sv->run->cap, S2L_EINVAL, "solver_admit: bad arguments");
+        require(!sv->used[static_cast<size_t>(slot)], S2L_ESTATE, "solver_admit: slot " + std::to_string(slot) + " is busy");
+        HIPCHECK(hipSetDevice(c->device));
+        const int64_t n = c->n;
+        const hipMemcpyKind k = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        double* bs = sv->B.p + static_cast<int64_t>(slot) * n;
+        double* xs = sv->X.p + static_cast<int64_t>(slot) * n;
+        HIPCHECK(hipMemcpyAsync(bs, b, sizeof(double) * n, k, c->stream));
+        if (x0)
+            HIPCHECK(hipMemcpyAsync(xs, x0, sizeof(double) * n, k, c->stream));
+        else
+            launchFill(n, 0.0, xs, c->stream);
+        if (host) HIPCHECK(hipStreamSynchronize(c->stream));  // the caller may reuse its buffers
+        sv->used[static_cast<size_t>(slot)] = 1;
+        sv->pending[static_cast<size_t>(slot)] = 1;
+    });
+}
+
+int s2l_solver_step(s2l_solver* sv, int32_t* finished)
+{
+    s2l_ctx* c = sv ? sv->c : nullptr;
+    return guarded(c, [&] {
+        require(sv != nullptr, S2L_EINVAL, "solver_step: null solver");
+        HIPCHECK(hipSetDevice(c->device));
+        GmresRun& R = *sv->run;
+        R.begin(sv->pending, sv->B.p, sv->X.p, c->stream);
+        std::fill(sv->pending.begin(), sv->pending.end(), 0);
+        R.cycle(sv->B.p, sv->X.p, c->stream);
+        if (finished)
+            for (int q = 0; q < R.cap; ++q)
+                finished[q] = (sv->used[static_cast<size_t>(q)] && !R.sl[static_cast<size_t>(q)].active(R.o.maxit)) ? 1 : 0;
+    });
+}
+
+int s2l_solver_take(s2l_solver* sv, int32_t slot, s2l_krylov_result* res, double* history, double* x, int host)
+{
+    s2l_ctx* c = sv ? sv->c : nullptr;
+    return guarded(c, [&] {
+        require(sv && res && slot >= 0 && slot < sv->run->cap, S2L_EINVAL, "solver_take: bad arguments");
+        GmresRun& R = *sv->run;
+        require(sv->used[static_cast<size_t>(slot)] && !sv->pending[static_cast<size_t>(slot)] &&
+                    !R.sl[static_cast<size_t>(slot)].active(R.o.maxit),
+                S2L_ESTATE, "solver_take: slot " + std::to_string(slot) + " holds no finished solve");
+        HIPCHECK(hipSetDevice(c->device));
+        R.result(slot, res, history);
+        if (x) {
+            const int64_t n = c->n;
+            HIPCHECK(hipMemcpyAsync(x, sv->X.p + static_cast<int64_t>(slot) * n, sizeof(double) * n,
+                                    host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->stream));
+            HIPCHECK(hipStreamSynchronize(c->stream));
+        }
+        sv->used[static_cast<size_t>(slot)] = 0;
+        R.sl[static_cast<size_t>(slot)].live = false;
+    });
 }
 
 int s2l_apply_host(s2l_ctx* c, const double* r, double* z, int32_t nrhs)

@@ -74,6 +74,8 @@ struct CoarseArgs {
 /// Per right-hand side state of the device GMRES loop.
 struct GmState {
     double bnorm, beta, rel, hn;
+    double thr;         ///< OpenFOAM test: threshold on sum|r| (max(rtol sum|r0|, atol F))
+    double l1;          ///< OpenFOAM test: sum|r_j| of the last iteration's GMRES residual
     int32_t k;          ///< columns of the current cycle
     int32_t iters;      ///< total iterations
     int32_t done;       ///< cycle finished (converged estimate, breakdown or maxit)
@@ -153,11 +155,15 @@ struct GmArgs {
     double rtol;
     double* partial;    ///< block partial sums
     int32_t nblocks;
+    int32_t of;         ///< 1: OpenFOAM stopping test (gmres.cpp:106-122), run by gmOpenFoamTest
 };
 void gmDots(const GmArgs& a, int j, hipStream_t s);
 void gmUpdateDots(const GmArgs& a, int j, hipStream_t s);
 void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s);
 void gmNormalize(const GmArgs& a, int j, hipStream_t s);
+/// OpenFOAM stopping test of iteration j: sum|g_{j+1} V_{j+2} Q^T e_{j+1}| against st.thr; sets
+/// the convergence and the flags the l2 test sets in gmNormalize.
+void gmOpenFoamTest(const GmArgs& a, int j, hipStream_t s);
 void gmStartCycle(const GmArgs& a, const double* r, hipStream_t s);
 void gmLsq(const GmArgs& a, hipStream_t s);
 /// u = B_k y for the basis B = V (or Z for flexible GMRES)

@@ -1265,7 +1265,9 @@ __device__ void gmGivensStep(const GmArgs& a, int j, int r, double hn)
     st.rel = fabs(g[j + 1]) / st.bnorm;
     a.hist[static_cast<int64_t>(r) * a.hist_ld + st.iters] = st.rel;
     int done = 0;
-    if (st.rel <= a.rtol) {
+    if (a.of) {
+        // the OpenFOAM test needs v_{j+1}: k_gm_of_final decides and sets the flag
+    } else if (st.rel <= a.rtol) {
         st.conv = 1;
         done = 1;
     } else if (hn == 0.0) {
@@ -1274,7 +1276,7 @@ __device__ void gmGivensStep(const GmArgs& a, int j, int r, double hn)
     }
     if (st.iters >= st.maxit || j + 1 >= m) done = 1;
     st.done = done;
-    a.flags[r] = done;
+    if (!a.of) a.flags[r] = done;
 }
 
 // hn = ||w|| from the partials of k_gm_update_norm (every workgroup sums them in the same order,
@@ -1298,6 +1300,62 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_givens_normalize(GmArgs a,
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     basisW(a, r, j + 1)[i] = a.w[static_cast<int64_t>(r) * a.n + i] / hn;
+}
+
+// OpenFOAM stopping test of iteration j (gmres.cpp:106-122): the GMRES residual is
+// r_j = g_{j+1} V_{j+2} Q^T e_{j+1}, Q^T = G_0^T ... G_j^T; its 1-norm is compared with st.thr.
+// Partials of sum_i |sum_k V_k[i] q_k| per block; every block forms q from the rotations in the
+// same order.
+__global__ __launch_bounds__(kGmThreads) void k_gm_of_partial(GmArgs a, int j)
+{
+    const int r = blockIdx.y;
+    if (a.flags[r]) return;
+    __shared__ double q[kGmMaxK + 1];
+    const double* cs = a.cs + r * a.m;
+    const double* sn = a.sn + r * a.m;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i <= j; ++i) q[i] = 0.0;
+        q[j + 1] = 1.0;
+        for (int i = j; i >= 0; --i) {
+            const double u = q[i], v = q[i + 1];
+            q[i] = cs[i] * u - sn[i] * v;
+            q[i + 1] = sn[i] * u + cs[i] * v;
+        }
+    }
+    __syncthreads();
+    double s1 = 0.0;
+    if (a.st[r].hn > 0.0) {  // hn == 0: v_{j+1} was not formed and the residual is zero
+        const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+        for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+            double v = 0.0;
+            for (int k = 0; k <= j + 1; ++k) v += basis(a, r, k)[i] * q[k];
+            s1 += fabs(v);
+        }
+    }
+    const double t = blockSum(s1);
+    if (threadIdx.x == 0) a.partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kGmMaxK] = t;
+}
+
+// sum|r_j| = |g_{j+1}| x (the partials in block order); convergence and the right-hand side's flag.
+__global__ __launch_bounds__(kBlockThreads) void k_gm_of_final(GmArgs a, int j)
+{
+    const int r = blockIdx.y;
+    if (a.flags[r]) return;
+    const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+    const double t = blockSum(v);
+    if (threadIdx.x == 0) {
+        GmState& st = a.st[r];
+        const double gj = a.g[r * (a.m + 1) + j + 1];
+        const double l1 = (st.hn == 0.0 || gj == 0.0) ? 0.0 : fabs(gj) * t;
+        st.l1 = l1;
+        if (l1 <= st.thr) {
+            st.conv = 1;
+            st.done = 1;
+        }
+        a.flags[r] = st.done;
+    }
 }
 
 // V_0 = r / beta, g = beta e_1, H = 0 for the right-hand sides that run this cycle.
@@ -1511,6 +1569,11 @@ void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s)
 void gmNormalize(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_givens_normalize, dim3(nblk(a.n, kBlockThreads), a.nrhs), dim3(kBlockThreads), 0, s, a, j);
+}
+void gmOpenFoamTest(const GmArgs& a, int j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gm_of_partial, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);
+    hipLaunchKernelGGL(k_gm_of_final, dim3(1, a.nrhs), dim3(kBlockThreads), 0, s, a, j);
 }
 void gmStartCycle(const GmArgs& a, const double* r, hipStream_t s)
 {

@@ -19,6 +19,11 @@ void check(s2l_ctx* ctx, int rc)
 
 } // namespace
 
+bool SetupData::samePattern(const SetupData& a, const SetupData& b)
+{
+    return a.n == b.n && a.rowptr == b.rowptr && a.col == b.col;
+}
+
 namespace prec {
 
 const char* toString(UpdateLevel l)
@@ -31,7 +36,16 @@ const char* toString(UpdateLevel l)
     return "?";
 }
 
-HipTwoLevelSchwarz::HipTwoLevelSchwarz(const SetupData& s, int device)
+HipTwoLevelSchwarz::HipTwoLevelSchwarz(const SetupData& s, int device) { create(s, device); }
+
+HipTwoLevelSchwarz::HipTwoLevelSchwarz(std::shared_ptr<SetupSource> source, int device) : source_(std::move(source))
+{
+    if (!source_) throw std::invalid_argument("HipTwoLevelSchwarz: null setup source");
+    create(source_->build(), device);
+    version_ = source_->valuesVersion();
+}
+
+void HipTwoLevelSchwarz::create(const SetupData& s, int device)
 {
     if (s.one_level != "ras" && s.one_level != "asm") throw std::invalid_argument("one-level variant must be ras or asm");
     if (s.correction != "deflated" && s.correction != "additive")
@@ -74,6 +88,8 @@ void HipTwoLevelSchwarz::upload(const SetupData& s)
     check(ctx_, s2l_upload_coarse(ctx_, s.n0, s.n0 > 0 ? s.coarse_offsets.data() : nullptr, s.n0 > 0 ? s.V.data() : nullptr,
                                   s.n0 > 0 ? s.E.data() : nullptr, s.symmetric ? 1 : 0, s.correction == "additive" ? 1 : 0));
     n_ = s.n;
+    rowptr_ = s.rowptr;
+    col_ = s.col;
 }
 
 void HipTwoLevelSchwarz::apply(const core::Vec& r, core::Vec& z) const
@@ -99,19 +115,36 @@ std::string HipTwoLevelSchwarz::name() const { return name_; }
 
 UpdateLevel HipTwoLevelSchwarz::refresh(const SetupData& s)
 {
-    if (s.n != n_) throw std::invalid_argument("refresh: the system size changed (a new setup is a Rebuild)");
+    if (s.n != n_) throw std::invalid_argument("refresh: the system size changed; rebuild instead");
+    if (s.rowptr != rowptr_ || s.col != col_)
+        throw std::invalid_argument("refresh: the sparsity pattern changed; rebuild instead");
     upload(s);
     up_to_date_ = true;
     return UpdateLevel::Refactorize;
 }
 
+bool HipTwoLevelSchwarz::upToDate() const { return source_ ? version_ == source_->valuesVersion() : up_to_date_; }
+
 UpdateLevel HipTwoLevelSchwarz::update(UpdateLevel requested)
 {
-    if (requested == UpdateLevel::Rebuild)
-        throw std::logic_error("Rebuild needs a new host setup: construct a new HipTwoLevelSchwarz");
-    if (up_to_date_) return UpdateLevel::Reuse;
-    if (requested == UpdateLevel::Reuse) return UpdateLevel::Reuse;
-    throw std::logic_error("the preconditioner is stale: refresh() it with the refactorized values");
+    // two_level_schwarz.cpp:30-47: Reuse, or Refactorize when up to date, does nothing
+    if (requested == UpdateLevel::Reuse || (requested == UpdateLevel::Refactorize && upToDate())) return UpdateLevel::Reuse;
+    if (!source_)
+        throw std::logic_error(std::string(toString(requested)) +
+                               " needs a setup source: build the preconditioner from a SetupSource, or refresh() it");
+    if (requested == UpdateLevel::Refactorize) {
+        refresh(source_->refactorize());
+        version_ = source_->valuesVersion();
+        return UpdateLevel::Refactorize;
+    }
+    // Rebuild is always performed: new partition-dependent data, factors and coarse basis
+    const SetupData s = source_->build();
+    if (s.one_level != "ras" && s.one_level != "asm") throw std::invalid_argument("one-level variant must be ras or asm");
+    upload(s);
+    name_ = std::string(s.one_level == "ras" ? "RAS" : "ASM") + (s.correction == "deflated" ? ",deflated" : ",additive");
+    version_ = source_->valuesVersion();
+    up_to_date_ = true;
+    return UpdateLevel::Rebuild;
 }
 
 // ---- ReusePolicy (src/preconditioner/reuse_policy.cpp:14-84): the same rules, in this file's terms
@@ -213,15 +246,18 @@ const char* toString(StopNorm s) { return s == StopNorm::L2 ? "l2" : "openfoam";
 
 namespace {
 
-const prec::HipTwoLevelSchwarz& deviceSide(const HipOperator& A, const prec::Preconditioner& M)
+const prec::HipTwoLevelSchwarz& deviceSide(const linalg::DistributedMatrix& A, const prec::Preconditioner& M)
 {
     const auto* hm = dynamic_cast<const prec::HipTwoLevelSchwarz*>(&M);
     if (!hm) throw std::invalid_argument("the device Krylov solvers need a HipTwoLevelSchwarz preconditioner");
-    if (&A.owner() != hm) throw std::invalid_argument("the operator and the preconditioner must share their context");
+    const auto* op = dynamic_cast<const HipOperator*>(&A);
+    if (!op) throw std::invalid_argument("the device Krylov solvers need the HipOperator of the preconditioner");
+    if (op->owner().context() != hm->context())
+        throw std::invalid_argument("the operator and the preconditioner must share their context");
     return *hm;
 }
 
-KrylovResult run(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x, int method,
+KrylovResult run(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x, int method,
                  int restart, double rtol, double atol, int maxit, StopNorm stop, bool flexible)
 {
     const prec::HipTwoLevelSchwarz& hm = deviceSide(A, M);
@@ -258,10 +294,10 @@ HipGmres::HipGmres(int restart, double rtol, int maxit, StopNorm stop, double at
     : m_(restart), maxit_(maxit), rtol_(rtol), atol_(atol), stop_(stop), flexible_(flexible)
 {
     if (restart < 1) throw std::invalid_argument("GMRES restart must be >= 1");
-    if (stop != StopNorm::L2) throw std::invalid_argument("the device GMRES implements the l2 stopping norm");
+    if (atol > 0.0 && stop != StopNorm::OpenFOAM) throw std::invalid_argument("an absolute tolerance needs the OpenFOAM stopping norm");
 }
 
-KrylovResult HipGmres::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x) const
+KrylovResult HipGmres::solve(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b, core::Vec& x) const
 {
     return run(A, M, b, x, 0, m_, rtol_, atol_, maxit_, stop_, flexible_);
 }
@@ -271,7 +307,7 @@ HipBiCgStab::HipBiCgStab(double rtol, int maxit, StopNorm stop, double atol)
 {
 }
 
-KrylovResult HipBiCgStab::solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+KrylovResult HipBiCgStab::solve(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b,
                                 core::Vec& x) const
 {
     return run(A, M, b, x, 1, 0, rtol_, atol_, maxit_, stop_, false);
@@ -286,6 +322,14 @@ std::unique_ptr<KrylovSolver> makeKrylovSolver(const std::string& krylov, int re
     if (krylov == "bicgstab") return std::make_unique<HipBiCgStab>(rtol, maxit, stop, atol);
     throw std::invalid_argument("unknown Krylov solver '" + krylov + "' (gmres|bicgstab)");
 }
+
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const core::SolverConfig& cfg, double rtol, double atol)
+{
+    if (cfg.backend != "hip") throw std::invalid_argument("unknown backend '" + cfg.backend + "' (this library: hip)");
+    return makeKrylovSolver(cfg.krylov, cfg.restart, rtol, atol, cfg.maxit, cfg.stop_norm);
+}
+
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const core::SolverConfig& cfg) { return makeKrylovSolver(cfg, cfg.rtol, cfg.atol); }
 
 } // namespace solv
 

@@ -58,7 +58,8 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_krylov_solve", "s2l_apply_host", "s2l_spmv_host", "s2l_krylov_solve_host", "s2l_local_solve_host",
            "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_copy_d2d", "s2l_synchronize",
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
-           "s2l_set_tasks_per_subdomain", "s2l_set_graphs"]
+           "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
+           "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take"]
 
 
 def hip_lib():
@@ -96,6 +97,11 @@ def hip_lib():
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
         "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
         "s2l_set_graphs": [vp, ctypes.c_int],
+        "s2l_solver_create": [vp, P(KrylovOpts), ctypes.c_int32, P(vp)],
+        "s2l_solver_destroy": [vp],
+        "s2l_solver_admit": [vp, ctypes.c_int32, vp, vp, ctypes.c_int],
+        "s2l_solver_step": [vp, _i32p],
+        "s2l_solver_take": [vp, ctypes.c_int32, P(KrylovResultC), _dp, vp, ctypes.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -198,6 +204,7 @@ class HipContext:
     def __init__(self, device: int = 0):
         self.L = hip_lib()
         self.h = ctypes.c_void_p()
+        _lib.gpu_initialised = True
         rc = self.L.s2l_ctx_create(device, ctypes.byref(self.h))
         if rc != 0:
             raise RuntimeError(f"s2l_ctx_create: {self.L.s2l_last_error(None).decode()}")
@@ -316,8 +323,8 @@ class HipContext:
         return x, (out[0] if nrhs == 1 else out)
 
     def gmres(self, b: np.ndarray, x: np.ndarray | None = None, restart=30, rtol=1e-8, maxit=1000,
-              flexible: bool = False):
-        return self.krylov(b, x, "gmres", restart, rtol, maxit, flexible=flexible)
+              flexible: bool = False, stop: str = "l2", atol: float = 0.0):
+        return self.krylov(b, x, "gmres", restart, rtol, maxit, stop, atol, flexible=flexible)
 
     def bicgstab(self, b: np.ndarray, x: np.ndarray | None = None, rtol=1e-8, maxit=1000, stop="l2", atol=0.0):
         return self.krylov(b, x, "bicgstab", 0, rtol, maxit, stop, atol)
@@ -338,6 +345,11 @@ class HipContext:
         res = (KrylovResultC * nrhs)()
         self._check(self.L.s2l_krylov_solve(self.h, ctypes.byref(o), b.ptr, x.ptr, nrhs, res, None, None))
         return [res[r] for r in range(nrhs)]
+
+    def solver(self, slots: int, restart=30, rtol=1e-8, maxit=1000, stop: str = "l2", atol: float = 0.0,
+               flexible: bool = False) -> "SlotSolver":
+        """A GMRES run over ``slots`` right-hand-side slots (s2l_solver_*, continuous batching)."""
+        return SlotSolver(self, slots, KrylovOpts(0, restart, rtol, atol, maxit, _STOPS[stop], 1 if flexible else 0))
 
     def copy_device(self, src: DeviceArray, dst: DeviceArray, count: int | None = None):
         count = min(src.n, dst.n) if count is None else count
@@ -371,6 +383,54 @@ class HipContext:
 
     def set_launch_mode(self, mode: int):
         self._check(self.L.s2l_set_local_launch_mode(self.h, mode))
+
+
+class SlotSolver:
+    """s2l_solver: GMRES over a fixed number of right-hand-side slots sharing every launch. A right-hand
+    side enters a free slot (admit) and starts at the next step, which runs one restart cycle of
+    every busy slot; a finished slot's result is taken (take), which frees it. A slot's results do
+    not depend on the other slots. ctypes releases the GIL during the calls, so solvers of different
+    contexts stepped from different threads overlap on the device."""
+
+    def __init__(self, ctx: HipContext, slots: int, opts: KrylovOpts):
+        self.ctx, self.slots, self.maxit, self.n = ctx, slots, opts.maxit, None
+        self.h = ctypes.c_void_p()
+        self.L = ctx.L
+        ctx._check(self.L.s2l_solver_create(ctx.h, ctypes.byref(opts), slots, ctypes.byref(self.h)))
+        self.n = ctx.stats().n
+        self._fin = np.zeros(slots, np.int32)
+
+    def admit(self, slot: int, b: np.ndarray, x0: np.ndarray | None = None):
+        b = np.ascontiguousarray(b, np.float64)
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        self.ctx._check(self.L.s2l_solver_admit(self.h, slot, b.ctypes.data_as(ctypes.c_void_p),
+                                                None if x0 is None else x0.ctypes.data_as(ctypes.c_void_p), 1))
+
+    def step(self) -> np.ndarray:
+        """One restart cycle; returns the slots that hold a finished solve."""
+        self.ctx._check(self.L.s2l_solver_step(self.h, _p(self._fin, ctypes.c_int32)))
+        return np.flatnonzero(self._fin)
+
+    def take(self, slot: int):
+        R = KrylovResultC()
+        hist = np.zeros(self.maxit + 1)
+        x = np.empty(self.n)
+        self.ctx._check(self.L.s2l_solver_take(self.h, slot, ctypes.byref(R), _p(hist, ctypes.c_double),
+                                               x.ctypes.data_as(ctypes.c_void_p), 1))
+        return x, KrylovResult(bool(R.converged), bool(R.breakdown), R.iterations, R.restarts, R.rel_residual,
+                               R.true_rel_residual, R.initial_residual_of, R.final_residual_of,
+                               list(hist[:R.history_len]))
+
+    def close(self):
+        if self.h:
+            self.L.s2l_solver_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -437,15 +497,18 @@ class HipGmres:
     (flexible GMRES, an extension without reference counterpart)."""
 
     def __init__(self, restart=30, rtol=1e-8, maxit=1000, stop="l2", atol=0.0, flexible=False):
-        if stop != "l2":
-            raise ValueError("the device GMRES implements the l2 stopping norm")
+        if stop not in _STOPS:
+            raise ValueError(f"unknown stopping norm '{stop}' (l2|openfoam)")
+        if atol > 0.0 and stop != "openfoam":
+            raise ValueError("an absolute tolerance needs the OpenFOAM stopping norm")
         self.m, self.rtol, self.maxit, self.flexible = restart, rtol, maxit, flexible
+        self.stop, self.atol = stop, atol
 
     def name(self) -> str:
         return f"GMRES({self.m})"
 
     def solve(self, M: HipTwoLevelSchwarz, b, x=None):
-        return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit, flexible=self.flexible)
+        return M.ctx.gmres(b, x, self.m, self.rtol, self.maxit, flexible=self.flexible, stop=self.stop, atol=self.atol)
 
 
 class HipBiCgStab:

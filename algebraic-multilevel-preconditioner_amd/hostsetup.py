@@ -617,7 +617,7 @@ def build_coarse(dec: Decomposition, subs: list[Subdomain], nev: int, tau: float
     sym = dec.symmetric
     if Z is None:
         tasks = [(sd, nev, tau, sym, tol) for sd in subs]
-        if workers and workers > 1 and len(subs) > 1:
+        if workers and workers > 1 and len(subs) > 1 and _lib.fork_is_safe():
             import multiprocessing as mp
             ctx = mp.get_context("fork")
             with ctx.Pool(min(workers, len(subs))) as pool:

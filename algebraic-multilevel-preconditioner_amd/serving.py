@@ -8,14 +8,23 @@ process; the request loop is this library's: it reuses the reference's per-solve
 (GMRES(m) or BiCGStab with the two-level preconditioner; several right-hand sides of a request share
 every launch). Service times are measured on the device request by request, back to back; the
 queue is then replayed on a virtual clock for the offered load (start = max(arrival, previous
-finish)), which is the timeline of a single FIFO server with those service times, without idle
-waits on the GPU. serve_batched is the continuous-batching server: requests waiting for the same
-family join one solve that shares every launch, on the same virtual clock with measured times."""
+finish)), which is the timeline of a single FIFO server with those service times.
+
+serve_live is the server on the wall clock: requests arrive in real time at the offered load. In
+"batched" mode (continuous batching) every family has a slot solver (s2l_solver: GMRES over a
+fixed number of right-hand-side slots sharing every launch) stepped by a thread of its own; between
+two restart cycles the thread admits the requests of its family that have arrived into free slots,
+and the families' threads run concurrently, so the solves of different families overlap on the
+device (one context and stream per family). A slot's results do not depend on the other slots, so
+every request gets the bits it gets alone (serve_alone). In "fifo" mode one thread serves the
+requests one at a time in arrival order."""
 from __future__ import annotations
 
+import hashlib
 import math
+import threading
 import time
-from collections import OrderedDict
+from collections import OrderedDict, deque
 from dataclasses import dataclass
 
 import numpy as np
@@ -174,63 +183,169 @@ def timeline(records: list, load: float, *, exclude_builds: bool = True) -> dict
                 requests_per_s=len(records) / span, capacity_solves_per_s=nrhs / float(svc.sum()))
 
 
-def serve_batched(fams: list, reqs: list, resident: ResidentSet, *, scale: float, max_rhs: int = 16,
-                  krylov: str = "gmres", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
-                  solve=None) -> list:
-    """Continuous batching on a virtual clock: arrivals at ``arrival * scale`` seconds; whenever the
-    server is free it takes the oldest waiting request and every other waiting request of the same
-    family (up to ``max_rhs`` right-hand sides), solves them together (each right-hand side's
-    result is that of its solve alone: the launches are shared, the arithmetic per right-hand side
-    is not) and advances the clock by the measured time. Returns per request its arrival, finish,
-    latency, batch size, iterations, convergence and true residual. ``solve(M, B) -> list`` replaces
-    the device solve in tests."""
-    def device_solve(M, B):
-        solver = device.HipGmres(restart, rtol, maxit) if krylov == "gmres" else device.HipBiCgStab(rtol, maxit)
-        _, res = solver.solve(M, B if B.shape[0] > 1 else B[0])
-        return res if isinstance(res, list) else [res]
-    solve = solve or device_solve
-    pending = sorted(reqs, key=lambda q: q.arrival)
-    waiting: list = []
+def _rhs(M, q):
+    """The right-hand sides of a request in the family's renumbered order."""
+    n = M.setup.n
+    B = np.stack([hs.random_rhs(n, q.seed + r) for r in range(q.nrhs)])
+    return B[:, M.setup.dec.new_to_old]
+
+
+def _digest(x: np.ndarray) -> str:
+    return hashlib.blake2b(np.ascontiguousarray(x).tobytes(), digest_size=8).hexdigest()
+
+
+def slot_solver(M, slots: int, restart: int, rtol: float, maxit: int):
+    """The family's slot solver (created once, kept with its preconditioner; its device memory counts
+    in the family's footprint)."""
+    sv = getattr(M, "slot_solver", None)
+    key = (slots, restart, rtol, maxit)
+    if sv is None or M.slot_solver_key != key:
+        M.slot_solver = sv = M.ctx.solver(slots, restart=restart, rtol=rtol, maxit=maxit)
+        M.slot_solver_key = key
+    return sv
+
+
+def _finish(rec, results):
+    rec["iterations"] = [r.iterations for _, r in results]
+    rec["converged"] = all(r.converged for _, r in results)
+    rec["true_rel_residual"] = max(r.true_rel_residual for _, r in results)
+    rec["digest"] = [_digest(x) for x, _ in results]
+
+
+def serve_alone(fams: list, reqs: list, resident: ResidentSet, *, slots: int = 16, restart: int = 50,
+                rtol: float = 1e-8, maxit: int = 1000, log=None) -> list:
+    """Every request by itself through its family's slot solver, one after the other (no other slot
+    busy): the reference results of the continuous-batching server (iterations and solution digests
+    per right-hand side) and the service time of each request."""
     out = []
-    t = 0.0
-    i = 0
-    while i < len(pending) or waiting:
-        while i < len(pending) and pending[i].arrival * scale <= t:
-            waiting.append(pending[i])
-            i += 1
-        if not waiting:
-            t = pending[i].arrival * scale
-            continue
-        fam = waiting[0].family
-        batch, rhs = [], 0
-        for q in waiting:
-            if q.family == fam and rhs + q.nrhs <= max(max_rhs, waiting[0].nrhs):
-                batch.append(q)
-                rhs += q.nrhs
-        waiting = [q for q in waiting if q not in batch]
+    for q in reqs:
         t0 = time.perf_counter()
-        M = resident.get(fam)
-        n = M.setup.n
-        B = np.stack([hs.random_rhs(n, q.seed + r) for q in batch for r in range(q.nrhs)])
-        if hasattr(M, "setup") and hasattr(M.setup, "dec"):
-            B = B[:, M.setup.dec.new_to_old]
-        res = solve(M, B)
-        resident.fit(fam)
-        t += time.perf_counter() - t0
-        k = 0
-        for q in batch:
-            rr = res[k:k + q.nrhs]
-            k += q.nrhs
-            out.append(dict(id=q.id, family=fam, nrhs=q.nrhs, arrival=q.arrival * scale, finish=t,
-                            latency=t - q.arrival * scale, batch=rhs, iterations=max(r.iterations for r in rr),
-                            converged=all(r.converged for r in rr),
-                            true_rel_residual=max(r.true_rel_residual for r in rr)))
-    return sorted(out, key=lambda r: r["id"])
+        M = resident.get(q.family)
+        sv = slot_solver(M, slots, restart, rtol, maxit)
+        B = _rhs(M, q)
+        for r in range(q.nrhs):
+            sv.admit(r, B[r])
+        done = set()
+        results = [None] * q.nrhs
+        while len(done) < q.nrhs:
+            for slot in sv.step():
+                slot = int(slot)
+                if slot not in done:
+                    results[slot] = sv.take(slot)
+                    done.add(slot)
+        resident.fit(q.family)
+        rec = dict(id=q.id, family=q.family, nrhs=q.nrhs, service=time.perf_counter() - t0)
+        _finish(rec, results)
+        out.append(rec)
+        if log:
+            log(rec)
+    return out
 
 
-def batched_summary(records: list) -> dict:
+def serve_live(fams: list, reqs: list, resident: ResidentSet, *, scale: float, mode: str = "batched",
+               slots: int = 16, restart: int = 50, rtol: float = 1e-8, maxit: int = 1000, log=None) -> list:
+    """Serves the requests on the wall clock: request q arrives at q.arrival * scale seconds after the
+    start. "batched": one thread per family steps its slot solver one restart cycle at a time and
+    admits, between cycles, the arrived requests of its family (first come, first served) whose
+    right-hand sides fit the free slots; "fifo": one thread solves the requests one at a time in
+    arrival order (s2l_krylov_solve with the request's right-hand sides). Every family must be
+    resident (built before the call). Returns per request its arrival, admission and finish times
+    [s], latency, iterations per right-hand side, convergence, the largest true residual and the
+    digests of the solutions."""
+    if mode not in ("batched", "fifo"):
+        raise ValueError(f"unknown serving mode '{mode}' (batched|fifo)")
+    members = {k: resident.get(k) for k in sorted({q.family for q in reqs})}
+    solvers = {k: slot_solver(M, slots, restart, rtol, maxit) for k, M in members.items()} if mode == "batched" else {}
+    recs = {q.id: dict(id=q.id, family=q.family, nrhs=q.nrhs, arrival=q.arrival * scale) for q in reqs}
+    done_count = [0]
+    lock = threading.Lock()
+    errors = []
+    t_start = time.perf_counter()
+
+    def clock():
+        return time.perf_counter() - t_start
+
+    def completed(q, results):
+        rec = recs[q.id]
+        rec["finish"] = clock()
+        rec["latency"] = rec["finish"] - rec["arrival"]
+        _finish(rec, results)
+        with lock:
+            done_count[0] += 1
+            if log and done_count[0] % 100 == 0:
+                log(f"{mode}: {done_count[0]} requests done at {rec['finish']:.1f} s")
+
+    def family_worker(k):
+        M, sv = members[k], solvers[k]
+        pending = deque(sorted((q for q in reqs if q.family == k), key=lambda q: q.arrival))
+        free = list(range(slots))
+        owner = {}      # slot -> (request, index of its right-hand side)
+        partial = {}    # request id -> results so far
+        while pending or owner:
+            now = clock()
+            while pending and pending[0].arrival * scale <= now and len(free) >= pending[0].nrhs:
+                q = pending.popleft()
+                recs[q.id]["admitted"] = now
+                B = _rhs(M, q)
+                partial[q.id] = [None] * q.nrhs
+                for r in range(q.nrhs):
+                    slot = free.pop(0)
+                    sv.admit(slot, B[r])
+                    owner[slot] = (q, r)
+            if not owner:
+                time.sleep(max(0.0, pending[0].arrival * scale - clock()))
+                continue
+            for slot in sv.step():
+                slot = int(slot)
+                q, r = owner.pop(slot)
+                partial[q.id][r] = sv.take(slot)
+                free.append(slot)
+                if all(v is not None for v in partial[q.id]):
+                    completed(q, partial.pop(q.id))
+
+    def fifo_worker():
+        for q in sorted(reqs, key=lambda q: q.arrival):
+            wait = q.arrival * scale - clock()
+            if wait > 0:
+                time.sleep(wait)
+            recs[q.id]["admitted"] = clock()
+            M = members[q.family]
+            B = _rhs(M, q)
+            X, res = M.ctx.gmres(B if q.nrhs > 1 else B[0], None, restart, rtol, maxit)
+            res = res if isinstance(res, list) else [res]
+            X = X.reshape(q.nrhs, -1)
+            completed(q, list(zip(X, res)))
+
+    def guard(fn, *a):
+        try:
+            fn(*a)
+        except BaseException as e:  # reported after the join
+            errors.append(e)
+
+    threads = ([threading.Thread(target=guard, args=(family_worker, k)) for k in members] if mode == "batched"
+               else [threading.Thread(target=guard, args=(fifo_worker,))])
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return [recs[q.id] for q in sorted(reqs, key=lambda q: q.id)]
+
+
+def live_summary(records: list) -> dict:
+    """Wall-clock latency percentiles, right-hand sides solved per second over the run (first
+    arrival to last finish) and requests per second."""
     lat = np.array([r["latency"] for r in records])
     span = max(r["finish"] for r in records) - min(r["arrival"] for r in records)
     return dict(p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 max_ms=1e3 * float(lat.max()), solves_per_s=sum(r["nrhs"] for r in records) / span,
-                mean_batch_rhs=float(np.mean([r["batch"] for r in records])))
+                requests_per_s=len(records) / span, span_s=span)
+
+
+def same_as_alone(alone: list, live: list) -> dict:
+    """Per request: the same iterations and the same solution bits (digests) for every right-hand
+    side, alone and in the mix. Returns the count of requests that differ and their ids."""
+    diff = [b["id"] for a, b in zip(alone, live)
+            if a["id"] != b["id"] or a["iterations"] != b["iterations"] or a["digest"] != b["digest"]]
+    return dict(requests=len(live), differing=len(diff), differing_ids=diff[:20])

@@ -15,13 +15,15 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                   correctors: int = 2, reuse: str = "auto", growth: float = 4.0, max_age: int = 0,
                   precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
                   seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
-                  log=None, matrix: str | None = None, coordinates: bool = True, krylov: str = "gmres") -> list:
+                  log=None, matrix: str | None = None, coordinates: bool = True, krylov: str = "gmres",
+                  eig_workers: int = 0) -> list:
     """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
     performed before it, iterations, convergence, true relative residual, update and solve times).
     With ``matrix`` (a file name or a step pattern) the matrices are read, not generated; a step
     whose size, pattern or symmetry differs from the first one raises ValueError on refactorize.
     ``coordinates=False`` partitions generated matrices by their graph only, as read ones are.
-    ``krylov``: "gmres" (GMRES(restart)) or "bicgstab"."""
+    ``krylov``: "gmres" (GMRES(restart)) or "bicgstab". ``eig_workers``: processes of the local
+    eigenproblems of a rebuild (forked only while this process has no GPU context)."""
     gen_kw = dict(gen_kw or {})
     policy = ReusePolicy(reuse, growth, max_age)
     setup = M = None
@@ -40,7 +42,7 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
             t0 = time.perf_counter()
             if level == REBUILD:
                 setup = hs.build_setup(A, b, nparts, xyz=xyz, symmetric=sym, block_size=bs, overlap=overlap, pou=pou,
-                                       nev=nev)
+                                       nev=nev, eig_workers=eig_workers)
                 M = device.HipTwoLevelSchwarz(setup, precision)
             elif level == REFACTORIZE:
                 setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou, symmetric=sym)
@@ -50,8 +52,11 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
             solver = device.HipGmres(restart, rtol, maxit) if krylov == "gmres" else device.HipBiCgStab(rtol, maxit)
             x, res = solver.solve(M, b[setup.dec.new_to_old])
             t_solve = time.perf_counter() - t0
-            # the solves start from 0: the reduction is the final relative residual
-            policy.record(level, res.iterations, res.true_rel_residual, bool(res.converged), t_update, t_solve, kind=c)
+            # the solves start from 0: the reduction is the final relative residual. Every corrector is
+            # the same kind of solve (one reference q): the reference gives the correctors their own
+            # kinds only with a warm start or final tolerances (time_loop.cpp:154-158), neither of
+            # which this loop has
+            policy.record(level, res.iterations, res.true_rel_residual, bool(res.converged), t_update, t_solve, kind=0)
             rec = dict(step=step, corrector=c, level=level, iterations=res.iterations, converged=bool(res.converged),
                        true_rel_residual=res.true_rel_residual, t_update=t_update, t_solve=t_solve,
                        wasted=policy.wasted())

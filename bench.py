@@ -34,15 +34,20 @@ def _load():
 
 HBM_PEAK_GBS = 8000.0
 
+# the bubbly pressure sequence: coefficient 1 in the liquid and 1e-3 in the gas (rho_ratio 1e-3:
+# beta = 1 / (alpha rho_g + 1 - alpha), rho_g = 1 / rho_ratio), 200 bubbles of radii uniform in
+# 3-8 cells rising 1 cell per step (cfl 1), bubble seed 2024
+BUBBLES_128 = dict(nx=128, nbubbles=200, rho_ratio=1e-3, rmin_cells=3.0, rmax_cells=8.0, cfl=1.0, seed=2024)
+
 WORKLOADS = {
     # name: (generator, kwargs, subdomains, overlap, nev, restart, description)
     "poisson3d-64": ("poisson3d", dict(nx=64), 64, 2, 24, 50,
                      "3D Poisson 7-point 64^3, 64 subdomains, overlap 2, nev 24, GMRES(50), rtol 1e-8"),
     "poisson2d-256": ("poisson2d", dict(nx=256), 8, 1, 20, 30,
                       "2D Poisson 5-point 256^2, 8 subdomains, overlap 1, nev 20, additive, GMRES(30), rtol 1e-8"),
-    "bubbles3d-128": ("bubbles3d", dict(nx=128, nbubbles=200, rho_ratio=1000.0, seed=2024), 256, 1, 8, 50,
-                      "bubbly pressure 7-point 128^3 (200 seeded bubbles, density ratio 1000), 256 subdomains, "
-                      "overlap 1, nev 8, GMRES(50), rtol 1e-8"),
+    "bubbles3d-128": ("bubbles3d", dict(BUBBLES_128), 256, 1, 8, 50,
+                      "bubbly pressure 7-point 128^3 (coefficient 1 in the liquid, 1e-3 inside 200 seeded bubbles "
+                      "of radii uniform in 3-8 cells), 256 subdomains, overlap 1, nev 8, GMRES(50), rtol 1e-8"),
     "highcontrast3d-96": ("highcontrast3d", dict(nx=96, seed=2024, contrast=1e6, channel_fraction=0.1), 128, 1, 32, 50,
                           "3D diffusion 7-point 96^3 with seeded axis-aligned channels of coefficient 1e6 (10 % of "
                           "the volume), 128 subdomains, overlap 1, nev 32, 16 right-hand sides solved together, "
@@ -152,10 +157,13 @@ def log(*a):
 
 
 def serving_main(args):
-    """The serving mix: a seeded Poisson-arrival stream of requests over 6 resident families
-    (2D 5-point, 3D 7-point, 2D anisotropic 1e-3; n log-uniform in [1e4, 2e6]), 1-8 right-hand
-    sides each, GMRES(50) to rtol with FP32 local factors, admitted by device footprint. Service
-    times are measured back to back on the GPU; the FIFO queue is replayed at 70 % offered load."""
+    """The serving mix: a seeded Poisson-arrival stream of requests over 6 resident families (2D
+    5-point, 3D 7-point, 2D anisotropic 1e-3; n log-uniform in [1e4, 2e6]), 1-8 right-hand sides
+    each, GMRES(50) to rtol with FP32 local factors, admitted by device footprint. Passes: (1) every
+    request back to back on the plain device solve (the FIFO service times and capacity); (2) every
+    request alone through its family's slot solver (the reference bits); (3) the wall-clock server
+    with continuous batching at 0.7x and 1.5x the FIFO capacity, and the wall-clock FIFO server at
+    0.7x. Latencies are measured on the wall clock from arrival to the solution on the host."""
     from amp_amd import serving as sv
     fams, reqs = sv.draw_mix(args.requests, 6, (1e4, 2e6), 8, seed=args.seed)
     log("families:", ", ".join(f"{f.kind} nx={f.nx} (n={f.n})" for f in fams))
@@ -163,42 +171,54 @@ def serving_main(args):
     resident = sv.ResidentSet(budget, lambda k: sv.build_family(fams[k], precision="single"))
     t0 = time.perf_counter()
     for k in range(len(fams)):  # a server admits its resident families before taking requests
-        resident.get(k)
-        log(f"admitted family {k}: {resident.footprint(resident.live[k]) / 2**20:.1f} MiB on the device")
+        M = resident.get(k)
+        sv.slot_solver(M, args.slots, 50, args.rtol, 1000)
+        resident.fit(k)
+        log(f"admitted family {k}: {resident.footprint(M) / 2**20:.1f} MiB on the device (slot solver included)")
     admit_s = time.perf_counter() - t0
+    every = max(1, len(reqs) // 10)
     recs = sv.serve(fams, reqs, resident, restart=50, rtol=args.rtol,
-                    log=lambda r: log(f"request {r['id']}: family {r['family']} nrhs {r['nrhs']} "
-                                      f"{1e3 * r['service']:.2f} ms {r['iterations']} its")
-                    if r["id"] % 100 == 0 else None)
-    t = sv.timeline(recs, 0.7)
-    svc_mean = t["mean_service_ms"] / 1e3
-    batched = {}
-    for load in (0.7, 1.5):  # offered load relative to the FIFO server's capacity
-        rb = sv.serve_batched(fams, reqs, resident, scale=svc_mean / load, max_rhs=16, rtol=args.rtol)
-        bs = sv.batched_summary(rb)
-        same = all(a["iterations"] == b["iterations"] and b["converged"]
-                   and abs(a["true_rel_residual"] - b["true_rel_residual"]) <= 0.05 * a["true_rel_residual"]
-                   for a, b in zip(recs, rb))
-        batched[f"load_{load}"] = {k: round(v, 3) for k, v in bs.items()}
-        batched[f"load_{load}"]["results_same_as_alone"] = same
-        log(f"continuous batching at load {load}: {bs}")
-    out = {"metric": "serving mix: solves/s + p50/p99 request latency (FIFO, 70 % offered load)",
-           "value": round(t["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1, "steps": len(recs), "warmup": 0,
+                    log=lambda r: log(f"fifo back to back: request {r['id']} {1e3 * r['service']:.2f} ms")
+                    if r["id"] % every == 0 else None)
+    svc = np.array([r["service"] for r in recs])
+    capacity_rps = 1.0 / float(svc.mean())  # requests per second of one FIFO server
+    alone = sv.serve_alone(fams, reqs, resident, slots=args.slots, restart=50, rtol=args.rtol,
+                           log=lambda r: log(f"alone: request {r['id']}") if r["id"] % every == 0 else None)
+    runs = {}
+    for mode, load in (("batched", 0.7), ("batched", 1.5), ("fifo", 0.7)):
+        scale = 1.0 / (load * capacity_rps)   # seconds per unit of the seeded inter-arrival times
+        live = sv.serve_live(fams, reqs, resident, scale=scale, mode=mode, slots=args.slots, restart=50,
+                             rtol=args.rtol, log=log)
+        summ = {k: round(v, 3) for k, v in sv.live_summary(live).items()}
+        summ["offered_requests_per_s"] = round(load * capacity_rps, 2)
+        summ["all_converged"] = all(r["converged"] for r in live)
+        summ["max_true_rel_residual"] = max(r["true_rel_residual"] for r in live)
+        if mode == "batched":
+            same = sv.same_as_alone(alone, live)
+            summ["results_same_as_alone"] = same["differing"] == 0
+            summ["requests_differing_from_alone"] = same["differing"]
+            summ["differing_ids"] = same["differing_ids"]
+        runs[f"{mode}_load_{load}"] = summ
+        log(f"live {mode} at load {load}: {summ}")
+    head = runs["batched_load_0.7"]
+    out = {"metric": "serving mix: solves/s + p50/p99 request latency on the wall clock (continuous batching, "
+                     "70 % of the FIFO capacity offered)",
+           "value": head["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 local factors / f64 Krylov",
            "data": "synthetic (generated matrices, seeded right-hand sides, seeded arrivals)",
-           "config": {"workload": f"serving mix: {len(recs)} requests, 6 resident families "
+           "config": {"workload": f"serving mix: {len(reqs)} requests, 6 resident families "
                                   f"(2D 5-point, 3D 7-point, 2D anisotropic 1e-3; n log-uniform in [1e4, 2e6]), "
                                   f"1-8 right-hand sides each, GMRES(50) to {args.rtol}",
-                      "families": [f"{f.kind}:{f.nx}" for f in fams], "parallelism": "single GPU, FIFO"},
-           "p50_ms": round(t["p50_ms"], 3), "p99_ms": round(t["p99_ms"], 3), "max_ms": round(t["max_ms"], 3),
-           "mean_service_ms": round(t["mean_service_ms"], 3), "arrival_rate_per_s": round(t["rate_per_s"], 2),
-           "capacity_solves_per_s": round(t["capacity_solves_per_s"], 2),
-           "continuous_batching": batched,
-           "all_converged": all(r["converged"] for r in recs),
-           "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
+                      "families": [f"{f.kind}:{f.nx}" for f in fams], "slots_per_family": args.slots,
+                      "parallelism": "single GPU, one context, stream and host thread per family"},
+           "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
+           "results_same_as_alone": all(v.get("results_same_as_alone", True) for v in runs.values()),
+           "fifo_back_to_back": {"mean_service_ms": round(1e3 * float(svc.mean()), 3),
+                                 "capacity_requests_per_s": round(capacity_rps, 2),
+                                 "capacity_solves_per_s": round(sum(r["nrhs"] for r in recs) / float(svc.sum()), 2)},
+           "live": runs,
            "admission": {"budget_bytes": budget, "builds": resident.builds, "evictions": resident.evictions,
-                         "admit_s": round(admit_s, 1),
-                         "resident_bytes": int(resident.used())}}
+                         "admit_s": round(admit_s, 1), "resident_bytes": int(resident.used())}}
     print(json.dumps(out), flush=True)
 
 
@@ -208,18 +228,22 @@ def reuse_main(args):
     builds the preconditioner, every later step refactorizes (new local factors and A_00, the same
     coarse basis Z) and its second corrector reuses; GMRES(50) and BiCGStab, FP32 local factors."""
     from amp_amd.timeloop import run_time_loop
-    kw = dict(nx=args.seq_nx, nbubbles=200, rho_ratio=1000.0, seed=2024)
+    kw = dict(BUBBLES_128, nx=args.seq_nx)
+    if args.seq_nx != 128:  # the same bubble density on a smaller grid
+        kw["nbubbles"] = max(1, round(200 * (args.seq_nx / 128.0) ** 3))
     nparts = max(4, (args.seq_nx ** 3) // 8192)
     out = {"metric": "setup-reuse sequence: solves/s over the sequence (updates + solves)", "unit": "solves/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 local factors / f64 Krylov", "data": "synthetic (generated matrices, seeded right-hand sides)",
-           "config": {"workload": f"bubbly pressure 7-point {args.seq_nx}^3, 200 bubbles drifting with the step, "
+           "config": {"workload": f"bubbly pressure 7-point {args.seq_nx}^3, coefficient 1e-3 inside {kw['nbubbles']} "
+                                  f"bubbles of radii 3-8 cells rising 1 cell per step, "
                                   f"{args.seq_steps} steps x 2 correctors, {nparts} subdomains, overlap 1, nev 8, "
-                                  f"deflated, reuse policy 'refactor' (factors refreshed, coarse basis kept)",
+                                  f"deflated, reuse policy '{args.reuse_policy}'",
                       "parallelism": "single GPU, all subdomains resident"}}
     for krylov in ("gmres", "bicgstab"):
-        recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse="refactor",
+        recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse=args.reuse_policy,
                              precision="single", restart=50, rtol=args.rtol, nev=8, krylov=krylov,
+                             eig_workers=args.setup_workers or 16,
                              log=lambda r: log(f"{krylov} step {r['step']} corrector {r['corrector']}: {r['level']} "
                                                f"{r['t_update']:.2f} s + {1e3 * r['t_solve']:.1f} ms, "
                                                f"{r['iterations']} its"))
@@ -233,8 +257,7 @@ def reuse_main(args):
                        "rebuild_s": round(recs[0]["t_update"], 2),
                        "refactorize_s_mean": round(float(np.mean(refac)), 2) if refac else None,
                        "solve_ms_mean": round(1e3 * sol / len(recs), 2),
-                       "continuous_batching": batched,
-           "all_converged": all(r["converged"] for r in recs),
+                       "all_converged": all(r["converged"] for r in recs),
                        "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
                        "levels": "".join({"rebuild": "2", "refactorize": "1", "reuse": "0"}[r["level"]] for r in recs)}
     out["value"] = out["gmres"]["solves_per_s"]
@@ -277,10 +300,14 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="requests of the serving mix")
     ap.add_argument("--serving-budget-gb", type=float, default=64.0,
                     help="device memory the serving mix may keep resident (GiB)")
+    ap.add_argument("--slots", type=int, default=16,
+                    help="right-hand-side slots of every family's solver in the serving mix")
     ap.add_argument("--reuse-sequence", action="store_true",
                     help="run the setup-reuse sequence (bubbly pressure, refactorized every step) instead")
     ap.add_argument("--seq-steps", type=int, default=20, help="steps of the reuse sequence")
     ap.add_argument("--seq-nx", type=int, default=128, help="grid of the reuse sequence (nx^3)")
+    ap.add_argument("--reuse-policy", default="refactor", choices=("off", "refactor", "auto"),
+                    help="reuse policy of the sequence")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the apply path runs on one GPU; --gpus {args.gpus} is not supported (it would "

@@ -6,7 +6,8 @@
 //   prec::TwoLevelSchwarz  src/preconditioner/two_level_schwarz.cpp:129-163 (here HipTwoLevelSchwarz)
 //   solv::KrylovResult     include/solver/krylov.hpp:43-54
 //   solv::KrylovSolver     include/solver/krylov.hpp:60-76, Gmres / BiCgStab (gmres.hpp, bicgstab.hpp)
-//   solv::makeKrylovSolver src/solver/krylov.cpp:23-36
+//   solv::makeKrylovSolver src/solver/krylov.cpp:23-36 (from core::SolverConfig, include/core/config.hpp:19-117)
+//   linalg::DistributedMatrix include/linalg/distributed_matrix.hpp (the operator argument of solve)
 // Vectors are std::vector<double> with the owned entries of every subdomain in the renumbered
 // global order (all subdomains live in one process and on one GPU). Errors of the C-ABI become
 // std::runtime_error with the library's message; invalid arguments std::invalid_argument.
@@ -24,7 +25,43 @@ namespace s2l {
 
 namespace core {
 using Vec = std::vector<double>;
-}
+
+/// The run-time parameters the device path reads, with the names, meanings and defaults of the
+/// reference's SolverConfig (include/core/config.hpp:19-117); the setup parameters (partitioning,
+/// eigenproblems, local factorization) are the host setup's.
+struct SolverConfig {
+    // preconditioner
+    std::string pou = "boolean";            ///< "boolean" | "multiplicity"
+    std::string one_level = "ras";          ///< "ras" | "asm"
+    std::string coarse = "deflated";        ///< "deflated" | "additive" | "none"
+    std::string local_precision = "double"; ///< "double" | "single" (FP32 factors, solveIn<float>)
+    // Krylov solver
+    std::string krylov = "gmres";           ///< "gmres" (restarted, right preconditioning) | "bicgstab"
+    int restart = 30;                       ///< GMRES restart length
+    double rtol = 1e-8;                     ///< relative tolerance
+    double atol = 0.0;                      ///< OpenFOAM norm only: absolute tolerance on the normalised residual
+    std::string stop_norm = "l2";           ///< "l2" | "openfoam"
+    int maxit = 1000;                       ///< maximum applications of the preconditioner
+    // added for the device path (the switch at src/main.cpp:143-168 and src/solver/time_loop.cpp:128-151)
+    std::string backend = "hip";            ///< "hip": this library
+    int device = 0;                         ///< HIP device of the context
+};
+} // namespace core
+
+namespace linalg {
+
+/// The operator of a Krylov solve (linalg::DistributedMatrix, include/linalg/distributed_matrix.hpp:
+/// y = A x on the owned entries). The device Krylov solvers take the operator uploaded with their
+/// preconditioner (solv::HipOperator).
+class DistributedMatrix {
+public:
+    virtual ~DistributedMatrix() = default;
+    virtual void apply(const core::Vec& x, core::Vec& y) const = 0;
+    /// owned rows (all rows: every subdomain lives in this process)
+    virtual int32_t nOwned() const = 0;
+};
+
+} // namespace linalg
 
 /// What the host setup produced, in the form the device layout is built from (the reference's
 /// setup objects: LocalSystem rows, Subdomain index maps and partition of unity, LocalSolver
@@ -58,6 +95,9 @@ struct SetupData {
     std::string correction = "deflated"; ///< "deflated" | "additive"
     std::string precision = "double";    ///< local factors: "double" | "single" | "mixed" (FP32 entries, FP64 arithmetic)
     int nsub() const { return static_cast<int>(local_ptr.size()) - 1; }
+    /// Same size and sparsity pattern of A (rowptr and columns): the condition of a refactorization
+    /// (DomainDecomposer::updateValues, src/partition/domain_decomposer.cpp:345-352).
+    static bool samePattern(const SetupData& a, const SetupData& b);
 };
 
 namespace prec {
@@ -78,11 +118,30 @@ public:
     virtual long coarseFallbacks() const { return 0; }
 };
 
+/// The host setup a device preconditioner is built from and brought up to date by. The
+/// reference's TwoLevelSchwarz runs this setup itself from its Subdomain (partition, overlap, local
+/// factorizations, eigenproblems, A_00: two_level_schwarz.cpp:14-47); here it stays host code
+/// behind this interface and the device side uploads what it returns.
+class SetupSource {
+public:
+    virtual ~SetupSource() = default;
+    /// Full setup from the current values of A (Rebuild: factors, new coarse basis and A_00).
+    virtual SetupData build() = 0;
+    /// New local factors and A_00 = V^T A V from the current values, with the subdomains and the
+    /// coarse basis of the last build() (Refactorize, two_level_schwarz.cpp:35-41).
+    virtual SetupData refactorize() = 0;
+    /// Changes whenever the values of A change (Subdomain::valuesVersion).
+    virtual uint64_t valuesVersion() const = 0;
+};
+
 /// The two-level Schwarz preconditioner with every subdomain resident on one GPU.
 class HipTwoLevelSchwarz final : public Preconditioner {
 public:
     /// Creates a context on @p device and uploads the setup (copies it; @p s may be freed).
     explicit HipTwoLevelSchwarz(const SetupData& s, int device = 0);
+    /// Builds from @p source (source->build()) and keeps it for update(): the drop-in for
+    /// TwoLevelSchwarz(sd, A, cfg) with its update levels.
+    explicit HipTwoLevelSchwarz(std::shared_ptr<SetupSource> source, int device = 0);
     ~HipTwoLevelSchwarz() override;
     HipTwoLevelSchwarz(const HipTwoLevelSchwarz&) = delete;
     HipTwoLevelSchwarz& operator=(const HipTwoLevelSchwarz&) = delete;
@@ -96,12 +155,16 @@ public:
     /// The preconditioner was built for other values of A (Subdomain::updateValues in the reference).
     void markStale() { up_to_date_ = false; }
     /// Refactorize on the device side: uploads the new values of A, of the local factors and of
-    /// A_00 computed by the host (same structure, same coarse basis). Returns Refactorize.
+    /// A_00 computed by the host (same size, pattern and coarse basis: std::invalid_argument
+    /// otherwise). Returns Refactorize.
     UpdateLevel refresh(const SetupData& s);
-    /// Reuse when up to date; a stale preconditioner needs refresh() (Refactorize) or a new object
-    /// (Rebuild: the host setup is not part of this library): std::logic_error.
+    /// The semantics of Preconditioner::update (preconditioner.hpp:75-87, two_level_schwarz.cpp:
+    /// 30-47): Reuse, or Refactorize of an up-to-date preconditioner, does nothing and returns
+    /// Reuse; Refactorize uploads source->refactorize(); Rebuild is always performed and uploads
+    /// source->build(). Without a source (built from SetupData) a stale preconditioner is brought
+    /// up to date by refresh(), and Refactorize / Rebuild throw std::logic_error.
     UpdateLevel update(UpdateLevel requested) override;
-    bool upToDate() const override { return up_to_date_; }
+    bool upToDate() const override;
 
     int32_t n() const { return n_; }
     s2l_ctx* context() const { return ctx_; }
@@ -109,11 +172,15 @@ public:
     void spmv(const core::Vec& x, core::Vec& y) const;
 
 private:
+    void create(const SetupData& s, int device);
     void upload(const SetupData& s);
     s2l_ctx* ctx_ = nullptr;
     int32_t n_ = 0;
     std::string name_;
     bool up_to_date_ = true;
+    std::shared_ptr<SetupSource> source_;
+    uint64_t version_ = 0;
+    std::vector<int32_t> rowptr_, col_;  ///< pattern of the uploaded system (refresh checks it)
 };
 
 /// Chooses the update level before each solve of a sequence of matrices with one sparsity pattern
@@ -180,22 +247,27 @@ struct KrylovResult {
     std::vector<double> history;
 };
 
-/// The operator A of a HipTwoLevelSchwarz (the system uploaded with its setup).
-class HipOperator {
+/// The operator A of a HipTwoLevelSchwarz (the system uploaded with its setup), as the
+/// linalg::DistributedMatrix built from the same Subdomain as the preconditioner in the reference.
+class HipOperator final : public linalg::DistributedMatrix {
 public:
     explicit HipOperator(const prec::HipTwoLevelSchwarz& M) : M_(M) {}
-    void apply(const core::Vec& x, core::Vec& y) const { M_.spmv(x, y); }
+    void apply(const core::Vec& x, core::Vec& y) const override { M_.spmv(x, y); }
+    int32_t nOwned() const override { return M_.n(); }
     const prec::HipTwoLevelSchwarz& owner() const { return M_; }
 
 private:
     const prec::HipTwoLevelSchwarz& M_;
 };
 
-/// Solves A x = b with a preconditioner M; x is the initial guess in and the solution out.
+/// Solves A x = b with a preconditioner M; x is the initial guess in and the solution out
+/// (KrylovSolver::solve, include/solver/krylov.hpp:60-76). The device solvers run the whole loop
+/// on the GPU: A must be the HipOperator of M's context and M a HipTwoLevelSchwarz, else
+/// std::invalid_argument (there is no host loop to fall back to).
 class KrylovSolver {
 public:
     virtual ~KrylovSolver() = default;
-    virtual KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+    virtual KrylovResult solve(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b,
                                core::Vec& x) const = 0;
     virtual std::string name() const = 0;
 };
@@ -205,7 +277,7 @@ public:
 class HipGmres final : public KrylovSolver {
 public:
     HipGmres(int restart, double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0, bool flexible = false);
-    KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+    KrylovResult solve(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b,
                        core::Vec& x) const override;
     std::string name() const override { return "GMRES(" + std::to_string(m_) + ")"; }
 
@@ -220,7 +292,7 @@ private:
 class HipBiCgStab final : public KrylovSolver {
 public:
     HipBiCgStab(double rtol, int maxit, StopNorm stop = StopNorm::L2, double atol = 0.0);
-    KrylovResult solve(const HipOperator& A, const prec::Preconditioner& M, const core::Vec& b,
+    KrylovResult solve(const linalg::DistributedMatrix& A, const prec::Preconditioner& M, const core::Vec& b,
                        core::Vec& x) const override;
     std::string name() const override { return "BiCGStab"; }
 
@@ -230,7 +302,13 @@ private:
     StopNorm stop_;
 };
 
-/// --krylov gmres (with restart) | bicgstab; an absolute tolerance needs the OpenFOAM norm.
+/// Krylov solver of a configuration (src/solver/krylov.cpp:23-36): --krylov gmres (with --restart)
+/// or bicgstab; --rtol, --atol, --maxit, --stop-norm. std::invalid_argument for an unknown solver
+/// or stopping norm, an absolute tolerance without the OpenFOAM norm, or another backend than "hip".
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const core::SolverConfig& cfg);
+/// Same, with the tolerances @p rtol and @p atol instead of those of @p cfg (e.g. the last corrector).
+std::unique_ptr<KrylovSolver> makeKrylovSolver(const core::SolverConfig& cfg, double rtol, double atol);
+/// The same factory from the individual parameters.
 std::unique_ptr<KrylovSolver> makeKrylovSolver(const std::string& krylov, int restart, double rtol, double atol,
                                                int maxit, const std::string& stop_norm = "l2");
 

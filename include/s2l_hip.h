@@ -50,7 +50,9 @@ const char* s2l_last_error(const s2l_ctx* ctx);
  * with the subdomain ownership (N + 1 offsets). Every row is reordered to the local column
  * order of its owner (owned columns, then ghosts by increasing global id), which is the order
  * of A(Omega_I, Omega_i) in DistributedMatrix::apply (include/linalg/distributed_matrix.hpp:33-38,
- * include/partition/subdomain.hpp:112). Uploading again with the same pattern refreshes the values. */
+ * include/partition/subdomain.hpp:112). Uploading again with the same n, row pointers, columns and
+ * ownership refreshes the values only; any other upload is a new system, and the subdomains,
+ * factors and coarse space must be uploaded again before the next apply (S2L_ESTATE otherwise). */
 int s2l_upload_system(s2l_ctx* ctx, int32_t n, const int32_t* rowptr, const int32_t* col, const double* val,
                       int32_t nsub, const int32_t* ownership);
 
@@ -137,6 +139,30 @@ typedef struct s2l_krylov_result {
 
 int s2l_krylov_solve(s2l_ctx* ctx, const s2l_krylov_opts* opts, const double* d_b, double* d_x, int32_t nrhs,
                      s2l_krylov_result* results, double* history, void* stream);
+
+/* ---- Continuous batching: a GMRES run over a fixed number of right-hand-side slots.
+ * No single reference counterpart: it runs Gmres::solve (src/solver/gmres.cpp:18-161) for every
+ * slot, and its callers are the request loops around the solve (src/solver/time_loop.cpp:57-195
+ * solves one right-hand side after another). Every slot is an independent solve whose kernels are
+ * shared with the other slots (one launch per kernel for all of them); a slot's results do not
+ * depend on which other slots are busy, so a request gets the same bits alone and in a mix. A
+ * request enters a free slot with s2l_solver_admit and starts at the next s2l_solver_step, which
+ * runs one restart cycle of every busy slot; a finished slot's result is taken with
+ * s2l_solver_take, which frees the slot. The solver owns its right-hand sides, solutions and
+ * Krylov workspace (counted in s2l_device_footprint of its context) and uses the context's stream
+ * and apply buffers: the solvers of one context run one at a time (one host thread); solvers of
+ * different contexts run concurrently and their launches overlap on the device. */
+typedef struct s2l_solver s2l_solver;
+int s2l_solver_create(s2l_ctx* ctx, const s2l_krylov_opts* opts, int32_t slots, s2l_solver** out);
+int s2l_solver_destroy(s2l_solver* solver);
+/* b (and x0, NULL = 0): n values, host memory if @p host, else device memory. */
+int s2l_solver_admit(s2l_solver* solver, int32_t slot, const double* b, const double* x0, int host);
+/* One restart cycle of every busy slot (admitted ones start first). finished[slot] (slots
+ * entries, may be NULL) = 1 for a slot that holds a finished solve not yet taken. */
+int s2l_solver_step(s2l_solver* solver, int32_t* finished);
+/* KrylovResult of a finished slot, its history (maxit + 1 entries, may be NULL) and its solution
+ * (n values to host memory if @p host, else device memory; may be NULL); frees the slot. */
+int s2l_solver_take(s2l_solver* solver, int32_t slot, s2l_krylov_result* result, double* history, double* x, int host);
 
 /* ---- Host-memory conveniences (copy in, run on the context's stream, copy out, synchronize). */
 int s2l_apply_host(s2l_ctx* ctx, const double* r, double* z, int32_t nrhs);

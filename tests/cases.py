@@ -1,5 +1,11 @@
 """Shared problem builders and independent dense references for the tests (CPU only)."""
 import functools
+import json
+import os
+import pickle
+import subprocess
+import sys
+import tempfile
 
 import numpy as np
 
@@ -20,6 +26,20 @@ def setup(name, kw_items, N, opts_items=(), rhs="random"):
     """Host setup of a generated problem on N subdomains (cached across tests)."""
     A, b, xyz, sym, bs = problem(name, kw_items, N, rhs=rhs)
     return hs.build_setup(A, b, N, xyz=xyz, symmetric=sym, block_size=bs, **dict(opts_items))
+
+
+@functools.lru_cache(maxsize=4)
+def setup_large(name, kw_items, N, opts_items=(), rhs="random", workers=16):
+    """Host setup of a large problem built in a child process (tests/setup_child.py), whose
+    eigensolver pool may fork: this process may have initialised the HIP runtime. Cached."""
+    args = dict(name=name, kw=dict(kw_items), N=N, opts=dict(opts_items), rhs=rhs, workers=workers)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "setup.pkl")
+        here = os.path.dirname(os.path.abspath(__file__))
+        subprocess.run([sys.executable, os.path.join(here, "setup_child.py"), out, json.dumps(args)], check=True,
+                       timeout=600)
+        with open(out, "rb") as f:   # written by setup_child.py just now
+            return pickle.load(f)
 
 
 def items(d):

@@ -4,8 +4,10 @@
 // BiCGStab accounting (test/test_bicgstab.cpp:82-109), solver names and configuration
 // (test/test_bicgstab.cpp:247-261), update semantics.
 //
-// Usage: test_backend <setup file>   (written by tests/test_gpu_backend.py: the setup arrays and the
-// oracle's expected results). Prints one line per check; exit status 1 if any failed.
+// Usage: test_backend <setup file> [<refactorized setup> <rebuilt setup>]   (written by
+// tests/test_gpu_backend.py: the setup arrays and the oracle's expected results; the optional
+// files drive the update levels through a SetupSource). Prints one line per check; exit status 1
+// if any failed.
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
@@ -87,12 +89,50 @@ SetupData readSetup(const s2ltest::ArrayFile& f)
     return s;
 }
 
+// The host setup behind the update levels: build() gives the first setup, then the rebuilt one;
+// refactorize() the refactorized one; bump() stands for Subdomain::updateValues.
+class FileSource final : public prec::SetupSource {
+public:
+    FileSource(const std::string& first, const std::string& refac, const std::string& rebuilt)
+        : first_(first), refac_(refac), rebuilt_(rebuilt)
+    {
+    }
+    SetupData build() override
+    {
+        ++builds;
+        return readSetup(s2ltest::ArrayFile(builds == 1 ? first_ : rebuilt_));
+    }
+    SetupData refactorize() override
+    {
+        ++refactorizations;
+        return readSetup(s2ltest::ArrayFile(refac_));
+    }
+    uint64_t valuesVersion() const override { return version_; }
+    void bump() { ++version_; }
+    int builds = 0, refactorizations = 0;
+
+private:
+    std::string first_, refac_, rebuilt_;
+    uint64_t version_ = 1;
+};
+
+// An operator that is not the device one: the device solvers refuse it (no host loop).
+class HostOperator final : public linalg::DistributedMatrix {
+public:
+    explicit HostOperator(int32_t n) : n_(n) {}
+    void apply(const core::Vec& x, core::Vec& y) const override { y = x; }
+    int32_t nOwned() const override { return n_; }
+
+private:
+    int32_t n_;
+};
+
 } // namespace
 
 int main(int argc, char** argv)
 {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: %s <setup file>\n", argv[0]);
+    if (argc != 2 && argc != 4) {
+        std::fprintf(stderr, "usage: %s <setup file> [<refactorized setup> <rebuilt setup>]\n", argv[0]);
         return 2;
     }
     try {
@@ -179,6 +219,87 @@ int main(int argc, char** argv)
         M.apply(r, z2);
         ok = ok && relDiff(z2, z) == 0.0;
         check(ok, "update: Reuse when up to date, refresh after new values, same apply bits after refresh");
+
+        // 6. a refresh whose pattern differs (two columns of a row swapped, same nnz) is refused
+        {
+            SetupData p = s;
+            size_t row = 0;
+            while (row + 1 < p.rowptr.size() && p.rowptr[row + 1] - p.rowptr[row] < 2) ++row;
+            std::swap(p.col[static_cast<size_t>(p.rowptr[row])], p.col[static_cast<size_t>(p.rowptr[row]) + 1]);
+            check(!SetupData::samePattern(p, s) && rejected([&] { M.refresh(p); }) && M.upToDate(),
+                  "refresh with another sparsity pattern of the same size rejected");
+        }
+
+        // 7. the solvers of a configuration (makeKrylovSolver(cfg[, rtol, atol])); the device loop
+        //    refuses an operator that is not the HIP operator of the preconditioner
+        {
+            core::SolverConfig cfg;
+            cfg.krylov = "gmres";
+            cfg.restart = 30;
+            cfg.maxit = 500;
+            auto k1 = solv::makeKrylovSolver(cfg);
+            cfg.krylov = "bicgstab";
+            auto k2 = solv::makeKrylovSolver(cfg, 1e-6, 0.0);
+            core::SolverConfig bad = cfg;
+            bad.backend = "cpu";
+            core::SolverConfig bad2 = cfg;
+            bad2.atol = 1e-3;
+            const HostOperator H(M.n());
+            core::Vec xx;
+            check(k1->name() == "GMRES(30)" && k2->name() == "BiCGStab" && rejected([&] { solv::makeKrylovSolver(bad); }) &&
+                      rejected([&] { solv::makeKrylovSolver(bad2); }) && rejected([&] { k1->solve(H, M, b, xx); }),
+                  "makeKrylovSolver(SolverConfig): names, backend and atol checks; a host operator is rejected");
+        }
+
+        // 8. GMRES with the OpenFOAM stopping norm (gmres.cpp:106-122): same iterations as the
+        //    oracle, sum|b - A x| <= rtol sum|b| from x0 = 0, normalised residuals reported
+        if (f.has("gmres_of_iterations")) {
+            core::SolverConfig cfg;
+            cfg.restart = 30;
+            cfg.maxit = 500;
+            cfg.stop_norm = "openfoam";
+            const double rtol = 1e-6;
+            auto gof = solv::makeKrylovSolver(cfg, rtol, 0.0);
+            core::Vec xo;
+            const solv::KrylovResult ro = gof->solve(A, M, b, xo);
+            core::Vec Axo;
+            A.apply(xo, Axo);
+            double l1 = 0.0, b1 = 0.0;
+            for (size_t i = 0; i < b.size(); ++i) {
+                l1 += std::abs(b[i] - Axo[i]);
+                b1 += std::abs(b[i]);
+            }
+            const int want = static_cast<int>(f.scalar("gmres_of_iterations"));
+            check(ro.converged && (s.precision != "double" || ro.iterations == want) && l1 <= 1.1 * rtol * b1 &&
+                      std::abs(ro.initial_residual_of - 1.0) < 1e-12 && std::abs(ro.final_residual_of - l1 / b1) < 1e-9 * rtol,
+                  "GMRES, OpenFOAM norm: " + std::to_string(ro.iterations) + " it. (oracle " + std::to_string(want) +
+                      "), sum|r| / sum|b| = " + sci(l1 / b1));
+        }
+
+        // 9. update levels through a SetupSource (two_level_schwarz.cpp:30-47)
+        if (argc == 4) {
+            auto src = std::make_shared<FileSource>(argv[1], argv[2], argv[3]);
+            prec::HipTwoLevelSchwarz P(src);
+            const s2ltest::ArrayFile f1(argv[2]), f2(argv[3]);
+            core::Vec za, zb, zc, zd;
+            P.apply(r, za);
+            const bool fresh = P.upToDate() && P.update(prec::UpdateLevel::Refactorize) == prec::UpdateLevel::Reuse &&
+                               relDiff(za, zref) < tol_apply;
+            src->bump();
+            const bool stale = !P.upToDate();
+            const bool refac = P.update(prec::UpdateLevel::Refactorize) == prec::UpdateLevel::Refactorize && P.upToDate() &&
+                               src->refactorizations == 1;
+            P.apply(r, zb);
+            const double e1 = relDiff(zb, f1.f64("z_expected"));
+            const bool rebuilt = P.update(prec::UpdateLevel::Rebuild) == prec::UpdateLevel::Rebuild && src->builds == 2;
+            P.apply(r, zc);
+            const double e2 = relDiff(zc, f2.f64("z_expected"));
+            const bool reuse = P.update(prec::UpdateLevel::Reuse) == prec::UpdateLevel::Reuse;
+            P.apply(r, zd);
+            check(fresh && stale && refac && e1 < tol_apply && rebuilt && e2 < tol_apply && reuse && relDiff(zd, zc) == 0.0,
+                  "SetupSource: Refactorize after new values (apply vs oracle " + sci(e1) + "), Rebuild always performed (" +
+                      sci(e2) + "), Reuse keeps the bits");
+        }
     } catch (const std::exception& ex) {
         std::printf("FAIL exception: %s\n", ex.what());
         return 1;

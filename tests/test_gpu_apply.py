@@ -55,7 +55,8 @@ def test_gmres_fp64_matches_oracle(name, kw, N):
     assert res.iterations == ro.iterations
     assert res.true_rel_residual < 1e-7
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
-    assert len(res.history) == res.iterations + 1 + 0 * len(ro.history)
+    assert len(res.history) == len(ro.history) == res.iterations + 1
+    assert np.allclose(res.history, ro.history, rtol=1e-6, atol=1e-14)
 
 
 @pytest.mark.parametrize("name,kw,N", [CASES[0], CASES[1], CASES[4]])

@@ -264,22 +264,106 @@ def test_serving_mix_small():
     assert t["p99_ms"] >= t["p50_ms"] > 0 and t["solves_per_s"] > 0
 
 
-def test_serving_results_same_alone_and_batched():
-    """Continuous batching: with every request waiting (arrivals compressed to 0) the requests of a
-    family are solved together, and each one's result is that of its solve alone within the stated
-    tolerance: the same iteration count, both converged to a true residual <= 1.1e-8, true
-    residuals within 5 % of each other (two or more FP32 right-hand sides run the float-pair
-    kernel, whose rounding may differ from the one-RHS kernel's)."""
+def test_serving_results_same_alone_and_in_the_live_mix():
+    """Continuous batching on the wall clock: three families, each stepped by its own thread with a
+    slot solver, requests admitted between restart cycles while others run (arrivals compressed so
+    that they overlap). Every request's right-hand sides have the iterations and the solution bits
+    they have when the request is served alone, and each converges to a true residual <= 1.1e-8."""
     from amp_amd import serving as sv
-    print("case serving batched vs alone, 2 families, 10 requests", flush=True)
-    fams, reqs = sv.draw_mix(10, 2, (2e3, 1e4), 4, seed=5)
+    print("case live serving mix, 3 families, 24 requests", flush=True)
+    fams, reqs = sv.draw_mix(24, 3, (2e3, 2e4), 4, seed=5)
     rs = sv.ResidentSet(1 << 40, lambda k: sv.build_family(fams[k], precision="single", rows_per_subdomain=1500,
                                                           nev=6))
-    alone = sv.serve(fams, reqs, rs, restart=30)
-    mixed = sv.serve_batched(fams, reqs, rs, scale=0.0, max_rhs=16, restart=30)
-    assert max(r["batch"] for r in mixed) > max(q.nrhs for q in reqs)
-    for a, b in zip(alone, mixed):
-        assert a["id"] == b["id"] and b["converged"]
-        assert a["iterations"] == b["iterations"]
-        assert max(a["true_rel_residual"], b["true_rel_residual"]) <= 1.1e-8
-        assert abs(a["true_rel_residual"] - b["true_rel_residual"]) <= 0.05 * a["true_rel_residual"]
+    alone = sv.serve_alone(fams, reqs, rs, slots=8, restart=30)
+    mean = float(np.mean([r["service"] for r in alone]))
+    live = sv.serve_live(fams, reqs, rs, scale=0.3 * mean, slots=8, restart=30)
+    same = sv.same_as_alone(alone, live)
+    assert same["differing"] == 0, same
+    for r in live:
+        assert r["converged"] and r["true_rel_residual"] <= 1.1e-8
+    assert len({r["family"] for r in live}) == 3
+    overlap = sum(1 for a in live for b in live if a["id"] < b["id"] and a["admitted"] < b["finish"]
+                  and b["admitted"] < a["finish"])
+    print(f"live mix: {overlap} overlapping request pairs, {sv.live_summary(live)}", flush=True)
+    assert overlap > 0
+
+
+@pytest.mark.parametrize("rtol", [1e-1, 1e-2, 1e-4, 1e-8])
+def test_gmres_openfoam_matches_oracle(rtol):
+    """GMRES with the OpenFOAM stopping norm (gmres.cpp:106-122: the 1-norm of the GMRES residual
+    g_{j+1} V Q^T e_{j+1} against rtol sum|r0|): the oracle's iterations and solution, the
+    normalised residuals, and the true 1-norm within the threshold (10 % slack of gmres.cpp:149)."""
+    st = _st(*CASES[2])
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    xo, ro = orc.gmres(O.spmv, O.apply, b, restart=30, rtol=rtol, maxit=500, stop="openfoam")
+    x, r = M.ctx.gmres(b, None, 30, rtol, 500, stop="openfoam")
+    assert r.converged == ro.converged and r.iterations == ro.iterations
+    assert np.allclose(r.history, ro.history, rtol=1e-6, atol=1e-14)
+    assert abs(r.initial_residual_of - 1.0) < 1e-12
+    t1 = np.abs(b - st.dec.A @ x).sum() / np.abs(b).sum()
+    assert t1 <= 1.1 * rtol and abs(r.final_residual_of - t1) < 1e-9 * max(rtol, t1)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_gmres_openfoam_nonzero_guess_and_atol():
+    """Nonzero initial guess (normalisation with the mean of x0 and A 1), and an absolute tolerance
+    on the normalised residual (OpenFOAM `tolerance`): the oracle's iterations and F."""
+    st = _st(*CASES[2])
+    M = _ctx(st)
+    O = orc.OracleSchwarz(st)
+    b = st.dec.b
+    x0 = 0.7 + 0.3 * np.sin(0.37 * np.arange(st.n))
+    xo, ro = orc.gmres(O.spmv, O.apply, b, x0, restart=30, rtol=1e-3, maxit=500, stop="openfoam")
+    x, r = M.ctx.gmres(b, x0, 30, 1e-3, 500, stop="openfoam")
+    assert r.converged and r.iterations == ro.iterations
+    assert abs(r.initial_residual_of - ro.initial_residual_of) < 1e-12 * ro.initial_residual_of
+    assert abs(r.final_residual_of - ro.final_residual_of) < 1e-9 * ro.initial_residual_of
+    xa, ra = orc.gmres(O.spmv, O.apply, b, x0, restart=30, rtol=0.0, atol=1e-4 * ro.initial_residual_of, maxit=500,
+                       stop="openfoam")
+    x, r = M.ctx.gmres(b, x0, 30, 0.0, 500, stop="openfoam", atol=1e-4 * ro.initial_residual_of)
+    assert r.converged and r.iterations == ra.iterations and r.final_residual_of <= 1.1e-4 * ro.initial_residual_of
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_slot_solver_same_bits_alone_and_in_a_mix(precision):
+    """Continuous batching (s2l_solver): a right-hand side solved alone in an otherwise empty solver
+    and the same right-hand side admitted into a solver whose other slots are busy with solves that
+    started earlier or later give the same bits, iterations and history; in FP64 the oracle's
+    iterations and solution."""
+    from amp_amd import hostsetup as hs
+    st = _st(*CASES[1])
+    M = _ctx(st, precision)
+    B = [hs.random_rhs(st.n, 700 + k) for k in range(6)]
+    B[2] = B[2] * 1e-4
+    alone = []
+    sv = M.ctx.solver(4, restart=30, rtol=1e-8, maxit=500)
+    for bk in B:
+        sv.admit(0, bk)
+        while 0 not in sv.step():
+            pass
+        alone.append(sv.take(0))
+    # the mix: slots filled as they free up, new requests admitted between cycles
+    queue = list(range(len(B)))
+    owner = {}
+    mixed = {}
+    steps = 0
+    while queue or owner:
+        for slot in range(4):
+            if slot not in owner and queue and (steps % 2 == 0 or slot > 0):
+                k = queue.pop(0)
+                sv.admit(slot, B[k])
+                owner[slot] = k
+        for slot in sv.step():
+            mixed[owner.pop(int(slot))] = sv.take(int(slot))
+        steps += 1
+    O = orc.OracleSchwarz(st)
+    for k in range(len(B)):
+        (xa, ra), (xm, rm) = alone[k], mixed[k]
+        assert np.array_equal(xa, xm) and ra.iterations == rm.iterations and ra.history == rm.history, k
+        assert rm.converged and rm.true_rel_residual <= 1.1e-8
+        if precision == "double":
+            xo, ro = orc.gmres(O.spmv, O.apply, B[k], restart=30, rtol=1e-8, maxit=500)
+            assert rm.iterations == ro.iterations and np.linalg.norm(xm - xo) / np.linalg.norm(xo) < 1e-8
+    sv.close()

@@ -40,6 +40,19 @@ def test_cpp_backend_links_and_runs_without_a_gpu_call():
     assert p.returncode == 2 and "usage" in p.stderr
 
 
+def test_cpp_setup_checks_without_a_gpu(tmp_path):
+    """The pattern condition of a refactorization and the SolverConfig factory of the C++ backend."""
+    import shutil
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is missing")
+    exe = str(tmp_path / "test_setup_checks")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(REPO, "include"), "-o", exe,
+                           os.path.join(REPO, "tests", "cpp", "test_setup_checks.cpp"), "-L", PKG, "-ls2l_backend",
+                           "-ls2l_hip", f"-Wl,-rpath,{PKG}"])
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0 and "0 failure(s)" in p.stdout, p.stdout + p.stderr
+
+
 def test_setup_file_round_trip(tmp_path):
     """The setup-file writer produces the layout the C++ reader expects (header, records)."""
     import struct
@@ -77,7 +90,9 @@ def _gpu_present():
 def test_null_context_is_rejected():
     L = device.hip_lib()
     for rc in (L.s2l_apply(None, None, None, 1, None), L.s2l_spmv(None, None, None, 1, None),
-               L.s2l_finalize_local(None, 0), L.s2l_synchronize(None)):
+               L.s2l_finalize_local(None, 0), L.s2l_synchronize(None),
+               L.s2l_solver_create(None, None, 1, None), L.s2l_solver_step(None, None),
+               L.s2l_solver_admit(None, 0, None, None, 1)):
         assert rc < 0
 
 

@@ -50,33 +50,56 @@ def test_timeline_is_a_fifo_queue():
     assert light["max_ms"] == pytest.approx(3000.0)
 
 
-def test_batched_server_groups_waiting_requests_of_a_family():
-    """Requests waiting for the same family share one solve, oldest first, within the right-hand
-    side cap; every request gets its own results back; an idle server waits for the next arrival."""
-    from types import SimpleNamespace as NS
-    reqs = [sv.Request(0, 0, 2, 0.0, 1), sv.Request(1, 1, 1, 0.1, 2), sv.Request(2, 0, 3, 0.2, 3),
-            sv.Request(3, 0, 8, 0.3, 4), sv.Request(4, 1, 1, 50.0, 5)]
-    calls = []
+class _FakeSolver:
+    """Slot solver stand-in: a slot finishes two steps after its admission; x = b."""
 
-    class FakeM:
-        def __init__(self, k):
-            self.setup = NS(n=4)
-            self.k = k
+    def __init__(self, slots):
+        self.slots, self.busy, self.log = slots, {}, []
 
-    def solve(M, B):
-        calls.append((M.k, B.shape[0]))
-        return [NS(iterations=10 + M.k, converged=True, true_rel_residual=1e-9) for _ in range(B.shape[0])]
+    def admit(self, slot, b, x0=None):
+        assert 0 <= slot < self.slots and slot not in self.busy
+        self.busy[slot] = [2, np.array(b)]
 
-    rs = sv.ResidentSet(1 << 30, build=FakeM, footprint=lambda M: 1)
-    # scale 1, each solve costs almost nothing: request 0 is served alone at t=0, then at t~0.1+
-    # requests 1 (family 1) ... ; with a huge scale nothing waits
-    recs = sv.serve_batched([None, None], reqs, rs, scale=1.0, max_rhs=8, solve=solve)
-    assert [r["id"] for r in recs] == [0, 1, 2, 3, 4]
-    assert all(r["iterations"] == 10 + r["family"] for r in recs)
-    # make every request wait: arrivals compressed to 0 -> one batch per family (within the cap)
-    calls.clear()
-    recs = sv.serve_batched([None, None], reqs, rs, scale=0.0, max_rhs=8, solve=solve)
-    assert calls == [(0, 5), (1, 2), (0, 8)]
-    assert [r["batch"] for r in recs] == [5, 2, 5, 8, 2]
-    s = sv.batched_summary(recs)
-    assert s["p99_ms"] >= s["p50_ms"] >= 0
+    def step(self):
+        self.log.append(sorted(self.busy))
+        for v in self.busy.values():
+            v[0] -= 1
+        return np.array([s for s, v in self.busy.items() if v[0] <= 0], dtype=np.int64)
+
+    def take(self, slot):
+        from types import SimpleNamespace as NS
+        v = self.busy.pop(slot)
+        return v[1].copy(), NS(iterations=7, converged=True, true_rel_residual=1e-9)
+
+
+class _FakeM:
+    def __init__(self, k):
+        from types import SimpleNamespace as NS
+        self.k = k
+        self.setup = NS(n=4, dec=NS(new_to_old=np.arange(4)))
+        self.ctx = NS(solver=lambda slots, **kw: _FakeSolver(slots))
+
+
+def test_live_server_admits_between_cycles_and_matches_alone():
+    """The wall-clock server: a family's requests enter free slots between steps, first come first
+    served, a request that does not fit waits for slots; every request is completed with its own
+    results, and the digests and iterations equal those of the requests served alone."""
+    reqs = [sv.Request(0, 0, 2, 0.0, 1), sv.Request(1, 1, 1, 0.0, 2), sv.Request(2, 0, 2, 0.0, 3),
+            sv.Request(3, 0, 1, 0.0, 4), sv.Request(4, 1, 3, 0.0, 5)]
+    rs = sv.ResidentSet(1 << 30, build=_FakeM, footprint=lambda M: 1)
+    alone = sv.serve_alone([None, None], reqs, rs, slots=3)
+    rs.live[0].slot_solver.log.clear()
+    live = sv.serve_live([None, None], reqs, rs, scale=0.0, slots=3)
+    assert [r["id"] for r in live] == [0, 1, 2, 3, 4]
+    assert sv.same_as_alone(alone, live)["differing"] == 0
+    # family 0 with 3 slots: requests 0 (2 rhs) and 3 cannot both... request 0 takes slots 0, 1;
+    # request 2 (2 rhs) waits; request 3 (1 rhs) waits behind it (first come first served)
+    log0 = rs.live[0].slot_solver.log
+    assert log0[0] == [0, 1] and log0[2] == [0, 1, 2]
+    assert live[2]["admitted"] >= live[0]["finish"] - 1e-9 and live[3]["admitted"] >= live[2]["admitted"]
+    s = sv.live_summary(live)
+    assert s["p99_ms"] >= s["p50_ms"] >= 0 and s["solves_per_s"] > 0
+    # arrivals spread over the wall clock: a request is admitted no earlier than it arrives
+    late = [sv.Request(0, 0, 1, 0.0, 1), sv.Request(1, 0, 1, 2.0, 2)]
+    rec = sv.serve_live([None], late, sv.ResidentSet(1 << 30, build=_FakeM, footprint=lambda M: 1), scale=0.05, slots=2)
+    assert rec[1]["admitted"] >= 0.1 - 1e-3 and rec[1]["latency"] >= 0.0
