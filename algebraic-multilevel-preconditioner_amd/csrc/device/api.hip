@@ -278,6 +278,17 @@ struct s2l_ctx {
         graphs.clear();
     }
 
+    // per-phase timing with HIP events (s2l_phase_timing_enable): pairs recorded on the work's
+    // stream, summed when read
+    struct PhaseMark {
+        int id;
+        hipEvent_t e0, e1;
+    };
+    bool phase_on = false;
+    std::vector<PhaseMark> phase_pending;
+    double phase_ms[S2L_NPHASES] = {};
+    int64_t phase_n[S2L_NPHASES] = {};
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -523,6 +534,50 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
 }
 
 
+hipEvent_t takeEvent(s2l_ctx* c)
+{
+    if (c->ev_pool.empty()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+    }
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+}
+
+// Phase @p id starts on stream s (a token for phaseEnd; -1 when phase timing is off).
+int phaseBegin(s2l_ctx* c, int id, hipStream_t s)
+{
+    if (!c->phase_on) return -1;
+    s2l_ctx::PhaseMark m{id, takeEvent(c), takeEvent(c)};
+    HIPCHECK(hipEventRecord(m.e0, s));
+    c->phase_pending.push_back(m);
+    return static_cast<int>(c->phase_pending.size()) - 1;
+}
+
+void phaseEnd(s2l_ctx* c, int tok, hipStream_t s)
+{
+    if (tok < 0) return;
+    HIPCHECK(hipEventRecord(c->phase_pending[static_cast<size_t>(tok)].e1, s));
+}
+
+void collectPhases(s2l_ctx* c)
+{
+    for (auto& m : c->phase_pending) {
+        HIPCHECK(hipEventSynchronize(m.e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, m.e0, m.e1));
+        c->phase_ms[m.id] += ms;
+        c->phase_n[m.id] += 1;
+        c->ev_pool.push_back(m.e0);
+        c->ev_pool.push_back(m.e1);
+    }
+    c->phase_pending.clear();
+}
+
 void collectTiming(s2l_ctx* c)
 {
     for (auto& pr : c->ev_pending) {
@@ -553,30 +608,48 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
     if (!direct) launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, x0, z, nrhs, skip, s);
 }
 
+// The parts of one application are timed under the reference's timer names when phase timing is
+// on (two_level_schwarz.cpp:144-159, coarse_space.cpp:237-244).
 void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* skip, hipStream_t s)
 {
     require(c->system_ready && c->subs_ready && c->local_ready, S2L_ESTATE, "apply: upload the system, subdomains and factors first");
     require(c->coarse_ready, S2L_ESTATE, "apply: upload the coarse space first (n0 = 0 for one level)");
     c->ensureWork(nrhs);
     if (c->n0 == 0) {
+        const int tl = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
         oneLevel(c, r, nullptr, z, nrhs, skip, s);
+        phaseEnd(c, tl, s);
         return;
     }
     // x0 = R_0^T A_00^{-1} R_0 r
+    const int tc = phaseBegin(c, S2L_PHASE_PREC_COARSE, s);
+    int t = phaseBegin(c, S2L_PHASE_PREC_COARSE_RESTRICT, s);
     launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, c->cpart.p, nrhs, skip, s);
+    phaseEnd(c, t, s);
+    t = phaseBegin(c, S2L_PHASE_PREC_COARSE_SOLVE, s);
     launchTriGemv(c->n0, c->tri_ld, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
     launchTriGemv(c->n0, c->tri_ld, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
+    phaseEnd(c, t, s);
+    t = phaseBegin(c, S2L_PHASE_PREC_COARSE_PROLONG, s);
     if (c->boolean) {
         launchCoarseProlong(c->ca, c->yvec.p, c->x0.p, c->n, 1, nrhs, skip, s);
     } else {
         launchCoarseProlong(c->ca, c->yvec.p, c->loc.p, c->local_total, 0, nrhs, skip, s);
         launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, nullptr, c->x0.p, nrhs, skip, s);
     }
+    phaseEnd(c, t, s);
+    phaseEnd(c, tc, s);
     if (c->correction == 1) {
+        t = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
         oneLevel(c, r, c->x0.p, z, nrhs, skip, s); // z = x0 + M1^{-1} r
+        phaseEnd(c, t, s);
     } else {
+        t = phaseBegin(c, S2L_PHASE_PREC_SPMV, s);
         launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->x0.p, c->w.p, r, nrhs, skip, s); // w = r - A x0
+        phaseEnd(c, t, s);
+        t = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
         oneLevel(c, c->w.p, c->x0.p, z, nrhs, skip, s);                                       // z = x0 + M1^{-1} w
+        phaseEnd(c, t, s);
     }
     HIPCHECK(hipGetLastError());
 }
@@ -593,9 +666,13 @@ void spmvImpl(s2l_ctx* c, const double* x, double* y, const double* b, int nrhs,
 // (or while the local solve is timed with events) the kernels are launched one by one.
 void applySpmv(s2l_ctx* c, const double* in, double* z, double* w, int nrhs, const int32_t* skip, hipStream_t s)
 {
-    if (!c->use_graphs || c->timing) {
+    if (!c->use_graphs || c->timing || c->phase_on) {
+        int t = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);  // gmres.cpp:67
         applyImpl(c, in, z, nrhs, skip, s);
+        phaseEnd(c, t, s);
+        t = phaseBegin(c, S2L_PHASE_SPMV, s);                // gmres.cpp:71
         spmvImpl(c, z, w, nullptr, nrhs, skip, s);
+        phaseEnd(c, t, s);
         return;
     }
     c->ensureWork(nrhs);  // no allocation while capturing
@@ -881,11 +958,17 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
             const double* vj = V.p + static_cast<int64_t>(j) * N;
             double* zj = flex ? Z.p + static_cast<int64_t>(j) * N : z.p;
             applySpmv(c, vj, zj, w.p, cap, live, s);  // z = M^{-1} v_j (kept: flexible); w = A z
+            const int t = phaseBegin(c, S2L_PHASE_ORTHOGONALIZATION, s);  // gmres.cpp:77
             gmDots(a, j, s);                          // CGS2 pass 1
             gmUpdateDots(a, j, s);                    // w -= V h; pass 2
             gmUpdateNorm(a, j, s);                    // w -= V h2; partials of ||w||^2
             gmNormalize(a, j, s);                     // ||w||, Givens, v_{j+1} = w / ||w||
-            if (of) gmOpenFoamTest(a, j, s);          // sum|r_j| against the threshold
+            phaseEnd(c, t, s);
+            if (of) {
+                const int ts = phaseBegin(c, S2L_PHASE_STOPPING_TEST, s);  // gmres.cpp:108
+                gmOpenFoamTest(a, j, s);              // sum|r_j| against the threshold
+                phaseEnd(c, ts, s);
+            }
         }
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(pflags.p, flags.p, sizeof(int32_t) * cap, hipMemcpyDeviceToHost, s));
@@ -900,7 +983,9 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
         gmCombine(a, Z.p, z.p, s);      // x += Z y
     } else {
         gmCombine(a, V.p, u.p, s);      // x += M^{-1}(V y)
+        const int t = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);  // gmres.cpp:139
         applyImpl(c, u.p, z.p, cap, idl, s);
+        phaseEnd(c, t, s);
     }
     launchAxpby(n, 1.0, z.p, 1.0, d_x, cap, idl, s);
     spmvImpl(c, d_x, r.p, d_b, cap, idl, s);
@@ -1149,8 +1234,12 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                 return 0.0;
             })) {
             bicgP(n, r, p, v, c->bcoef.p, nrhs, skip, s);
+            int tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
             applyImpl(c, p, y, nrhs, skip, s);
+            phaseEnd(c, tp, s);
+            tp = phaseBegin(c, S2L_PHASE_SPMV, s);
             spmvImpl(c, y, v, nullptr, nrhs, skip, s);
+            phaseEnd(c, tp, s);
             launchReduce(3, n, v, rh, c->kpart.p, c->kred.p, nrhs, skip, s);
             readSums();
             std::fill(half.begin(), half.end(), 0);
@@ -1192,8 +1281,12 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
                     bicgHalf(n, d_x, y, sv, r, c->bcoef.p, nrhs, skip, s);
                 // second half: z = M s, t = A z, omega = (t, s) / (t, t), x and r
                 if (stage(full, zero)) {
+                    int tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
                     applyImpl(c, sv, z, nrhs, skip, s);
+                    phaseEnd(c, tp, s);
+                    tp = phaseBegin(c, S2L_PHASE_SPMV, s);
                     spmvImpl(c, z, t, nullptr, nrhs, skip, s);
+                    phaseEnd(c, tp, s);
                     launchReduce(4, n, t, sv, c->kpart.p, c->kred.p, nrhs, skip, s);
                     readSums();
                     for (int q = 0; q < nrhs; ++q) {
@@ -1332,6 +1425,10 @@ int s2l_ctx_destroy(s2l_ctx* ctx)
     for (auto& pr : ctx->ev_pending) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
+    }
+    for (auto& m : ctx->phase_pending) {
+        (void)hipEventDestroy(m.e0);
+        (void)hipEventDestroy(m.e1);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     ctx->invalidateGraphs();
@@ -2085,6 +2182,41 @@ int s2l_timing_read(s2l_ctx* c, int64_t* launches, double* ms_total, int reset)
         if (reset) {
             c->launches = 0;
             c->ms_total = 0.0;
+        }
+    });
+}
+
+const char* s2l_phase_name(int phase)
+{
+    static const char* names[S2L_NPHASES] = {
+        "solve.preconditioner",         "solve.preconditioner.coarse",         "solve.preconditioner.coarse.restrict",
+        "solve.preconditioner.coarse.solve", "solve.preconditioner.coarse.prolong", "solve.preconditioner.spmv",
+        "solve.preconditioner.local",   "solve.spmv",                          "solve.orthogonalization",
+        "solve.stopping_test"};
+    return phase >= 0 && phase < S2L_NPHASES ? names[phase] : nullptr;
+}
+
+int s2l_phase_timing_enable(s2l_ctx* c, int on)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "phase_timing_enable: null context");
+        c->invalidateGraphs();
+        c->phase_on = on != 0;
+    });
+}
+
+int s2l_phase_timing_read(s2l_ctx* c, double* ms, int64_t* counts, int reset)
+{
+    return guarded(c, [&] {
+        require(c != nullptr, S2L_EINVAL, "phase_timing_read: null context");
+        collectPhases(c);
+        for (int q = 0; q < S2L_NPHASES; ++q) {
+            if (ms) ms[q] = c->phase_ms[q];
+            if (counts) counts[q] = c->phase_n[q];
+            if (reset) {
+                c->phase_ms[q] = 0.0;
+                c->phase_n[q] = 0;
+            }
         }
     });
 }

@@ -59,7 +59,10 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_device_alloc", "s2l_device_free", "s2l_memcpy_h2d", "s2l_memcpy_d2h", "s2l_copy_d2d", "s2l_synchronize",
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
-           "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take"]
+           "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
+           "s2l_phase_timing_read"]
+#: S2L_NPHASES of include/s2l_hip.h
+NPHASES = 10
 
 
 def hip_lib():
@@ -103,12 +106,16 @@ def hip_lib():
         "s2l_solver_step": [vp, _i32p],
         "s2l_solver_take": [vp, ctypes.c_int32, P(KrylovResultC), _dp, vp, ctypes.c_int],
     }
+    sig["s2l_phase_timing_enable"] = [vp, ctypes.c_int]
+    sig["s2l_phase_timing_read"] = [vp, _dp, _i64p, ctypes.c_int]
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = ctypes.c_int
     L.s2l_last_error.argtypes = [vp]
     L.s2l_last_error.restype = ctypes.c_char_p
+    L.s2l_phase_name.argtypes = [ctypes.c_int]
+    L.s2l_phase_name.restype = ctypes.c_char_p
     _hip = L
     return L
 
@@ -377,6 +384,18 @@ class HipContext:
         ms = ctypes.c_double()
         self._check(self.L.s2l_timing_read(self.h, ctypes.byref(n), ctypes.byref(ms), 1 if reset else 0))
         return n.value, ms.value
+
+    def phase_timing(self, on: bool):
+        """Per-phase event timing under the reference's timer names (s2l_phase_timing_enable)."""
+        self._check(self.L.s2l_phase_timing_enable(self.h, 1 if on else 0))
+
+    def phase_timing_read(self, reset=True) -> dict:
+        """{timer name: (milliseconds summed, intervals)} (s2l_phase_timing_read)."""
+        ms = np.zeros(NPHASES)
+        cnt = np.zeros(NPHASES, np.int64)
+        self._check(self.L.s2l_phase_timing_read(self.h, _p(ms, ctypes.c_double), _p(cnt, ctypes.c_int64),
+                                                 1 if reset else 0))
+        return {self.L.s2l_phase_name(q).decode(): (float(ms[q]), int(cnt[q])) for q in range(NPHASES)}
 
     def set_graphs(self, on: bool):
         self._check(self.L.s2l_set_graphs(self.h, 1 if on else 0))

@@ -60,6 +60,7 @@ DEFAULT_NRHS = {"highcontrast3d-96": 16}
 COPY_COUNT = 1 << 25   # doubles of the calibration copy (256 MiB read + 256 MiB written)
 PMC_APPLIES = 3        # applies of a counter pass
 PROFILE_WARM = 2       # untimed applies before the timed ones in --profile-applies runs
+LATENCY_SOLVES = 100   # solves of the latency pass (p50 / p99)
 LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float>")
 
 
@@ -446,6 +447,25 @@ def main():
     tm = time.perf_counter() - tms
     xm = dX[args.warmup].download()[:n]
 
+    # ---- latency distribution: p50 / p99 over a separate pass of at least LATENCY_SOLVES solves
+    # (cycling over the resident right-hand sides)
+    nlat = max(LATENCY_SOLVES, args.steps)
+    lat_pass = {}
+    for name_, c_, flex_ in (("fp32", ctx, False), ("fp64", ctx64, False)):
+        lat = [run(c_, args.warmup + q % args.steps, flex_)[0] for q in range(nlat)]
+        lat_pass[name_] = (1e3 * float(np.percentile(lat, 50)), 1e3 * float(np.percentile(lat, 99)))
+
+    # ---- per-phase times of one FP32 solve under the reference's timer names (events; a separate
+    # pass: the events and the one-by-one launches are not in the timed region)
+    ctx.phase_timing(True)
+    ctx.phase_timing_read(reset=True)
+    nph = min(args.steps, 10)
+    for q in range(nph):
+        run(ctx, args.warmup + q)
+    ph = ctx.phase_timing_read(reset=True)
+    ctx.phase_timing(False)
+    phases = {k: round(v[0] / (nph * nrhs), 4) for k, v in ph.items() if v[1] > 0}
+
     # ---- solution error against a tight FP64 solve
     xr, rr = ctx64.gmres(B[args.warmup * nrhs], None, restart, 1e-12, 4000)
     err32 = float(np.linalg.norm(x32 - xr) / np.linalg.norm(xr))
@@ -485,6 +505,8 @@ def main():
         cpu = {"value": round(1.0 / tc, 4), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"1 GMRES({restart}) solve of the same workload with FP64 local factors, "
                          f"{N} subdomains on {threads} threads ({ro.iterations} its, {tc:.1f} s)",
+               "cores_rule": "min(subdomains, 16): one subdomain per thread as the reference's one rank per "
+                             "subdomain, capped at the 16 host cores one GPU is given",
                "iterations": ro.iterations,
                "x_rel_diff_vs_gpu_fp64": float(np.linalg.norm(xo - x64) / np.linalg.norm(x64))}
 
@@ -505,13 +527,15 @@ def main():
         "data": "synthetic (generated matrix, seeded uniform[-1,1] right-hand sides)",
         "config": {"workload": desc, "n": n, "subdomains": N, "overlap": overlap, "nev": nev, "n0": st.coarse.n0,
                    "restart": restart, "rtol": args.rtol, "parallelism": "single GPU, all subdomains resident"},
-        "p50_ms": round(1e3 * float(np.percentile(lat32, 50)), 3),
-        "p99_ms": round(1e3 * float(np.percentile(lat32, 99)), 3),
+        "p50_ms": round(lat_pass["fp32"][0], 3),
+        "p99_ms": round(lat_pass["fp32"][1], 3),
+        "latency_solves": nlat,
+        "phases_ms_per_solve": phases,
         "iterations_fp32": float(np.mean(its32)),
         "converged_fp32": all(conv32),
         "fp64_plain": {"value": round(args.steps * nrhs / t64, 3), "unit": "solves/s",
-                       "p50_ms": round(1e3 * float(np.percentile(lat64, 50)), 3),
-                       "p99_ms": round(1e3 * float(np.percentile(lat64, 99)), 3),
+                       "p50_ms": round(lat_pass["fp64"][0], 3),
+                       "p99_ms": round(lat_pass["fp64"][1], 3),
                        "iterations": float(np.mean(its64))},
         "speedup_fp32_vs_fp64": round(t64 / t_total, 3),
         "fgmres_fp32": {"value": round(args.steps * nrhs / tf, 3), "unit": "solves/s",

@@ -210,6 +210,32 @@ int s2l_device_footprint(s2l_ctx* ctx, int64_t* bytes);
 int s2l_timing_enable(s2l_ctx* ctx, int on);
 int s2l_timing_read(s2l_ctx* ctx, int64_t* solves, double* ms_total, int reset);
 
+/* Per-phase timing with HIP events on the work's stream, under the timer names of the reference
+ * (utils::ScopedTimer in src/solver/gmres.cpp:67,71,77,108,
+ * src/preconditioner/two_level_schwarz.cpp:144,149,155,159 and
+ * src/preconditioner/coarse_space.cpp:237,241,244; s2l_phase_name gives them). The parts nest as in
+ * the reference: solve.preconditioner contains the coarse, spmv and local parts of each apply.
+ * The halo exchange of the one-level part is fused into the local solve (its gather), so
+ * solve.preconditioner.local covers solve.preconditioner.local.{halo,lu}. Enabling it launches the
+ * kernels one by one (no HIP graphs); s2l_phase_timing_read synchronizes with the recorded events
+ * and returns the summed milliseconds and the number of timed intervals per phase. */
+enum {
+    S2L_PHASE_PRECONDITIONER = 0,
+    S2L_PHASE_PREC_COARSE = 1,
+    S2L_PHASE_PREC_COARSE_RESTRICT = 2,
+    S2L_PHASE_PREC_COARSE_SOLVE = 3,
+    S2L_PHASE_PREC_COARSE_PROLONG = 4,
+    S2L_PHASE_PREC_SPMV = 5,
+    S2L_PHASE_PREC_LOCAL = 6,
+    S2L_PHASE_SPMV = 7,
+    S2L_PHASE_ORTHOGONALIZATION = 8,
+    S2L_PHASE_STOPPING_TEST = 9,
+    S2L_NPHASES = 10
+};
+const char* s2l_phase_name(int phase);
+int s2l_phase_timing_enable(s2l_ctx* ctx, int on);
+int s2l_phase_timing_read(s2l_ctx* ctx, double* ms, int64_t* counts, int reset);
+
 /* Schedule of the batched local solve: 0 = task queue, one launch per depth group of all
  * subdomains' task trees, a subtree per workgroup with its vector in LDS (default); 1 = the same
  * tasks with one launch per subdomain (the comparison point); 2 = level-synchronous, one launch per

@@ -303,7 +303,8 @@ def test_gmres_openfoam_matches_oracle(rtol):
     assert np.allclose(r.history, ro.history, rtol=1e-6, atol=1e-14)
     assert abs(r.initial_residual_of - 1.0) < 1e-12
     t1 = np.abs(b - st.dec.A @ x).sum() / np.abs(b).sum()
-    assert t1 <= 1.1 * rtol and abs(r.final_residual_of - t1) < 1e-9 * max(rtol, t1)
+    # sum|b - A x| at a residual of 1e-8 carries the cancellation of b - A x: 1e-6 relative
+    assert t1 <= 1.1 * rtol and abs(r.final_residual_of - t1) < 1e-6 * t1
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
 
 
@@ -367,3 +368,31 @@ def test_slot_solver_same_bits_alone_and_in_a_mix(precision):
             xo, ro = orc.gmres(O.spmv, O.apply, B[k], restart=30, rtol=1e-8, maxit=500)
             assert rm.iterations == ro.iterations and np.linalg.norm(xm - xo) / np.linalg.norm(xo) < 1e-8
     sv.close()
+
+
+def test_phase_timers_count_the_reference_timers():
+    """Per-phase event timing under the reference's timer names: one solve.preconditioner interval
+    per application (iterations + one per cycle), one solve.spmv and one solve.orthogonalization per
+    iteration, one solve.stopping_test per iteration with the OpenFOAM norm; the coarse parts nest
+    inside the preconditioner; the results are the same bits with the timers on."""
+    st = _st(*CASES[1])
+    M = _ctx(st)
+    b = st.dec.b
+    x0, r0 = M.ctx.gmres(b, None, 30, 1e-8, 500, stop="openfoam")
+    M.ctx.phase_timing(True)
+    x1, r1 = M.ctx.gmres(b, None, 30, 1e-8, 500, stop="openfoam")
+    ph = M.ctx.phase_timing_read()
+    M.ctx.phase_timing(False)
+    assert np.array_equal(x0, x1) and r0.iterations == r1.iterations
+    its = r1.iterations
+    cycles = (its + 29) // 30
+    assert ph["solve.preconditioner"][1] == its + cycles
+    assert ph["solve.spmv"][1] == its and ph["solve.orthogonalization"][1] == its
+    assert ph["solve.stopping_test"][1] == its
+    assert ph["solve.preconditioner.coarse"][1] == ph["solve.preconditioner.local"][1] == its + cycles
+    inner = sum(ph[k][0] for k in ("solve.preconditioner.coarse", "solve.preconditioner.spmv",
+                                   "solve.preconditioner.local"))
+    assert 0 < inner <= ph["solve.preconditioner"][0] * 1.05
+    parts = sum(ph[k][0] for k in ("solve.preconditioner.coarse.restrict", "solve.preconditioner.coarse.solve",
+                                   "solve.preconditioner.coarse.prolong"))
+    assert 0 < parts <= ph["solve.preconditioner.coarse"][0] * 1.05
