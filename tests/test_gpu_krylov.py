@@ -386,10 +386,13 @@ def test_phase_timers_count_the_reference_timers():
     assert np.array_equal(x0, x1) and r0.iterations == r1.iterations
     its = r1.iterations
     cycles = (its + 29) // 30
-    assert ph["solve.preconditioner"][1] == its + cycles
-    assert ph["solve.spmv"][1] == its and ph["solve.orthogonalization"][1] == its
-    assert ph["solve.stopping_test"][1] == its
-    assert ph["solve.preconditioner.coarse"][1] == ph["solve.preconditioner.local"][1] == its + cycles
+    # the device loop polls the convergence flags every 4 iterations: up to 3 launches of a cycle
+    # after convergence run masked (they return at once) and are timed as well
+    loop = ph["solve.spmv"][1]
+    assert its <= loop <= its + 3 * cycles
+    assert ph["solve.orthogonalization"][1] == ph["solve.stopping_test"][1] == loop
+    assert ph["solve.preconditioner"][1] == loop + cycles
+    assert ph["solve.preconditioner.coarse"][1] == ph["solve.preconditioner.local"][1] == loop + cycles
     inner = sum(ph[k][0] for k in ("solve.preconditioner.coarse", "solve.preconditioner.spmv",
                                    "solve.preconditioner.local"))
     assert 0 < inner <= ph["solve.preconditioner"][0] * 1.05
