@@ -10,6 +10,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -174,6 +175,12 @@ namespace {
 struct GmresRun;
 }
 
+// Scratch of s2l_apply_noalloc: one array per GPU, held until the process ends. Host code uses the
+// device address that hipGetSymbolAddress returns for the current device (s2l_ctx::scratch),
+// never the array's name.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 constexpr size_t kMaxGraphs = 160;  // cached apply graphs per context (GMRES(m): m inputs, m outputs if flexible)
 
 struct s2l_ctx {
@@ -251,9 +258,16 @@ struct s2l_ctx {
     int32_t tri_ld = 0;
     bool coarse_perm = false;
 
-    // work
+    // work: the apply's scratch buffers (grown by ensureWork) and the pointers the kernels use,
+    // which s2l_apply_noalloc points at the static scratch array for one call
     DevBuf<double> x0, w, loc, cvec, tvec, yvec, cpart;
     int work_nrhs = 0;
+    struct Work {
+        double *x0, *w, *loc, *cvec, *tvec, *yvec, *cpart;
+        unsigned char *tbuf, *xgbuf, *ls_slots, *ls_part, *ls_b, *ls_yf, *ls_xb;
+    };
+    Work work{};
+    void* scratch = nullptr;  ///< this device's copy of the static scratch array (s2l_apply_noalloc)
 
     // HIP graphs of one apply + SpMV in the Krylov loops, one per (input, output) pair, captured on
     // first use; every upload, schedule change or work-buffer growth invalidates them (gen)
@@ -328,6 +342,7 @@ struct s2l_ctx {
         ls_yf.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         ls_xb.reserve(static_cast<size_t>(std::max<int64_t>(local_total, 1)) * nrhs * esz);
         work_nrhs = nrhs;
+        work = Work{x0.p, w.p, loc.p, cvec.p, tvec.p, yvec.p, cpart.p, tbuf.p, xgbuf.p, ls_slots.p, ls_part.p, ls_b.p, ls_yf.p, ls_xb.p};
     }
 };
 
@@ -390,7 +405,7 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     a.ridx = c->ridx_d.p;
     a.cptr = c->cptr_d.p;
     a.cidx = c->cidx_d.p;
-    a.tbuf = reinterpret_cast<T*>(c->tbuf.p);
+    a.tbuf = reinterpret_cast<T*>(c->work.tbuf);
     a.gmap = c->gmap_d.p;
     a.lperm = c->lperm_d.p;
     a.rhs = rhs;
@@ -407,7 +422,7 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     a.task_rows = c->trows_d.p;
     a.ext_rows = c->ext_d.p;
     a.nrhs = nrhs;
-    a.xg = reinterpret_cast<T*>(c->xgbuf.p);
+    a.xg = reinterpret_cast<T*>(c->work.xgbuf);
     return a;
 }
 
@@ -448,11 +463,11 @@ template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const doub
     a.out = out;
     a.pou = (mode == 1 && c->one_level == 0 && !c->boolean) ? c->pou_d.p : nullptr;
     a.skip = skip;
-    a.slots = reinterpret_cast<T*>(c->ls_slots.p);
-    a.part = reinterpret_cast<T*>(c->ls_part.p);
-    a.bbuf = reinterpret_cast<T*>(c->ls_b.p);
-    a.yf = reinterpret_cast<T*>(c->ls_yf.p);
-    a.xb = reinterpret_cast<T*>(c->ls_xb.p);
+    a.slots = reinterpret_cast<T*>(c->work.ls_slots);
+    a.part = reinterpret_cast<T*>(c->work.ls_part);
+    a.bbuf = reinterpret_cast<T*>(c->work.ls_b);
+    a.yf = reinterpret_cast<T*>(c->work.ls_yf);
+    a.xb = reinterpret_cast<T*>(c->work.ls_xb);
     a.n_global = c->n;
     a.n_local_total = c->local_total;
     a.nslots = c->ls.slots;
@@ -596,7 +611,7 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
 {
     const bool direct = (c->one_level == 0 && c->boolean);
     const int mode = direct ? 0 : 1;
-    double* out = direct ? z : c->loc.p;
+    double* out = direct ? z : c->work.loc;
     if (c->precision == 1 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
         localSolve<float2, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else if (c->precision == 1)
@@ -605,16 +620,16 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
         localSolve<double, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else
         localSolve<double, double>(c, rhs, x0, out, mode, nrhs, skip, s);
-    if (!direct) launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, x0, z, nrhs, skip, s);
+    if (!direct) launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->work.loc, c->local_total, x0, z, nrhs, skip, s);
 }
 
 // The parts of one application are timed under the reference's timer names when phase timing is
 // on (two_level_schwarz.cpp:144-159, coarse_space.cpp:237-244).
-void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* skip, hipStream_t s)
+void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* skip, hipStream_t s, bool grow = true)
 {
     require(c->system_ready && c->subs_ready && c->local_ready, S2L_ESTATE, "apply: upload the system, subdomains and factors first");
     require(c->coarse_ready, S2L_ESTATE, "apply: upload the coarse space first (n0 = 0 for one level)");
-    c->ensureWork(nrhs);
+    if (grow) c->ensureWork(nrhs);
     if (c->n0 == 0) {
         const int tl = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
         oneLevel(c, r, nullptr, z, nrhs, skip, s);
@@ -624,31 +639,32 @@ void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* 
     // x0 = R_0^T A_00^{-1} R_0 r
     const int tc = phaseBegin(c, S2L_PHASE_PREC_COARSE, s);
     int t = phaseBegin(c, S2L_PHASE_PREC_COARSE_RESTRICT, s);
-    launchCoarseRestrict(c->ca, r, c->n, c->cvec.p, c->cpart.p, nrhs, skip, s);
+    const s2l_ctx::Work& W = c->work;
+    launchCoarseRestrict(c->ca, r, c->n, W.cvec, W.cpart, nrhs, skip, s);
     phaseEnd(c, t, s);
     t = phaseBegin(c, S2L_PHASE_PREC_COARSE_SOLVE, s);
-    launchTriGemv(c->n0, c->tri_ld, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, c->cvec.p, c->tvec.p, nrhs, skip, s);
-    launchTriGemv(c->n0, c->tri_ld, c->T2_d.p, 0, nullptr, c->tvec.p, c->yvec.p, nrhs, skip, s);
+    launchTriGemv(c->n0, c->tri_ld, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, W.cvec, W.tvec, nrhs, skip, s);
+    launchTriGemv(c->n0, c->tri_ld, c->T2_d.p, 0, nullptr, W.tvec, W.yvec, nrhs, skip, s);
     phaseEnd(c, t, s);
     t = phaseBegin(c, S2L_PHASE_PREC_COARSE_PROLONG, s);
     if (c->boolean) {
-        launchCoarseProlong(c->ca, c->yvec.p, c->x0.p, c->n, 1, nrhs, skip, s);
+        launchCoarseProlong(c->ca, W.yvec, W.x0, c->n, 1, nrhs, skip, s);
     } else {
-        launchCoarseProlong(c->ca, c->yvec.p, c->loc.p, c->local_total, 0, nrhs, skip, s);
-        launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->loc.p, c->local_total, nullptr, c->x0.p, nrhs, skip, s);
+        launchCoarseProlong(c->ca, W.yvec, W.loc, c->local_total, 0, nrhs, skip, s);
+        launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, W.loc, c->local_total, nullptr, W.x0, nrhs, skip, s);
     }
     phaseEnd(c, t, s);
     phaseEnd(c, tc, s);
     if (c->correction == 1) {
         t = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
-        oneLevel(c, r, c->x0.p, z, nrhs, skip, s); // z = x0 + M1^{-1} r
+        oneLevel(c, r, W.x0, z, nrhs, skip, s); // z = x0 + M1^{-1} r
         phaseEnd(c, t, s);
     } else {
         t = phaseBegin(c, S2L_PHASE_PREC_SPMV, s);
-        launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->x0.p, c->w.p, r, nrhs, skip, s); // w = r - A x0
+        launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, W.x0, W.w, r, nrhs, skip, s); // w = r - A x0
         phaseEnd(c, t, s);
         t = phaseBegin(c, S2L_PHASE_PREC_LOCAL, s);
-        oneLevel(c, c->w.p, c->x0.p, z, nrhs, skip, s);                                       // z = x0 + M1^{-1} w
+        oneLevel(c, W.w, W.x0, z, nrhs, skip, s);                                       // z = x0 + M1^{-1} w
         phaseEnd(c, t, s);
     }
     HIPCHECK(hipGetLastError());
@@ -1400,6 +1416,19 @@ int s2l_ctx_create(int device, s2l_ctx** out)
         auto* c = new s2l_ctx;
         c->device = device;
         try {
+            {
+                // the device address of this GPU's scratch array, looked up once per device
+                static std::mutex mu;
+                static std::vector<void*> per_device;
+                std::lock_guard<std::mutex> lock(mu);
+                if (per_device.size() <= static_cast<size_t>(device)) per_device.resize(static_cast<size_t>(device) + 1, nullptr);
+                if (!per_device[static_cast<size_t>(device)]) {
+                    void* ptr = nullptr;
+                    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+                    per_device[static_cast<size_t>(device)] = ptr;
+                }
+                c->scratch = per_device[static_cast<size_t>(device)];
+            }
             HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             HIPCHECK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
             HIPCHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -1838,6 +1867,76 @@ int s2l_apply(s2l_ctx* c, const double* d_r, double* d_z, int32_t nrhs, void* st
     });
 }
 
+// The apply's scratch for nrhs right-hand sides carved from @p base (256-byte aligned pieces, only
+// those the context's schedule and coarse space use; the others null); returns the bytes needed.
+size_t scratchLayout(const s2l_ctx* c, int nrhs, unsigned char* base, s2l_ctx::Work* w)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes, bool used) -> unsigned char* {
+        if (!used) return nullptr;
+        unsigned char* p = base + off;
+        off += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return p;
+    };
+    const size_t R = static_cast<size_t>(nrhs);
+    const size_t n = static_cast<size_t>(c->n), lt = static_cast<size_t>(std::max<int64_t>(c->local_total, 1));
+    const size_t n0 = static_cast<size_t>(std::max(c->n0, 1));
+    const bool coarse = c->n0 > 0;
+    const bool direct = c->one_level == 0 && c->boolean;
+    const bool lsm = c->launch_mode == 2;
+    const size_t esz = c->precision == 1 ? sizeof(float) : sizeof(double);
+    const size_t nr = (R + 1) / 2 * 2;
+    *w = s2l_ctx::Work{};
+    w->x0 = reinterpret_cast<double*>(take(n * R * 8, coarse));
+    w->w = reinterpret_cast<double*>(take(n * R * 8, coarse && c->correction == 0));
+    w->loc = reinterpret_cast<double*>(take(lt * R * 8, !direct || (coarse && !c->boolean)));
+    w->cvec = reinterpret_cast<double*>(take(n0 * R * 8, coarse));
+    w->tvec = reinterpret_cast<double*>(take(n0 * R * 8, coarse));
+    w->yvec = reinterpret_cast<double*>(take(n0 * R * 8, coarse));
+    w->cpart = reinterpret_cast<double*>(take(static_cast<size_t>(std::max(c->nsub, 1)) * std::max(c->ca.rchunks, 1) *
+                                                  std::max(c->ca.mmax, 1) * R * 8, coarse));
+    w->tbuf = take(static_cast<size_t>(std::max<int64_t>(c->pk.t_total, 1)) * nr * esz, !lsm);
+    w->xgbuf = take(lt * nr * esz, !lsm);
+    w->ls_slots = take(static_cast<size_t>(std::max<int64_t>(c->ls.slots, 1)) * R * esz, lsm);
+    w->ls_part = take(static_cast<size_t>(std::max<int64_t>(c->ls.partials, 1)) * R * esz, lsm);
+    w->ls_b = take(lt * R * esz, lsm);
+    w->ls_yf = take(lt * R * esz, lsm);
+    w->ls_xb = take(lt * R * esz, lsm);
+    return off;
+}
+
+int s2l_apply_noalloc(s2l_ctx* c, const double* d_r, double* d_z, int32_t nrhs, void* stream)
+{
+    return guarded(c, [&] {
+        require(c && d_r && d_z && nrhs >= 1, S2L_EINVAL, "apply_noalloc: bad arguments");
+        require(c->system_ready && c->subs_ready && c->local_ready && c->coarse_ready, S2L_ESTATE,
+                "apply_noalloc: upload the system, subdomains, factors and coarse space first");
+        require(c->scratch != nullptr, S2L_ESTATE, "apply_noalloc: no scratch address for this device");
+        HIPCHECK(hipSetDevice(c->device));
+        s2l_ctx::Work w{};
+        const size_t need = scratchLayout(c, nrhs, static_cast<unsigned char*>(c->scratch), &w);
+        require(need <= kScratchBytes, S2L_ELIMIT,
+                "apply_noalloc: " + std::to_string(nrhs) + " right-hand sides need " + std::to_string(need) +
+                    " bytes of scratch; the maximum is " + std::to_string(kScratchBytes));
+        // one call runs with the scratch pointers, without event timing (no event creation)
+        const s2l_ctx::Work saved = c->work;
+        const bool timing = c->timing, phases = c->phase_on;
+        c->work = w;
+        c->timing = c->phase_on = false;
+        try {
+            applyImpl(c, d_r, d_z, nrhs, nullptr, pick(c, stream), false);
+        } catch (...) {
+            c->work = saved;
+            c->timing = timing;
+            c->phase_on = phases;
+            throw;
+        }
+        c->work = saved;
+        c->timing = timing;
+        c->phase_on = phases;
+    });
+}
+
 int s2l_spmv(s2l_ctx* c, const double* d_x, double* d_y, int32_t nrhs, void* stream)
 {
     return guarded(c, [&] {
@@ -2016,14 +2115,14 @@ int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
             using T = decltype(tag);
             if (c->launch_mode == 2) {
                 // every subdomain is solved; the others see their share of the scattered b
-                LsArgs<T> a = lsArgs<T>(c, dg.p, nullptr, c->loc.p, 1, nullptr);
+                LsArgs<T> a = lsArgs<T>(c, dg.p, nullptr, c->work.loc, 1, nullptr);
                 a.pou = nullptr;
                 lsLaunches<T>(c, a, 1, c->stream);
                 return;
             }
             auto queue = [&](auto ftag) {
                 using F = decltype(ftag);
-                LocalArgs<T, F> a = localArgs<T, F>(c, dg.p, nullptr, c->loc.p, 1, 1, nullptr);
+                LocalArgs<T, F> a = localArgs<T, F>(c, dg.p, nullptr, c->work.loc, 1, 1, nullptr);
                 a.pou = nullptr; // y by local index, no partition of unity
                 launchQueue<T, F>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
                                   c->qsub_h[static_cast<size_t>(i) + 1] - c->qsub_h[static_cast<size_t>(i)], true, c->stream);
@@ -2039,7 +2138,7 @@ int s2l_local_solve_host(s2l_ctx* c, int32_t i, const double* b, double* y)
             run(double{});
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipStreamSynchronize(c->stream));
-        HIPCHECK(hipMemcpyAsync(y, c->loc.p + l0, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(y, c->work.loc + l0, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
     });
 }

@@ -60,7 +60,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
-           "s2l_phase_timing_read"]
+           "s2l_phase_timing_read", "s2l_apply_noalloc"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -82,6 +82,7 @@ def hip_lib():
         "s2l_upload_coarse": [vp, ctypes.c_int32, _i64p, _dp, _dp, ctypes.c_int, ctypes.c_int],
         "s2l_apply": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_spmv": [vp, vp, vp, ctypes.c_int32, vp],
+        "s2l_apply_noalloc": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_krylov_solve": [vp, P(KrylovOpts), vp, vp, ctypes.c_int32, P(KrylovResultC), _dp, vp],
         "s2l_apply_host": [vp, _dp, _dp, ctypes.c_int32],
         "s2l_spmv_host": [vp, _dp, _dp, ctypes.c_int32],

@@ -101,6 +101,16 @@ int s2l_upload_coarse(s2l_ctx* ctx, int32_t n0, const int64_t* offsets, const do
  * hipStream_t or NULL. */
 int s2l_apply(s2l_ctx* ctx, const double* d_r, double* d_z, int32_t nrhs, void* stream);
 
+/* z = M^{-1} r like s2l_apply, allocating nothing: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The apply's scratch (the coarse vectors, x0 and w, the local buffers of the
+ * local solves) lives in a static device array of 128 MiB per GPU, held until the process ends;
+ * S2L_ELIMIT when @p nrhs right-hand sides need more (about (2 n + 2 sum n_i) x 8 bytes per
+ * right-hand side). Every call on one GPU uses that GPU's array: calls that overlap on one GPU
+ * (from any contexts or streams) race. Nothing in it needs to start at zero. Event timing is not
+ * recorded for these calls. Same kernels and results as s2l_apply. */
+int s2l_apply_noalloc(s2l_ctx* ctx, const double* d_r, double* d_z, int32_t nrhs, void* stream);
+
 /* y = A x (device pointers). Replaces DistributedMatrix::apply
  * (include/linalg/distributed_matrix.hpp:33-38). */
 int s2l_spmv(s2l_ctx* ctx, const double* d_x, double* d_y, int32_t nrhs, void* stream);
