@@ -37,13 +37,14 @@ def build_device():
     objdir = os.path.join(HERE, "csrc", "device", "_obj")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for f in ("kernels.hip", "lsolve.hip", "api.hip", "pack.cpp", "lsched.cpp"):
+    for f in ("kernels.hip", "lsolve.hip", "refactor.hip", "api.hip", "pack.cpp", "lsched.cpp", "refactor.cpp"):
         o = os.path.join(objdir, f + ".o")
         lang = ["-x", "hip"] if f.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", src] + lang + ["-c", os.path.join(src, f), "-o", o])
         objs.append(o)
     out = os.path.join(HERE, "libs2l_hip.so")
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out] + objs +
+         ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 
 

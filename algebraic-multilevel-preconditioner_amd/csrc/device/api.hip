@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -16,6 +17,9 @@
 #include <vector>
 
 #include "device.hpp"
+#include "refactor.hpp"
+
+#include <rocsolver/rocsolver.h>
 
 namespace s2l {
 template <typename T, typename TF> const void* localSolveKernel();
@@ -200,6 +204,7 @@ struct s2l_ctx {
     int64_t nnz = 0;
     std::vector<int32_t> ownership;
     std::vector<int32_t> rp_h, col_h;  ///< uploaded pattern (columns in the owners' local order)
+    std::vector<int32_t> inpos_h;      ///< stored position -> position in the uploaded arrays
     DevBuf<int32_t> rp, col;
     DevBuf<double> val;
     bool system_ready = false;
@@ -229,11 +234,13 @@ struct s2l_ctx {
     bool queue_ok = false;
     Packed pk;
     DevBuf<SubDesc> sub_d;
-    DevBuf<SnDesc> sn_d;
-    DevBuf<int32_t> lev_d, levsn_d, levw_d, levh_d, levc_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
-    size_t y_off = 0, st_off = 0;
+    DevBuf<int32_t> levw_d, levh_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
+    DevBuf<SnDesc> levdesc_d;
+    DevBuf<int32_t> levrow_d, levcb_d, levce_d;
+    DevBuf<StepDesc> steps_d;
+    size_t y_off = 0, st_off = 0, step_off = 0;
     // FP32 factors, two right-hand sides per workgroup (a float2 vector in LDS): its LDS layout
-    size_t y_off2 = 0, st_off2 = 0, lds_bytes2 = 0;
+    size_t y_off2 = 0, st_off2 = 0, step_off2 = 0, lds_bytes2 = 0;
     bool pair_ok = false;
     DevBuf<float> diag_f, below_f;
     DevBuf<double> diag_dd, below_dd;
@@ -245,6 +252,18 @@ struct s2l_ctx {
     int cus = 0;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
+
+    // device refactorization (s2l_refactorize): its plan, device tables and work arrays
+    bool rf_ready = false;
+    bool ls_stale = false;  ///< the level-synchronous layout holds older factor values
+    RfPlan rf;
+    DevBuf<RfSn> rf_sn;
+    DevBuf<RfUpd> rf_upd;
+    DevBuf<int32_t> rf_rel, rf_src, rf_order, rf_status, rf_inpos;
+    DevBuf<int64_t> rf_pos, rf_invoff;
+    DevBuf<double> rf_work, rf_inv, rf_raw, rf_E, rf_Y, rf_X, rf_AX, rf_C, rf_part, rf_loc;
+    rocblas_handle rf_blas = nullptr;
+    double rf_ms[4] = {};   ///< last refactorization: values, local factors, A_00, total [ms]
 
     // coarse
     int32_t n0 = 0;
@@ -379,22 +398,25 @@ void require(bool c, int code, const std::string& m)
 
 hipStream_t pick(s2l_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
 
-// ---- local solve launch: the persistent work-queue kernel over every subdomain (mode 0) or one
-//      launch per subdomain over its slice of the queue (mode 1)
+// ---- local solve launch: the task-queue kernel, one launch per depth group over every
+//      subdomain's tasks (mode 0) or one single-workgroup launch per subdomain over its slice of
+//      the queue (mode 1)
 template <typename T, typename TF>
 LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode, int nrhs, const int32_t* skip)
 {
     LocalArgs<T, TF> a{};
     a.sub = c->sub_d.p;
-    a.sn = c->sn_d.p;
-    a.lev = c->lev_d.p;
-    a.lev_sn = c->levsn_d.p;
     a.lev_wpre = c->levw_d.p;
     a.lev_hpre = c->levh_d.p;
-    a.lev_chunk = c->levc_d.p;
     const bool pair = std::is_same_v<T, float2>;
     a.y_off = static_cast<int32_t>(pair ? c->y_off2 : c->y_off);
     a.st_off = static_cast<int32_t>(pair ? c->st_off2 : c->st_off);
+    a.step_off = static_cast<int32_t>(pair ? c->step_off2 : c->step_off);
+    a.lev_desc = c->levdesc_d.p;
+    a.lev_row = c->levrow_d.p;
+    a.lev_cb = c->levcb_d.p;
+    a.lev_ce = c->levce_d.p;
+    a.steps = c->steps_d.p;
     if constexpr (std::is_same_v<TF, float>) {
         a.diag = c->diag_f.p;
         a.below = c->below_f.p;
@@ -1465,6 +1487,7 @@ int s2l_ctx_destroy(s2l_ctx* ctx)
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    if (ctx->rf_blas) (void)rocblas_destroy_handle(ctx->rf_blas);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return S2L_OK;
@@ -1480,7 +1503,7 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
         for (int p = 0; p < nsub; ++p) require(ownership[p + 1] > ownership[p], S2L_EINVAL, "empty subdomain in ownership");
         HIPCHECK(hipSetDevice(c->device));
         const int64_t nnz = rowptr[n];
-        std::vector<int32_t> cl(static_cast<size_t>(nnz));
+        std::vector<int32_t> cl(static_cast<size_t>(nnz)), inpos(static_cast<size_t>(nnz));
         std::vector<double> vl(static_cast<size_t>(nnz));
         std::vector<std::pair<int64_t, int64_t>> key;
         int p = 0;
@@ -1499,6 +1522,7 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
             for (const auto& kv : key) {
                 cl[static_cast<size_t>(q)] = col[kv.second];
                 vl[static_cast<size_t>(q)] = val[kv.second];
+                inpos[static_cast<size_t>(q)] = static_cast<int32_t>(kv.second);
                 ++q;
             }
         }
@@ -1515,6 +1539,8 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
             c->ownership.assign(ownership, ownership + nsub + 1);
             c->rp_h.assign(rowptr, rowptr + n + 1);
             c->col_h = cl;
+            c->inpos_h = std::move(inpos);
+            c->rf_ready = false;
             c->rp.upload(rowptr, static_cast<size_t>(n) + 1, c->stream);
             c->col.upload(cl, c->stream);
             c->val.upload(vl, c->stream);
@@ -1677,11 +1703,13 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
         c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
         c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
-        c->lds_bytes = c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
+        c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+        c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
         c->queue_ok = c->lds_bytes <= c->max_lds;
         c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
         c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
-        c->lds_bytes2 = c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
+        c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+        c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
         c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
         if (c->pair_ok)
             HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1707,12 +1735,13 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
             }
         }
         c->sub_d.upload(P.sub, c->stream);
-        c->sn_d.upload(P.sn, c->stream);
-        c->lev_d.upload(P.lev, c->stream);
-        c->levsn_d.upload(P.lev_sn, c->stream);
         c->levw_d.upload(P.lev_wpre, c->stream);
         c->levh_d.upload(P.lev_hpre, c->stream);
-        c->levc_d.upload(P.lev_chunk, c->stream);
+        c->levdesc_d.upload(P.lev_desc, c->stream);
+        c->levrow_d.upload(P.lev_row, c->stream);
+        c->levcb_d.upload(P.lev_cb, c->stream);
+        c->levce_d.upload(P.lev_ce, c->stream);
+        c->steps_d.upload(P.steps, c->stream);
         c->ridx_d.upload(P.ridx, c->stream);
         c->cptr_d.upload(P.cptr, c->stream);
         c->cidx_d.upload(P.cidx, c->stream);
@@ -1750,6 +1779,8 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->precision = precision;
         c->work_nrhs = 0;
         c->local_ready = true;
+        c->rf_ready = false;
+        c->ls_stale = false;
     });
 }
 
@@ -1855,6 +1886,125 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
         }
         c->work_nrhs = 0;
         c->coarse_ready = true;
+    });
+}
+
+int s2l_refactorize(s2l_ctx* c, const double* val, int host)
+{
+    return guarded(c, [&] {
+        require(c && val, S2L_EINVAL, "refactorize: bad arguments");
+        require(c->system_ready && c->subs_ready && c->local_ready && c->coarse_ready, S2L_ESTATE,
+                "refactorize: upload the system, subdomains, factors and coarse space first");
+        for (const HostFactor& F : c->hf) require(F.cholesky, S2L_ESTATE, "refactorize: the device refactorization needs Cholesky factors");
+        require(c->queue_ok && c->launch_mode != 2, S2L_ESTATE, "refactorize: needs the task-queue schedule (launch modes 0 and 1)");
+        require(c->n0 == 0 || !c->coarse_perm, S2L_ESTATE, "refactorize: needs the Cholesky factor of A_00 (a symmetric coarse matrix)");
+        HIPCHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        c->invalidateGraphs();
+        using clk = std::chrono::steady_clock;
+        const auto t_start = clk::now();
+        auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
+        if (!c->rf_ready) {
+            c->rf = buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+            RfPlan& R = c->rf;
+            c->rf_sn.upload(R.sn, s);
+            c->rf_upd.upload(R.upd, s);
+            c->rf_rel.upload(R.rel, s);
+            c->rf_pos.upload(R.amap_pos, s);
+            c->rf_src.upload(R.amap_src, s);
+            c->rf_order.upload(R.order, s);
+            c->rf_invoff.upload(R.inv_off, s);
+            c->rf_inpos.upload(c->inpos_h, s);
+            c->rf_work.reserve(static_cast<size_t>(std::max<int64_t>(R.work, 1)));
+            c->rf_inv.reserve(static_cast<size_t>(std::max<int64_t>(R.inv, 1)));
+            c->rf_status.reserve(1);
+            HIPCHECK(hipStreamSynchronize(s));
+            c->rf_ready = true;
+        }
+        RfPlan& R = c->rf;
+        // the new values of A in the stored order
+        c->rf_raw.reserve(static_cast<size_t>(c->nnz));
+        HIPCHECK(hipMemcpyAsync(c->rf_raw.p, val, sizeof(double) * c->nnz, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+        launchGather(c->rf_raw.p, c->rf_inpos.p, c->val.p, c->nnz, s);
+        HIPCHECK(hipStreamSynchronize(s));
+        c->rf_ms[0] = ms_since(t_start);
+        const auto t_local = clk::now();
+        // local factors: one launch per height of the elimination trees, then the solve layout
+        const int32_t zero = 0;
+        HIPCHECK(hipMemcpyAsync(c->rf_status.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        RfArgs a{};
+        a.sn = c->rf_sn.p;
+        a.upd = c->rf_upd.p;
+        a.rel = c->rf_rel.p;
+        a.ridx = c->ridx_d.p;
+        a.amap_pos = c->rf_pos.p;
+        a.amap_src = c->rf_src.p;
+        a.val = c->val.p;
+        a.work = c->rf_work.p;
+        a.inv = c->rf_inv.p;
+        a.inv_off = c->rf_invoff.p;
+        a.status = c->rf_status.p;
+        for (size_t h = 0; h + 1 < R.level_ptr.size(); ++h)
+            launchRfLevel(a, c->rf_order.p + R.level_ptr[h], R.level_ptr[h + 1] - R.level_ptr[h], s);
+        const bool f32 = c->precision != 0;
+        launchRfPack(a, static_cast<int32_t>(R.sn.size()), f32 ? static_cast<void*>(c->diag_f.p) : static_cast<void*>(c->diag_dd.p),
+                     f32 ? static_cast<void*>(c->below_f.p) : static_cast<void*>(c->below_dd.p), f32, s);
+        HIPCHECK(hipGetLastError());
+        int32_t status = 0;
+        HIPCHECK(hipMemcpyAsync(&status, c->rf_status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        require(status == 0, S2L_ENUMERIC, "refactorize: a local factor lost positive definiteness at supernode " + std::to_string(status - 1));
+        c->ls_stale = true;
+        c->rf_ms[1] = ms_since(t_local);
+        const auto t_coarse = clk::now();
+        if (c->n0 > 0) {
+            // A_00 = W^T A W, 16 columns at a time: prolongation of unit vectors, SpMV, restriction
+            const int32_t n0 = c->n0, G = 16;
+            c->rf_E.reserve(static_cast<size_t>(n0) * n0);
+            c->rf_Y.reserve(static_cast<size_t>(n0) * G);
+            c->rf_C.reserve(static_cast<size_t>(n0) * G);
+            c->rf_X.reserve(static_cast<size_t>(c->n) * G);
+            c->rf_AX.reserve(static_cast<size_t>(c->n) * G);
+            c->rf_part.reserve(static_cast<size_t>(std::max(c->nsub, 1)) * std::max(c->ca.rchunks, 1) * std::max(c->ca.mmax, 1) * G);
+            if (!c->boolean) c->rf_loc.reserve(static_cast<size_t>(c->local_total) * G);
+            for (int32_t k0 = 0; k0 < n0; k0 += G) {
+                const int32_t cnt = std::min(G, n0 - k0);
+                launchUnitColumns(c->rf_Y.p, n0, k0, cnt, s);
+                if (c->boolean) {
+                    launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_X.p, c->n, 1, cnt, nullptr, s);
+                } else {
+                    launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_loc.p, c->local_total, 0, cnt, nullptr, s);
+                    launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->rf_loc.p, c->local_total, nullptr, c->rf_X.p, cnt, nullptr, s);
+                }
+                launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->rf_X.p, c->rf_AX.p, nullptr, cnt, nullptr, s);
+                launchCoarseRestrict(c->ca, c->rf_AX.p, c->n, c->rf_C.p, c->rf_part.p, cnt, nullptr, s);
+                launchStoreColumns(c->rf_C.p, n0, k0, cnt, c->rf_E.p, n0, s);
+            }
+            HIPCHECK(hipGetLastError());
+            // Cholesky factor and inverse of A_00 (dense FP64 library routines), then the layout of
+            // the coarse solve
+            if (!c->rf_blas) {
+                require(rocblas_create_handle(&c->rf_blas) == rocblas_status_success, S2L_EHIP, "refactorize: rocblas_create_handle failed");
+            }
+            require(rocblas_set_stream(c->rf_blas, s) == rocblas_status_success, S2L_EHIP, "refactorize: rocblas_set_stream failed");
+            rocblas_int* info = reinterpret_cast<rocblas_int*>(c->rf_status.p);
+            require(rocsolver_dpotrf(c->rf_blas, rocblas_fill_lower, n0, c->rf_E.p, n0, info) == rocblas_status_success, S2L_EHIP,
+                    "refactorize: rocsolver_dpotrf failed");
+            int32_t pinfo = 0;
+            HIPCHECK(hipMemcpyAsync(&pinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            require(pinfo == 0, S2L_ENUMERIC, "refactorize: A_00 is not positive definite (leading minor " + std::to_string(pinfo) + ")");
+            require(rocsolver_dtrtri(c->rf_blas, rocblas_fill_lower, rocblas_diagonal_non_unit, n0, c->rf_E.p, n0, info) ==
+                        rocblas_status_success,
+                    S2L_EHIP, "refactorize: rocsolver_dtrtri failed");
+            launchTriLayout(c->rf_E.p, n0, c->tri_ld, c->T1_d.p, c->T2_d.p, s);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(&pinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            require(pinfo == 0, S2L_ENUMERIC, "refactorize: the factor of A_00 is singular");
+        }
+        c->rf_ms[2] = ms_since(t_coarse);
+        c->rf_ms[3] = ms_since(t_start);
     });
 }
 
@@ -2259,6 +2409,7 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
             if (c->correction == 0) apply += spmv + static_cast<double>(c->n) * 8.0;
         }
         o->bytes_apply = apply;
+        for (int q = 0; q < 4; ++q) o->refactor_ms[q] = c->rf_ms[q];
     });
 }
 
@@ -2348,6 +2499,8 @@ int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
                 "the level-synchronous schedule has no FP32-storage / FP64-arithmetic mode");
         require(mode == 2 || !c->local_ready || c->queue_ok, S2L_ELIMIT,
                 "launch modes 0 and 1 need a subdomain vector in the LDS of one workgroup");
+        require(mode != 2 || !c->ls_stale, S2L_ESTATE,
+                "the level-synchronous layout predates the device refactorization; upload the factors again");
         c->launch_mode = mode;
     });
 }

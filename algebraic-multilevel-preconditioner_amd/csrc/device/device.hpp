@@ -13,20 +13,23 @@ namespace s2l {
 
 constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per queue entry
 constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
+constexpr int kStepBlock = 32;      ///< steps of a task's sweep staged in LDS at a time
 
 /// Arguments of the batched local solve (all pointers are device pointers).
 /// T: arithmetic type of the local solve (LDS vector, pushed products); TF: storage type of the
 /// factor entries (TF = float with T = double: FP32 factors applied in FP64 arithmetic).
 template <typename T, typename TF = T> struct LocalArgs {
     const SubDesc* sub;
-    const SnDesc* sn;
-    const int32_t* lev;
-    const int32_t* lev_sn;
     const int32_t* lev_wpre;  ///< per small supernode: widths of the small supernodes before it in its level
     const int32_t* lev_hpre;  ///< same for the heights
-    const int32_t* lev_chunk; ///< chunk starts of every level
     int32_t y_off;            ///< byte offset of the result buffer in the dynamic LDS
     int32_t st_off;           ///< byte offset of the staged supernode descriptors
+    int32_t step_off;         ///< byte offset of the staged steps (kStepBlock StepDesc)
+    const SnDesc* lev_desc;   ///< descriptors in level order
+    const int32_t* lev_row;   ///< forward: factor position of each small row, level order
+    const int32_t* lev_cb;    ///< forward: first pulled slot of each small row
+    const int32_t* lev_ce;    ///< forward: end of its pulled slots
+    const StepDesc* steps;    ///< the steps of every task's sweeps
     const TF* diag;
     const TF* below;
     const int32_t* ridx;

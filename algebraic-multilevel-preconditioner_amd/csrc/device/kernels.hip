@@ -136,6 +136,31 @@ __device__ inline T pullRow(const LocalArgs<T, TF>& a, const SubDesc& sd, const 
     return v;
 }
 
+// x[row] minus the products in the slots [cb, ce) pushed to it by its descendants, subtracted in
+// slot order, eight loads per batch (the slot range comes with the chunk's rows).
+template <typename T> __device__ inline T pullSlots(const T* tb, T v, int cb, int ce)
+{
+    int q = cb;
+    for (; q + kB <= ce; q += kB) {
+        T t[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) t[u] = tb[q + u];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kB; ++u) v -= t[u];
+    }
+    if (q < ce) {
+        T t[kB - 1];
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u) t[u] = q + u < ce ? tb[q + u] : T(0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kB - 1; ++u)
+            if (q + u < ce) v -= t[u];
+    }
+    return v;
+}
+
 // ------------------------------------------------------------------ wide supernodes (w > 64): whole workgroup
 // The blocks of a wide supernode are row-major with a leading dimension padded to four entries
 // and start 16-byte aligned, so a lane reads four consecutive columns with one vector load.
@@ -321,39 +346,35 @@ __device__ inline int findItem(const int* prefix, int count, int t)
     return lo;
 }
 
-// One level of a sweep. The level's small supernodes are processed by chunks, each with every
-// thread of the workgroup on one row (or column) at a time:
-//   forward:  pull the slots of every row; y = inverse(diagonal block) * row values; push the
-//             products of the rows below to their slots;
+// One chunk of small supernodes of one level (a step of the task's sweep), with every thread of the
+// workgroup on one row (or column) at a time:
+//   forward:  pull the slots of every row (its factor position and slot range come with the
+//             chunk, so the pulls start together with the staging of the descriptors);
+//             y = inverse(diagonal block) * row values; push the products of the rows below to
+//             their slots;
 //   backward: every column subtracts the products with the rows below; y = inverse(diagonal) * x.
-// Then the level's wide supernodes, one after the other.
+// The step's fields are uniform over the workgroup.
 template <typename T, typename TF>
-__device__ void runLevel(const LocalArgs<T, TF>& a, const SubDesc& sd, const int* lv, bool forward, T* x, T* ybuf, SnDesc* st,
-                         int* wp, int* hp, T* tb, int tid, int lane, int wave, int nt)
+__device__ void runChunk(const LocalArgs<T, TF>& a, const SubDesc& sd, int q0, int cnt, int wbase, int hbase, int W, int H,
+                         int roff, bool forward, T* x, T* ybuf, SnDesc* st, int* wp, int* hp, T* tb, int tid, int lane,
+                         int wave, int nt)
 {
-    const int end = lv[1], split = lv[2], wsum = lv[3], hsum = lv[4], coff = lv[5], nchunk = lv[6];
-    for (int ch = 0; ch < nchunk; ++ch) {
-        const int q0 = a.lev_chunk[coff + ch], q1 = a.lev_chunk[coff + ch + 1];
-        const int cnt = q1 - q0;
-        const int wbase = a.lev_wpre[q0], hbase = a.lev_hpre[q0];
-        const int W = (q1 < split ? a.lev_wpre[q1] : wsum) - wbase;
-        const int H = (q1 < split ? a.lev_hpre[q1] : hsum) - hbase;
-        for (int i = tid; i < cnt; i += nt) {
-            st[i] = a.sn[a.lev_sn[q0 + i]];
-            wp[i] = a.lev_wpre[q0 + i] - wbase;
-            hp[i] = a.lev_hpre[q0 + i] - hbase;
+    for (int i = tid; i < cnt; i += nt) {
+        st[i] = a.lev_desc[q0 + i];
+        wp[i] = a.lev_wpre[q0 + i] - wbase;
+        hp[i] = a.lev_hpre[q0 + i] - hbase;
+    }
+    if (forward)
+        for (int t = tid; t < W; t += nt) {
+            const int row = a.lev_row[roff + t];
+            x[row] = pullSlots(tb, x[row], a.lev_cb[roff + t], a.lev_ce[roff + t]);
         }
-        __syncthreads();
-        // the supernode of each thread's first row (t = tid) and first row below, found once
-        const int iw = tid < W ? findItem(wp, cnt, tid) : 0;
-        const int ih = forward && tid < H ? findItem(hp, cnt, tid) : 0;
-        if (forward) {
-            for (int t = tid; t < W; t += nt) {
-                const int i = t == tid ? iw : findItem(wp, cnt, t);
-                const int row = st[i].c0 + (t - wp[i]);
-                x[row] = pullRow(a, sd, tb, x, row);
-            }
-        } else if (W * kPullSplit > nt) {
+    __syncthreads();
+    // the supernode of each thread's first row (t = tid) and first row below, found once
+    const int iw = tid < W ? findItem(wp, cnt, tid) : 0;
+    const int ih = forward && tid < H ? findItem(hp, cnt, tid) : 0;
+    if (!forward) {
+        if (W * kPullSplit > nt) {
             for (int t = tid; t < W; t += nt) {
                 const int i = t == tid ? iw : findItem(wp, cnt, t);
                 const SnDesc& s = st[i];
@@ -382,49 +403,79 @@ __device__ void runLevel(const LocalArgs<T, TF>& a, const SubDesc& sd, const int
             }
         }
         __syncthreads();
-        for (int t = tid; t < W; t += nt) {
-            const int i = t == tid ? iw : findItem(wp, cnt, t);
-            const SnDesc& s = st[i];
-            const int j = t - wp[i];
-            const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
-            const TF* D = a.diag + s.diag + lo * s.w + j;
-            ybuf[t] = dot8(D, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
-        }
-        __syncthreads();
-        for (int t = tid; t < W; t += nt) {
-            const int i = t == tid ? iw : findItem(wp, cnt, t);
-            x[st[i].c0 + (t - wp[i])] = ybuf[t];
-        }
-        __syncthreads();
-        if (forward) {
-            for (int t = tid; t < H; t += nt) {
-                const int i = t == tid ? ih : findItem(hp, cnt, t);
-                const SnDesc& s = st[i];
-                const int k = t - hp[i];
-                const TF* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
-                tb[a.cidx[sd.dst_off + s.toff + k]] = dot8(B, 1, x + s.c0, 1, s.w, T(0));
-            }
-            __syncthreads();
-        }
     }
-    for (int q = split; q < end; ++q) {
-        if (forward)
-            fwdWide(a, sd, a.sn[a.lev_sn[q]], x, ybuf, tb, tid, lane, wave, nt);
-        else
-            bwdWide(a, a.sn[a.lev_sn[q]], x, ybuf, tid, lane, wave, nt);
+    for (int t = tid; t < W; t += nt) {
+        const int i = t == tid ? iw : findItem(wp, cnt, t);
+        const SnDesc& s = st[i];
+        const int j = t - wp[i];
+        const int lo = forward ? 0 : j, hi = forward ? j + 1 : s.w;
+        const TF* D = a.diag + s.diag + lo * s.w + j;
+        ybuf[t] = dot8(D, s.w, x + s.c0 + lo, 1, hi - lo, T(0));
     }
     __syncthreads();
+    for (int t = tid; t < W; t += nt) {
+        const int i = t == tid ? iw : findItem(wp, cnt, t);
+        x[st[i].c0 + (t - wp[i])] = ybuf[t];
+    }
+    __syncthreads();
+    if (forward) {
+        for (int t = tid; t < H; t += nt) {
+            const int i = t == tid ? ih : findItem(hp, cnt, t);
+            const SnDesc& s = st[i];
+            const int k = t - hp[i];
+            const TF* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
+            tb[a.cidx[sd.dst_off + s.toff + k]] = dot8(B, 1, x + s.c0, 1, s.w, T(0));
+        }
+        __syncthreads();
+    }
 }
 
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uniform64(int64_t v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(v & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32));
+    return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
+}
+
+// The steps of one sweep of a task, in order: chunks of the small supernodes of a level and the
+// level's wide supernodes, level after level. The steps are staged in LDS kStepBlock at a time;
+// every step ends with a barrier.
 template <typename T, typename TF>
-__device__ void runLevels(const LocalArgs<T, TF>& a, const SubDesc& sd, int lev0, int nlev, bool forward, T* x, unsigned char* smem,
-                          T* tb, int tid, int lane, int wave, int nt)
+__device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0, int nsteps, bool forward, T* x, unsigned char* smem,
+                         T* tb, int tid, int lane, int wave, int nt)
 {
     T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
     SnDesc* st = reinterpret_cast<SnDesc*>(smem + a.st_off);
     int* wp = reinterpret_cast<int*>(smem + a.st_off + kChunkSn * sizeof(SnDesc));
     int* hp = wp + kChunkSn;
-    for (int L = 0; L < nlev; ++L) runLevel(a, sd, a.lev + lev0 + 7 * L, forward, x, ybuf, st, wp, hp, tb, tid, lane, wave, nt);
+    StepDesc* sb = reinterpret_cast<StepDesc*>(smem + a.step_off);
+    for (int k0 = 0; k0 < nsteps; k0 += kStepBlock) {
+        const int kn = min(kStepBlock, nsteps - k0);
+        for (int i = tid; i < kn; i += nt) sb[i] = a.steps[step0 + k0 + i];
+        __syncthreads();
+        for (int k = 0; k < kn; ++k) {
+            const StepDesc& sp = sb[k];
+            if (uniform(sp.kind) == 0) {
+                runChunk(a, sd, uniform(sp.q0), uniform(sp.cnt), uniform(sp.wbase), uniform(sp.hbase), uniform(sp.W),
+                         uniform(sp.H), uniform(sp.roff), forward, x, ybuf, st, wp, hp, tb, tid, lane, wave, nt);
+            } else {
+                SnDesc d;
+                d.c0 = uniform(sp.wide.c0);
+                d.w = uniform(sp.wide.w);
+                d.h = uniform(sp.wide.h);
+                d.toff = uniform(sp.wide.toff);
+                d.diag = uniform64(sp.wide.diag);
+                d.below = uniform64(sp.wide.below);
+                d.ridx = uniform64(sp.wide.ridx);
+                if (forward)
+                    fwdWide(a, sd, d, x, ybuf, tb, tid, lane, wave, nt);
+                else
+                    bwdWide(a, d, x, ybuf, tid, lane, wave, nt);
+                __syncthreads();
+            }
+        }
+    }
 }
 
 // One entry of the level-ordered queue: a task's forward sweep, its backward sweep, or both for a
@@ -461,7 +512,7 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
                 x[row] = static_cast<T>(rhs[g]);
         }
         __syncthreads();
-        runLevels(a, sd, td.f_lev, td.f_nlev, true, x, smem, tb, tid, lane, wave, nt);
+        runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, tb, tid, lane, wave, nt);
         if (!combined) {
             for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
             __syncthreads();
@@ -473,7 +524,7 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
         for (int q = tid; q < td.ext_count; q += blockDim.x) x[ext[q]] = xg[ext[q]];
         __syncthreads();
     }
-    runLevels(a, sd, td.b_lev, td.b_nlev, false, x, smem, tb, tid, lane, wave, nt);
+    runSteps(a, sd, td.b_step, td.b_nsteps, false, x, smem, tb, tid, lane, wave, nt);
     if (td.nchild > 0)
         for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
     // the owned rows of the result (z = x0 + y), or the local buffer (scaled by D_i) for a reverse add

@@ -47,10 +47,16 @@ namespace {
 // (the size of the kernel's staging area and result buffer in LDS). lev_wpre / lev_hpre hold, for
 // every small supernode, the sum of the widths / heights of the small supernodes before it in its
 // level.
-void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, int32_t& lev_off, int32_t& nlev)
+//
+// The same levels are also emitted as steps (StepDesc): every chunk, then every wide supernode of
+// a level, in the order the kernel runs them. @p cptr: the subdomain's slot pointers for the
+// forward sweep (the pulled range of every row), null for a backward sweep.
+void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, int32_t& lev_off, int32_t& nlev,
+                  int32_t& step_off, int32_t& nsteps, const int32_t* cptr)
 {
     lev_off = static_cast<int32_t>(P.lev.size());
     nlev = static_cast<int32_t>(byLevel.size());
+    step_off = static_cast<int32_t>(P.steps.size());
     for (const auto& list : byLevel) {
         const int32_t start = static_cast<int32_t>(P.lev_sn.size());
         int32_t wsum = 0, hsum = 0;
@@ -84,7 +90,40 @@ void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, i
         const int32_t nchunk = static_cast<int32_t>(P.lev_chunk.size()) - coff;
         P.lev_chunk.push_back(split);
         P.lev.insert(P.lev.end(), {start, end, split, wsum, hsum, coff, nchunk});
+        // level-ordered descriptors, the rows of the small supernodes, and the steps
+        const int32_t roff = static_cast<int32_t>(P.lev_row.size());
+        for (int32_t q = start; q < end; ++q) P.lev_desc.push_back(P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(q)])]);
+        if (cptr)
+            for (int32_t q = start; q < split; ++q) {
+                const SnDesc& d = P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(q)])];
+                for (int32_t j = 0; j < d.w; ++j) {
+                    P.lev_row.push_back(d.c0 + j);
+                    P.lev_cb.push_back(cptr[d.c0 + j]);
+                    P.lev_ce.push_back(cptr[d.c0 + j + 1]);
+                }
+            }
+        for (int32_t ch = 0; ch < nchunk; ++ch) {
+            const int32_t q0 = P.lev_chunk[static_cast<std::size_t>(coff + ch)], q1 = P.lev_chunk[static_cast<std::size_t>(coff + ch + 1)];
+            StepDesc st{};
+            st.kind = 0;
+            st.q0 = q0;
+            st.cnt = q1 - q0;
+            st.wbase = P.lev_wpre[static_cast<std::size_t>(q0)];
+            st.hbase = P.lev_hpre[static_cast<std::size_t>(q0)];
+            st.W = (q1 < split ? P.lev_wpre[static_cast<std::size_t>(q1)] : wsum) - st.wbase;
+            st.H = (q1 < split ? P.lev_hpre[static_cast<std::size_t>(q1)] : hsum) - st.hbase;
+            st.roff = cptr ? roff + st.wbase : -1;
+            P.steps.push_back(st);
+        }
+        for (int32_t q = split; q < end; ++q) {
+            StepDesc st{};
+            st.kind = 1;
+            st.q0 = q;
+            st.wide = P.sn[static_cast<std::size_t>(P.lev_sn[static_cast<std::size_t>(q)])];
+            P.steps.push_back(st);
+        }
     }
+    nsteps = static_cast<int32_t>(P.steps.size()) - step_off;
 }
 
 } // namespace
@@ -243,7 +282,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                     P.cidx[static_cast<std::size_t>(sd.dst_off + d.toff + k)] = next[static_cast<std::size_t>(r)]++;
                 }
             }
-            P.bytes_index += static_cast<int64_t>(toff) * 4 + (static_cast<int64_t>(n) + 1) * 4;
+            // slots read and written once, the slot table of every product, and per forward row
+            // its factor position and slot range (lev_row, lev_cb, lev_ce; slot pointers for wide ones)
+            P.bytes_index += static_cast<int64_t>(toff) * 4 + static_cast<int64_t>(n) * 12;
         }
 
         // ---------------- backward sweep (U, given by U^T)
@@ -254,6 +295,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         for (int32_t s = 0; s < nsb; ++s)
             for (int32_t j = bst[s]; j < bst[s + 1]; ++j) snodeB[static_cast<std::size_t>(j)] = s;
         const int32_t b_base = static_cast<int32_t>(P.sn.size());
+        P.fwd_base.push_back(f_base);
+        P.fwd_count.push_back(nsf);
+        P.bwd_base.push_back(b_base);
         std::vector<int32_t> parentB(static_cast<std::size_t>(nsb), -1);
         for (int32_t s = 0; s < nsb; ++s) {
             const int32_t c0 = bst[s], w = bst[s + 1] - c0;
@@ -424,7 +468,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             for (int32_t s : list) hmax = std::max(hmax, heightT[static_cast<std::size_t>(s)]);
             std::vector<std::vector<int32_t>> byLev(static_cast<std::size_t>(hmax) + 1);
             for (int32_t s : list) byLev[static_cast<std::size_t>(heightT[static_cast<std::size_t>(s)])].push_back(f_base + s);
-            appendLevels(P, byLev, td.f_lev, td.f_nlev);
+            appendLevels(P, byLev, td.f_lev, td.f_nlev, td.f_step, td.f_nsteps, P.cptr.data() + sd.cptr_off);
             // backward levels: backward supernodes of the task (same columns when the partitions agree)
             std::vector<int32_t> blist;
             if (samePartition)
@@ -438,7 +482,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             }
             std::vector<std::vector<int32_t>> byLevB(static_cast<std::size_t>(dmax - dmin) + 1);
             for (int32_t s : blist) byLevB[static_cast<std::size_t>(depthB[static_cast<std::size_t>(s)] - dmin)].push_back(b_base + s);
-            appendLevels(P, byLevB, td.b_lev, td.b_nlev);
+            appendLevels(P, byLevB, td.b_lev, td.b_nlev, td.b_step, td.b_nsteps, nullptr);
             // rows of other tasks read by the backward sweep
             td.ext_first = static_cast<int32_t>(P.ext_rows.size());
             std::vector<int32_t> ext;
@@ -488,6 +532,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         P.queue_sub.insert(P.queue_sub.end(), sl.begin(), sl.end());
     }
     P.qsub.push_back(static_cast<int32_t>(P.queue_sub.size()));
+    P.bytes_index += static_cast<int64_t>(P.steps.size()) * static_cast<int64_t>(sizeof(StepDesc));
     for (const TaskDesc& td : P.task) {
         P.max_levels_fwd = std::max(P.max_levels_fwd, td.f_nlev);
         P.max_levels_bwd = std::max(P.max_levels_bwd, td.b_nlev);

@@ -27,6 +27,18 @@ struct SnDesc {
     int64_t ridx;           ///< offset of the h row indices (factor positions)
 };
 
+/// One step of a task's sweep, in execution order: a chunk of small supernodes of one level, or one
+/// wide supernode. A chunk lists its supernodes' descriptors in level order (lev_desc[q0 ..
+/// q0 + cnt)), the prefix sums of their widths and heights (lev_wpre / lev_hpre minus wbase /
+/// hbase), and, in the forward sweep, the factor position and the pulled slot range [cb, ce) of
+/// each of its W rows at lev_row / lev_cb / lev_ce [roff, roff + W): everything a chunk reads
+/// before its first product is one load away from the step. A task's steps are staged in LDS.
+struct StepDesc {
+    int32_t kind;                           ///< 0 chunk of small supernodes, 1 wide supernode
+    int32_t q0, cnt, wbase, hbase, W, H, roff;
+    SnDesc wide;                            ///< kind 1: the wide supernode
+};
+
 /// Per-subdomain descriptor.
 struct SubDesc {
     int32_t n;        ///< local size n_i
@@ -51,6 +63,8 @@ struct TaskDesc {
     int32_t child_first, nchild;    ///< children tasks (indices into task_child)
     int32_t depth;                  ///< depth of the task's root supernode
     int32_t rows_first, rows_count; ///< the task's rows (factor positions, indices into task_rows)
+    int32_t f_step, f_nsteps;       ///< forward steps (indices into steps)
+    int32_t b_step, b_nsteps;       ///< backward steps
     int32_t pad;
 };
 
@@ -80,10 +94,14 @@ struct Packed {
     std::vector<int32_t> lev_sn;      ///< supernode indices listed by level
     std::vector<int32_t> lev_wpre, lev_hpre; ///< prefix sums of small widths / heights per level
     std::vector<int32_t> lev_chunk;   ///< chunk starts (positions in lev_sn) of every level, + end
+    std::vector<SnDesc> lev_desc;     ///< the descriptors in lev_sn order
+    std::vector<int32_t> lev_row, lev_cb, lev_ce; ///< forward: factor position and slot range of each small row, level order
+    std::vector<StepDesc> steps;      ///< the steps of every task's sweeps
     std::vector<double> diag, below;  ///< values
     std::vector<int32_t> ridx;        ///< below-row indices
     std::vector<int32_t> cptr, cidx;  ///< forward contributions: slot ranges per row; slot of every product
     std::vector<SubDesc> sub;
+    std::vector<int32_t> fwd_base, fwd_count, bwd_base; ///< per subdomain: its forward and backward supernodes in sn
     int64_t t_total = 0;
     int64_t entries_fwd = 0, entries_bwd = 0;
     int32_t max_levels_fwd = 0, max_levels_bwd = 0, max_n = 0;

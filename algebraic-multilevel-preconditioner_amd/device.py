@@ -45,7 +45,7 @@ class Stats(ctypes.Structure):
                 ("coarse_rows", ctypes.c_int64), ("levels_fwd_max", ctypes.c_int32),
                 ("levels_bwd_max", ctypes.c_int32), ("supernodes", ctypes.c_int32),
                 ("bytes_local_solve", ctypes.c_double), ("bytes_apply", ctypes.c_double),
-                ("bytes_spmv", ctypes.c_double)]
+                ("bytes_spmv", ctypes.c_double), ("refactor_ms", ctypes.c_double * 4)]
 
 
 _hip = None
@@ -60,7 +60,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
-           "s2l_phase_timing_read", "s2l_apply_noalloc"]
+           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -83,6 +83,7 @@ def hip_lib():
         "s2l_apply": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_spmv": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_apply_noalloc": [vp, vp, vp, ctypes.c_int32, vp],
+        "s2l_refactorize": [vp, _dp, ctypes.c_int],
         "s2l_krylov_solve": [vp, P(KrylovOpts), vp, vp, ctypes.c_int32, P(KrylovResultC), _dp, vp],
         "s2l_apply_host": [vp, _dp, _dp, ctypes.c_int32],
         "s2l_spmv_host": [vp, _dp, _dp, ctypes.c_int32],
@@ -290,6 +291,11 @@ class HipContext:
             self._check(self.L.s2l_upload_coarse(self.h, 0, _p(off, ctypes.c_int64), None, None, 1,
                                                  1 if correction == "additive" else 0))
 
+    def refactorize(self, val: np.ndarray):
+        """s2l_refactorize: new values of A in the order of the uploaded arrays (host memory)."""
+        val = np.ascontiguousarray(val, np.float64)
+        self._check(self.L.s2l_refactorize(self.h, _p(val, ctypes.c_double), 1))
+
     # ------------------------------------------------------------------ operations (host memory)
     def apply(self, r: np.ndarray) -> np.ndarray:
         r = np.ascontiguousarray(r, np.float64)
@@ -487,6 +493,29 @@ class HipTwoLevelSchwarz:
     def name(self) -> str:
         """OneLevelSchwarz::name + ",deflated"/",additive" (two_level_schwarz.cpp:129-132)."""
         return self.one_level.upper() + ("," + self.correction)
+
+    def refactorize_device(self, A_new) -> str:
+        """Refactorize on the device (s2l_refactorize): ``A_new`` in the original ordering with the
+        sparsity pattern the setup was built from (ValueError otherwise: a rebuild); the local
+        factors and A_00 = W^T A W are recomputed on the GPU with the same subdomains and coarse
+        basis. The host setup keeps its old factors (``self.setup.factors`` no longer match)."""
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A_new)
+        A.sort_indices()
+        if getattr(self, "_ren", None) is None:
+            dec = self.setup.dec
+            idx = sp.csr_matrix((np.arange(A.nnz, dtype=np.float64), A.indices, A.indptr), shape=A.shape)
+            ren = idx[dec.new_to_old][:, dec.new_to_old].tocsr()
+            ren.sort_indices()
+            Ar = dec.A.tocsr()
+            if not (np.array_equal(ren.indptr, Ar.indptr) and np.array_equal(ren.indices, Ar.indices)):
+                raise ValueError("refactorize: the sparsity pattern changed; rebuild instead")
+            self._ren = ren.data.astype(np.int64)
+            self._pattern = (A.indptr.copy(), A.indices.copy())
+        elif not (np.array_equal(A.indptr, self._pattern[0]) and np.array_equal(A.indices, self._pattern[1])):
+            raise ValueError("refactorize: the sparsity pattern changed; rebuild instead")
+        self.ctx.refactorize(A.data[self._ren])
+        return "refactorize"
 
     def refresh(self, setup) -> str:
         """Refactorize: uploads the operator, the local factors and A_00 of ``setup`` (new values of

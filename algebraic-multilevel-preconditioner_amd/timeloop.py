@@ -16,14 +16,16 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                   precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
                   seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
                   log=None, matrix: str | None = None, coordinates: bool = True, krylov: str = "gmres",
-                  eig_workers: int = 0) -> list:
+                  eig_workers: int = 0, device_refactor: bool = True) -> list:
     """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
     performed before it, iterations, convergence, true relative residual, update and solve times).
     With ``matrix`` (a file name or a step pattern) the matrices are read, not generated; a step
     whose size, pattern or symmetry differs from the first one raises ValueError on refactorize.
     ``coordinates=False`` partitions generated matrices by their graph only, as read ones are.
     ``krylov``: "gmres" (GMRES(restart)) or "bicgstab". ``eig_workers``: processes of the local
-    eigenproblems of a rebuild (forked only while this process has no GPU context)."""
+    eigenproblems of a rebuild (forked only while this process has no GPU context).
+    ``device_refactor``: refactorize on the device (s2l_refactorize) when the factors allow it
+    (Cholesky local factors and a symmetric A_00), else on the host."""
     gen_kw = dict(gen_kw or {})
     policy = ReusePolicy(reuse, growth, max_age)
     setup = M = None
@@ -45,8 +47,11 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                                        nev=nev, eig_workers=eig_workers)
                 M = device.HipTwoLevelSchwarz(setup, precision)
             elif level == REFACTORIZE:
-                setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou, symmetric=sym)
-                M.refresh(setup)
+                if device_refactor and sym and setup.factors[0].kind == 0:
+                    M.refactorize_device(A)
+                else:
+                    setup = hs.refactorize(setup, A, b, overlap=overlap, pou=pou, symmetric=sym)
+                    M.refresh(setup)
             t_update = time.perf_counter() - t0 if level != REUSE else 0.0
             t0 = time.perf_counter()
             solver = device.HipGmres(restart, rtol, maxit) if krylov == "gmres" else device.HipBiCgStab(rtol, maxit)

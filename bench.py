@@ -224,43 +224,61 @@ def serving_main(args):
 
 
 def reuse_main(args):
-    """The setup-reuse sequence: bubbly pressure matrices 128^3 (200 seeded bubbles, density ratio
-    1000) at pseudo-times 0..steps-1, 256 subdomains, overlap 1, nev 8, deflated; the first solve
-    builds the preconditioner, every later step refactorizes (new local factors and A_00, the same
-    coarse basis Z) and its second corrector reuses; GMRES(50) and BiCGStab, FP32 local factors."""
-    from amp_amd.timeloop import run_time_loop
+    """The setup-reuse sequence: bubbly pressure matrices (coefficient 1e-3 inside 200 bubbles of radii
+    3-8 cells rising one cell per step) at pseudo-times 0..steps-1, 256 subdomains at 128^3, overlap 1,
+    nev 8, deflated; the first solve builds the preconditioner, every later step refactorizes (on the
+    device: new local factors and A_00 with the same coarse basis) and its second corrector reuses;
+    GMRES(50) and BiCGStab, FP32 local factors. Each Krylov method's sequence runs in a child process
+    of its own (a fresh process may fork the eigensolver pool of its rebuild)."""
+    import subprocess
     kw = dict(BUBBLES_128, nx=args.seq_nx)
     if args.seq_nx != 128:  # the same bubble density on a smaller grid
         kw["nbubbles"] = max(1, round(200 * (args.seq_nx / 128.0) ** 3))
     nparts = max(4, (args.seq_nx ** 3) // 8192)
+    if args.reuse_krylov:
+        from amp_amd.timeloop import run_time_loop
+        krylov = args.reuse_krylov
+        recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse=args.reuse_policy,
+                             precision="single", restart=50, rtol=args.rtol, nev=8, krylov=krylov,
+                             eig_workers=args.setup_workers or 16, device_refactor=not args.host_refactor,
+                             log=lambda r: log(f"{krylov} step {r['step']} corrector {r['corrector']}: {r['level']} "
+                                               f"{r['t_update']:.3f} s + {1e3 * r['t_solve']:.1f} ms, "
+                                               f"{r['iterations']} its"))
+        upd = sum(r["t_update"] for r in recs)
+        sol = sum(r["t_solve"] for r in recs)
+        refac = [r["t_update"] for r in recs if r["level"] == "refactorize"]
+        print(json.dumps({"solves_per_s": round(len(recs) / (upd + sol), 3), "solves": len(recs),
+                          "solve_only_per_s": round(len(recs) / sol, 2),
+                          "mean_iterations": round(float(np.mean([r["iterations"] for r in recs])), 2),
+                          "first_iterations": recs[0]["iterations"], "last_iterations": recs[-1]["iterations"],
+                          "rebuild_s": round(recs[0]["t_update"], 2),
+                          "refactorize_s_mean": round(float(np.mean(refac)), 3) if refac else None,
+                          "refactorize_s_first": round(refac[0], 3) if refac else None,
+                          "refactorize_s_median": round(float(np.median(refac)), 3) if refac else None,
+                          "refactorize_on": "host" if args.host_refactor else "device",
+                          "solve_ms_mean": round(1e3 * sol / len(recs), 2),
+                          "all_converged": all(r["converged"] for r in recs),
+                          "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
+                          "levels": "".join({"rebuild": "2", "refactorize": "1", "reuse": "0"}[r["level"]] for r in recs)}),
+              flush=True)
+        return
     out = {"metric": "setup-reuse sequence: solves/s over the sequence (updates + solves)", "unit": "solves/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 local factors / f64 Krylov", "data": "synthetic (generated matrices, seeded right-hand sides)",
            "config": {"workload": f"bubbly pressure 7-point {args.seq_nx}^3, coefficient 1e-3 inside {kw['nbubbles']} "
                                   f"bubbles of radii 3-8 cells rising 1 cell per step, "
                                   f"{args.seq_steps} steps x 2 correctors, {nparts} subdomains, overlap 1, nev 8, "
-                                  f"deflated, reuse policy '{args.reuse_policy}'",
+                                  f"deflated, reuse policy '{args.reuse_policy}', refactorizations on the "
+                                  f"{'host' if args.host_refactor else 'device'}",
                       "parallelism": "single GPU, all subdomains resident"}}
     for krylov in ("gmres", "bicgstab"):
-        recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse=args.reuse_policy,
-                             precision="single", restart=50, rtol=args.rtol, nev=8, krylov=krylov,
-                             eig_workers=args.setup_workers or 16,
-                             log=lambda r: log(f"{krylov} step {r['step']} corrector {r['corrector']}: {r['level']} "
-                                               f"{r['t_update']:.2f} s + {1e3 * r['t_solve']:.1f} ms, "
-                                               f"{r['iterations']} its"))
-        upd = sum(r["t_update"] for r in recs)
-        sol = sum(r["t_solve"] for r in recs)
-        refac = [r["t_update"] for r in recs if r["level"] == "refactorize"]
-        out[krylov] = {"solves_per_s": round(len(recs) / (upd + sol), 3), "solves": len(recs),
-                       "solve_only_per_s": round(len(recs) / sol, 2),
-                       "mean_iterations": round(float(np.mean([r["iterations"] for r in recs])), 2),
-                       "first_iterations": recs[0]["iterations"], "last_iterations": recs[-1]["iterations"],
-                       "rebuild_s": round(recs[0]["t_update"], 2),
-                       "refactorize_s_mean": round(float(np.mean(refac)), 2) if refac else None,
-                       "solve_ms_mean": round(1e3 * sol / len(recs), 2),
-                       "all_converged": all(r["converged"] for r in recs),
-                       "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
-                       "levels": "".join({"rebuild": "2", "refactorize": "1", "reuse": "0"}[r["level"]] for r in recs)}
+        cmd = [sys.executable, os.path.abspath(__file__), "--reuse-sequence", "--reuse-krylov", krylov, "--seq-nx",
+               str(args.seq_nx), "--seq-steps", str(args.seq_steps), "--reuse-policy", args.reuse_policy, "--rtol",
+               str(args.rtol)] + (["--host-refactor"] if args.host_refactor else [])
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+        if p.returncode != 0:
+            raise SystemExit(p.returncode)
+        out[krylov] = json.loads(p.stdout.strip().splitlines()[-1])
     out["value"] = out["gmres"]["solves_per_s"]
     print(json.dumps(out), flush=True)
 
@@ -282,6 +300,8 @@ def main():
                     help="replay each GMRES iteration's apply + SpMV from a captured HIP graph (s2l_set_graphs)")
     ap.add_argument("--tasks", type=int, default=0,
                     help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
+    ap.add_argument("--leaf", type=int, default=0,
+                    help="largest nested-dissection leaf of the local factorizations (0 = the setup's default, 16)")
     ap.add_argument("--local-mode", type=int, default=None, choices=(0, 1, 2),
                     help="schedule of the local solves (s2l_set_local_launch_mode; default: the library's)")
     ap.add_argument("--profile-applies", type=int, default=0,
@@ -291,6 +311,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) that measure the HBM traffic "
                          "of the local-solve launch")
+    ap.add_argument("--profile-precisions", action="store_true",
+                    help="with --profile-applies: also time the FP64 and FP32-storage / FP64-arithmetic local solves")
     ap.add_argument("--profile-one-level", action="store_true",
                     help="with --profile-applies: skip the coarse space (nev 0), for the counter passes")
     ap.add_argument("--setup-workers", type=int, default=0,
@@ -309,6 +331,10 @@ def main():
     ap.add_argument("--seq-nx", type=int, default=128, help="grid of the reuse sequence (nx^3)")
     ap.add_argument("--reuse-policy", default="refactor", choices=("off", "refactor", "auto"),
                     help="reuse policy of the sequence")
+    ap.add_argument("--reuse-krylov", default="", choices=("", "gmres", "bicgstab"),
+                    help="(child run) the sequence of one Krylov method")
+    ap.add_argument("--host-refactor", action="store_true",
+                    help="refactorize on the host (host factorization, A_00 on the host, upload) instead of the device")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the apply path runs on one GPU; --gpus {args.gpus} is not supported (it would "
@@ -337,7 +363,7 @@ def main():
     workers = args.setup_workers or min(16, os.cpu_count() or 1)
     st = hs.build_setup(A, b0, N, xyz=xyz, symmetric=sym, overlap=overlap, nev=nev, eig_workers=workers,
                         eig_tol=1e-6, correction=correction,
-                        factorization="lu" if args.workload == "poisson2d-256" else "auto")
+                        factorization="lu" if args.workload == "poisson2d-256" else "auto", leaf=args.leaf)
     t_setup = time.perf_counter() - t0
     log(f"setup {t_setup:.1f} s {st.times}; n0 = {st.coarse.n0}")
 
@@ -361,8 +387,17 @@ def main():
         for _ in range(3):
             ctx.copy_device(big_a, big_b)
         ctx.synchronize()
-        print(json.dumps({"mode": "profile", "applies": args.profile_applies,
-                          "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}), flush=True)
+        out = {"mode": "profile", "applies": args.profile_applies, "leaf": args.leaf, "tasks": args.tasks,
+               "levels": [stats.levels_fwd_max, stats.levels_bwd_max], "supernodes": stats.supernodes,
+               "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}
+        if args.profile_precisions:
+            for prec in ("double", "mixed"):
+                Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks)
+                Mq.ctx.set_graphs(False)
+                sq = Mq.ctx.stats()
+                ms_q, _ = time_applies(Mq.ctx, r, z, args.profile_applies, warm=PROFILE_WARM)
+                out[prec] = roofline(sq.bytes_local_solve, ms_q, args.profile_applies)
+        print(json.dumps(out), flush=True)
         return
 
     M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks)

@@ -95,6 +95,18 @@ int s2l_finalize_local(s2l_ctx* ctx, int precision);
 int s2l_upload_coarse(s2l_ctx* ctx, int32_t n0, const int64_t* offsets, const double* V, const double* E, int symmetric,
                       int correction);
 
+/* Refactorize on the device (UpdateLevel::Refactorize, src/preconditioner/two_level_schwarz.cpp:
+ * 35-41 with coarse_space.cpp:186-206 and subdomain.cpp:129-148): @p val holds new values of A in the
+ * order of the arrays last given to s2l_upload_system (same pattern; host memory if @p host, else
+ * device memory). The local matrices A_ii are gathered from them, every local Cholesky factor is
+ * computed again on the device over its symbolic structure (a left-looking supernodal factorization,
+ * one height of the elimination trees per launch) and repacked into the layout of the local solve,
+ * and A_00 = W^T A W is formed on the device with the same coarse basis, factored and inverted
+ * (dense FP64). Needs Cholesky local factors, a symmetric A_00 and the task-queue schedule; the
+ * level-synchronous layout is not refreshed (launch mode 2 is refused afterwards). S2L_ENUMERIC when
+ * a factor loses positive definiteness. The first call builds the symbolic plan. */
+int s2l_refactorize(s2l_ctx* ctx, const double* val, int host);
+
 /* z = M^{-1} r for @p nrhs right-hand sides (device pointers, n x nrhs). Replaces
  * Preconditioner::apply (include/preconditioner/preconditioner.hpp:73) as implemented by
  * TwoLevelSchwarz::apply (src/preconditioner/two_level_schwarz.cpp:134-163). @p stream: a
@@ -205,6 +217,7 @@ typedef struct s2l_stats {
     double bytes_local_solve;     /* algorithmic HBM bytes of one batched local-solve launch (1 rhs) */
     double bytes_apply;           /* algorithmic HBM bytes of one preconditioner application (1 rhs) */
     double bytes_spmv;            /* algorithmic HBM bytes of one SpMV (1 rhs) */
+    double refactor_ms[4];        /* the last s2l_refactorize: new values of A, local factors, A_00, total [ms] */
 } s2l_stats;
 int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
 /* Bytes of device memory the context holds (its uploaded system, subdomains, factors, coarse
