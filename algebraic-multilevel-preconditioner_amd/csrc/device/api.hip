@@ -10,6 +10,7 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -257,6 +258,28 @@ struct s2l_ctx {
     bool rf_ready = false;
     bool ls_stale = false;  ///< the level-synchronous layout holds older factor values
     RfPlan rf;
+    bool rf_planned = false;            ///< rf holds the plan of the current factors and pattern
+    std::future<RfPlan> rf_pending;     ///< the plan, built on a host thread after finalize_local
+    std::string rf_plan_err;            ///< why that plan could not be built
+    /// Waits for the background plan and keeps it (or its error). Every call that changes what the
+    /// plan reads (pattern, subdomains, factors, packing) calls this first.
+    void rfJoin()
+    {
+        if (!rf_pending.valid()) return;
+        try {
+            rf = rf_pending.get();
+            rf_planned = true;
+        } catch (const std::exception& e) {
+            rf_plan_err = e.what();
+            rf_planned = false;
+        }
+    }
+    void rfForget()
+    {
+        rfJoin();
+        rf_ready = rf_planned = false;
+        rf_plan_err.clear();
+    }
     DevBuf<RfSn> rf_sn;
     DevBuf<RfUpd> rf_upd;
     DevBuf<int32_t> rf_rel, rf_src, rf_order, rf_status, rf_inpos;
@@ -335,10 +358,12 @@ struct s2l_ctx {
     DevBuf<double> kpart, kred;
     PinBuf<double> pred;
     // BiCGStab
-    DevBuf<double> bR, bRh, bP, bV, bY, bS, bZ, bT, bcoef, bones;
-    DevBuf<int32_t> bskip;
-    PinBuf<double> pcoef;
-    PinBuf<int32_t> pskip;
+    DevBuf<double> bR, bRh, bP, bV, bY, bS, bZ, bT, bcoef, bones, bhist;
+    DevBuf<int32_t> bmask, blive;
+    DevBuf<BiState> bst;
+    PinBuf<int32_t> plive;
+    PinBuf<BiState> pst;
+    PinBuf<double> phist;
 
     void ensureWork(int nrhs)
     {
@@ -1095,16 +1120,10 @@ int gmresImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d
 
 
 // ---- device BiCGStab (bicgstab.cpp:15-154), nrhs independent solves sharing every launch. The
-// scalar recurrences and every decision run on the host in FP64 exactly as in the reference; the
-// vector updates, products and stopping norms are fused kernels whose sums come back per stage.
-struct BicgRhs {
-    double bnorm = 0, rn = 0, l1 = 0, l1_0 = 0, thr = 0, normF = 1, rho = 0, rho_old = 1, alpha = 1, omega = 1, rhn = 0;
-    int its = 0, restarts = 0;
-    bool conv = false, brk = false, candidate = false, first = true, done = false;
-    int phase = 0; // 0 cycle start, 1 step, 2 true residual
-    std::vector<double> hist;
-};
-
+// scalar recurrences and every decision run on the device (k_bicg_control, FP64, the reference's
+// order of operations), between the fused vector kernels, whose launches each right-hand side joins
+// or skips through per-stage masks the control kernel writes. The host reads one word per loop
+// iteration: how many right-hand sides are still running.
 int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_x, int nrhs, s2l_krylov_result* res,
              double* history, hipStream_t s)
 {
@@ -1114,16 +1133,19 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     const bool of = o->stop_norm == 1;
     const int64_t n = c->n;
     const int maxit = o->maxit;
-    const double rtol = o->rtol;
     const size_t N = static_cast<size_t>(n) * nrhs;
+    const size_t H = static_cast<size_t>(maxit + 1) * nrhs;
     for (DevBuf<double>* b : {&c->bR, &c->bRh, &c->bP, &c->bV, &c->bY, &c->bS, &c->bZ, &c->bT}) b->reserve(N);
-    c->bcoef.reserve(static_cast<size_t>(4) * nrhs);
-    c->bskip.reserve(static_cast<size_t>(nrhs));
-    c->pcoef.reserve(static_cast<size_t>(4) * nrhs);
-    c->pskip.reserve(static_cast<size_t>(nrhs));
+    c->bcoef.reserve(static_cast<size_t>(kBiMasks) * 4 * nrhs);
+    c->bmask.reserve(static_cast<size_t>(kBiMasks) * nrhs);
+    c->bst.reserve(static_cast<size_t>(nrhs));
+    c->bhist.reserve(H);
+    c->blive.reserve(1);
+    c->plive.reserve(1);
+    c->pst.reserve(static_cast<size_t>(nrhs));
+    c->phist.reserve(H);
     c->kpart.reserve(static_cast<size_t>(reduceBlocks(n)) * 3 * nrhs);
     c->kred.reserve(static_cast<size_t>(3) * nrhs);
-    c->pred.reserve(static_cast<size_t>(3) * nrhs);
     double* r = c->bR.p;
     double* rh = c->bRh.p;
     double* p = c->bP.p;
@@ -1132,281 +1154,107 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     double* sv = c->bS.p;
     double* z = c->bZ.p;
     double* t = c->bT.p;
-    std::vector<BicgRhs> R(static_cast<size_t>(nrhs));
+    BiArgs A{};
+    A.st = c->bst.p;
+    A.red = c->kred.p;
+    A.mask = c->bmask.p;
+    A.coef = c->bcoef.p;
+    A.hist = c->bhist.p;
+    A.live = c->blive.p;
+    A.n = n;
+    A.rtol = o->rtol;
+    A.atol = o->atol;
+    A.nrhs = nrhs;
+    A.maxit = maxit;
+    A.of = of ? 1 : 0;
+    auto mask = [&](int m) { return static_cast<const int32_t*>(c->bmask.p + static_cast<size_t>(m) * nrhs); };
+    auto coef = [&](int m) { return static_cast<const double*>(c->bcoef.p + static_cast<size_t>(m) * 4 * nrhs); };
+    auto ctl = [&](int stage) { launchBicgControl(A, stage, s); };
+    double* part = c->kpart.p;
+    double* red = c->kred.p;
 
-    // Every stage: the host writes the participation mask and coefficients, copies them, launches,
-    // and (for reductions) reads the sums back. The stream is synchronized before the pinned
-    // staging buffers are written again.
-    auto stage = [&](const std::vector<char>& on, const std::function<double(int, int)>& coef) -> bool {
-        HIPCHECK(hipStreamSynchronize(s));  // the previous stage's copies have read the staging buffers
-        bool any = false;
-        for (int q = 0; q < nrhs; ++q) {
-            c->pskip.p[q] = on[static_cast<size_t>(q)] ? 0 : 1;
-            any = any || on[static_cast<size_t>(q)];
-            for (int k = 0; k < 4; ++k) c->pcoef.p[4 * q + k] = on[static_cast<size_t>(q)] ? coef(q, k) : 0.0;
-        }
-        if (!any) return false;
-        HIPCHECK(hipMemcpyAsync(c->bskip.p, c->pskip.p, sizeof(int32_t) * nrhs, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(c->bcoef.p, c->pcoef.p, sizeof(double) * 4 * nrhs, hipMemcpyHostToDevice, s));
-        return true;
-    };
-    auto readSums = [&] {
-        HIPCHECK(hipMemcpyAsync(c->pred.p, c->kred.p, sizeof(double) * 3 * nrhs, hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-    };
-    auto sums = [&](int q, int k) { return c->pred.p[3 * q + k]; };
-    auto zero = [](int, int) { return 0.0; };
-    const int32_t* skip = c->bskip.p;
-    auto satisfied = [&](const BicgRhs& st, double slack) {
-        return of ? (st.l1 <= slack * st.thr) : (st.rn <= slack * rtol * st.bnorm);
-    };
-
-    // ||b|| for every right-hand side
-    std::vector<char> all(static_cast<size_t>(nrhs), 1);
-    stage(all, zero);
-    launchReduce(0, n, d_b, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-    readSums();
-    std::vector<char> live(static_cast<size_t>(nrhs), 0);
-    for (int q = 0; q < nrhs; ++q) {
-        BicgRhs& st = R[static_cast<size_t>(q)];
-        st.bnorm = std::sqrt(sums(q, 0));
-        if (st.bnorm == 0.0) { // x = 0, converged (bicgstab.cpp:24-29)
-            HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(q) * n, 0, sizeof(double) * n, s));
-            st.conv = st.done = true;
-            st.hist.push_back(0.0);
-        } else {
-            live[static_cast<size_t>(q)] = 1;
-        }
+    // ||b||; x = 0 where b = 0
+    launchReduce(0, n, d_b, nullptr, part, red, nrhs, nullptr, s);
+    ctl(kBiBnorm);
+    launchFillMasked(n, 0.0, d_x, nrhs, mask(kBiZero), s);
+    // r = A x0; the OpenFOAM normalisation from A x0; r = b - r
+    spmvImpl(c, d_x, r, nullptr, nrhs, mask(kBiLive), s);
+    if (of) {
+        launchReduce(5, n, d_x, nullptr, part, red, nrhs, mask(kBiLive), s);
+        ctl(kBiXmean);
+        launchReduce(1, n, r, nullptr, part, red, nrhs, mask(kBiNom), s);  // xm = 0: F = |A x0|_1 + |b|_1 + 1e-20
+        ctl(kBiNomAx);
+        launchReduce(1, n, d_b, nullptr, part, red, nrhs, mask(kBiNom), s);
+        ctl(kBiNomB);
+        c->bones.reserve(static_cast<size_t>(n) * 2);
+        launchFill(n, 1.0, c->bones.p, s);
+        spmvImpl(c, c->bones.p, c->bones.p + n, nullptr, 1, nullptr, s);
+        ofNorm(n, r, d_b, c->bones.p + n, coef(kBiWithm), part, red, nrhs, mask(kBiWithm), s);
+        ctl(kBiWithmNorm);
     }
-    // r = A x0; OpenFOAM normalisation from A x0; r = b - r
-    if (stage(live, zero)) {
-        spmvImpl(c, d_x, r, nullptr, nrhs, skip, s);
-        if (of) {
-            launchReduce(5, n, d_x, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-            readSums();
-            std::vector<double> xm(static_cast<size_t>(nrhs));
-            for (int q = 0; q < nrhs; ++q) xm[static_cast<size_t>(q)] = n ? sums(q, 1) / static_cast<double>(n) : 0.0;
-            std::vector<char> withm(static_cast<size_t>(nrhs), 0), nom(static_cast<size_t>(nrhs), 0);
-            for (int q = 0; q < nrhs; ++q) {
-                if (!live[static_cast<size_t>(q)]) continue;
-                (xm[static_cast<size_t>(q)] == 0.0 ? nom : withm)[static_cast<size_t>(q)] = 1;
-            }
-            if (stage(nom, zero)) {  // xm = 0: F = |A x0|_1 + |b|_1 + 1e-20
-                launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-                readSums();
-                for (int q = 0; q < nrhs; ++q)
-                    if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 1);
-                launchReduce(1, n, d_b, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-                readSums();
-                for (int q = 0; q < nrhs; ++q)
-                    if (nom[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF += sums(q, 1) + 1e-20;
-            }
-            if (stage(withm, [&](int q, int k) { return k == 0 ? xm[static_cast<size_t>(q)] : 0.0; })) {
-                c->bones.reserve(static_cast<size_t>(n) * 2);
-                launchFill(n, 1.0, c->bones.p, s);
-                spmvImpl(c, c->bones.p, c->bones.p + n, nullptr, 1, nullptr, s);
-                ofNorm(n, r, d_b, c->bones.p + n, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
-                readSums();
-                for (int q = 0; q < nrhs; ++q)
-                    if (withm[static_cast<size_t>(q)]) R[static_cast<size_t>(q)].normF = sums(q, 0) + sums(q, 1) + 1e-20;
-            }
-            stage(live, zero);
-        }
-        launchAxpby(n, 1.0, d_b, -1.0, r, nrhs, skip, s);  // r = b - A x0
-        launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-        readSums();
-        for (int q = 0; q < nrhs; ++q) {
-            if (!live[static_cast<size_t>(q)]) continue;
-            BicgRhs& st = R[static_cast<size_t>(q)];
-            st.rn = std::sqrt(sums(q, 0));
-            st.l1 = sums(q, 1);
-            st.l1_0 = st.l1;
-            if (of) st.thr = std::max(rtol * st.l1, o->atol * st.normF);
-            st.hist.push_back(st.rn / st.bnorm);
-        }
-    }
+    launchAxpby(n, 1.0, d_b, -1.0, r, nrhs, mask(kBiLive), s);
+    launchReduce(1, n, r, nullptr, part, red, nrhs, mask(kBiLive), s);
+    ctl(kBiR0);
 
-    std::vector<char> on(static_cast<size_t>(nrhs)), half(static_cast<size_t>(nrhs)), full(static_cast<size_t>(nrhs));
-    for (;;) {
-        // cycle starts: convergence / budget tests, shadow residual r^ = r (bicgstab.cpp:73-82)
-        std::fill(on.begin(), on.end(), 0);
-        for (int q = 0; q < nrhs; ++q) {
-            BicgRhs& st = R[static_cast<size_t>(q)];
-            if (st.done || st.phase != 0) continue;
-            if (satisfied(st, 1.0)) {
-                st.conv = st.done = true;
-                continue;
-            }
-            if (st.its >= maxit) {
-                st.done = true;
-                continue;
-            }
-            st.rhn = st.rn;
-            st.rho = st.rn * st.rn;
-            st.rho_old = st.alpha = st.omega = 1.0;
-            st.first = true;
-            st.candidate = false;
-            st.phase = 1;
-            on[static_cast<size_t>(q)] = 1;
-        }
-        if (stage(on, zero)) launchAxpby(n, 1.0, r, 0.0, rh, nrhs, skip, s);
-        // step, first half: p, y = M p, v = A y, (r^, v)
-        std::fill(on.begin(), on.end(), 0);
-        for (int q = 0; q < nrhs; ++q) {
-            BicgRhs& st = R[static_cast<size_t>(q)];
-            if (st.done || st.phase != 1) continue;
-            if (st.its >= maxit) {
-                st.phase = 2;
-                continue;
-            }
-            on[static_cast<size_t>(q)] = 1;
-        }
-        bool anyDone = true;
-        for (const BicgRhs& st : R) anyDone = anyDone && st.done;
-        if (anyDone) break;
-        if (stage(on, [&](int q, int k) {
-                const BicgRhs& st = R[static_cast<size_t>(q)];
-                if (k == 0) return st.first ? 0.0 : (st.rho / st.rho_old) * (st.alpha / st.omega);
-                if (k == 1) return st.omega;
-                if (k == 2) return st.first ? 1.0 : 0.0;
-                return 0.0;
-            })) {
-            bicgP(n, r, p, v, c->bcoef.p, nrhs, skip, s);
-            int tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
-            applyImpl(c, p, y, nrhs, skip, s);
-            phaseEnd(c, tp, s);
-            tp = phaseBegin(c, S2L_PHASE_SPMV, s);
-            spmvImpl(c, y, v, nullptr, nrhs, skip, s);
-            phaseEnd(c, tp, s);
-            launchReduce(3, n, v, rh, c->kpart.p, c->kred.p, nrhs, skip, s);
-            readSums();
-            std::fill(half.begin(), half.end(), 0);
-            for (int q = 0; q < nrhs; ++q) {
-                if (!on[static_cast<size_t>(q)]) continue;
-                BicgRhs& st = R[static_cast<size_t>(q)];
-                st.first = false;
-                ++st.its;
-                const double rhv = sums(q, 0);
-                if (rhv == 0.0 || !std::isfinite(rhv)) {
-                    st.brk = true;
-                    st.phase = 2;
-                    continue;
-                }
-                st.alpha = st.rho / rhv;
-                half[static_cast<size_t>(q)] = 1;
-            }
-            // s = r - alpha v, stopping norms of s (the half step)
-            if (stage(half, [&](int q, int k) { return k == 0 ? R[static_cast<size_t>(q)].alpha : 0.0; })) {
-                bicgS(n, r, v, sv, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
-                readSums();
-                std::fill(on.begin(), on.end(), 0);
-                std::fill(full.begin(), full.end(), 0);
-                for (int q = 0; q < nrhs; ++q) {
-                    if (!half[static_cast<size_t>(q)]) continue;
-                    BicgRhs& st = R[static_cast<size_t>(q)];
-                    st.rn = std::sqrt(sums(q, 0));
-                    st.l1 = sums(q, 1);
-                    st.hist.push_back(st.rn / st.bnorm);
-                    if (satisfied(st, 1.0) || st.its >= maxit) {
-                        st.candidate = satisfied(st, 1.0);
-                        st.phase = 2;
-                        on[static_cast<size_t>(q)] = 1;  // x += alpha y; r = s
-                    } else {
-                        full[static_cast<size_t>(q)] = 1;
-                    }
-                }
-                if (stage(on, [&](int q, int k) { return k == 0 ? R[static_cast<size_t>(q)].alpha : 0.0; }))
-                    bicgHalf(n, d_x, y, sv, r, c->bcoef.p, nrhs, skip, s);
-                // second half: z = M s, t = A z, omega = (t, s) / (t, t), x and r
-                if (stage(full, zero)) {
-                    int tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
-                    applyImpl(c, sv, z, nrhs, skip, s);
-                    phaseEnd(c, tp, s);
-                    tp = phaseBegin(c, S2L_PHASE_SPMV, s);
-                    spmvImpl(c, z, t, nullptr, nrhs, skip, s);
-                    phaseEnd(c, tp, s);
-                    launchReduce(4, n, t, sv, c->kpart.p, c->kred.p, nrhs, skip, s);
-                    readSums();
-                    for (int q = 0; q < nrhs; ++q) {
-                        if (!full[static_cast<size_t>(q)]) continue;
-                        BicgRhs& st = R[static_cast<size_t>(q)];
-                        ++st.its;
-                        const double d0 = sums(q, 0), d1 = sums(q, 1);
-                        st.omega = d0 > 0.0 ? d1 / d0 : 0.0;
-                    }
-                    stage(full, [&](int q, int k) {
-                        const BicgRhs& st = R[static_cast<size_t>(q)];
-                        return k == 0 ? st.alpha : (k == 1 ? st.omega : 0.0);
-                    });
-                    bicgXR(n, d_x, y, z, sv, t, r, rh, c->bcoef.p, c->kpart.p, c->kred.p, nrhs, skip, s);
-                    readSums();
-                    for (int q = 0; q < nrhs; ++q) {
-                        if (!full[static_cast<size_t>(q)]) continue;
-                        BicgRhs& st = R[static_cast<size_t>(q)];
-                        st.rho_old = st.rho;
-                        st.rho = sums(q, 2);
-                        st.rn = std::sqrt(sums(q, 0));
-                        st.l1 = sums(q, 1);
-                        st.hist.push_back(st.rn / st.bnorm);
-                        if (satisfied(st, 1.0)) {
-                            st.candidate = true;
-                            st.phase = 2;
-                        } else if (st.omega == 0.0 || std::abs(st.rho) < 1e-14 * st.rhn * st.rn) {
-                            st.phase = 2;  // restart with r^ = r
-                        } else if (st.its >= maxit) {
-                            st.phase = 2;
-                        }
-                    }
-                }
-            }
-        }
+    // every loop iteration ends a cycle or advances the iteration count of each running solve
+    for (int it = 0; it <= maxit + 1; ++it) {
+        ctl(kBiCycle);  // cycle starts: convergence / budget tests, r^ = r (bicgstab.cpp:73-82)
+        launchAxpby(n, 1.0, r, 0.0, rh, nrhs, mask(kBiStart), s);
+        // first half: p, y = M p, v = A y, alpha = rho / (r^, v)
+        bicgP(n, r, p, v, coef(kBiP), nrhs, mask(kBiP), s);
+        int tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
+        applyImpl(c, p, y, nrhs, mask(kBiP), s);
+        phaseEnd(c, tp, s);
+        tp = phaseBegin(c, S2L_PHASE_SPMV, s);
+        spmvImpl(c, y, v, nullptr, nrhs, mask(kBiP), s);
+        phaseEnd(c, tp, s);
+        launchReduce(3, n, v, rh, part, red, nrhs, mask(kBiP), s);
+        ctl(kBiAlpha);
+        // s = r - alpha v and its norms: the half step may end the cycle (x += alpha y; r = s)
+        bicgS(n, r, v, sv, coef(kBiHalf), part, red, nrhs, mask(kBiHalf), s);
+        ctl(kBiHalfStep);
+        bicgHalf(n, d_x, y, sv, r, coef(kBiHx), nrhs, mask(kBiHx), s);
+        // second half: z = M s, t = A z, omega = (t, s) / (t, t), x and r
+        tp = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);
+        applyImpl(c, sv, z, nrhs, mask(kBiFull), s);
+        phaseEnd(c, tp, s);
+        tp = phaseBegin(c, S2L_PHASE_SPMV, s);
+        spmvImpl(c, z, t, nullptr, nrhs, mask(kBiFull), s);
+        phaseEnd(c, tp, s);
+        launchReduce(4, n, t, sv, part, red, nrhs, mask(kBiFull), s);
+        ctl(kBiOmega);
+        bicgXR(n, d_x, y, z, sv, t, r, rh, coef(kBiFull), part, red, nrhs, mask(kBiFull), s);
+        ctl(kBiXr);
         // end of a cycle: true residual r = b - A x (bicgstab.cpp:138-151)
-        std::fill(on.begin(), on.end(), 0);
-        for (int q = 0; q < nrhs; ++q) {
-            const BicgRhs& st = R[static_cast<size_t>(q)];
-            on[static_cast<size_t>(q)] = !st.done && st.phase == 2;
-        }
-        if (stage(on, zero)) {
-            spmvImpl(c, d_x, r, d_b, nrhs, skip, s);
-            launchReduce(1, n, r, nullptr, c->kpart.p, c->kred.p, nrhs, skip, s);
-            readSums();
-            for (int q = 0; q < nrhs; ++q) {
-                if (!on[static_cast<size_t>(q)]) continue;
-                BicgRhs& st = R[static_cast<size_t>(q)];
-                st.rn = std::sqrt(sums(q, 0));
-                st.l1 = sums(q, 1);
-                st.phase = 0;
-                if (st.brk) {
-                    st.done = true;
-                } else if (st.candidate && satisfied(st, 1.1)) {
-                    st.conv = st.done = true;
-                } else if (st.its >= maxit) {
-                    st.done = true;
-                } else {
-                    ++st.restarts;
-                }
-            }
-        }
+        spmvImpl(c, d_x, r, d_b, nrhs, mask(kBiEnd), s);
+        launchReduce(1, n, r, nullptr, part, red, nrhs, mask(kBiEnd), s);
+        ctl(kBiEndCycle);
+        HIPCHECK(hipMemcpyAsync(c->plive.p, c->blive.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (c->plive.p[0] == 0) break;
     }
+    HIPCHECK(hipMemcpyAsync(c->pst.p, c->bst.p, sizeof(BiState) * nrhs, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(c->phist.p, c->bhist.p, sizeof(double) * H, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     for (int q = 0; q < nrhs; ++q) {
-        const BicgRhs& st = R[static_cast<size_t>(q)];
+        const BiState& st = c->pst.p[q];
+        const double* hq = c->phist.p + static_cast<size_t>(q) * (maxit + 1);
+        const int nh = std::min(st.nhist, maxit + 1);
         s2l_krylov_result& out = res[q];
         std::memset(&out, 0, sizeof(out));
-        out.converged = st.conv;
-        out.breakdown = st.brk;
+        out.converged = st.conv != 0;
+        out.breakdown = st.brk != 0;
         out.iterations = st.its;
         out.restarts = st.restarts;
-        out.rel_residual = st.hist.empty() ? 0.0 : st.hist.back();
+        out.rel_residual = nh > 0 ? hq[nh - 1] : 0.0;
         out.true_rel_residual = st.bnorm > 0.0 ? st.rn / st.bnorm : 0.0;
         if (of && st.bnorm > 0.0) {
             out.initial_residual_of = st.l1_0 / st.normF;
             out.final_residual_of = st.l1 / st.normF;
         }
-        out.history_len = static_cast<int>(st.hist.size());
+        out.history_len = st.nhist;
         if (history)
-            for (size_t k = 0; k < st.hist.size() && k < static_cast<size_t>(maxit + 1); ++k)
-                history[static_cast<size_t>(q) * (maxit + 1) + k] = st.hist[k];
+            for (int k = 0; k < nh; ++k) history[static_cast<size_t>(q) * (maxit + 1) + k] = hq[k];
     }
     return S2L_OK;
 }
@@ -1482,6 +1330,7 @@ int s2l_ctx_destroy(s2l_ctx* ctx)
         (void)hipEventDestroy(m.e1);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    ctx->rfJoin();
     ctx->invalidateGraphs();
     if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
@@ -1540,7 +1389,7 @@ int s2l_upload_system(s2l_ctx* c, int32_t n, const int32_t* rowptr, const int32_
             c->rp_h.assign(rowptr, rowptr + n + 1);
             c->col_h = cl;
             c->inpos_h = std::move(inpos);
-            c->rf_ready = false;
+            c->rfForget();
             c->rp.upload(rowptr, static_cast<size_t>(n) + 1, c->stream);
             c->col.upload(cl, c->stream);
             c->val.upload(vl, c->stream);
@@ -1559,6 +1408,7 @@ int s2l_upload_subdomains(s2l_ctx* c, int32_t nsub, const int32_t* local_ptr, co
     return guarded(c, [&] {
         if (c) c->invalidateGraphs();
         require(c && c->system_ready, S2L_ESTATE, "upload_subdomains: upload the system first");
+        c->rfForget();
         require(nsub == c->nsub && local_ptr && gid, S2L_EINVAL, "upload_subdomains: bad arguments");
         require(one_level == 0 || one_level == 1, S2L_EINVAL, "one_level must be 0 (RAS) or 1 (ASM)");
         HIPCHECK(hipSetDevice(c->device));
@@ -1618,6 +1468,7 @@ int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* per
     return guarded(c, [&] {
         if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "upload_local_factor: upload the subdomains first");
+        c->rfForget();
         require(i >= 0 && i < c->nsub, S2L_EINVAL, "subdomain index out of range");
         require(n == c->local_ptr[i + 1] - c->local_ptr[i], S2L_EINVAL, "factor size differs from n_i");
         require(perm && l_colptr && l_row && l_val, S2L_EINVAL, "upload_local_factor: null factor");
@@ -1652,6 +1503,7 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
     return guarded(c, [&] {
         if (c) c->invalidateGraphs();
         require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
+        c->rfForget();
         require(precision >= 0 && precision <= 2, S2L_EINVAL,
                 "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 factors, FP64 arithmetic)");
         for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
@@ -1779,8 +1631,14 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         c->precision = precision;
         c->work_nrhs = 0;
         c->local_ready = true;
-        c->rf_ready = false;
         c->ls_stale = false;
+        // the symbolic plan of a later device refactorization, on a host thread meanwhile
+        bool chol = true;
+        for (const HostFactor& F : c->hf) chol = chol && F.cholesky;
+        if (chol && c->queue_ok && !c->rp_h.empty())
+            c->rf_pending = std::async(std::launch::async, [c] {
+                return buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+            });
     });
 }
 
@@ -1905,7 +1763,10 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
         const auto t_start = clk::now();
         auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
         if (!c->rf_ready) {
-            c->rf = buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+            c->rfJoin();
+            if (!c->rf_plan_err.empty()) throw std::invalid_argument(c->rf_plan_err);
+            if (!c->rf_planned) c->rf = buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+            c->rf_planned = true;
             RfPlan& R = c->rf;
             c->rf_sn.upload(R.sn, s);
             c->rf_upd.upload(R.upd, s);

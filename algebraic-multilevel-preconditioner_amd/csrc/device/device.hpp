@@ -88,6 +88,31 @@ struct GmState {
     int32_t pad[2];
 };
 
+/// Per right-hand side state of the device BiCGStab loop (bicgstab.cpp:15-154).
+struct BiState {
+    double bnorm, rn, l1, l1_0, thr, normF, rho, rho_old, alpha, omega, rhn, pad0;
+    int32_t its, restarts, conv, brk, candidate, first, done, phase, nhist, pad1;
+};
+/// Participation masks of the BiCGStab launches (1 = skip that right-hand side), each with its
+/// own four coefficients per right-hand side.
+enum BiMask { kBiZero, kBiLive, kBiNom, kBiWithm, kBiStart, kBiP, kBiHalf, kBiHx, kBiFull, kBiEnd, kBiMasks };
+/// Stages of the BiCGStab control kernel: each reads the sums of the launch before it.
+enum BiStage { kBiBnorm, kBiXmean, kBiNomAx, kBiNomB, kBiWithmNorm, kBiR0, kBiCycle, kBiAlpha, kBiHalfStep, kBiOmega, kBiXr, kBiEndCycle };
+struct BiArgs {
+    BiState* st;
+    const double* red;  ///< sums of the last reduction, 3 per right-hand side
+    int32_t* mask;      ///< [kBiMasks][nrhs]
+    double* coef;       ///< [kBiMasks][4 nrhs]
+    double* hist;       ///< [nrhs][maxit + 1] relative residual norms
+    int32_t* live;      ///< right-hand sides not finished (written by kBiEndCycle)
+    int64_t n;
+    double rtol, atol;
+    int32_t nrhs, maxit, of, pad;
+};
+void launchBicgControl(const BiArgs& a, int stage, hipStream_t s);
+/// y_r = v for the right-hand sides r that skip[r] does not exclude
+void launchFillMasked(int64_t n, double v, double* y, int nrhs, const int32_t* skip, hipStream_t s);
+
 template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, hipStream_t s);
 template <typename T, typename TF> const void* localSolveKernel();
 

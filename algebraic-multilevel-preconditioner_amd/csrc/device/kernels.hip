@@ -1151,6 +1151,204 @@ __global__ __launch_bounds__(kRedThreads) void k_of_norm(int64_t n, const double
     writePartial3(a0, b0, 0.0, partial, rr);
 }
 
+// The scalar recurrences and decisions of BiCGStab (bicgstab.cpp:15-154) for every right-hand
+// side, one thread each, in FP64 with the reference's order of operations; stage by stage they
+// read the sums of the preceding launch and write the masks and coefficients of the next ones.
+__device__ inline bool biSatisfied(const BiArgs& a, const BiState& st, double slack)
+{
+    return a.of ? (st.l1 <= slack * st.thr) : (st.rn <= slack * a.rtol * st.bnorm);
+}
+__device__ inline void biPush(const BiArgs& a, BiState& st, int q, double v)
+{
+    if (st.nhist < a.maxit + 1) a.hist[static_cast<int64_t>(q) * (a.maxit + 1) + st.nhist] = v;
+    ++st.nhist;
+}
+
+__global__ __launch_bounds__(64) void k_bicg_control(BiArgs a, int stage)
+{
+    __shared__ int live;
+    if (threadIdx.x == 0) live = 0;
+    __syncthreads();
+    const int R = a.nrhs;
+    auto M = [&](int m, int q) -> int32_t& { return a.mask[m * R + q]; };
+    auto C = [&](int m, int q, int k) -> double& { return a.coef[static_cast<int64_t>(m) * 4 * R + 4 * q + k]; };
+    for (int q = threadIdx.x; q < R; q += blockDim.x) {
+        BiState st = a.st[q];
+        const double* red = a.red + 3 * q;
+        switch (stage) {
+        case kBiBnorm: {
+            st = BiState{};
+            st.bnorm = sqrt(red[0]);
+            st.normF = 1.0;
+            st.rho_old = st.alpha = st.omega = 1.0;
+            st.first = 1;
+            const bool zero = st.bnorm == 0.0;  // x = 0, converged (bicgstab.cpp:24-29)
+            if (zero) {
+                st.conv = st.done = 1;
+                biPush(a, st, q, 0.0);
+            }
+            M(kBiZero, q) = zero ? 0 : 1;
+            M(kBiLive, q) = zero ? 1 : 0;
+            for (int m = kBiNom; m < kBiMasks; ++m) M(m, q) = 1;
+            for (int m = 0; m < kBiMasks; ++m)
+                for (int k = 0; k < 4; ++k) C(m, q, k) = 0.0;
+            break;
+        }
+        case kBiXmean: {  // OpenFOAM normalisation: the mean of x0 decides the form of F
+            const bool on = !M(kBiLive, q);
+            const double xm = a.n ? red[1] / static_cast<double>(a.n) : 0.0;
+            M(kBiNom, q) = on && xm == 0.0 ? 0 : 1;
+            M(kBiWithm, q) = on && xm != 0.0 ? 0 : 1;
+            C(kBiWithm, q, 0) = on ? xm : 0.0;
+            break;
+        }
+        case kBiNomAx:
+            if (!M(kBiNom, q)) st.normF = red[1];
+            break;
+        case kBiNomB:
+            if (!M(kBiNom, q)) st.normF += red[1] + 1e-20;
+            break;
+        case kBiWithmNorm:
+            if (!M(kBiWithm, q)) st.normF = red[0] + red[1] + 1e-20;
+            break;
+        case kBiR0:
+            if (!M(kBiLive, q)) {
+                st.rn = sqrt(red[0]);
+                st.l1 = red[1];
+                st.l1_0 = st.l1;
+                if (a.of) st.thr = fmax(a.rtol * st.l1, a.atol * st.normF);
+                biPush(a, st, q, st.rn / st.bnorm);
+            }
+            break;
+        case kBiCycle: {  // cycle start (bicgstab.cpp:73-82), then the first half's participants
+            bool start = false;
+            if (!st.done && st.phase == 0) {
+                if (biSatisfied(a, st, 1.0)) {
+                    st.conv = st.done = 1;
+                } else if (st.its >= a.maxit) {
+                    st.done = 1;
+                } else {
+                    st.rhn = st.rn;
+                    st.rho = st.rn * st.rn;
+                    st.rho_old = st.alpha = st.omega = 1.0;
+                    st.first = 1;
+                    st.candidate = 0;
+                    st.phase = 1;
+                    start = true;
+                }
+            }
+            bool step = false;
+            if (!st.done && st.phase == 1) {
+                if (st.its >= a.maxit)
+                    st.phase = 2;
+                else
+                    step = true;
+            }
+            M(kBiStart, q) = start ? 0 : 1;
+            M(kBiP, q) = step ? 0 : 1;
+            C(kBiP, q, 0) = step ? (st.first ? 0.0 : (st.rho / st.rho_old) * (st.alpha / st.omega)) : 0.0;
+            C(kBiP, q, 1) = step ? st.omega : 0.0;
+            C(kBiP, q, 2) = step ? (st.first ? 1.0 : 0.0) : 0.0;
+            break;
+        }
+        case kBiAlpha: {  // alpha = rho / (r^, v)
+            bool half = false;
+            if (!M(kBiP, q)) {
+                st.first = 0;
+                ++st.its;
+                const double rhv = red[0];
+                if (rhv == 0.0 || !isfinite(rhv)) {
+                    st.brk = 1;
+                    st.phase = 2;
+                } else {
+                    st.alpha = st.rho / rhv;
+                    half = true;
+                }
+            }
+            M(kBiHalf, q) = half ? 0 : 1;
+            C(kBiHalf, q, 0) = half ? st.alpha : 0.0;
+            break;
+        }
+        case kBiHalfStep: {  // norms of s: the half step may end the cycle
+            bool hx = false, full = false;
+            if (!M(kBiHalf, q)) {
+                st.rn = sqrt(red[0]);
+                st.l1 = red[1];
+                biPush(a, st, q, st.rn / st.bnorm);
+                const bool sat = biSatisfied(a, st, 1.0);
+                if (sat || st.its >= a.maxit) {
+                    st.candidate = sat ? 1 : 0;
+                    st.phase = 2;
+                    hx = true;  // x += alpha y; r = s
+                } else {
+                    full = true;
+                }
+            }
+            M(kBiHx, q) = hx ? 0 : 1;
+            C(kBiHx, q, 0) = hx ? st.alpha : 0.0;
+            M(kBiFull, q) = full ? 0 : 1;
+            break;
+        }
+        case kBiOmega:  // omega = (t, s) / (t, t)
+            if (!M(kBiFull, q)) {
+                ++st.its;
+                const double d0 = red[0], d1 = red[1];
+                st.omega = d0 > 0.0 ? d1 / d0 : 0.0;
+                C(kBiFull, q, 0) = st.alpha;
+                C(kBiFull, q, 1) = st.omega;
+            }
+            break;
+        case kBiXr:
+            if (!M(kBiFull, q)) {
+                st.rho_old = st.rho;
+                st.rho = red[2];
+                st.rn = sqrt(red[0]);
+                st.l1 = red[1];
+                biPush(a, st, q, st.rn / st.bnorm);
+                if (biSatisfied(a, st, 1.0)) {
+                    st.candidate = 1;
+                    st.phase = 2;
+                } else if (st.omega == 0.0 || fabs(st.rho) < 1e-14 * st.rhn * st.rn) {
+                    st.phase = 2;  // restart with r^ = r
+                } else if (st.its >= a.maxit) {
+                    st.phase = 2;
+                }
+            }
+            M(kBiEnd, q) = (!st.done && st.phase == 2) ? 0 : 1;
+            break;
+        case kBiEndCycle:  // true residual at the end of a cycle (bicgstab.cpp:138-151)
+            if (!M(kBiEnd, q)) {
+                st.rn = sqrt(red[0]);
+                st.l1 = red[1];
+                st.phase = 0;
+                if (st.brk)
+                    st.done = 1;
+                else if (st.candidate && biSatisfied(a, st, 1.1))
+                    st.conv = st.done = 1;
+                else if (st.its >= a.maxit)
+                    st.done = 1;
+                else
+                    ++st.restarts;
+            }
+            if (!st.done) atomicAdd(&live, 1);
+            break;
+        default:
+            break;
+        }
+        a.st[q] = st;
+    }
+    __syncthreads();
+    if (stage == kBiEndCycle && threadIdx.x == 0) a.live[0] = live;
+}
+
+__global__ void k_fill_masked(int64_t n, double v, double* y, const int32_t* skip)
+{
+    const int rr = blockIdx.y;
+    if (skip && skip[rr]) return;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[static_cast<int64_t>(rr) * n + i] = v;
+}
+
 __global__ void k_fill(int64_t n, double v, double* y)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1598,6 +1796,16 @@ void ofNorm(int64_t n, const double* ax, const double* b, const double* a1, cons
     hipLaunchKernelGGL(k_of_norm, dim3(nb, nrhs), dim3(kRedThreads), 0, s, n, ax, b, a1, coef, partial, skip);
     finalSum3(partial, nb, out, nrhs, skip, s);
 }
+void launchBicgControl(const BiArgs& a, int stage, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bicg_control, dim3(1), dim3(64), 0, s, a, stage);
+}
+
+void launchFillMasked(int64_t n, double v, double* y, int nrhs, const int32_t* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_masked, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, v, y, skip);
+}
+
 void launchFill(int64_t n, double v, double* y, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, y);
