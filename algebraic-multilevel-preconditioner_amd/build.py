@@ -37,7 +37,8 @@ def build_device():
     objdir = os.path.join(HERE, "csrc", "device", "_obj")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for f in ("kernels.hip", "lsolve.hip", "refactor.hip", "api.hip", "pack.cpp", "lsched.cpp", "refactor.cpp"):
+    for f in ("kernels.hip", "lsolve.hip", "refactor.hip", "eigen.hip", "api.hip", "pack.cpp", "lsched.cpp", "refactor.cpp",
+              "eigen.cpp"):
         o = os.path.join(objdir, f + ".o")
         lang = ["-x", "hip"] if f.endswith(".hip") else []
         _run([HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", src] + lang + ["-c", os.path.join(src, f), "-o", o])

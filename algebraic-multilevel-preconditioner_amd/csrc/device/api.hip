@@ -19,6 +19,7 @@
 
 #include "device.hpp"
 #include "refactor.hpp"
+#include "eigen.hpp"
 
 #include <rocsolver/rocsolver.h>
 
@@ -279,7 +280,16 @@ struct s2l_ctx {
         rfJoin();
         rf_ready = rf_planned = false;
         rf_plan_err.clear();
+        eg_ready = false;
     }
+
+    // device local eigensolver (s2l_local_eigensolve): its tables, shifted factors and Krylov bases
+    bool eg_ready = false;
+    EigPlan eg;
+    DevBuf<int32_t> eg_chunk, eg_cptr, eg_lrp, eg_lcol, eg_lsrc, eg_orp, eg_osrc, eg_ghost, eg_lptr, eg_done, eg_conv,
+        eg_live, eg_info;
+    DevBuf<double> eg_D, eg_adj, eg_rowsum, eg_sigma, eg_diag, eg_below, eg_V, eg_AV, eg_H, eg_Hw, eg_theta, eg_E,
+        eg_part, eg_hb, eg_hb2, eg_beta, eg_u, eg_w, eg_Aw, eg_lam;
     DevBuf<RfSn> rf_sn;
     DevBuf<RfUpd> rf_upd;
     DevBuf<int32_t> rf_rel, rf_src, rf_order, rf_status, rf_inpos;
@@ -1258,6 +1268,48 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     }
     return S2L_OK;
 }
+// The plan of the device refactorization and its device tables (built once per pattern, subdomains
+// and factors; the host part usually comes from the thread started by finalize_local).
+void ensureRfPlan(s2l_ctx* c, hipStream_t s)
+{
+    if (c->rf_ready) return;
+    c->rfJoin();
+    if (!c->rf_plan_err.empty()) throw std::invalid_argument(c->rf_plan_err);
+    if (!c->rf_planned) c->rf = buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+    c->rf_planned = true;
+    RfPlan& R = c->rf;
+    c->rf_sn.upload(R.sn, s);
+    c->rf_upd.upload(R.upd, s);
+    c->rf_rel.upload(R.rel, s);
+    c->rf_pos.upload(R.amap_pos, s);
+    c->rf_src.upload(R.amap_src, s);
+    c->rf_order.upload(R.order, s);
+    c->rf_invoff.upload(R.inv_off, s);
+    c->rf_inpos.upload(c->inpos_h, s);
+    c->rf_work.reserve(static_cast<size_t>(std::max<int64_t>(R.work, 1)));
+    c->rf_inv.reserve(static_cast<size_t>(std::max<int64_t>(R.inv, 1)));
+    c->rf_status.reserve(1);
+    HIPCHECK(hipStreamSynchronize(s));
+    c->rf_ready = true;
+}
+
+RfArgs rfArgs(s2l_ctx* c)
+{
+    RfArgs a{};
+    a.sn = c->rf_sn.p;
+    a.upd = c->rf_upd.p;
+    a.rel = c->rf_rel.p;
+    a.ridx = c->ridx_d.p;
+    a.amap_pos = c->rf_pos.p;
+    a.amap_src = c->rf_src.p;
+    a.val = c->val.p;
+    a.work = c->rf_work.p;
+    a.inv = c->rf_inv.p;
+    a.inv_off = c->rf_invoff.p;
+    a.status = c->rf_status.p;
+    return a;
+}
+
 } // namespace
 
 // A GMRES run over a fixed number of right-hand-side slots with its own right-hand sides and
@@ -1747,6 +1799,205 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
     });
 }
 
+int s2l_local_eigensolve(s2l_ctx* c, int32_t nev, int32_t ncv, double shift_rel, double tol, int32_t max_restarts,
+                         const int32_t* skip, double* lambda, double* Z, int32_t* converged, int32_t* info)
+{
+    return guarded(c, [&] {
+        require(c && lambda && Z, S2L_EINVAL, "local_eigensolve: bad arguments");
+        require(c->system_ready && c->subs_ready && c->local_ready, S2L_ESTATE,
+                "local_eigensolve: upload the system, subdomains and local factors first");
+        for (const HostFactor& F : c->hf) require(F.cholesky, S2L_ESTATE, "local_eigensolve: needs Cholesky factors (symmetric A)");
+        require(c->queue_ok && c->launch_mode != 2, S2L_ESTATE, "local_eigensolve: needs the task-queue schedule (launch modes 0 and 1)");
+        require(c->precision == 0, S2L_ESTATE, "local_eigensolve: needs FP64 local factors (precision 0)");
+        require(nev >= 1 && ncv >= nev + 2 && ncv <= kEigMaxM, S2L_EINVAL,
+                "local_eigensolve: need 1 <= nev, nev + 2 <= ncv <= " + std::to_string(kEigMaxM));
+        require(tol > 0.0 && shift_rel > 0.0 && max_restarts >= 1, S2L_EINVAL, "local_eigensolve: tol, shift_rel > 0, max_restarts >= 1");
+        const int nsub = c->nsub;
+        for (int i = 0; i < nsub; ++i) {
+            const int32_t ni = c->local_ptr[static_cast<size_t>(i) + 1] - c->local_ptr[static_cast<size_t>(i)];
+            require((skip && skip[i]) || ni >= ncv + 1, S2L_EINVAL,
+                    "local_eigensolve: subdomain " + std::to_string(i) + " has " + std::to_string(ni) +
+                        " rows; the Krylov basis needs ncv + 1 (skip it)");
+        }
+        HIPCHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        c->invalidateGraphs();
+        ensureRfPlan(c, s);
+        const int64_t nl = c->local_total;
+        if (!c->eg_ready) {
+            c->eg = buildEigPlan(c->local_ptr, c->gid_h, c->ownership, c->rp_h, c->col_h, c->pou_h, c->boolean);
+            EigPlan& E = c->eg;
+            c->eg_chunk.upload(E.chunk, s);
+            c->eg_cptr.upload(E.chunk_ptr, s);
+            c->eg_lrp.upload(E.lrp, s);
+            c->eg_lcol.upload(E.lcol, s);
+            c->eg_lsrc.upload(E.lsrc, s);
+            c->eg_orp.upload(E.orp, s);
+            c->eg_osrc.upload(E.osrc, s);
+            c->eg_ghost.upload(E.ghost, s);
+            c->eg_D.upload(E.D, s);
+            c->eg_lptr.upload(c->local_ptr, s);
+            HIPCHECK(hipStreamSynchronize(s));
+            c->eg_ready = true;
+        }
+        const int m = ncv, kk = std::max(nev + 1, (nev + ncv) / 2);
+        const int nchunk = static_cast<int>(c->eg.chunk.size() / 3);
+        const size_t ld = static_cast<size_t>(kEigMaxM) + 1;
+        c->eg_adj.reserve(static_cast<size_t>(nl));
+        c->eg_rowsum.reserve(static_cast<size_t>(nl));
+        c->eg_sigma.reserve(static_cast<size_t>(nsub));
+        c->eg_diag.reserve(c->diag_dd.n);
+        c->eg_below.reserve(c->below_dd.n);
+        c->eg_V.reserve(static_cast<size_t>(m + 1) * nl);
+        c->eg_AV.reserve(static_cast<size_t>(m + 1) * nl);
+        c->eg_H.reserve(static_cast<size_t>(nsub) * m * m);
+        c->eg_Hw.reserve(static_cast<size_t>(nsub) * m * m);
+        c->eg_theta.reserve(static_cast<size_t>(nsub) * m);
+        c->eg_E.reserve(static_cast<size_t>(nsub) * m);
+        c->eg_part.reserve(static_cast<size_t>(nchunk) * ld);
+        c->eg_hb.reserve(static_cast<size_t>(nsub) * ld);
+        c->eg_hb2.reserve(static_cast<size_t>(nsub) * ld);
+        c->eg_beta.reserve(static_cast<size_t>(nsub));
+        c->eg_u.reserve(static_cast<size_t>(nl));
+        c->eg_w.reserve(static_cast<size_t>(nl));
+        c->eg_Aw.reserve(static_cast<size_t>(nl));
+        c->eg_lam.reserve(static_cast<size_t>(nsub) * nev);
+        c->eg_done.reserve(static_cast<size_t>(nsub));
+        c->eg_conv.reserve(static_cast<size_t>(nsub));
+        c->eg_live.reserve(1);
+        c->eg_info.reserve(static_cast<size_t>(nsub));
+        std::vector<int32_t> done0(static_cast<size_t>(nsub), 0);
+        for (int i = 0; i < nsub; ++i) done0[static_cast<size_t>(i)] = (skip && skip[i]) ? 1 : 0;
+        HIPCHECK(hipMemcpyAsync(c->eg_done.p, done0.data(), sizeof(int32_t) * nsub, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemsetAsync(c->eg_H.p, 0, sizeof(double) * nsub * m * m, s));
+        HIPCHECK(hipMemsetAsync(c->eg_V.p, 0, sizeof(double) * static_cast<size_t>(m + 1) * nl, s));  // skipped subdomains stay 0
+        HIPCHECK(hipMemsetAsync(c->eg_lam.p, 0, sizeof(double) * nsub * nev, s));
+        EigArgs A{};
+        A.nsub = nsub;
+        A.nchunk = nchunk;
+        A.m = m;
+        A.nl = nl;
+        A.lptr = c->eg_lptr.p;
+        A.chunk = c->eg_chunk.p;
+        A.chunk_ptr = c->eg_cptr.p;
+        A.lrp = c->eg_lrp.p;
+        A.lcol = c->eg_lcol.p;
+        A.lsrc = c->eg_lsrc.p;
+        A.orp = c->eg_orp.p;
+        A.osrc = c->eg_osrc.p;
+        A.ghost = c->eg_ghost.p;
+        A.D = c->eg_D.p;
+        A.val = c->val.p;
+        A.adj = c->eg_adj.p;
+        A.done = c->eg_done.p;
+
+        // factors of A~_ii + sigma_i I on the structure of A_ii's (local_eigensolver.cpp:285-303:
+        // the shift grows by 1e3 when a factorization fails, at most four attempts)
+        RfArgs ra = rfArgs(c);
+        ra.diag_adj = c->eg_adj.p;
+        ra.lperm = c->lperm_d.p;
+        ra.sub_off = c->eg_lptr.p;
+        const RfPlan& R = c->rf;
+        int attempt = 0;
+        for (;; ++attempt) {
+            launchEigLump(A, c->eg_rowsum.p, s);
+            launchEigShift(A, c->eg_rowsum.p, shift_rel, c->eg_sigma.p, s);
+            launchEigAddShift(A, c->eg_sigma.p, std::pow(1e3, attempt), s);
+            const int32_t zero = 0;
+            HIPCHECK(hipMemcpyAsync(c->rf_status.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+            for (size_t h = 0; h + 1 < R.level_ptr.size(); ++h)
+                launchRfLevel(ra, c->rf_order.p + R.level_ptr[h], R.level_ptr[h + 1] - R.level_ptr[h], s);
+            launchRfPack(ra, static_cast<int32_t>(R.sn.size()), c->eg_diag.p, c->eg_below.p, false, s);
+            HIPCHECK(hipGetLastError());
+            int32_t status = 0;
+            HIPCHECK(hipMemcpyAsync(&status, c->rf_status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (status == 0) break;
+            require(attempt < 3, S2L_ENUMERIC, "local_eigensolve: cannot factorize A~_ii + sigma I (supernode " + std::to_string(status - 1) + ")");
+        }
+
+        // the operator's local solve: local input and output, the shifted factors
+        c->ensureWork(1);
+        LocalArgs<double, double> la = localArgs<double, double>(c, c->eg_u.p, nullptr, c->eg_w.p, 1, 1, nullptr);
+        la.diag = c->eg_diag.p;
+        la.below = c->eg_below.p;
+        la.pou = nullptr;
+        la.local_in = 1;
+        auto solve = [&] {
+            for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g)
+                launchQueue<double, double>(c, la, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
+        };
+        double* V = c->eg_V.p;
+        double* AV = c->eg_AV.p;
+        double* u = c->eg_u.p;
+        double* w = c->eg_w.p;
+        double* Aw = c->eg_Aw.p;
+        // v_0: a seeded vector normalized in the (A~ + sigma I) norm
+        launchEigStart(A, w, s);
+        launchEigSpmv(A, 0, w, Aw, s);
+        launchEigDots(A, Aw, 1, w, c->eg_part.p, s);
+        launchEigDotsFinal(A, c->eg_part.p, 1, 2, nullptr, nullptr, nullptr, 0, c->eg_beta.p, s);
+        launchEigScale(A, V, AV, 0, w, Aw, c->eg_beta.p, s);
+        if (!c->rf_blas)
+            require(rocblas_create_handle(&c->rf_blas) == rocblas_status_success, S2L_EHIP, "local_eigensolve: rocblas_create_handle failed");
+        require(rocblas_set_stream(c->rf_blas, s) == rocblas_status_success, S2L_EHIP, "local_eigensolve: rocblas_set_stream failed");
+        int k = 0, restarts = 0, steps = 0;
+        int32_t live = nsub;
+        for (;;) {
+            // Lanczos steps j = k .. m-1 with full reorthogonalization (two classical Gram-Schmidt
+            // passes in the A~ + sigma I inner product): w = (A~ + sigma I)^{-1} B v_j
+            for (int j = k; j < m; ++j, ++steps) {
+                launchEigSpmv(A, 1, V + static_cast<int64_t>(j) * nl, u, s);
+                solve();
+                launchEigDots(A, AV, j + 1, w, c->eg_part.p, s);
+                launchEigDotsFinal(A, c->eg_part.p, j + 1, 0, c->eg_hb.p, c->eg_hb2.p, c->eg_H.p, j, nullptr, s);
+                launchEigUpdate(A, V, j + 1, c->eg_hb.p, w, s);
+                launchEigDots(A, AV, j + 1, w, c->eg_part.p, s);
+                launchEigDotsFinal(A, c->eg_part.p, j + 1, 1, c->eg_hb.p, c->eg_hb2.p, c->eg_H.p, j, nullptr, s);
+                launchEigUpdate(A, V, j + 1, c->eg_hb2.p, w, s);
+                launchEigSpmv(A, 0, w, Aw, s);
+                launchEigDots(A, Aw, 1, w, c->eg_part.p, s);
+                launchEigDotsFinal(A, c->eg_part.p, 1, 2, nullptr, nullptr, nullptr, 0, c->eg_beta.p, s);
+                launchEigScale(A, V, AV, j + 1, w, Aw, c->eg_beta.p, s);
+            }
+            HIPCHECK(hipGetLastError());
+            // Ritz pairs of the projected matrix, convergence, thick restart with kk Ritz vectors
+            HIPCHECK(hipMemcpyAsync(c->eg_Hw.p, c->eg_H.p, sizeof(double) * nsub * m * m, hipMemcpyDeviceToDevice, s));
+            require(rocsolver_dsyevd_strided_batched(c->rf_blas, rocblas_evect_original, rocblas_fill_upper, m, c->eg_Hw.p, m,
+                                                     static_cast<rocblas_stride>(m) * m, c->eg_theta.p, m, c->eg_E.p, m,
+                                                     c->eg_info.p, nsub) == rocblas_status_success,
+                    S2L_EHIP, "local_eigensolve: rocsolver_dsyevd_strided_batched failed");
+            launchEigCheck(A, c->eg_Hw.p, c->eg_theta.p, c->eg_beta.p, nev, tol, c->eg_conv.p, c->eg_lam.p, c->eg_live.p, s);
+            launchEigRotate(A, V, AV, c->eg_Hw.p, kk, s);
+            launchEigReset(A, c->eg_H.p, c->eg_theta.p, kk, c->eg_conv.p, c->eg_done.p, s);
+            HIPCHECK(hipGetLastError());
+            ++restarts;
+            HIPCHECK(hipMemcpyAsync(&live, c->eg_live.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (live == 0 || restarts >= max_restarts) break;
+            k = kk;
+        }
+        // the nev leading Ritz vectors are the first columns of every basis
+        HIPCHECK(hipMemcpyAsync(Z, V, sizeof(double) * static_cast<size_t>(nev) * nl, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(lambda, c->eg_lam.p, sizeof(double) * static_cast<size_t>(nsub) * nev, hipMemcpyDeviceToHost, s));
+        std::vector<int32_t> done(static_cast<size_t>(nsub));
+        HIPCHECK(hipMemcpyAsync(done.data(), c->eg_done.p, sizeof(int32_t) * nsub, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        int nconv = 0;
+        for (int i = 0; i < nsub; ++i) {
+            const bool ok = done[static_cast<size_t>(i)] && !(skip && skip[i]);
+            if (converged) converged[i] = ok ? 1 : 0;
+            nconv += ok ? 1 : 0;
+        }
+        if (info) {
+            info[0] = restarts;
+            info[1] = steps;
+            info[2] = attempt + 1;
+            info[3] = nconv;
+        }
+    });
+}
+
 int s2l_refactorize(s2l_ctx* c, const double* val, int host)
 {
     return guarded(c, [&] {
@@ -1762,26 +2013,7 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
         using clk = std::chrono::steady_clock;
         const auto t_start = clk::now();
         auto ms_since = [](clk::time_point t) { return std::chrono::duration<double, std::milli>(clk::now() - t).count(); };
-        if (!c->rf_ready) {
-            c->rfJoin();
-            if (!c->rf_plan_err.empty()) throw std::invalid_argument(c->rf_plan_err);
-            if (!c->rf_planned) c->rf = buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
-            c->rf_planned = true;
-            RfPlan& R = c->rf;
-            c->rf_sn.upload(R.sn, s);
-            c->rf_upd.upload(R.upd, s);
-            c->rf_rel.upload(R.rel, s);
-            c->rf_pos.upload(R.amap_pos, s);
-            c->rf_src.upload(R.amap_src, s);
-            c->rf_order.upload(R.order, s);
-            c->rf_invoff.upload(R.inv_off, s);
-            c->rf_inpos.upload(c->inpos_h, s);
-            c->rf_work.reserve(static_cast<size_t>(std::max<int64_t>(R.work, 1)));
-            c->rf_inv.reserve(static_cast<size_t>(std::max<int64_t>(R.inv, 1)));
-            c->rf_status.reserve(1);
-            HIPCHECK(hipStreamSynchronize(s));
-            c->rf_ready = true;
-        }
+        ensureRfPlan(c, s);
         RfPlan& R = c->rf;
         // the new values of A in the stored order
         c->rf_raw.reserve(static_cast<size_t>(c->nnz));
@@ -1793,18 +2025,7 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
         // local factors: one launch per height of the elimination trees, then the solve layout
         const int32_t zero = 0;
         HIPCHECK(hipMemcpyAsync(c->rf_status.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
-        RfArgs a{};
-        a.sn = c->rf_sn.p;
-        a.upd = c->rf_upd.p;
-        a.rel = c->rf_rel.p;
-        a.ridx = c->ridx_d.p;
-        a.amap_pos = c->rf_pos.p;
-        a.amap_src = c->rf_src.p;
-        a.val = c->val.p;
-        a.work = c->rf_work.p;
-        a.inv = c->rf_inv.p;
-        a.inv_off = c->rf_invoff.p;
-        a.status = c->rf_status.p;
+        RfArgs a = rfArgs(c);
         for (size_t h = 0; h + 1 < R.level_ptr.size(); ++h)
             launchRfLevel(a, c->rf_order.p + R.level_ptr[h], R.level_ptr[h + 1] - R.level_ptr[h], s);
         const bool f32 = c->precision != 0;

@@ -47,6 +47,7 @@ template <typename T, typename TF = T> struct LocalArgs {
     int64_t n_local_total;
     int64_t t_total;
     int32_t mode;         ///< 0 RAS with Boolean PoU (direct write), 1 local buffer for a reverse add
+    int32_t local_in;     ///< 1: rhs is a local buffer (sum n_i x nrhs, by local index) instead of global
     // ---- level-ordered work queue
     const TaskDesc* task;
     const int32_t* task_child;

@@ -502,12 +502,13 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
     const int32_t* rows = a.task_rows + td.rows_first;
     if (!backward) {
-        const double* rhs = a.rhs + static_cast<int64_t>(rA) * a.n_global;
+        const int64_t ld = a.local_in ? a.n_local_total : a.n_global;
+        const double* rhs = a.rhs + static_cast<int64_t>(rA) * ld;
         for (int q = tid; q < td.rows_count; q += blockDim.x) {
             const int row = rows[q];
-            const int g = a.gmap[sd.map_off + row];
+            const int g = a.local_in ? sd.map_off + a.lperm[sd.map_off + row] : a.gmap[sd.map_off + row];
             if constexpr (kPair)
-                x[row] = float2(static_cast<float>(rhs[g]), onB ? static_cast<float>(rhs[a.n_global + g]) : 0.0f);
+                x[row] = float2(static_cast<float>(rhs[g]), onB ? static_cast<float>(rhs[ld + g]) : 0.0f);
             else
                 x[row] = static_cast<T>(rhs[g]);
         }

@@ -27,6 +27,11 @@ __global__ __launch_bounds__(kRfThreads) void k_rf_level(RfArgs a, const int32_t
     __syncthreads();
     for (int64_t e = tid; e < r.amap_count; e += nt) a.work[a.amap_pos[r.amap_first + e]] = a.val[a.amap_src[r.amap_first + e]];
     __syncthreads();
+    if (a.diag_adj) {  // a shifted matrix with the same structure (the local eigenproblems' A~_ii + sigma I)
+        const int32_t off = a.sub_off[r.sub];
+        for (int c = tid; c < w; c += nt) D[static_cast<int64_t>(c) * w + c] += a.diag_adj[off + a.lperm[off + r.c0 + c]];
+        __syncthreads();
+    }
     for (int u = 0; u < r.upd_count; ++u) {
         const RfUpd U = a.upd[r.upd_first + u];
         const RfSn dn = a.sn[U.d];

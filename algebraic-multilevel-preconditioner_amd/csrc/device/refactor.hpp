@@ -69,6 +69,9 @@ struct RfArgs {
     double* inv;
     const int64_t* inv_off;
     int32_t* status;          ///< 0 ok; 1 + the first failing supernode: a pivot not > 0
+    const double* diag_adj;   ///< null, or added to the diagonal of A_ii (by local index)
+    const int32_t* lperm;     ///< local index of every factor position (with diag_adj)
+    const int32_t* sub_off;   ///< first local position of every subdomain (with diag_adj)
 };
 
 /// The panels of supernodes order[first .. first + count) (one height): A entries, descendants'

@@ -60,7 +60,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
-           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize"]
+           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -84,6 +84,8 @@ def hip_lib():
         "s2l_spmv": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_apply_noalloc": [vp, vp, vp, ctypes.c_int32, vp],
         "s2l_refactorize": [vp, _dp, ctypes.c_int],
+        "s2l_local_eigensolve": [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                 _i32p, _dp, _dp, _i32p, _i32p],
         "s2l_krylov_solve": [vp, P(KrylovOpts), vp, vp, ctypes.c_int32, P(KrylovResultC), _dp, vp],
         "s2l_apply_host": [vp, _dp, _dp, ctypes.c_int32],
         "s2l_spmv_host": [vp, _dp, _dp, ctypes.c_int32],
@@ -160,9 +162,9 @@ def setup_arrays(setup, precision: str = "double", one_level: str | None = None,
             d[f"f{i}_urow"] = np.ascontiguousarray(f.ut_row, np.int32)
             d[f"f{i}_uval"] = np.ascontiguousarray(f.ut_val, np.float64)
     cs = setup.coarse
-    d["n0"] = np.int64(cs.n0)
-    d["coarse_offsets"] = np.ascontiguousarray(cs.offsets, np.int64)
-    if cs.n0 > 0:
+    d["n0"] = np.int64(cs.n0 if cs is not None else 0)
+    d["coarse_offsets"] = np.ascontiguousarray(cs.offsets if cs is not None else np.zeros(len(setup.subs) + 1), np.int64)
+    if cs is not None and cs.n0 > 0:
         blocks = []
         for sd, V in zip(setup.subs, cs.V):
             rows = V[:sd.n_owned] if sd.boolean else V
@@ -235,8 +237,9 @@ class HipContext:
 
     # ------------------------------------------------------------------ uploads
     def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
-                     correction: str | None = None, tasks_per_subdomain: int | None = None):
-        """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products)."""
+                     correction: str | None = None, tasks_per_subdomain: int | None = None, coarse: bool = True):
+        """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products);
+        ``coarse=False`` stops after the local factors (a setup whose coarse space is not built yet)."""
         if tasks_per_subdomain is not None:
             self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
         d = setup_arrays(setup, precision, one_level, correction)
@@ -251,7 +254,8 @@ class HipContext:
         for i, f in enumerate(setup.factors):
             self.upload_factor(i, f)
         self.finalize(precision)
-        self.upload_coarse(setup, d["correction"])
+        if coarse:
+            self.upload_coarse(setup, d["correction"])
         self._keep = d
 
     def upload_factor(self, i, f):
@@ -295,6 +299,24 @@ class HipContext:
         """s2l_refactorize: new values of A in the order of the uploaded arrays (host memory)."""
         val = np.ascontiguousarray(val, np.float64)
         self._check(self.L.s2l_refactorize(self.h, _p(val, ctypes.c_double), 1))
+
+    def local_eigensolve(self, nev: int, ncv: int, shift_rel: float = 1e-10, tol: float = 1e-8,
+                         max_restarts: int = 100, skip=None):
+        """s2l_local_eigensolve: the nev largest eigenpairs of B_i u = lambda (A~_ii + sigma_i I) u of
+        every subdomain not in ``skip`` (a boolean per subdomain). Returns (lam nsub x nev,
+        Z nev x local_total, converged per subdomain, info [restarts, steps, attempts, converged])."""
+        lptr = self._keep["local_ptr"]
+        nsub, nl = len(lptr) - 1, int(lptr[-1])
+        lam = np.zeros((nsub, nev))
+        Z = np.zeros((nev, nl))
+        conv = np.zeros(nsub, np.int32)
+        info = np.zeros(4, np.int32)
+        sk = None if skip is None else np.ascontiguousarray(skip, np.int32)
+        self._check(self.L.s2l_local_eigensolve(self.h, nev, ncv, shift_rel, tol, max_restarts,
+                                                None if sk is None else _p(sk, ctypes.c_int32), _p(lam, ctypes.c_double),
+                                                _p(Z, ctypes.c_double), _p(conv, ctypes.c_int32),
+                                                _p(info, ctypes.c_int32)))
+        return lam, Z, conv, info
 
     # ------------------------------------------------------------------ operations (host memory)
     def apply(self, r: np.ndarray) -> np.ndarray:

@@ -650,10 +650,57 @@ class SchwarzSetup:
         return self.dec.A.shape[0]
 
 
+def device_eigenvectors(dec: Decomposition, subs: list, factors: list, nev: int, tau: float = 0.6, tol: float = 1e-8,
+                        shift_rel: float = 1e-10, dense_threshold: int = 400, max_restarts: int = 100,
+                        device_index: int = 0) -> tuple[list, dict]:
+    """Z_i of every subdomain from the device eigensolver (s2l_local_eigensolve: the operator
+    (A~_ii + sigma_i I)^{-1} B_i of ProjectedOperator, local_eigensolver.cpp:174-211, applied by
+    batched device solves inside one thick-restart Lanczos process per subdomain), selected as
+    local_eigenvectors does (|lambda| >= 1/tau among the nev largest, unit 2-norm). Subdomains of at
+    most ``dense_threshold`` rows take the dense host path, as in the reference
+    (local_eigensolver.cpp:223-225); a subdomain whose Lanczos process did not converge within
+    ``max_restarts`` is solved again on the host. Needs symmetric A (Cholesky local factors)."""
+    from . import device
+    if not dec.symmetric or any(f.kind != 0 for f in factors):
+        raise ValueError("the device eigensolver needs a symmetric matrix (Cholesky local factors)")
+    n = len(subs)
+    if nev <= 0:
+        return [np.zeros((sd.n_local, 0)) for sd in subs], {}
+    ncv = max(2 * nev + 1, nev + 20)
+    small = np.array([sd.n_local <= dense_threshold or ncv >= sd.n_local for sd in subs], np.int32)
+    stats = {"device_subdomains": int(n - small.sum())}
+    Zs = [None] * n
+    if small.sum() < n:
+        t0 = time.perf_counter()
+        ctx = device.HipContext(device_index)
+        try:
+            ctx.upload_setup(SchwarzSetup(dec, subs, factors, None), "double", coarse=False)
+            t1 = time.perf_counter()
+            lam, Zall, conv, info = ctx.local_eigensolve(nev, ncv, shift_rel, tol, max_restarts, skip=small)
+            stats.update(upload_s=t1 - t0, eigensolve_s=time.perf_counter() - t1, restarts=int(info[0]),
+                         lanczos_steps=int(info[1]), shift_attempts=int(info[2]), converged=int(info[3]))
+            lptr = ctx._keep["local_ptr"]
+        finally:
+            ctx.close()
+        for i, sd in enumerate(subs):
+            if small[i] or not conv[i]:
+                continue
+            U = Zall[:, lptr[i]:lptr[i + 1]].T
+            vecs = _select(lam[i], U, nev, tau)
+            Zs[i] = (np.stack([v / (np.linalg.norm(v) or 1.0) for v in vecs], axis=1) if vecs
+                     else np.zeros((sd.n_local, 0)))
+    stats["host_subdomains"] = sum(z is None for z in Zs)
+    for i, sd in enumerate(subs):
+        if Zs[i] is None:
+            Zs[i] = local_eigenvectors(sd, nev, tau, True, tol=tol)
+    return Zs, stats
+
+
 def build_setup(A, b, nparts, *, xyz=None, symmetric=None, block_size=1, overlap=1, pou="boolean",
                 nev=20, tau=0.6, factorization="auto", one_level="ras", correction="deflated",
-                eig_workers=0, eig_tol=1e-8, part=None, leaf=0) -> SchwarzSetup:
-    """Full host setup of the two-level preconditioner on ``nparts`` subdomains."""
+                eig_workers=0, eig_tol=1e-8, part=None, leaf=0, eig_backend: str = "host") -> SchwarzSetup:
+    """Full host setup of the two-level preconditioner on ``nparts`` subdomains. ``eig_backend``
+    "device" solves the local eigenproblems on the GPU (symmetric matrices; device_eigenvectors)."""
     t = {}
     t0 = time.perf_counter()
     A = sp.csr_matrix(A)
@@ -672,8 +719,16 @@ def build_setup(A, b, nparts, *, xyz=None, symmetric=None, block_size=1, overlap
     factors = factorize_all(subs, kind, leaf=leaf)
     t["factorization"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    coarse = build_coarse(dec, subs, nev, tau, eig_workers, eig_tol) if correction != "none" else \
-        CoarseSpace([np.zeros((s.n_local, 0)) for s in subs], np.zeros(len(subs) + 1, np.int64), np.zeros((0, 0)))
+    if correction == "none":
+        coarse = CoarseSpace([np.zeros((s.n_local, 0)) for s in subs], np.zeros(len(subs) + 1, np.int64), np.zeros((0, 0)))
+    elif eig_backend == "device":
+        Z, st = device_eigenvectors(dec, subs, factors, nev, tau, eig_tol)
+        t["eigensolver_device"] = st
+        coarse = build_coarse(dec, subs, nev, tau, eig_workers, eig_tol, Z=Z)
+    elif eig_backend == "host":
+        coarse = build_coarse(dec, subs, nev, tau, eig_workers, eig_tol)
+    else:
+        raise ValueError(f"eig_backend must be 'host' or 'device', not {eig_backend!r}")
     t["coarse"] = time.perf_counter() - t0
     return SchwarzSetup(dec, subs, factors, coarse, one_level, correction if correction != "none" else "additive", t)
 

@@ -107,6 +107,27 @@ int s2l_upload_coarse(s2l_ctx* ctx, int32_t n0, const int64_t* offsets, const do
  * a factor loses positive definiteness. The first call builds the symbolic plan. */
 int s2l_refactorize(s2l_ctx* ctx, const double* val, int host);
 
+/* Local eigenproblems of the coarse space on the device (LocalEigensolver::compute and
+ * computeSparse, src/preconditioner/local_eigensolver.cpp:215-226 and :277-421, the operator of
+ * ProjectedOperator :174-211): for every subdomain i not marked in @p skip (may be NULL), the @p nev
+ * largest eigenpairs of B_i u = lambda (A~_ii + sigma_i I) u with B_i = D_i A_ii D_i, A~_ii the
+ * lumped block (src/preconditioner/local_block_splitting.cpp:9-21) and sigma_i = shift_rel *
+ * sqrt(|A~_ii|_1 |A~_ii|_inf), multiplied by 1e3 while a factorization fails (four attempts, :285-303).
+ * The values of A are the context's current ones (upload or refactorize). A~_ii + sigma_i I is
+ * factored on the device over the structure of A_ii's Cholesky factor; every subdomain runs a
+ * thick-restart Lanczos process (basis @p ncv, restarted with (nev + ncv) / 2 Ritz vectors, at most
+ * @p max_restarts restarts, full reorthogonalization in the A~ + sigma I inner product) until the
+ * Ritz estimates |beta_m s_mk| of the nev largest are <= tol |theta_k|. The kernel of A~_ii is not
+ * detected separately: its directions with B q != 0 get eigenvalues of order 1/sigma and come first.
+ * Outputs (host memory): @p lambda nsub x nev (decreasing), @p Z the Ritz vectors (nev columns of
+ * local_total rows: column c of subdomain i at Z[c * local_total + local_ptr[i] + l], normalized in
+ * the A~ + sigma I norm), @p converged per subdomain (may be NULL), @p info [restarts, Lanczos steps,
+ * factorization attempts, subdomains converged] (may be NULL). Needs symmetric A (Cholesky local
+ * factors), FP64 local factors (precision 0), the task-queue schedule, 1 <= nev, nev + 2 <= ncv <= 96
+ * and ncv + 1 rows in every subdomain solved here. */
+int s2l_local_eigensolve(s2l_ctx* ctx, int32_t nev, int32_t ncv, double shift_rel, double tol, int32_t max_restarts,
+                         const int32_t* skip, double* lambda, double* Z, int32_t* converged, int32_t* info);
+
 /* z = M^{-1} r for @p nrhs right-hand sides (device pointers, n x nrhs). Replaces
  * Preconditioner::apply (include/preconditioner/preconditioner.hpp:73) as implemented by
  * TwoLevelSchwarz::apply (src/preconditioner/two_level_schwarz.cpp:134-163). @p stream: a
