@@ -257,6 +257,7 @@ struct s2l_ctx {
 
     // device refactorization (s2l_refactorize): its plan, device tables and work arrays
     bool rf_ready = false;
+    bool ls_built = false;  ///< the level-synchronous layout exists (built for launch mode 2 only)
     bool ls_stale = false;  ///< the level-synchronous layout holds older factor values
     RfPlan rf;
     bool rf_planned = false;            ///< rf holds the plan of the current factors and pattern
@@ -1268,6 +1269,45 @@ int bicgImpl(s2l_ctx* c, const s2l_krylov_opts* o, const double* d_b, double* d_
     }
     return S2L_OK;
 }
+// The level-synchronous schedule (launch mode 2) of the uploaded factors with values in @p precision.
+void buildLs(s2l_ctx* c, int precision)
+{
+    c->ls = buildLsPlan(c->hf, c->local_ptr);
+    {
+        LsPlan& L = c->ls;
+        c->ls_sn.upload(L.sn, c->stream);
+        c->ls_units.upload(L.units, c->stream);
+        c->ls_items.upload(L.items, c->stream);
+        c->ls_pull.upload(L.pull_rows, c->stream);
+        c->ls_ridx.upload(L.ridx, c->stream);
+        c->ls_cptr.upload(L.cptr, c->stream);
+        c->ls_cidx.upload(L.cidx, c->stream);
+        std::vector<int32_t> owned(static_cast<size_t>(c->nsub)), begin(static_cast<size_t>(c->nsub));
+        for (int i = 0; i < c->nsub; ++i) {
+            owned[static_cast<size_t>(i)] = c->ownership[i + 1] - c->ownership[i];
+            begin[static_cast<size_t>(i)] = c->ownership[i];
+        }
+        c->ls_owned.upload(owned, c->stream);
+        c->ls_begin.upload(begin, c->stream);
+        auto values = [&](DevBuf<unsigned char>& dst, std::vector<double>& src) {
+            if (precision != 0) {
+                std::vector<float> f(src.begin(), src.end());
+                dst.upload(reinterpret_cast<const unsigned char*>(f.data()), f.size() * sizeof(float), c->stream);
+                HIPCHECK(hipStreamSynchronize(c->stream));
+            } else {
+                dst.upload(reinterpret_cast<const unsigned char*>(src.data()), src.size() * sizeof(double), c->stream);
+                HIPCHECK(hipStreamSynchronize(c->stream));
+            }
+            std::vector<double>().swap(src);
+        };
+        values(c->ls_diag, L.diag);
+        values(c->ls_bf, L.belowF);
+        values(c->ls_bb, L.belowB);
+    }
+    c->ls_built = true;
+    c->work_nrhs = 0;  // the scratch of the level-synchronous schedule is sized again
+}
+
 // The plan of the device refactorization and its device tables (built once per pattern, subdomains
 // and factors; the host part usually comes from the thread started by finalize_local).
 void ensureRfPlan(s2l_ctx* c, hipStream_t s)
@@ -1562,39 +1602,10 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         HIPCHECK(hipSetDevice(c->device));
         std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
         const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type (LDS)
-        // level-synchronous schedule (launch mode 2)
-        c->ls = buildLsPlan(c->hf, c->local_ptr);
-        {
-            LsPlan& L = c->ls;
-            c->ls_sn.upload(L.sn, c->stream);
-            c->ls_units.upload(L.units, c->stream);
-            c->ls_items.upload(L.items, c->stream);
-            c->ls_pull.upload(L.pull_rows, c->stream);
-            c->ls_ridx.upload(L.ridx, c->stream);
-            c->ls_cptr.upload(L.cptr, c->stream);
-            c->ls_cidx.upload(L.cidx, c->stream);
-            std::vector<int32_t> owned(static_cast<size_t>(c->nsub)), begin(static_cast<size_t>(c->nsub));
-            for (int i = 0; i < c->nsub; ++i) {
-                owned[static_cast<size_t>(i)] = c->ownership[i + 1] - c->ownership[i];
-                begin[static_cast<size_t>(i)] = c->ownership[i];
-            }
-            c->ls_owned.upload(owned, c->stream);
-            c->ls_begin.upload(begin, c->stream);
-            auto values = [&](DevBuf<unsigned char>& dst, std::vector<double>& src) {
-                if (precision != 0) {
-                    std::vector<float> f(src.begin(), src.end());
-                    dst.upload(reinterpret_cast<const unsigned char*>(f.data()), f.size() * sizeof(float), c->stream);
-                    HIPCHECK(hipStreamSynchronize(c->stream));
-                } else {
-                    dst.upload(reinterpret_cast<const unsigned char*>(src.data()), src.size() * sizeof(double), c->stream);
-                    HIPCHECK(hipStreamSynchronize(c->stream));
-                }
-                std::vector<double>().swap(src);
-            };
-            values(c->ls_diag, L.diag);
-            values(c->ls_bf, L.belowF);
-            values(c->ls_bb, L.belowB);
-        }
+        // the level-synchronous schedule is built when launch mode 2 needs it
+        c->ls_built = false;
+        c->ls = LsPlan{};
+        for (DevBuf<unsigned char>* b : {&c->ls_diag, &c->ls_bf, &c->ls_bb}) b->release();
         // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
         // the subtree tasks of all subdomains run in one launch, one workgroup per compute unit
         const int tasks = c->tasks_per_sub > 0 ? c->tasks_per_sub : std::max(1, c->cus / std::max(1, c->nsub));
@@ -1628,6 +1639,7 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
                     "FP32 factors with FP64 arithmetic need the task queue, whose subdomain vector does not fit the LDS");
             c->launch_mode = 2;  // the task queue cannot hold these subdomains in LDS
         }
+        if (c->launch_mode == 2) buildLs(c, precision);
         // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
         std::vector<int32_t> gmap(c->gid_h.size()), lperm(c->gid_h.size());
         for (int i = 0; i < c->nsub; ++i) {
@@ -2583,6 +2595,7 @@ int s2l_set_local_launch_mode(s2l_ctx* c, int mode)
                 "launch modes 0 and 1 need a subdomain vector in the LDS of one workgroup");
         require(mode != 2 || !c->ls_stale, S2L_ESTATE,
                 "the level-synchronous layout predates the device refactorization; upload the factors again");
+        if (mode == 2 && c->local_ready && !c->ls_built) buildLs(c, c->precision);
         c->launch_mode = mode;
     });
 }

@@ -12,10 +12,12 @@
 #include "pack.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace s2l {
 
@@ -168,22 +170,56 @@ namespace {
 // Diagonal blocks as applied by the kernel: the inverse of the whole block, computed in FP64.
 // A small supernode's stays column-major (a thread per row walks it: coalesced over the rows); a
 // wide one's is stored row-major with the padded leading dimension wideLd(w), starting at a
-// multiple of four entries (a wavefront reads a row in 16-byte pieces).
-void appendDiagonal(std::vector<double>& out, std::vector<double>& B, int w, bool lower)
+// multiple of four entries (a wavefront reads a row in 16-byte pieces). The inversions (w^3 / 3
+// flops each, most of the packing time for wide separators) run after the packing loop, spread
+// over host threads: appendDiagonal reserves the block's place and keeps the block.
+struct DiagJob {
+    int64_t dst;
+    int32_t w;
+    bool lower;
+};
+
+inline std::size_t diagSize(int w) { return w <= kSmallWidth ? static_cast<std::size_t>(w) * w : static_cast<std::size_t>(w) * wideLd(w); }
+
+// the block's place in out holds the block itself (column-major) until runDiagonalJobs
+void appendDiagonal(std::vector<double>& out, std::vector<DiagJob>& jobs, const std::vector<double>& B, int w, bool lower)
 {
-    if (lower)
-        invertLower(B.data(), w, w);
-    else
-        invertUpper(B.data(), w, w);
-    if (w <= kSmallWidth) {
-        out.insert(out.end(), B.begin(), B.end());
-        return;
-    }
-    const int ld = wideLd(w);
-    for (int r = 0; r < w; ++r) {
-        for (int c = 0; c < w; ++c) out.push_back(B[static_cast<std::size_t>(c) * w + r]);
-        out.insert(out.end(), static_cast<std::size_t>(ld - w), 0.0);
-    }
+    jobs.push_back(DiagJob{static_cast<int64_t>(out.size()), w, lower});
+    out.resize(out.size() + diagSize(w), 0.0);
+    std::copy(B.begin(), B.end(), out.begin() + jobs.back().dst);
+}
+
+void runDiagonalJobs(std::vector<double>& out, std::vector<DiagJob>& jobs)
+{
+    // widest first, so that the long inversions do not end up last on one thread
+    std::stable_sort(jobs.begin(), jobs.end(), [](const DiagJob& a, const DiagJob& b) { return a.w > b.w; });
+    std::atomic<std::size_t> next{0};
+    auto work = [&] {
+        std::vector<double> B;
+        for (std::size_t k = next++; k < jobs.size(); k = next++) {
+            const DiagJob& j = jobs[k];
+            const int w = j.w;
+            B.assign(out.begin() + j.dst, out.begin() + j.dst + static_cast<int64_t>(w) * w);
+            if (j.lower)
+                invertLower(B.data(), w, w);
+            else
+                invertUpper(B.data(), w, w);
+            double* d = out.data() + j.dst;
+            if (w <= kSmallWidth) {
+                std::copy(B.begin(), B.end(), d);
+                continue;
+            }
+            const int ld = wideLd(w);
+            for (int r = 0; r < w; ++r)
+                for (int c = 0; c < w; ++c) d[static_cast<std::size_t>(r) * ld + c] = B[static_cast<std::size_t>(c) * w + r];
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const std::size_t nt = std::min<std::size_t>(hw, std::max<std::size_t>(1, jobs.size() / 64));
+    std::vector<std::thread> pool;
+    for (std::size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
 }
 
 // Pads a value array to a multiple of four entries (the start of a wide supernode's blocks).
@@ -197,6 +233,38 @@ void alignFour(std::vector<double>& v)
 Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off, int tasks_per_sub)
 {
     Packed P;
+    std::vector<DiagJob> jobs;  // diagonal blocks to invert
+    // supernodes of every factor first, so that the value arrays are allocated once
+    std::vector<std::vector<int32_t>> fstAll(fs.size()), bstAll(fs.size());
+    {
+        std::size_t ndiag = 0, nbelow = 0, nridx = 0, nsn = 0;
+        for (std::size_t i = 0; i < fs.size(); ++i) {
+            const HostFactor& F = fs[i];
+            fstAll[i] = detectSupernodes(F.n, F.lcp, F.lr);
+            if (!F.cholesky) bstAll[i] = detectSupernodes(F.n, F.ucp, F.ur);
+            auto add = [&](const std::vector<int32_t>& st, const std::vector<int64_t>& cp, bool below) {
+                for (std::size_t s = 0; s + 1 < st.size(); ++s) {
+                    const int32_t w = st[s + 1] - st[s];
+                    const int64_t h = cp[static_cast<std::size_t>(st[s]) + 1] - cp[static_cast<std::size_t>(st[s])] - w;
+                    ndiag += diagSize(w) + 3;
+                    if (below) {
+                        nbelow += static_cast<std::size_t>(h) * static_cast<std::size_t>(w > kSmallWidth ? wideLd(w) : w) + 3;
+                        nridx += static_cast<std::size_t>(h);
+                    }
+                }
+                nsn += st.size() - 1;
+            };
+            add(fstAll[i], F.lcp, true);
+            if (F.cholesky)
+                nsn += fstAll[i].size() - 1;
+            else
+                add(bstAll[i], F.ucp, true);
+        }
+        P.diag.reserve(ndiag);
+        P.below.reserve(nbelow);
+        P.ridx.reserve(nridx);
+        P.sn.reserve(nsn);
+    }
     // queue groups, keyed by (phase, depth): forward groups in decreasing depth, then backward
     std::map<std::pair<int, int>, std::vector<int32_t>> groups; // key (0, -depth) / (1, depth)
     std::vector<std::vector<int32_t>> perSub(fs.size());
@@ -214,7 +282,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         const std::vector<double>& uv = chol ? F.lv : F.uv;
 
         // ---------------- forward sweep (L)
-        const std::vector<int32_t> fst = detectSupernodes(n, F.lcp, F.lr);
+        const std::vector<int32_t>& fst = fstAll[i];
         const int32_t nsf = static_cast<int32_t>(fst.size()) - 1;
         std::vector<int32_t> snodeOf(static_cast<std::size_t>(n));
         for (int32_t s = 0; s < nsf; ++s)
@@ -243,7 +311,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             blk.assign(static_cast<std::size_t>(w) * w, 0.0);
             for (int32_t c = 0; c < w; ++c)
                 for (int32_t r = c; r < w; ++r) blk[static_cast<std::size_t>(c) * w + r] = F.lv[F.lcp[c0 + c] + (r - c)];
-            appendDiagonal(P.diag, blk, w, true);
+            appendDiagonal(P.diag, jobs, blk, w, true);
             if (wide) alignFour(P.below);
             d.below = static_cast<int64_t>(P.below.size());
             const int32_t ldb = wide ? wideLd(w) : w;
@@ -288,7 +356,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         }
 
         // ---------------- backward sweep (U, given by U^T)
-        const std::vector<int32_t> bst = chol ? fst : detectSupernodes(n, ucp, ur);
+        const std::vector<int32_t>& bst = chol ? fst : bstAll[i];
         const bool samePartition = (bst == fst);
         const int32_t nsb = static_cast<int32_t>(bst.size()) - 1;
         std::vector<int32_t> snodeB(static_cast<std::size_t>(n));
@@ -315,7 +383,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             blk.assign(static_cast<std::size_t>(w) * w, 0.0);
             for (int32_t c = 0; c < w; ++c)
                 for (int32_t r = 0; r <= c; ++r) blk[static_cast<std::size_t>(c) * w + r] = uv[ucp[c0 + r] + (c - r)];
-            appendDiagonal(P.diag, blk, w, false);
+            appendDiagonal(P.diag, jobs, blk, w, false);
             if (chol) {
                 d.below = belowOff[static_cast<std::size_t>(s)];
                 d.ridx = ridxOff[static_cast<std::size_t>(s)];
@@ -537,6 +605,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         P.max_levels_fwd = std::max(P.max_levels_fwd, td.f_nlev);
         P.max_levels_bwd = std::max(P.max_levels_bwd, td.b_nlev);
     }
+    runDiagonalJobs(P.diag, jobs);
     return P;
 }
 

@@ -16,7 +16,7 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
                   precision: str = "single", restart: int = 50, rtol: float = 1e-8, maxit: int = 1000,
                   seed: int = 12345, dt: float = 1.0, overlap: int = 1, pou: str = "boolean", nev: int = 8,
                   log=None, matrix: str | None = None, coordinates: bool = True, krylov: str = "gmres",
-                  eig_workers: int = 0, device_refactor: bool = True) -> list:
+                  eig_workers: int = 0, device_refactor: bool = True, eig_backend: str = "host") -> list:
     """Runs steps x correctors solves; returns one record per solve (step, corrector, the level
     performed before it, iterations, convergence, true relative residual, update and solve times).
     With ``matrix`` (a file name or a step pattern) the matrices are read, not generated; a step
@@ -25,7 +25,8 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
     ``krylov``: "gmres" (GMRES(restart)) or "bicgstab". ``eig_workers``: processes of the local
     eigenproblems of a rebuild (forked only while this process has no GPU context).
     ``device_refactor``: refactorize on the device (s2l_refactorize) when the factors allow it
-    (Cholesky local factors and a symmetric A_00), else on the host."""
+    (Cholesky local factors and a symmetric A_00), else on the host. ``eig_backend``: "device" solves
+    the local eigenproblems of a rebuild on the GPU (symmetric matrices), "host" with ARPACK."""
     gen_kw = dict(gen_kw or {})
     policy = ReusePolicy(reuse, growth, max_age)
     setup = M = None
@@ -44,7 +45,8 @@ def run_time_loop(family: str = "bubbles3d", gen_kw: dict | None = None, nparts:
             t0 = time.perf_counter()
             if level == REBUILD:
                 setup = hs.build_setup(A, b, nparts, xyz=xyz, symmetric=sym, block_size=bs, overlap=overlap, pou=pou,
-                                       nev=nev, eig_workers=eig_workers)
+                                       nev=nev, eig_workers=eig_workers,
+                                       eig_backend=eig_backend if sym else "host")
                 M = device.HipTwoLevelSchwarz(setup, precision)
             elif level == REFACTORIZE:
                 if device_refactor and sym and setup.factors[0].kind == 0:

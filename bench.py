@@ -241,6 +241,7 @@ def reuse_main(args):
         recs = run_time_loop("bubbles3d", kw, nparts=nparts, steps=args.seq_steps, correctors=2, reuse=args.reuse_policy,
                              precision="single", restart=50, rtol=args.rtol, nev=8, krylov=krylov,
                              eig_workers=args.setup_workers or 16, device_refactor=not args.host_refactor,
+                             eig_backend="host" if args.host_eigensolver else "device",
                              log=lambda r: log(f"{krylov} step {r['step']} corrector {r['corrector']}: {r['level']} "
                                                f"{r['t_update']:.3f} s + {1e3 * r['t_solve']:.1f} ms, "
                                                f"{r['iterations']} its"))
@@ -256,6 +257,7 @@ def reuse_main(args):
                           "refactorize_s_first": round(refac[0], 3) if refac else None,
                           "refactorize_s_median": round(float(np.median(refac)), 3) if refac else None,
                           "refactorize_on": "host" if args.host_refactor else "device",
+                          "eigensolver_on": "host" if args.host_eigensolver else "device",
                           "solve_ms_mean": round(1e3 * sol / len(recs), 2),
                           "all_converged": all(r["converged"] for r in recs),
                           "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
@@ -269,12 +271,14 @@ def reuse_main(args):
                                   f"bubbles of radii 3-8 cells rising 1 cell per step, "
                                   f"{args.seq_steps} steps x 2 correctors, {nparts} subdomains, overlap 1, nev 8, "
                                   f"deflated, reuse policy '{args.reuse_policy}', refactorizations on the "
-                                  f"{'host' if args.host_refactor else 'device'}",
+                                  f"{'host' if args.host_refactor else 'device'}, local eigenproblems of the rebuild "
+                                  f"on the {'host' if args.host_eigensolver else 'device'}",
                       "parallelism": "single GPU, all subdomains resident"}}
     for krylov in ("gmres", "bicgstab"):
         cmd = [sys.executable, os.path.abspath(__file__), "--reuse-sequence", "--reuse-krylov", krylov, "--seq-nx",
                str(args.seq_nx), "--seq-steps", str(args.seq_steps), "--reuse-policy", args.reuse_policy, "--rtol",
-               str(args.rtol)] + (["--host-refactor"] if args.host_refactor else [])
+               str(args.rtol)] + (["--host-refactor"] if args.host_refactor else []) + \
+              (["--host-eigensolver"] if args.host_eigensolver else [])
         p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
         if p.returncode != 0:
             raise SystemExit(p.returncode)
@@ -333,6 +337,8 @@ def main():
                     help="reuse policy of the sequence")
     ap.add_argument("--reuse-krylov", default="", choices=("", "gmres", "bicgstab"),
                     help="(child run) the sequence of one Krylov method")
+    ap.add_argument("--host-eigensolver", action="store_true",
+                    help="solve the local eigenproblems of a rebuild on the host (ARPACK) instead of the device")
     ap.add_argument("--host-refactor", action="store_true",
                     help="refactorize on the host (host factorization, A_00 on the host, upload) instead of the device")
     args = ap.parse_args()
@@ -363,7 +369,9 @@ def main():
     workers = args.setup_workers or min(16, os.cpu_count() or 1)
     st = hs.build_setup(A, b0, N, xyz=xyz, symmetric=sym, overlap=overlap, nev=nev, eig_workers=workers,
                         eig_tol=1e-6, correction=correction,
-                        factorization="lu" if args.workload == "poisson2d-256" else "auto", leaf=args.leaf)
+                        factorization="lu" if args.workload == "poisson2d-256" else "auto", leaf=args.leaf,
+                        eig_backend="device" if (sym and args.workload != "poisson2d-256" and not args.host_eigensolver)
+                        else "host")
     t_setup = time.perf_counter() - t0
     log(f"setup {t_setup:.1f} s {st.times}; n0 = {st.coarse.n0}")
 
@@ -599,6 +607,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "setup_s": round(t_setup, 2),
+        "setup_stages_s": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.times.items()},
     }
     print(json.dumps(out), flush=True)
 
