@@ -251,6 +251,7 @@ struct s2l_ctx {
     DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
     std::vector<int32_t> qsub_h;
     int tasks_per_sub = 0;  // 0: one subtree task per compute unit (compute units / subdomains)
+    int dense_top_max = kDenseTopMax;  // widest skeleton replaced by a dense top (0: none)
     int cus = 0;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
@@ -297,6 +298,12 @@ struct s2l_ctx {
     DevBuf<int64_t> rf_pos, rf_invoff;
     DevBuf<double> rf_work, rf_inv, rf_raw, rf_E, rf_Y, rf_X, rf_AX, rf_C, rf_part, rf_loc;
     rocblas_handle rf_blas = nullptr;
+    // dense tops of the refactorized factors: their tables, L_SS / S^-1 scratch
+    DevBuf<DenseTop> dt_d;
+    DevBuf<int32_t> dt_rf_d, dt_q_d;
+    DevBuf<int64_t> dt_off_d;
+    DevBuf<double> dt_L, dt_C;
+    DevBuf<rocblas_int> dt_info;
     double rf_ms[4] = {};   ///< last refactorization: values, local factors, A_00, total [ms]
 
     // coarse
@@ -1328,9 +1335,46 @@ void ensureRfPlan(s2l_ctx* c, hipStream_t s)
     c->rf_inpos.upload(c->inpos_h, s);
     c->rf_work.reserve(static_cast<size_t>(std::max<int64_t>(R.work, 1)));
     c->rf_inv.reserve(static_cast<size_t>(std::max<int64_t>(R.inv, 1)));
+    if (!c->pk.dtop.empty()) {
+        c->dt_d.upload(c->pk.dtop, s);
+        c->dt_rf_d.upload(R.dtop_rf, s);
+        c->dt_q_d.upload(c->pk.dtop_q, s);
+        c->dt_off_d.upload(R.dtop_off, s);
+        c->dt_L.reserve(static_cast<size_t>(R.dtop_total));
+        c->dt_C.reserve(static_cast<size_t>(R.dtop_total));
+        c->dt_info.reserve(1);
+    }
     c->rf_status.reserve(1);
     HIPCHECK(hipStreamSynchronize(s));
     c->rf_ready = true;
+}
+
+// The dense tops of refactorized factors (after launchRfPack): L_SS assembled from the panels,
+// X = L_SS^-1 (rocSOLVER), S^-1 = X^T X (rocBLAS), then the solve layout in @p diag.
+void rfDenseTops(s2l_ctx* c, const RfArgs& ra, void* diag, bool fp32, hipStream_t s)
+{
+    const std::vector<DenseTop>& D = c->pk.dtop;
+    if (D.empty()) return;
+    const RfPlan& R = c->rf;
+    const int32_t nd = static_cast<int32_t>(D.size());
+    launchDtAssemble(ra, c->dt_d.p, nd, c->dt_rf_d.p, c->dt_q_d.p, c->dt_off_d.p, c->dt_L.p, s);
+    HIPCHECK(hipGetLastError());
+    if (!c->rf_blas)
+        require(rocblas_create_handle(&c->rf_blas) == rocblas_status_success, S2L_EHIP, "dense tops: rocblas_create_handle failed");
+    require(rocblas_set_stream(c->rf_blas, s) == rocblas_status_success, S2L_EHIP, "dense tops: rocblas_set_stream failed");
+    const double one = 1.0, zero = 0.0;
+    for (int32_t t = 0; t < nd; ++t) {
+        const int S = D[static_cast<size_t>(t)].S;
+        double* X = c->dt_L.p + R.dtop_off[static_cast<size_t>(t)];
+        require(rocsolver_dtrtri(c->rf_blas, rocblas_fill_lower, rocblas_diagonal_non_unit, S, X, S, c->dt_info.p) ==
+                    rocblas_status_success,
+                S2L_EHIP, "dense tops: rocsolver_dtrtri failed");
+        require(rocblas_dgemm(c->rf_blas, rocblas_operation_transpose, rocblas_operation_none, S, S, S, &one, X, S, X, S, &zero,
+                              c->dt_C.p + R.dtop_off[static_cast<size_t>(t)], S) == rocblas_status_success,
+                S2L_EHIP, "dense tops: rocblas_dgemm failed");
+    }
+    launchDtLayout(c->dt_d.p, nd, c->dt_off_d.p, c->dt_C.p, diag, fp32, s);
+    HIPCHECK(hipGetLastError());
 }
 
 RfArgs rfArgs(s2l_ctx* c)
@@ -1609,7 +1653,7 @@ int s2l_finalize_local(s2l_ctx* c, int precision)
         // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
         // the subtree tasks of all subdomains run in one launch, one workgroup per compute unit
         const int tasks = c->tasks_per_sub > 0 ? c->tasks_per_sub : std::max(1, c->cus / std::max(1, c->nsub));
-        c->pk = packFactors(c->hf, moff, tasks);
+        c->pk = packFactors(c->hf, moff, tasks, c->dense_top_max);
         Packed& P = c->pk;
         for (int i = 0; i < c->nsub; ++i) {
             P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
@@ -1927,6 +1971,7 @@ int s2l_local_eigensolve(s2l_ctx* c, int32_t nev, int32_t ncv, double shift_rel,
             if (status == 0) break;
             require(attempt < 3, S2L_ENUMERIC, "local_eigensolve: cannot factorize A~_ii + sigma I (supernode " + std::to_string(status - 1) + ")");
         }
+        rfDenseTops(c, ra, c->eg_diag.p, false, s);
 
         // the operator's local solve: local input and output, the shifted factors
         c->ensureWork(1);
@@ -2048,6 +2093,7 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
         HIPCHECK(hipMemcpyAsync(&status, c->rf_status.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         require(status == 0, S2L_ENUMERIC, "refactorize: a local factor lost positive definiteness at supernode " + std::to_string(status - 1));
+        rfDenseTops(c, a, f32 ? static_cast<void*>(c->diag_f.p) : static_cast<void*>(c->diag_dd.p), f32, s);
         c->ls_stale = true;
         c->rf_ms[1] = ms_since(t_local);
         const auto t_coarse = clk::now();
@@ -2571,6 +2617,17 @@ int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
         if (c) c->invalidateGraphs();
         require(c && tasks >= 0, S2L_EINVAL, "tasks per subdomain must be >= 0 (0 = automatic)");
         c->tasks_per_sub = tasks;
+        c->local_ready = false;
+    });
+}
+
+int s2l_set_dense_top(s2l_ctx* c, int32_t max_width)
+{
+    return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
+        require(c && max_width >= 0 && max_width <= kDenseTopMax, S2L_EINVAL,
+                "dense_top: the widest dense top must be in [0, " + std::to_string(kDenseTopMax) + "]");
+        c->dense_top_max = max_width;
         c->local_ready = false;
     });
 }

@@ -331,6 +331,21 @@ __device__ void bwdWide(const LocalArgs<T, TF>& a, const SnDesc s, T* x, T* ybuf
     wideDiag(a.diag + s.diag, w, false, c0, x, ybuf, tid, lane, wave, nw, nt);
 }
 
+// Dense top, the rows [qlo, qhi) of part of a skeleton task: every row of the skeleton pulls the
+// products of the subtrees (b_S - L_SR y_R, staged in order in ybuf), then x_q = sum over c of
+// S^-1(q, c) b_c for the part's rows, four rows per wavefront pass with 16-byte loads of S^-1.
+template <typename T, typename TF>
+__device__ void denseTop(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, const int32_t* rows, int qlo, int qhi, T* x,
+                         T* ybuf, const T* tb, int tid, int lane, int wave, int nt)
+{
+    const int S = s.w, ld = wideLd(S), nw = nt / kWave;
+    for (int q = tid; q < S; q += nt) ybuf[q] = pullRow(a, sd, tb, x, rows[q]);
+    __syncthreads();
+    denseRows<T, TF, 0>(a.diag + s.diag + static_cast<int64_t>(qlo) * ld, ld, qhi - qlo, S, ybuf, lane, wave, nw,
+                        [&](int r, T v) { x[rows[qlo + r]] = v; });
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------ small supernodes of one level, all at once
 // Index of the last prefix entry <= t (prefix[0] = 0, nondecreasing).
 __device__ inline int findItem(const int* prefix, int count, int t)
@@ -443,7 +458,7 @@ __device__ inline int64_t uniform64(int64_t v)
 // every step ends with a barrier.
 template <typename T, typename TF>
 __device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0, int nsteps, bool forward, T* x, unsigned char* smem,
-                         T* tb, int tid, int lane, int wave, int nt)
+                         T* tb, const int32_t* rows, int qlo, int qhi, int tid, int lane, int wave, int nt)
 {
     T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
     SnDesc* st = reinterpret_cast<SnDesc*>(smem + a.st_off);
@@ -468,7 +483,9 @@ __device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0
                 d.diag = uniform64(sp.wide.diag);
                 d.below = uniform64(sp.wide.below);
                 d.ridx = uniform64(sp.wide.ridx);
-                if (forward)
+                if (uniform(sp.kind) == 2)
+                    denseTop(a, sd, d, rows, qlo, qhi, x, ybuf, tb, tid, lane, wave, nt);
+                else if (forward)
                     fwdWide(a, sd, d, x, ybuf, tb, tid, lane, wave, nt);
                 else
                     bwdWide(a, d, x, ybuf, tid, lane, wave, nt);
@@ -493,7 +510,8 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     const bool onB = kPair && rB < a.nrhs && !(a.skip && a.skip[rB]);
     if (!onA && !onB) return;
     const int code = a.queue[e % a.qcount];
-    const int tk = code >> 1;
+    const int tk = (code & ((1 << kPartShift) - 1)) >> 1;
+    const int part = code >> kPartShift;
     const bool backward = (code & 1) != 0;
     const TaskDesc td = a.task[tk];
     const bool combined = td.parent < 0;
@@ -501,6 +519,9 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     T* tb = a.tbuf + static_cast<int64_t>(r) * a.t_total + sd.t_off;
     T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
     const int32_t* rows = a.task_rows + td.rows_first;
+    // the rows this entry computes: all of the task's, or a dense top part's range
+    const int qlo = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * part / td.nparts) : 0;
+    const int qhi = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * (part + 1) / td.nparts) : td.rows_count;
     if (!backward) {
         const int64_t ld = a.local_in ? a.n_local_total : a.n_global;
         const double* rhs = a.rhs + static_cast<int64_t>(rA) * ld;
@@ -513,7 +534,7 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
                 x[row] = static_cast<T>(rhs[g]);
         }
         __syncthreads();
-        runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, tb, tid, lane, wave, nt);
+        runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, tb, rows, qlo, qhi, tid, lane, wave, nt);
         if (!combined) {
             for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
             __syncthreads();
@@ -525,9 +546,9 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
         for (int q = tid; q < td.ext_count; q += blockDim.x) x[ext[q]] = xg[ext[q]];
         __syncthreads();
     }
-    runSteps(a, sd, td.b_step, td.b_nsteps, false, x, smem, tb, tid, lane, wave, nt);
+    runSteps(a, sd, td.b_step, td.b_nsteps, false, x, smem, tb, rows, qlo, qhi, tid, lane, wave, nt);
     if (td.nchild > 0)
-        for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
+        for (int q = qlo + tid; q < qhi; q += blockDim.x) xg[rows[q]] = x[rows[q]];
     // the owned rows of the result (z = x0 + y), or the local buffer (scaled by D_i) for a reverse add
     auto store = [&](int rr, int row, double y) {
         const int l = a.lperm[sd.map_off + row];
@@ -540,7 +561,7 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
             a.out[static_cast<int64_t>(rr) * a.n_local_total + sd.map_off + l] = a.pou ? a.pou[sd.map_off + l] * y : y;
         }
     };
-    for (int q = tid; q < td.rows_count; q += blockDim.x) {
+    for (int q = qlo + tid; q < qhi; q += blockDim.x) {
         const int row = rows[q];
         if constexpr (kPair) {
             if (onA) store(rA, row, static_cast<double>(x[row].x));

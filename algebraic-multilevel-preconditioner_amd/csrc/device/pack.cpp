@@ -9,6 +9,12 @@
 // skeleton with the root). The queue lists the forward entries of all subtree tasks first, then
 // the top tasks (both sweeps in one entry), then the subtrees' backward entries, all subdomains
 // interleaved (a level-ordered work queue).
+//
+// Dense tops: the skeleton of a tree (the separators above its subtree tasks) is a chain of
+// dependent wide supernodes in both sweeps. When its widths sum to at most dense_top_max, the
+// skeleton is replaced by the dense inverse of its Schur complement, S^-1 = U_SS^-1 L_SS^-1: the
+// top task pulls the subtrees' products into its S rows and multiplies by S^-1 (both sweeps of the
+// skeleton in one product), its rows split over several workgroups. Its supernodes push nothing.
 #include "pack.hpp"
 
 #include <algorithm>
@@ -228,12 +234,92 @@ void alignFour(std::vector<double>& v)
     while (v.size() % 4 != 0) v.push_back(0.0);
 }
 
+// A dense top to compute: S^-1 of the skeleton rows @p rows (factor positions, increasing) of factor
+// @p f, row-major with leading dimension wideLd(S) at diag[dst].
+struct DenseJob {
+    int64_t dst;
+    int32_t f;
+    std::vector<int32_t> rows;
+};
+
+// The S x S block of a factor in compressed-column form on the rows @p rows (column-major, lower
+// triangular in their order); every entry of these columns must lie in these rows.
+std::vector<double> skeletonBlock(int32_t n, const std::vector<int64_t>& cp, const std::vector<int32_t>& row,
+                                  const std::vector<double>& val, const std::vector<int32_t>& rows)
+{
+    const int S = static_cast<int>(rows.size());
+    std::vector<int32_t> qOf(static_cast<std::size_t>(n), -1);
+    for (int q = 0; q < S; ++q) qOf[static_cast<std::size_t>(rows[static_cast<std::size_t>(q)])] = q;
+    std::vector<double> B(static_cast<std::size_t>(S) * S, 0.0);
+    for (int q = 0; q < S; ++q) {
+        const int32_t j = rows[static_cast<std::size_t>(q)];
+        for (int64_t p = cp[static_cast<std::size_t>(j)]; p < cp[static_cast<std::size_t>(j) + 1]; ++p) {
+            const int32_t qr = qOf[static_cast<std::size_t>(row[static_cast<std::size_t>(p)])];
+            if (qr < q) throw std::invalid_argument("dense top: a skeleton column has a row outside the skeleton");
+            B[static_cast<std::size_t>(q) * S + qr] = val[static_cast<std::size_t>(p)];
+        }
+    }
+    return B;
+}
+
+// S^-1 = U_SS^-1 L_SS^-1 in FP64 for every dense top, spread over host threads: X = L_SS^-1 and
+// Y = (U_SS^T)^-1 (Y = X for Cholesky), both lower triangular, and S^-1(i, j) = sum over k >=
+// max(i, j) of Y(k, i) X(k, j).
+void runDenseJobs(std::vector<double>& out, const std::vector<DenseJob>& jobs, const std::vector<HostFactor>& fs)
+{
+    std::atomic<std::size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::string error;
+    auto work = [&] {
+        try {
+            for (std::size_t k = next++; k < jobs.size() && !failed; k = next++) {
+                const DenseJob& jb = jobs[k];
+                const HostFactor& F = fs[static_cast<std::size_t>(jb.f)];
+                const int S = static_cast<int>(jb.rows.size()), ld = wideLd(S);
+                std::vector<double> X = skeletonBlock(F.n, F.lcp, F.lr, F.lv, jb.rows);
+                invertLower(X.data(), S, S);
+                std::vector<double> Y;
+                if (!F.cholesky) {
+                    Y = skeletonBlock(F.n, F.ucp, F.ur, F.uv, jb.rows);
+                    invertLower(Y.data(), S, S);
+                }
+                const std::vector<double>& Yc = F.cholesky ? X : Y;  // column-major: Y(k, i) at i S + k
+                std::vector<double> Xr(static_cast<std::size_t>(S) * S);   // row-major: X(k, j) at k S + j
+                for (int c = 0; c < S; ++c)
+                    for (int r = 0; r < S; ++r) Xr[static_cast<std::size_t>(r) * S + c] = X[static_cast<std::size_t>(c) * S + r];
+                std::vector<double> acc(static_cast<std::size_t>(S));
+                for (int i = 0; i < S; ++i) {
+                    std::fill(acc.begin(), acc.end(), 0.0);
+                    for (int kk = i; kk < S; ++kk) {
+                        const double y = Yc[static_cast<std::size_t>(i) * S + kk];
+                        if (y == 0.0) continue;
+                        const double* xr = Xr.data() + static_cast<std::size_t>(kk) * S;
+                        for (int j = 0; j <= kk; ++j) acc[static_cast<std::size_t>(j)] += y * xr[j];
+                    }
+                    std::copy(acc.begin(), acc.end(), out.begin() + jb.dst + static_cast<int64_t>(i) * ld);
+                }
+            }
+        } catch (const std::exception& e) {
+            if (!failed.exchange(true)) error = e.what();
+        }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const std::size_t nt = std::min<std::size_t>(hw, jobs.size());
+    std::vector<std::thread> pool;
+    for (std::size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    if (!jobs.empty()) work();
+    for (std::thread& t : pool) t.join();
+    if (failed) throw std::invalid_argument(error);
+}
+
 } // namespace
 
-Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off, int tasks_per_sub)
+Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>& map_off, int tasks_per_sub,
+                   int dense_top_max)
 {
     Packed P;
     std::vector<DiagJob> jobs;  // diagonal blocks to invert
+    std::vector<DenseJob> djobs; // dense tops to compute
     // supernodes of every factor first, so that the value arrays are allocated once
     std::vector<std::vector<int32_t>> fstAll(fs.size()), bstAll(fs.size());
     {
@@ -328,84 +414,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             P.entries_fwd += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
         }
         P.t_total += toff;
-        // contributions: the product of row k below supernode d goes to a slot of its target row;
-        // the slots of one row are contiguous (ordered by d), so a row pulls one contiguous range
-        {
-            std::vector<int32_t> cnt(static_cast<std::size_t>(n) + 1, 0);
-            for (int32_t s = 0; s < nsf; ++s) {
-                const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
-                for (int32_t k = 0; k < d.h; ++k) ++cnt[static_cast<std::size_t>(P.ridx[static_cast<std::size_t>(d.ridx + k)]) + 1];
-            }
-            sd.cptr_off = static_cast<int32_t>(P.cptr.size());
-            sd.dst_off = static_cast<int32_t>(P.cidx.size());
-            std::vector<int32_t> ptr(static_cast<std::size_t>(n) + 1, 0);
-            for (int32_t r = 0; r < n; ++r) ptr[static_cast<std::size_t>(r) + 1] = ptr[static_cast<std::size_t>(r)] + cnt[static_cast<std::size_t>(r) + 1];
-            for (int32_t r = 0; r <= n; ++r) P.cptr.push_back(ptr[static_cast<std::size_t>(r)]);
-            P.cidx.resize(static_cast<std::size_t>(sd.dst_off) + static_cast<std::size_t>(toff));
-            std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
-            for (int32_t s = 0; s < nsf; ++s) {
-                const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
-                for (int32_t k = 0; k < d.h; ++k) {
-                    const int32_t r = P.ridx[static_cast<std::size_t>(d.ridx + k)];
-                    P.cidx[static_cast<std::size_t>(sd.dst_off + d.toff + k)] = next[static_cast<std::size_t>(r)]++;
-                }
-            }
-            // slots read and written once, the slot table of every product, and per forward row
-            // its factor position and slot range (lev_row, lev_cb, lev_ce; slot pointers for wide ones)
-            P.bytes_index += static_cast<int64_t>(toff) * 4 + static_cast<int64_t>(n) * 12;
-        }
-
-        // ---------------- backward sweep (U, given by U^T)
+        // ---------------- task split (before the contributions: the supernodes of a dense top push nothing)
         const std::vector<int32_t>& bst = chol ? fst : bstAll[i];
         const bool samePartition = (bst == fst);
-        const int32_t nsb = static_cast<int32_t>(bst.size()) - 1;
-        std::vector<int32_t> snodeB(static_cast<std::size_t>(n));
-        for (int32_t s = 0; s < nsb; ++s)
-            for (int32_t j = bst[s]; j < bst[s + 1]; ++j) snodeB[static_cast<std::size_t>(j)] = s;
-        const int32_t b_base = static_cast<int32_t>(P.sn.size());
-        P.fwd_base.push_back(f_base);
-        P.fwd_count.push_back(nsf);
-        P.bwd_base.push_back(b_base);
-        std::vector<int32_t> parentB(static_cast<std::size_t>(nsb), -1);
-        for (int32_t s = 0; s < nsb; ++s) {
-            const int32_t c0 = bst[s], w = bst[s + 1] - c0;
-            const int64_t p0 = ucp[c0];
-            const int32_t h = static_cast<int32_t>(ucp[c0 + 1] - p0) - w;
-            if (w > kChunkRows) throw std::invalid_argument("supernode wider than " + std::to_string(kChunkRows) + " columns");
-            SnDesc d{};
-            d.c0 = c0;
-            d.w = w;
-            d.h = h;
-            const bool wide = w > kSmallWidth;
-            if (wide) alignFour(P.diag);
-            d.diag = static_cast<int64_t>(P.diag.size());
-            // G[c * w + r] = U(c0 + r, c0 + c) = U^T(c0 + c, c0 + r), r <= c
-            blk.assign(static_cast<std::size_t>(w) * w, 0.0);
-            for (int32_t c = 0; c < w; ++c)
-                for (int32_t r = 0; r <= c; ++r) blk[static_cast<std::size_t>(c) * w + r] = uv[ucp[c0 + r] + (c - r)];
-            appendDiagonal(P.diag, jobs, blk, w, false);
-            if (chol) {
-                d.below = belowOff[static_cast<std::size_t>(s)];
-                d.ridx = ridxOff[static_cast<std::size_t>(s)];
-            } else {
-                if (wide) alignFour(P.below);
-                d.below = static_cast<int64_t>(P.below.size());
-                const int32_t ldb = wide ? wideLd(w) : w;
-                for (int32_t k = 0; k < h; ++k) {
-                    for (int32_t c = 0; c < w; ++c) P.below.push_back(uv[ucp[c0 + c] + (w - c) + k]);
-                    P.below.insert(P.below.end(), static_cast<std::size_t>(ldb - w), 0.0);
-                }
-                d.ridx = static_cast<int64_t>(P.ridx.size());
-                for (int32_t k = 0; k < h; ++k) P.ridx.push_back(ur[p0 + w + k]);
-            }
-            if (h > 0) parentB[static_cast<std::size_t>(s)] = snodeB[static_cast<std::size_t>(ur[p0 + w])];
-            P.sn.push_back(d);
-            P.entries_bwd += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
-            P.bytes_index += static_cast<int64_t>(h) * 4;
-        }
-        P.bytes_index += static_cast<int64_t>(nsf + nsb) * static_cast<int64_t>(sizeof(SnDesc));
-
-        // ---------------- tasks
         std::vector<int32_t> depth(static_cast<std::size_t>(nsf), 0);
         for (int32_t s = nsf - 1; s >= 0; --s) {
             const int32_t p = parent[static_cast<std::size_t>(s)];
@@ -456,6 +467,102 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 }
             }
         }
+        // dense tops: the skeleton supernodes below every root that was cut, when their widths sum
+        // to at most dense_top_max (a skeleton is closed upwards: every supernode above a skeleton
+        // supernode belongs to it)
+        std::vector<char> dense(static_cast<std::size_t>(nsf), 0);
+        if (dense_top_max > 0) {
+            std::vector<int32_t> rootOf(static_cast<std::size_t>(nsf));
+            std::vector<int64_t> width(static_cast<std::size_t>(nsf), 0);
+            for (int32_t s = nsf - 1; s >= 0; --s) {
+                const int32_t p = parent[static_cast<std::size_t>(s)];
+                rootOf[static_cast<std::size_t>(s)] = p < 0 ? s : rootOf[static_cast<std::size_t>(p)];
+                if (skel[static_cast<std::size_t>(s)]) width[static_cast<std::size_t>(rootOf[static_cast<std::size_t>(s)])] += P.sn[static_cast<std::size_t>(f_base + s)].w;
+            }
+            for (int32_t s = 0; s < nsf; ++s)
+                dense[static_cast<std::size_t>(s)] = skel[static_cast<std::size_t>(s)] &&
+                                                     width[static_cast<std::size_t>(rootOf[static_cast<std::size_t>(s)])] <= dense_top_max;
+        }
+        // contributions: the product of row k below supernode d goes to a slot of its target row;
+        // the slots of one row are contiguous (ordered by d), so a row pulls one contiguous range.
+        // The supernodes of a dense top push nothing (their products are inside S^-1).
+        {
+            std::vector<int32_t> cnt(static_cast<std::size_t>(n) + 1, 0);
+            int64_t pushed = 0;
+            for (int32_t s = 0; s < nsf; ++s) {
+                if (dense[static_cast<std::size_t>(s)]) continue;
+                const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
+                for (int32_t k = 0; k < d.h; ++k) ++cnt[static_cast<std::size_t>(P.ridx[static_cast<std::size_t>(d.ridx + k)]) + 1];
+                pushed += d.h;
+            }
+            sd.cptr_off = static_cast<int32_t>(P.cptr.size());
+            sd.dst_off = static_cast<int32_t>(P.cidx.size());
+            std::vector<int32_t> ptr(static_cast<std::size_t>(n) + 1, 0);
+            for (int32_t r = 0; r < n; ++r) ptr[static_cast<std::size_t>(r) + 1] = ptr[static_cast<std::size_t>(r)] + cnt[static_cast<std::size_t>(r) + 1];
+            for (int32_t r = 0; r <= n; ++r) P.cptr.push_back(ptr[static_cast<std::size_t>(r)]);
+            P.cidx.resize(static_cast<std::size_t>(sd.dst_off) + static_cast<std::size_t>(toff), 0);
+            std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
+            for (int32_t s = 0; s < nsf; ++s) {
+                if (dense[static_cast<std::size_t>(s)]) continue;
+                const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
+                for (int32_t k = 0; k < d.h; ++k) {
+                    const int32_t r = P.ridx[static_cast<std::size_t>(d.ridx + k)];
+                    P.cidx[static_cast<std::size_t>(sd.dst_off + d.toff + k)] = next[static_cast<std::size_t>(r)]++;
+                }
+            }
+            // slots read and written once, the slot table of every product, and per forward row
+            // its factor position and slot range (lev_row, lev_cb, lev_ce; slot pointers for wide ones)
+            P.bytes_index += pushed * 4 + static_cast<int64_t>(n) * 12;
+        }
+
+        // ---------------- backward sweep (U, given by U^T)
+        const int32_t nsb = static_cast<int32_t>(bst.size()) - 1;
+        std::vector<int32_t> snodeB(static_cast<std::size_t>(n));
+        for (int32_t s = 0; s < nsb; ++s)
+            for (int32_t j = bst[s]; j < bst[s + 1]; ++j) snodeB[static_cast<std::size_t>(j)] = s;
+        const int32_t b_base = static_cast<int32_t>(P.sn.size());
+        P.fwd_base.push_back(f_base);
+        P.fwd_count.push_back(nsf);
+        P.bwd_base.push_back(b_base);
+        std::vector<int32_t> parentB(static_cast<std::size_t>(nsb), -1);
+        for (int32_t s = 0; s < nsb; ++s) {
+            const int32_t c0 = bst[s], w = bst[s + 1] - c0;
+            const int64_t p0 = ucp[c0];
+            const int32_t h = static_cast<int32_t>(ucp[c0 + 1] - p0) - w;
+            if (w > kChunkRows) throw std::invalid_argument("supernode wider than " + std::to_string(kChunkRows) + " columns");
+            SnDesc d{};
+            d.c0 = c0;
+            d.w = w;
+            d.h = h;
+            const bool wide = w > kSmallWidth;
+            if (wide) alignFour(P.diag);
+            d.diag = static_cast<int64_t>(P.diag.size());
+            // G[c * w + r] = U(c0 + r, c0 + c) = U^T(c0 + c, c0 + r), r <= c
+            blk.assign(static_cast<std::size_t>(w) * w, 0.0);
+            for (int32_t c = 0; c < w; ++c)
+                for (int32_t r = 0; r <= c; ++r) blk[static_cast<std::size_t>(c) * w + r] = uv[ucp[c0 + r] + (c - r)];
+            appendDiagonal(P.diag, jobs, blk, w, false);
+            if (chol) {
+                d.below = belowOff[static_cast<std::size_t>(s)];
+                d.ridx = ridxOff[static_cast<std::size_t>(s)];
+            } else {
+                if (wide) alignFour(P.below);
+                d.below = static_cast<int64_t>(P.below.size());
+                const int32_t ldb = wide ? wideLd(w) : w;
+                for (int32_t k = 0; k < h; ++k) {
+                    for (int32_t c = 0; c < w; ++c) P.below.push_back(uv[ucp[c0 + c] + (w - c) + k]);
+                    P.below.insert(P.below.end(), static_cast<std::size_t>(ldb - w), 0.0);
+                }
+                d.ridx = static_cast<int64_t>(P.ridx.size());
+                for (int32_t k = 0; k < h; ++k) P.ridx.push_back(ur[p0 + w + k]);
+            }
+            if (h > 0) parentB[static_cast<std::size_t>(s)] = snodeB[static_cast<std::size_t>(ur[p0 + w])];
+            P.sn.push_back(d);
+            P.entries_bwd += static_cast<int64_t>(w) * (w + 1) / 2 + static_cast<int64_t>(h) * w;
+            P.bytes_index += static_cast<int64_t>(h) * 4;
+        }
+        P.bytes_index += static_cast<int64_t>(nsf + nsb) * static_cast<int64_t>(sizeof(SnDesc));
+
         const int32_t t_base = static_cast<int32_t>(P.task.size());
         std::vector<int32_t> taskOf(static_cast<std::size_t>(nsf), -1);
         std::vector<int32_t> taskRoot;
@@ -514,6 +621,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             depthB[static_cast<std::size_t>(s)] = (p >= 0) ? depthB[static_cast<std::size_t>(p)] + 1 : 0;
         }
         std::vector<std::vector<int32_t>> children(static_cast<std::size_t>(nt));
+        int32_t qOff = -1;
         for (int32_t t = 0; t < nt; ++t) {
             const int32_t r = taskRoot[static_cast<std::size_t>(t)];
             TaskDesc td{};
@@ -531,6 +639,54 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 for (int32_t j = fst[s]; j < fst[s + 1]; ++j) P.task_rows.push_back(j);
             }
             td.rows_count = static_cast<int32_t>(P.task_rows.size()) - td.rows_first;
+            if (dense[static_cast<std::size_t>(r)]) {
+                // a dense top: one step x_S = S^-1 (b_S - pulled), no levels and no backward steps;
+                // its rows are split over nparts workgroups of at least 64 rows each
+                const int32_t S = td.rows_count;
+                td.nparts = std::max(1, std::min({tasks_per_sub, kMaxParts, (S + 63) / 64}));
+                td.f_lev = td.b_lev = static_cast<int32_t>(P.lev.size());
+                td.f_nlev = td.b_nlev = 0;
+                alignFour(P.diag);
+                StepDesc st{};
+                st.kind = 2;
+                st.W = S;
+                st.wide.w = S;
+                st.wide.diag = static_cast<int64_t>(P.diag.size());
+                td.f_step = static_cast<int32_t>(P.steps.size());
+                td.f_nsteps = 1;
+                P.steps.push_back(st);
+                td.b_step = static_cast<int32_t>(P.steps.size());
+                td.b_nsteps = 0;
+                td.ext_first = static_cast<int32_t>(P.ext_rows.size());
+                td.ext_count = 0;
+                P.diag.resize(P.diag.size() + static_cast<std::size_t>(S) * static_cast<std::size_t>(wideLd(S)), 0.0);
+                DenseTop dt{};
+                dt.sub = static_cast<int32_t>(i);
+                dt.task = t_base + t;
+                dt.S = S;
+                dt.sn_first = static_cast<int32_t>(P.dtop_sn.size());
+                dt.sn_count = static_cast<int32_t>(list.size());
+                for (int32_t sk : list) {
+                    P.dtop_sn.push_back(f_base + sk);
+                    const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + sk)];
+                    const int64_t e = static_cast<int64_t>(d.w) * (d.w + 1) / 2 + static_cast<int64_t>(d.h) * d.w;
+                    P.entries_skel_fwd += e;
+                    P.entries_skel_bwd += e;  // the same partition: the same entries backward
+                }
+                if (qOff < 0) {  // the subdomain's position -> row map, shared by its dense tops
+                    qOff = static_cast<int32_t>(P.dtop_q.size());
+                    P.dtop_q.resize(P.dtop_q.size() + static_cast<std::size_t>(n), -1);
+                }
+                dt.q_off = qOff;
+                for (int32_t q = 0; q < S; ++q) P.dtop_q[static_cast<std::size_t>(qOff + P.task_rows[static_cast<std::size_t>(td.rows_first + q)])] = q;
+                dt.sinv = st.wide.diag;
+                P.entries_dense += static_cast<int64_t>(S) * S;
+                P.dtop.push_back(dt);
+                djobs.push_back(DenseJob{dt.sinv, static_cast<int32_t>(i),
+                                         std::vector<int32_t>(P.task_rows.begin() + td.rows_first, P.task_rows.end())});
+                P.task.push_back(td);
+                continue;
+            }
             // forward levels: height inside the task
             int32_t hmax = 0;
             for (int32_t s : list) hmax = std::max(hmax, heightT[static_cast<std::size_t>(s)]);
@@ -580,8 +736,10 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             const TaskDesc& td = P.task[static_cast<std::size_t>(t_base + t)];
             const int32_t id = t_base + t;
             const bool leaf = td.nchild == 0;
-            groups[{0, leaf ? std::numeric_limits<int32_t>::min() : -td.depth}].push_back(id << 1);
-            if (td.parent >= 0) groups[{1, leaf ? std::numeric_limits<int32_t>::max() : td.depth}].push_back((id << 1) | 1);
+            if (id >= kMaxTasks) throw std::invalid_argument("more than " + std::to_string(kMaxTasks) + " local-solve tasks");
+            auto& fg = groups[{0, leaf ? std::numeric_limits<int32_t>::min() : -td.depth}];
+            for (int32_t part = 0; part < std::max(1, td.nparts); ++part) fg.push_back(queueCode(id, 0, part));
+            if (td.parent >= 0) groups[{1, leaf ? std::numeric_limits<int32_t>::max() : td.depth}].push_back(queueCode(id, 1, 0));
         }
         P.max_levels_fwd = std::max(P.max_levels_fwd, 0);
         P.sub.push_back(sd);
@@ -594,7 +752,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
     P.qgroup.push_back(static_cast<int32_t>(P.queue.size()));
     // per-subdomain slices in the same order
     std::vector<std::vector<int32_t>> slice(fs.size());
-    for (int32_t e : P.queue) slice[static_cast<std::size_t>(P.task[static_cast<std::size_t>(e >> 1)].sub)].push_back(e);
+    for (int32_t e : P.queue) slice[static_cast<std::size_t>(P.task[static_cast<std::size_t>(codeTask(e))].sub)].push_back(e);
     for (const auto& sl : slice) {
         P.qsub.push_back(static_cast<int32_t>(P.queue_sub.size()));
         P.queue_sub.insert(P.queue_sub.end(), sl.begin(), sl.end());
@@ -606,6 +764,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         P.max_levels_bwd = std::max(P.max_levels_bwd, td.b_nlev);
     }
     runDiagonalJobs(P.diag, jobs);
+    runDenseJobs(P.diag, djobs, fs);
     return P;
 }
 

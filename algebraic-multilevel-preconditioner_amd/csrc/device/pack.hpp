@@ -12,6 +12,15 @@ constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide a
 constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
 constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
 constexpr int32_t kMaxSkeleton = 16;       ///< most supernodes cut from the subtrees into the top task
+constexpr int32_t kDenseTopMax = 2048;     ///< widest skeleton replaced by a dense inverse (default limit)
+constexpr int32_t kMaxParts = 32;          ///< most workgroups sharing the rows of a dense top
+
+/// Queue entry codes: bit 0 the phase (0 forward or combined, 1 backward), bits 1-24 the task,
+/// bits 25-30 the part of a dense top task (each part computes a range of its rows).
+constexpr int kPartShift = 25;
+constexpr int32_t kMaxTasks = 1 << (kPartShift - 1);
+inline int32_t queueCode(int32_t task, int phase, int part) { return (part << kPartShift) | (task << 1) | phase; }
+inline int32_t codeTask(int32_t code) { return (code & ((1 << kPartShift) - 1)) >> 1; }
 
 /// Leading dimension of a wide supernode's blocks: w rounded up to a multiple of four entries.
 constexpr int32_t wideLd(int32_t w) { return (w + 3) & ~3; }
@@ -28,15 +37,18 @@ struct SnDesc {
 };
 
 /// One step of a task's sweep, in execution order: a chunk of small supernodes of one level, or one
-/// wide supernode. A chunk lists its supernodes' descriptors in level order (lev_desc[q0 ..
-/// q0 + cnt)), the prefix sums of their widths and heights (lev_wpre / lev_hpre minus wbase /
-/// hbase), and, in the forward sweep, the factor position and the pulled slot range [cb, ce) of
-/// each of its W rows at lev_row / lev_cb / lev_ce [roff, roff + W): everything a chunk reads
-/// before its first product is one load away from the step. A task's steps are staged in LDS.
+/// wide supernode, or (kind 2) the dense top: x_S = S^-1 (b_S - pulled products) over the S rows
+/// of a skeleton task, S^-1 the inverse of the skeleton's Schur complement (wide.w = S, wide.diag
+/// its row-major offset with leading dimension wideLd(S)).
+/// A chunk lists its supernodes' descriptors in level order (lev_desc[q0 .. q0 + cnt)), the prefix
+/// sums of their widths and heights (lev_wpre / lev_hpre minus wbase / hbase), and, in the forward
+/// sweep, the factor position and the pulled slot range [cb, ce) of each of its W rows at lev_row /
+/// lev_cb / lev_ce [roff, roff + W): everything a chunk reads before its first product is one load
+/// away from the step. A task's steps are staged in LDS.
 struct StepDesc {
-    int32_t kind;                           ///< 0 chunk of small supernodes, 1 wide supernode
+    int32_t kind;                           ///< 0 chunk of small supernodes, 1 wide supernode, 2 dense top
     int32_t q0, cnt, wbase, hbase, W, H, roff;
-    SnDesc wide;                            ///< kind 1: the wide supernode
+    SnDesc wide;                            ///< kind 1: the wide supernode; kind 2: w = S, diag = S^-1
 };
 
 /// Per-subdomain descriptor.
@@ -55,7 +67,7 @@ struct SubDesc {
 /// task without parent runs both sweeps in one entry.
 struct TaskDesc {
     int32_t sub;
-    int32_t f_lev, f_nlev;          ///< forward level table of the task's supernodes
+    int32_t f_lev, f_nlev;          ///< forward level table of the task's supernodes (none for a dense top)
     int32_t b_lev, b_nlev;          ///< backward level table
     int32_t sn_first, sn_count;     ///< the task's supernodes (indices into task_sn)
     int32_t ext_first, ext_count;   ///< rows of ancestor tasks read by the backward sweep
@@ -65,7 +77,17 @@ struct TaskDesc {
     int32_t rows_first, rows_count; ///< the task's rows (factor positions, indices into task_rows)
     int32_t f_step, f_nsteps;       ///< forward steps (indices into steps)
     int32_t b_step, b_nsteps;       ///< backward steps
-    int32_t pad;
+    int32_t nparts;                 ///< dense top: workgroups sharing its rows (one queue entry each); else 0
+};
+
+/// A dense top: the skeleton task of one elimination tree whose supernodes (all above the subtree
+/// tasks) are replaced by the dense inverse of their Schur complement. The inverse is
+/// U_SS^-1 L_SS^-1 (L_SS^-T L_SS^-1 for Cholesky), computed in FP64 and stored row-major.
+struct DenseTop {
+    int32_t sub, task, S;
+    int32_t sn_first, sn_count;     ///< its forward supernodes (global indices into sn), in dtop_sn
+    int32_t q_off;                  ///< offset of its subdomain's position -> row map in dtop_q
+    int64_t sinv;                   ///< offset of S^-1 in diag
 };
 
 /// Host copy of one uploaded factor (FP64 values).
@@ -86,7 +108,7 @@ struct Packed {
     std::vector<int32_t> task_child;
     std::vector<int32_t> ext_rows;    ///< factor positions
     std::vector<int32_t> task_rows;   ///< factor positions of every task's columns
-    std::vector<int32_t> queue;       ///< entries (task << 1 | phase), phase 0 forward/combined, 1 backward
+    std::vector<int32_t> queue;       ///< entries (queueCode: task, phase 0 forward/combined or 1 backward, part)
     std::vector<int32_t> qgroup;      ///< start of every depth group in the queue (+ end)
     std::vector<int32_t> qsub;        ///< start of every subdomain's slice in queue_sub (+ end)
     std::vector<int32_t> queue_sub;   ///< the same entries grouped by subdomain
@@ -102,6 +124,11 @@ struct Packed {
     std::vector<int32_t> cptr, cidx;  ///< forward contributions: slot ranges per row; slot of every product
     std::vector<SubDesc> sub;
     std::vector<int32_t> fwd_base, fwd_count, bwd_base; ///< per subdomain: its forward and backward supernodes in sn
+    std::vector<DenseTop> dtop;
+    std::vector<int32_t> dtop_sn;     ///< forward supernodes of the dense tops
+    std::vector<int32_t> dtop_q;      ///< per subdomain with a dense top: row of every factor position in its top (-1 outside)
+    int64_t entries_dense = 0;        ///< entries of the dense tops' inverses (S^2 each, read once per solve)
+    int64_t entries_skel_fwd = 0, entries_skel_bwd = 0; ///< factor entries of the supernodes they replace
     int64_t t_total = 0;
     int64_t entries_fwd = 0, entries_bwd = 0;
     int32_t max_levels_fwd = 0, max_levels_bwd = 0, max_n = 0;
@@ -118,7 +145,9 @@ void invertLower(double* B, int ld, int m);
 void invertUpper(double* B, int ld, int m);
 
 /// @p tasks_per_sub: wanted subtree tasks per subdomain (1 = one task per subdomain).
-Packed packFactors(const std::vector<HostFactor>& f, const std::vector<int32_t>& map_off, int tasks_per_sub);
+/// @p dense_top_max: widest skeleton (sum of its supernode widths) replaced by a dense top; 0 = none.
+Packed packFactors(const std::vector<HostFactor>& f, const std::vector<int32_t>& map_off, int tasks_per_sub,
+                   int dense_top_max = kDenseTopMax);
 
 } // namespace s2l
 

@@ -13,6 +13,7 @@ RfPlan buildRefactorPlan(const std::vector<HostFactor>& fs, const std::vector<in
 {
     RfPlan R;
     const int nsub = static_cast<int>(fs.size());
+    std::vector<int32_t> planBase(static_cast<std::size_t>(nsub), 0);  // plan index of every subdomain's first supernode
     const int32_t n_global = static_cast<int32_t>(rp.size()) - 1;
     std::vector<int32_t> g2l(static_cast<std::size_t>(n_global), -1);
     std::vector<int32_t> height;
@@ -25,6 +26,7 @@ RfPlan buildRefactorPlan(const std::vector<HostFactor>& fs, const std::vector<in
         const int32_t fb = P.fwd_base[static_cast<std::size_t>(i)], nsf = P.fwd_count[static_cast<std::size_t>(i)];
         const int32_t bb = P.bwd_base[static_cast<std::size_t>(i)];
         const int32_t base = static_cast<int32_t>(R.sn.size());
+        planBase[static_cast<std::size_t>(i)] = base;
         std::vector<int32_t> snodeOf(static_cast<std::size_t>(n));
         for (int32_t s = 0; s < nsf; ++s) {
             const SnDesc& d = P.sn[static_cast<std::size_t>(fb + s)];
@@ -148,6 +150,14 @@ RfPlan buildRefactorPlan(const std::vector<HostFactor>& fs, const std::vector<in
     R.order.assign(R.sn.size(), 0);
     std::vector<int32_t> next(R.level_ptr.begin(), R.level_ptr.end() - 1);
     for (std::size_t s = 0; s < R.sn.size(); ++s) R.order[static_cast<std::size_t>(next[static_cast<std::size_t>(height[s])]++)] = static_cast<int32_t>(s);
+    // dense tops: the plan index of each of their supernodes, and S x S scratch blocks
+    for (const DenseTop& dt : P.dtop) {
+        for (int32_t k = dt.sn_first; k < dt.sn_first + dt.sn_count; ++k)
+            R.dtop_rf.push_back(planBase[static_cast<std::size_t>(dt.sub)] + P.dtop_sn[static_cast<std::size_t>(k)] -
+                                P.fwd_base[static_cast<std::size_t>(dt.sub)]);
+        R.dtop_off.push_back(R.dtop_total);
+        R.dtop_total += static_cast<int64_t>(dt.S) * dt.S;
+    }
     return R;
 }
 

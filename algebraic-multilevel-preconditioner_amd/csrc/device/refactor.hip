@@ -154,6 +154,48 @@ template <typename TF> __global__ __launch_bounds__(kRfThreads) void k_rf_pack(R
     }
 }
 
+// L_SS of dense top blockIdx.x: its supernodes' diagonal blocks and rows below (all inside the
+// skeleton) at the skeleton rows dtop_q gives their factor positions.
+__global__ __launch_bounds__(kRfThreads) void k_dt_assemble(RfArgs a, const DenseTop* dt, const int32_t* dt_rf, const int32_t* dtop_q,
+                                                             const int64_t* off, double* Lall)
+{
+    const DenseTop t = dt[blockIdx.x];
+    const int S = t.S, tid = threadIdx.x, nt = blockDim.x;
+    double* L = Lall + off[blockIdx.x];  // L(q, c) at c S + q
+    const int32_t* qOf = dtop_q + t.q_off;
+    for (int64_t e = tid; e < static_cast<int64_t>(S) * S; e += nt) L[e] = 0.0;
+    __syncthreads();
+    for (int k = 0; k < t.sn_count; ++k) {
+        const RfSn r = a.sn[dt_rf[t.sn_first + k]];
+        const int w = r.w, h = r.h, q0 = qOf[r.c0];
+        const double* D = a.work + r.wd;  // L_ss(q, c) at c w + q
+        const double* B = a.work + r.wb;  // L_rs(k, c) at c h + k
+        const int32_t* R = a.ridx + r.ridx;
+        for (int64_t e = tid; e < static_cast<int64_t>(w) * (w + h); e += nt) {
+            const int c = static_cast<int>(e / (w + h)), q = static_cast<int>(e - static_cast<int64_t>(c) * (w + h));
+            if (q < w) {
+                if (q >= c) L[static_cast<int64_t>(q0 + c) * S + q0 + q] = D[static_cast<int64_t>(c) * w + q];
+            } else {
+                const int kk = q - w;
+                L[static_cast<int64_t>(q0 + c) * S + qOf[R[kk]]] = B[static_cast<int64_t>(c) * h + kk];
+            }
+        }
+    }
+}
+
+// S^-1 (column-major, symmetric blocks of C) into the solve layout of every dense top.
+template <typename TF> __global__ void k_dt_layout(const DenseTop* dt, const int64_t* off, const double* C, TF* diag)
+{
+    const DenseTop t = dt[blockIdx.y];
+    const int S = t.S, ld = wideLd(S);
+    const double* Ct = C + off[blockIdx.y];
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < static_cast<int64_t>(S) * ld;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int i = static_cast<int>(e / ld), j = static_cast<int>(e - static_cast<int64_t>(i) * ld);
+        diag[t.sinv + e] = j < S ? static_cast<TF>(Ct[static_cast<int64_t>(j) * S + i]) : TF(0);
+    }
+}
+
 __global__ void k_unit_columns(double* Y, int32_t n0, int32_t k0, int32_t cnt)
 {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -203,6 +245,22 @@ void launchRfPack(const RfArgs& a, int32_t nsn, void* diag, void* below, bool fp
     else
         hipLaunchKernelGGL(k_rf_pack<double>, dim3(nsn), dim3(kRfThreads), 0, s, a, static_cast<double*>(diag),
                            static_cast<double*>(below));
+}
+
+void launchDtAssemble(const RfArgs& a, const DenseTop* dt, int32_t ndt, const int32_t* dt_rf, const int32_t* dtop_q,
+                      const int64_t* off, double* L, hipStream_t s)
+{
+    if (ndt > 0) hipLaunchKernelGGL(k_dt_assemble, dim3(ndt), dim3(kRfThreads), 0, s, a, dt, dt_rf, dtop_q, off, L);
+}
+
+void launchDtLayout(const DenseTop* dt, int32_t ndt, const int64_t* off, const double* C, void* diag, bool fp32, hipStream_t s)
+{
+    if (ndt <= 0) return;
+    const dim3 grid(64, static_cast<unsigned>(ndt));
+    if (fp32)
+        hipLaunchKernelGGL(k_dt_layout<float>, grid, dim3(256), 0, s, dt, off, C, static_cast<float*>(diag));
+    else
+        hipLaunchKernelGGL(k_dt_layout<double>, grid, dim3(256), 0, s, dt, off, C, static_cast<double*>(diag));
 }
 
 void launchGather(const double* src, const int32_t* idx, double* dst, int64_t n, hipStream_t s)

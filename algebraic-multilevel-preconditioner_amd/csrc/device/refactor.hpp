@@ -46,6 +46,10 @@ struct RfPlan {
     int64_t work = 0;                ///< FP64 entries of the work array (panels)
     int64_t inv = 0;                 ///< FP64 entries of the inverse scratch (sum of w^2)
     std::vector<int64_t> inv_off;    ///< per supernode
+    // dense tops (Packed::dtop): their supernodes' plan indices and the offsets of their S x S blocks
+    std::vector<int32_t> dtop_rf;    ///< plan index of every entry of Packed::dtop_sn
+    std::vector<int64_t> dtop_off;   ///< per dense top: offset of its S x S scratch blocks
+    int64_t dtop_total = 0;          ///< FP64 entries of one scratch array of all dense tops
 };
 
 /// The plan of all subdomains' factors (Cholesky only). local_ptr / gid: the subdomains' maps;
@@ -80,6 +84,14 @@ void launchRfLevel(const RfArgs& a, const int32_t* order, int32_t count, hipStre
 /// Inverse of every L_ss and the solve layout of all supernodes: forward inverse (lower),
 /// backward inverse (its transpose), block below; @p fp32: float storage.
 void launchRfPack(const RfArgs& a, int32_t nsn, void* diag, void* below, bool fp32, hipStream_t s);
+
+/// Dense tops: L_SS of every dense top assembled from the refactorized panels into the column-major
+/// S x S blocks of @p L (zero outside the factor's entries).
+void launchDtAssemble(const RfArgs& a, const DenseTop* dt, int32_t ndt, const int32_t* dt_rf, const int32_t* dtop_q,
+                      const int64_t* off, double* L, hipStream_t s);
+/// S^-1 (column-major blocks of @p C) into the solve layout: row-major with leading dimension
+/// wideLd(S) at each top's offset of @p diag (FP32 when @p fp32).
+void launchDtLayout(const DenseTop* dt, int32_t ndt, const int64_t* off, const double* C, void* diag, bool fp32, hipStream_t s);
 
 /// dst[q] = src[idx[q]], q < n.
 void launchGather(const double* src, const int32_t* idx, double* dst, int64_t n, hipStream_t s);

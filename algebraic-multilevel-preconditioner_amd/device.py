@@ -60,7 +60,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
-           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve"]
+           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve", "s2l_set_dense_top"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -103,6 +103,7 @@ def hip_lib():
         "s2l_timing_read": [vp, P(ctypes.c_int64), P(ctypes.c_double), ctypes.c_int],
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
         "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
+        "s2l_set_dense_top": [vp, ctypes.c_int32],
         "s2l_set_graphs": [vp, ctypes.c_int],
         "s2l_solver_create": [vp, P(KrylovOpts), ctypes.c_int32, P(vp)],
         "s2l_solver_destroy": [vp],
@@ -237,11 +238,16 @@ class HipContext:
 
     # ------------------------------------------------------------------ uploads
     def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
-                     correction: str | None = None, tasks_per_subdomain: int | None = None, coarse: bool = True):
+                     correction: str | None = None, tasks_per_subdomain: int | None = None, coarse: bool = True,
+                     dense_top: int | None = None):
         """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products);
-        ``coarse=False`` stops after the local factors (a setup whose coarse space is not built yet)."""
+        ``coarse=False`` stops after the local factors (a setup whose coarse space is not built yet).
+        ``dense_top``: widest skeleton replaced by a dense top (s2l_set_dense_top; None = the
+        library's default, 0 = none)."""
         if tasks_per_subdomain is not None:
             self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
+        if dense_top is not None:
+            self._check(self.L.s2l_set_dense_top(self.h, dense_top))
         d = setup_arrays(setup, precision, one_level, correction)
         rp, col, val, own = d["rowptr"], d["col"], d["val"], d["ownership"]
         N = len(own) - 1
@@ -500,14 +506,15 @@ class HipTwoLevelSchwarz:
     """prec::Preconditioner of the reference, applied on the GPU (two_level_schwarz.cpp:134-163)."""
 
     def __init__(self, setup, precision: str = "double", device: int = 0, one_level: str | None = None,
-                 correction: str | None = None, tasks_per_subdomain: int | None = None):
+                 correction: str | None = None, tasks_per_subdomain: int | None = None, dense_top: int | None = None):
         self.setup = setup
         self.precision = precision
         self.one_level = one_level or setup.one_level
         self.correction = correction or setup.correction
         self.tasks_per_subdomain = tasks_per_subdomain
+        self.dense_top = dense_top
         self.ctx = HipContext(device)
-        self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain)
+        self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain, dense_top=dense_top)
 
     def apply(self, r):
         return self.ctx.apply(r)
@@ -543,7 +550,8 @@ class HipTwoLevelSchwarz:
         """Refactorize: uploads the operator, the local factors and A_00 of ``setup`` (new values of
         A with the same pattern and coarse basis, hostsetup.refactorize) into this context."""
         self.setup = setup
-        self.ctx.upload_setup(setup, self.precision, self.one_level, self.correction, self.tasks_per_subdomain)
+        self.ctx.upload_setup(setup, self.precision, self.one_level, self.correction, self.tasks_per_subdomain,
+                              dense_top=self.dense_top)
         return "refactorize"
 
     def update(self, level: str = "reuse", setup=None) -> str:

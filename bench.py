@@ -113,7 +113,8 @@ def measure_traffic(args):
         d = tempfile.mkdtemp(prefix="s2l_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
                os.path.abspath(__file__), "--profile-applies", str(PMC_APPLIES), "--profile-one-level", "--setup-workers", "1",
-               "--workload", args.workload, "--tasks", str(args.tasks)]
+               "--workload", args.workload, "--tasks", str(args.tasks)] + \
+              ([] if args.dense_top is None else ["--dense-top", str(args.dense_top)])
         log(f"traffic pass {counter}")
         try:
             p = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd="/tmp",
@@ -304,6 +305,8 @@ def main():
                     help="replay each GMRES iteration's apply + SpMV from a captured HIP graph (s2l_set_graphs)")
     ap.add_argument("--tasks", type=int, default=0,
                     help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
+    ap.add_argument("--dense-top", type=int, default=None,
+                    help="widest skeleton replaced by a dense top (s2l_set_dense_top; default: the library's, 0 = none)")
     ap.add_argument("--leaf", type=int, default=0,
                     help="largest nested-dissection leaf of the local factorizations (0 = the setup's default, 16)")
     ap.add_argument("--local-mode", type=int, default=None, choices=(0, 1, 2),
@@ -375,7 +378,7 @@ def main():
     t_setup = time.perf_counter() - t0
     log(f"setup {t_setup:.1f} s {st.times}; n0 = {st.coarse.n0}")
 
-    M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks)
+    M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
     ctx = M32.ctx
     ctx.set_graphs(args.graphs != 0)
     if args.local_mode is not None:
@@ -400,7 +403,7 @@ def main():
                "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}
         if args.profile_precisions:
             for prec in ("double", "mixed"):
-                Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks)
+                Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
                 Mq.ctx.set_graphs(False)
                 sq = Mq.ctx.stats()
                 ms_q, _ = time_applies(Mq.ctx, r, z, args.profile_applies, warm=PROFILE_WARM)
@@ -408,11 +411,11 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks)
+    M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
     ctx64 = M64.ctx
     ctx64.set_graphs(args.graphs != 0)
     # FP32 factor entries applied in FP64 arithmetic (no reference counterpart)
-    Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks)
+    Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
     ctxm = Mmx.ctx
     ctxm.set_graphs(args.graphs != 0)
     nsteps = args.steps + args.warmup

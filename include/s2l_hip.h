@@ -296,6 +296,17 @@ int s2l_set_local_launch_mode(s2l_ctx* ctx, int mode);
  * holds about one workgroup per compute unit. Takes effect at the next s2l_finalize_local(). */
 int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
 
+/* Dense tops (no reference counterpart; a layout of LocalSolver::solve,
+ * src/linalg/local_solver.cpp:119-126): when a subdomain is split into tasks, the skeleton above its
+ * subtree tasks (the top separators, a chain of dependent dense supernodes in both sweeps) is
+ * replaced by the dense inverse of its Schur complement S^-1 = U_SS^-1 L_SS^-1, computed in FP64 and
+ * stored in the factors' precision, whenever the skeleton holds at most @p max_width columns: the
+ * top task pulls the subtrees' products into its S rows and multiplies by S^-1, its rows split over
+ * as many workgroups as there are tasks per subdomain. Results agree with the sparse skeleton to
+ * rounding. 0 turns it off; at most 2048; default 2048. Takes effect at the next
+ * s2l_finalize_local(). */
+int s2l_set_dense_top(s2l_ctx* ctx, int32_t max_width);
+
 /* HIP graphs in the device Krylov loops (no counterpart in the reference, whose apply is a host
  * call): on, the kernels of one preconditioner apply and the following SpMV of a GMRES iteration
  * are captured once per input/output buffer pair and replayed as one graph launch; every upload,

@@ -159,10 +159,12 @@ def test_apply_bitwise_repeatable_and_launch_modes_agree(precision):
 def test_subtree_tasks_agree(precision):
     """Splitting every subdomain's elimination tree into subtree tasks (several workgroups per
     subdomain, several launches) changes the schedule, and with it only the rounding of the sums
-    below narrow chunks: 1 to 4 tasks per subdomain agree to rounding and with the oracle, on
-    Cholesky (Poisson) and LU (convection) factors, and every split is bitwise repeatable."""
+    below narrow chunks and the operator of the dense top (the skeleton's S^-1, whose rows depend
+    on the split; stored in FP32 by the single and mixed precisions): 1 to 4 tasks per subdomain
+    agree to rounding and with the oracle, on Cholesky (Poisson) and LU (convection) factors, and
+    every split is bitwise repeatable."""
     from amp_amd import device
-    tol = 2e-6 if precision == "single" else 1e-13
+    tol = 1e-13 if precision == "double" else 2e-6
     for name, kw, N in (("poisson3d", dict(nx=16), 8), ("convdiff2d", dict(nx=24), 4)):
         st = _setup(name, kw, N, nev=8, overlap=2)
         print(f"case {name} {kw} N={N} {precision}", flush=True)

@@ -1,0 +1,74 @@
+"""Dense tops (s2l_set_dense_top): the skeleton above a subdomain's subtree tasks replaced by the dense
+inverse of its Schur complement, S^-1 = U_SS^-1 L_SS^-1, its rows split over several workgroups.
+The local solves are those of the sparse skeleton up to rounding (LocalSolver::solve,
+src/linalg/local_solver.cpp:119-126): checked against the sparse skeleton, the oracle and the
+oracle's local solves, on Cholesky and LU factors, in every precision and launch mode."""
+import numpy as np
+import pytest
+
+from amp_amd import hostsetup as hs
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _setup(name, kw, N, **opts):
+    A, _, xyz, sym, bs = hs.generate(name, **kw)
+    return hs.build_setup(A, hs.random_rhs(A.shape[0], 12345), N, xyz=xyz, symmetric=sym, block_size=bs, **opts)
+
+
+@pytest.mark.parametrize("precision", ["double", "single", "mixed"])
+@pytest.mark.parametrize("name,kw,N", [("poisson3d", dict(nx=16), 8), ("convdiff2d", dict(nx=24), 4)])
+def test_dense_top_equals_sparse_skeleton(name, kw, N, precision):
+    from amp_amd import device
+    st = _setup(name, kw, N, nev=8, overlap=2)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    zo = orc.OracleSchwarz(st).apply(r)
+    for t in (2, 4):
+        print(f"case dense top {name} {kw} N={N} {precision} tasks {t}", flush=True)
+        Md = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t)
+        Ms = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t, dense_top=0)
+        zd, zs = Md.apply(r), Ms.apply(r)
+        assert np.array_equal(zd, Md.apply(r))
+        # FP32 storage (single, mixed) rounds S^-1 where the sparse skeleton rounds its factor blocks
+        tol = 1e-13 if precision == "double" else 2e-6
+        assert _rel(zd, zs) < tol, (t, _rel(zd, zs))
+        if precision == "double":
+            assert _rel(zd, zo) < 1e-12
+        # one launch per subdomain (a workgroup runs the parts of a dense top one after another)
+        Md.ctx.set_launch_mode(1)
+        assert np.array_equal(Md.apply(r), zd)
+
+
+def test_dense_top_local_solves_match_the_oracle():
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=24), 4, nev=8, overlap=2)
+    M = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=4)
+    rng = np.random.default_rng(5)
+    for i in range(4):
+        f = st.factors[i]
+        b = rng.standard_normal(f.n)
+        assert _rel(M.ctx.local_solve(i, b), orc.OracleLocalFactor(f, "double").solve(b)) < 1e-12, i
+
+
+def test_dense_top_gmres_iterations_and_paired_rhs():
+    """GMRES with dense tops takes the oracle's iterations (FP64); in FP32 two right-hand sides
+    sharing a workgroup give the bits of each alone."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=20), 8, nev=12, overlap=2)
+    M = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=4)
+    x, res = device.HipGmres(50, 1e-8, 500).solve(M, st.dec.b)
+    O = orc.OracleSchwarz(st)
+    xo, ro = orc.gmres(O.spmv, O.apply, st.dec.b, restart=50, rtol=1e-8, maxit=500)
+    assert res.converged and res.iterations == ro.iterations
+    assert _rel(x, xo) < 1e-8
+    M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=4)
+    r1 = np.cos(np.linspace(0.0, 7.0, st.n))
+    r2 = np.sin(np.linspace(-3.0, 2.0, st.n))
+    z = M32.ctx.apply(np.stack([r1, r2]))
+    assert np.array_equal(z[0], M32.apply(r1)) and np.array_equal(z[1], M32.apply(r2))
