@@ -1407,6 +1407,122 @@ struct s2l_solver {
 };
 
 // ============================================================================ C-ABI
+// Packs the uploaded local factors for the local solve in @p precision (the body of
+// s2l_finalize_local; the autotuner repacks through it).
+void finalizeLocal(s2l_ctx* c, int precision)
+{
+    if (c) c->invalidateGraphs();
+    require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
+    c->rfForget();
+    require(precision >= 0 && precision <= 2, S2L_EINVAL,
+            "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 factors, FP64 arithmetic)");
+    for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
+    HIPCHECK(hipSetDevice(c->device));
+    std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
+    const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type (LDS)
+    // the level-synchronous schedule is built when launch mode 2 needs it
+    c->ls_built = false;
+    c->ls = LsPlan{};
+    for (DevBuf<unsigned char>* b : {&c->ls_diag, &c->ls_bf, &c->ls_bb}) b->release();
+    // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
+    // the subtree tasks of all subdomains run in one launch, one workgroup per compute unit
+    const int tasks = c->tasks_per_sub > 0 ? c->tasks_per_sub : std::max(1, c->cus / std::max(1, c->nsub));
+    c->pk = packFactors(c->hf, moff, tasks, c->dense_top_max);
+    Packed& P = c->pk;
+    for (int i = 0; i < c->nsub; ++i) {
+        P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
+        P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
+    }
+    // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
+    c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
+    c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
+    c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+    c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
+    c->queue_ok = c->lds_bytes <= c->max_lds;
+    c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
+    c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
+    c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+    c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
+    c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
+    if (c->pair_ok)
+        HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(c->lds_bytes2)));
+    if (c->queue_ok) {
+        const void* kern = precision == 1   ? localSolveKernel<float, float>()
+                           : precision == 2 ? localSolveKernel<double, float>()
+                                            : localSolveKernel<double, double>();
+        HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
+    } else if (c->launch_mode != 2) {
+        require(precision != 2, S2L_ELIMIT,
+                "FP32 factors with FP64 arithmetic need the task queue, whose subdomain vector does not fit the LDS");
+        c->launch_mode = 2;  // the task queue cannot hold these subdomains in LDS
+    }
+    if (c->launch_mode == 2) buildLs(c, precision);
+    // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
+    std::vector<int32_t> gmap(c->gid_h.size()), lperm(c->gid_h.size());
+    for (int i = 0; i < c->nsub; ++i) {
+        const int32_t off = c->local_ptr[i];
+        const HostFactor& F = c->hf[static_cast<size_t>(i)];
+        for (int32_t p = 0; p < F.n; ++p) {
+            lperm[static_cast<size_t>(off + p)] = F.perm[static_cast<size_t>(p)];
+            gmap[static_cast<size_t>(off + p)] = c->gid_h[static_cast<size_t>(off + F.perm[static_cast<size_t>(p)])];
+        }
+    }
+    c->sub_d.upload(P.sub, c->stream);
+    c->levw_d.upload(P.lev_wpre, c->stream);
+    c->levh_d.upload(P.lev_hpre, c->stream);
+    c->levdesc_d.upload(P.lev_desc, c->stream);
+    c->levrow_d.upload(P.lev_row, c->stream);
+    c->levcb_d.upload(P.lev_cb, c->stream);
+    c->levce_d.upload(P.lev_ce, c->stream);
+    c->steps_d.upload(P.steps, c->stream);
+    c->ridx_d.upload(P.ridx, c->stream);
+    c->cptr_d.upload(P.cptr, c->stream);
+    c->cidx_d.upload(P.cidx, c->stream);
+    c->gmap_d.upload(gmap, c->stream);
+    c->lperm_d.upload(lperm, c->stream);
+    c->task_d.upload(P.task, c->stream);
+    c->tchild_d.upload(P.task_child, c->stream);
+    c->trows_d.upload(P.task_rows, c->stream);
+    c->ext_d.upload(P.ext_rows, c->stream);
+    c->queue_d.upload(P.queue, c->stream);
+    c->qsubq_d.upload(P.queue_sub, c->stream);
+    c->qsub_h = P.qsub;
+    if (!c->queue_ok) {
+        c->diag_f.release();
+        c->below_f.release();
+        c->diag_dd.release();
+        c->below_dd.release();
+    } else if (precision != 0) {
+        std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
+        c->diag_f.upload(d, c->stream);
+        c->below_f.upload(b, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->diag_dd.release();
+        c->below_dd.release();
+    } else {
+        c->diag_dd.upload(P.diag, c->stream);
+        c->below_dd.upload(P.below, c->stream);
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        c->diag_f.release();
+        c->below_f.release();
+    }
+    // the host copies of the values are no longer needed
+    std::vector<double>().swap(P.diag);
+    std::vector<double>().swap(P.below);
+    c->precision = precision;
+    c->work_nrhs = 0;
+    c->local_ready = true;
+    c->ls_stale = false;
+    // the symbolic plan of a later device refactorization, on a host thread meanwhile
+    bool chol = true;
+    for (const HostFactor& F : c->hf) chol = chol && F.cholesky;
+    if (chol && c->queue_ok && !c->rp_h.empty())
+        c->rf_pending = std::async(std::launch::async, [c] {
+            return buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
+        });
+}
+
 extern "C" {
 
 const char* s2l_last_error(const s2l_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
@@ -1636,118 +1752,7 @@ int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* per
 
 int s2l_finalize_local(s2l_ctx* c, int precision)
 {
-    return guarded(c, [&] {
-        if (c) c->invalidateGraphs();
-        require(c && c->subs_ready, S2L_ESTATE, "finalize_local: upload the subdomains first");
-        c->rfForget();
-        require(precision >= 0 && precision <= 2, S2L_EINVAL,
-                "precision must be 0 (FP64), 1 (FP32) or 2 (FP32 factors, FP64 arithmetic)");
-        for (int i = 0; i < c->nsub; ++i) require(c->hf[static_cast<size_t>(i)].n > 0, S2L_ESTATE, "factor of subdomain " + std::to_string(i) + " missing");
-        HIPCHECK(hipSetDevice(c->device));
-        std::vector<int32_t> moff(c->local_ptr.begin(), c->local_ptr.end() - 1);
-        const size_t esz = precision == 1 ? sizeof(float) : sizeof(double);  // arithmetic type (LDS)
-        // the level-synchronous schedule is built when launch mode 2 needs it
-        c->ls_built = false;
-        c->ls = LsPlan{};
-        for (DevBuf<unsigned char>* b : {&c->ls_diag, &c->ls_bf, &c->ls_bb}) b->release();
-        // task-queue schedule (launch modes 0 and 1): only when a subdomain vector fits the LDS
-        // the subtree tasks of all subdomains run in one launch, one workgroup per compute unit
-        const int tasks = c->tasks_per_sub > 0 ? c->tasks_per_sub : std::max(1, c->cus / std::max(1, c->nsub));
-        c->pk = packFactors(c->hf, moff, tasks, c->dense_top_max);
-        Packed& P = c->pk;
-        for (int i = 0; i < c->nsub; ++i) {
-            P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
-            P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
-        }
-        // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
-        c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
-        c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
-        c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
-        c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
-        c->queue_ok = c->lds_bytes <= c->max_lds;
-        c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
-        c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
-        c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
-        c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
-        c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
-        if (c->pair_ok)
-            HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         static_cast<int>(c->lds_bytes2)));
-        if (c->queue_ok) {
-            const void* kern = precision == 1   ? localSolveKernel<float, float>()
-                               : precision == 2 ? localSolveKernel<double, float>()
-                                                : localSolveKernel<double, double>();
-            HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
-        } else if (c->launch_mode != 2) {
-            require(precision != 2, S2L_ELIMIT,
-                    "FP32 factors with FP64 arithmetic need the task queue, whose subdomain vector does not fit the LDS");
-            c->launch_mode = 2;  // the task queue cannot hold these subdomains in LDS
-        }
-        if (c->launch_mode == 2) buildLs(c, precision);
-        // maps: gmap[p] = global row of factor position p, lperm[p] = its local index
-        std::vector<int32_t> gmap(c->gid_h.size()), lperm(c->gid_h.size());
-        for (int i = 0; i < c->nsub; ++i) {
-            const int32_t off = c->local_ptr[i];
-            const HostFactor& F = c->hf[static_cast<size_t>(i)];
-            for (int32_t p = 0; p < F.n; ++p) {
-                lperm[static_cast<size_t>(off + p)] = F.perm[static_cast<size_t>(p)];
-                gmap[static_cast<size_t>(off + p)] = c->gid_h[static_cast<size_t>(off + F.perm[static_cast<size_t>(p)])];
-            }
-        }
-        c->sub_d.upload(P.sub, c->stream);
-        c->levw_d.upload(P.lev_wpre, c->stream);
-        c->levh_d.upload(P.lev_hpre, c->stream);
-        c->levdesc_d.upload(P.lev_desc, c->stream);
-        c->levrow_d.upload(P.lev_row, c->stream);
-        c->levcb_d.upload(P.lev_cb, c->stream);
-        c->levce_d.upload(P.lev_ce, c->stream);
-        c->steps_d.upload(P.steps, c->stream);
-        c->ridx_d.upload(P.ridx, c->stream);
-        c->cptr_d.upload(P.cptr, c->stream);
-        c->cidx_d.upload(P.cidx, c->stream);
-        c->gmap_d.upload(gmap, c->stream);
-        c->lperm_d.upload(lperm, c->stream);
-        c->task_d.upload(P.task, c->stream);
-        c->tchild_d.upload(P.task_child, c->stream);
-        c->trows_d.upload(P.task_rows, c->stream);
-        c->ext_d.upload(P.ext_rows, c->stream);
-        c->queue_d.upload(P.queue, c->stream);
-        c->qsubq_d.upload(P.queue_sub, c->stream);
-        c->qsub_h = P.qsub;
-        if (!c->queue_ok) {
-            c->diag_f.release();
-            c->below_f.release();
-            c->diag_dd.release();
-            c->below_dd.release();
-        } else if (precision != 0) {
-            std::vector<float> d(P.diag.begin(), P.diag.end()), b(P.below.begin(), P.below.end());
-            c->diag_f.upload(d, c->stream);
-            c->below_f.upload(b, c->stream);
-            HIPCHECK(hipStreamSynchronize(c->stream));
-            c->diag_dd.release();
-            c->below_dd.release();
-        } else {
-            c->diag_dd.upload(P.diag, c->stream);
-            c->below_dd.upload(P.below, c->stream);
-            HIPCHECK(hipStreamSynchronize(c->stream));
-            c->diag_f.release();
-            c->below_f.release();
-        }
-        // the host copies of the values are no longer needed
-        std::vector<double>().swap(P.diag);
-        std::vector<double>().swap(P.below);
-        c->precision = precision;
-        c->work_nrhs = 0;
-        c->local_ready = true;
-        c->ls_stale = false;
-        // the symbolic plan of a later device refactorization, on a host thread meanwhile
-        bool chol = true;
-        for (const HostFactor& F : c->hf) chol = chol && F.cholesky;
-        if (chol && c->queue_ok && !c->rp_h.empty())
-            c->rf_pending = std::async(std::launch::async, [c] {
-                return buildRefactorPlan(c->hf, c->local_ptr, c->gid_h, c->rp_h, c->col_h, c->pk);
-            });
-    });
+    return guarded(c, [&] { finalizeLocal(c, precision); });
 }
 
 int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const double* V, const double* E, int symmetric,
@@ -2629,6 +2634,104 @@ int s2l_set_dense_top(s2l_ctx* c, int32_t max_width)
                 "dense_top: the widest dense top must be in [0, " + std::to_string(kDenseTopMax) + "]");
         c->dense_top_max = max_width;
         c->local_ready = false;
+    });
+}
+
+int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry* entries, int32_t max_entries,
+                       int32_t* count, int32_t* best)
+{
+    return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
+        require(c && c->local_ready && c->system_ready, S2L_ESTATE, "autotune_local: upload the system and the factors first");
+        require(nrhs >= 1 && applies >= 1 && entries && max_entries >= 1 && count && best, S2L_EINVAL,
+                "autotune_local: bad arguments");
+        HIPCHECK(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        const int precision = c->precision;
+        // candidates: tasks per subdomain around one workgroup per compute unit, each with and
+        // without dense tops, and the level-synchronous schedule
+        const int b = std::max(1, c->cus / std::max(1, c->nsub));
+        std::vector<int> tset{1, std::max(1, b / 2), std::max(1, b - 1), b, b + 1, 2 * b};
+        std::sort(tset.begin(), tset.end());
+        tset.erase(std::unique(tset.begin(), tset.end()), tset.end());
+        std::vector<s2l_tune_entry> cand;
+        for (int t : tset)
+            for (int d : {static_cast<int>(kDenseTopMax), 0}) {
+                if (t == 1 && d != 0) continue;  // one task per subdomain: no skeleton
+                cand.push_back(s2l_tune_entry{t, d, 0, 0, 0.0});
+            }
+        if (precision != 2) cand.push_back(s2l_tune_entry{b, static_cast<int>(kDenseTopMax), 2, 0, 0.0});
+        require(static_cast<int32_t>(cand.size()) <= max_entries, S2L_ELIMIT,
+                "autotune_local: " + std::to_string(cand.size()) + " candidates, max_entries too small");
+        // input: a fixed vector per right-hand side; output: the owned rows
+        std::vector<double> h(static_cast<size_t>(c->n) * nrhs);
+        for (size_t q = 0; q < h.size(); ++q) h[q] = std::cos(1e-3 * static_cast<double>(q));
+        DevBuf<double> in, out;
+        in.upload(h, s);
+        out.reserve(h.size());
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        auto run = [&](int cnt) {
+            for (int k = 0; k < cnt; ++k) {
+                if (precision == 1 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
+                    localSolve<float2, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
+                else if (precision == 1)
+                    localSolve<float, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
+                else if (precision == 2)
+                    localSolve<double, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
+                else
+                    localSolve<double, double>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
+            }
+        };
+        const bool timing = c->timing;
+        c->timing = false;
+        int32_t ibest = -1;
+        try {
+            for (size_t k = 0; k < cand.size(); ++k) {
+                s2l_tune_entry& e = cand[k];
+                c->tasks_per_sub = e.tasks;
+                c->dense_top_max = e.dense_top;
+                c->launch_mode = 0;
+                finalizeLocal(c, precision);
+                if (e.launch_mode == 2 || !c->queue_ok) {
+                    if (!c->ls_built) buildLs(c, precision);
+                    c->launch_mode = 2;
+                    e.launch_mode = 2;
+                }
+                c->ensureWork(nrhs);
+                run(3);
+                HIPCHECK(hipEventRecord(e0, s));
+                run(applies);
+                HIPCHECK(hipEventRecord(e1, s));
+                HIPCHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                e.us = 1e3 * static_cast<double>(ms) / applies;
+                if (ibest < 0 || e.us < cand[static_cast<size_t>(ibest)].us) ibest = static_cast<int32_t>(k);
+            }
+        } catch (...) {
+            c->timing = timing;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        c->timing = timing;
+        HIPCHECK(hipEventDestroy(e0));
+        HIPCHECK(hipEventDestroy(e1));
+        // keep the fastest
+        const s2l_tune_entry& w = cand[static_cast<size_t>(ibest)];
+        c->tasks_per_sub = w.tasks;
+        c->dense_top_max = w.dense_top;
+        c->launch_mode = 0;
+        finalizeLocal(c, precision);
+        if (w.launch_mode == 2) {
+            if (!c->ls_built) buildLs(c, precision);
+            c->launch_mode = 2;
+        }
+        std::copy(cand.begin(), cand.end(), entries);
+        *count = static_cast<int32_t>(cand.size());
+        *best = ibest;
     });
 }
 

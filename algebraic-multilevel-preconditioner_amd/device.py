@@ -38,6 +38,12 @@ class KrylovResultC(ctypes.Structure):
                 ("final_residual_of", ctypes.c_double), ("history_len", ctypes.c_int)]
 
 
+class TuneEntry(ctypes.Structure):
+    """s2l_tune_entry of include/s2l_hip.h."""
+    _fields_ = [("tasks", ctypes.c_int32), ("dense_top", ctypes.c_int32), ("launch_mode", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("us", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("nsub", ctypes.c_int32), ("n0", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("nnz_a", ctypes.c_int64), ("factor_entries_fwd", ctypes.c_int64),
@@ -60,7 +66,8 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_get_stats", "s2l_device_footprint", "s2l_timing_enable", "s2l_timing_read", "s2l_set_local_launch_mode",
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
-           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve", "s2l_set_dense_top"]
+           "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve", "s2l_set_dense_top",
+           "s2l_autotune_local"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -104,6 +111,8 @@ def hip_lib():
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
         "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
         "s2l_set_dense_top": [vp, ctypes.c_int32],
+        "s2l_autotune_local": [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TuneEntry), ctypes.c_int32,
+                               ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
         "s2l_set_graphs": [vp, ctypes.c_int],
         "s2l_solver_create": [vp, P(KrylovOpts), ctypes.c_int32, P(vp)],
         "s2l_solver_destroy": [vp],
@@ -438,6 +447,16 @@ class HipContext:
     def set_launch_mode(self, mode: int):
         self._check(self.L.s2l_set_local_launch_mode(self.h, mode))
 
+    def autotune(self, nrhs: int = 1, applies: int = 20):
+        """s2l_autotune_local: times the candidate local-solve schedules and keeps the fastest.
+        Returns (candidates as dicts with tasks, dense_top, launch_mode, us; index of the kept one)."""
+        ent = (TuneEntry * 16)()
+        cnt = ctypes.c_int32()
+        best = ctypes.c_int32()
+        self._check(self.L.s2l_autotune_local(self.h, nrhs, applies, ent, 16, ctypes.byref(cnt), ctypes.byref(best)))
+        out = [dict(tasks=e.tasks, dense_top=e.dense_top, launch_mode=e.launch_mode, us=e.us) for e in ent[:cnt.value]]
+        return out, best.value
+
 
 class SlotSolver:
     """s2l_solver: GMRES over a fixed number of right-hand-side slots sharing every launch. A right-hand
@@ -518,6 +537,14 @@ class HipTwoLevelSchwarz:
 
     def apply(self, r):
         return self.ctx.apply(r)
+
+    def autotune(self, nrhs: int = 1, applies: int = 20):
+        """Picks the local-solve schedule by timing (HipContext.autotune) and keeps it for later
+        refreshes. Call before refactorize_device (it repacks the uploaded host factors)."""
+        cands, best = self.ctx.autotune(nrhs, applies)
+        self.tasks_per_subdomain = cands[best]["tasks"]
+        self.dense_top = cands[best]["dense_top"]
+        return cands, best
 
     def name(self) -> str:
         """OneLevelSchwarz::name + ",deflated"/",additive" (two_level_schwarz.cpp:129-132)."""

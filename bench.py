@@ -113,7 +113,7 @@ def measure_traffic(args):
         d = tempfile.mkdtemp(prefix="s2l_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
                os.path.abspath(__file__), "--profile-applies", str(PMC_APPLIES), "--profile-one-level", "--setup-workers", "1",
-               "--workload", args.workload, "--tasks", str(args.tasks)] + \
+               "--workload", args.workload, "--tasks", str(args.tasks), "--no-autotune"] + \
               ([] if args.dense_top is None else ["--dense-top", str(args.dense_top)])
         log(f"traffic pass {counter}")
         try:
@@ -305,6 +305,8 @@ def main():
                     help="replay each GMRES iteration's apply + SpMV from a captured HIP graph (s2l_set_graphs)")
     ap.add_argument("--tasks", type=int, default=0,
                     help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default local-solve schedule instead of the autotuner's choice (s2l_autotune_local)")
     ap.add_argument("--dense-top", type=int, default=None,
                     help="widest skeleton replaced by a dense top (s2l_set_dense_top; default: the library's, 0 = none)")
     ap.add_argument("--leaf", type=int, default=0,
@@ -379,6 +381,14 @@ def main():
     log(f"setup {t_setup:.1f} s {st.times}; n0 = {st.coarse.n0}")
 
     M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    tune = {}
+    if not args.no_autotune and args.profile_applies == 0 and args.tasks == 0 and args.dense_top is None:
+        t0 = time.perf_counter()
+        cands, best = M32.autotune(nrhs, 20)
+        tune["fp32"] = {"candidates": cands, "kept": best, "s": round(time.perf_counter() - t0, 2)}
+        log(f"autotune fp32: kept {cands[best]} of {len(cands)} candidates")
+        # the traffic passes measure the schedule kept
+        args.tasks, args.dense_top = cands[best]["tasks"], cands[best]["dense_top"]
     ctx = M32.ctx
     ctx.set_graphs(args.graphs != 0)
     if args.local_mode is not None:
@@ -412,10 +422,16 @@ def main():
         return
 
     M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    if tune:
+        cands, best = M64.autotune(nrhs, 20)
+        tune["fp64"] = {"candidates": cands, "kept": best}
     ctx64 = M64.ctx
     ctx64.set_graphs(args.graphs != 0)
     # FP32 factor entries applied in FP64 arithmetic (no reference counterpart)
     Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    if tune:
+        cands, best = Mmx.autotune(nrhs, 20)
+        tune["mixed"] = {"candidates": cands, "kept": best}
     ctxm = Mmx.ctx
     ctxm.set_graphs(args.graphs != 0)
     nsteps = args.steps + args.warmup
@@ -609,6 +625,7 @@ def main():
                            "apply_wall_us_per_subdomain": _us(mode_ms[1][1])},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": tune or None,
         "setup_s": round(t_setup, 2),
         "setup_stages_s": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.times.items()},
     }

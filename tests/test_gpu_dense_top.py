@@ -72,3 +72,28 @@ def test_dense_top_gmres_iterations_and_paired_rhs():
     r2 = np.sin(np.linspace(-3.0, 2.0, st.n))
     z = M32.ctx.apply(np.stack([r1, r2]))
     assert np.array_equal(z[0], M32.apply(r1)) and np.array_equal(z[1], M32.apply(r2))
+
+
+@pytest.mark.parametrize("precision", ["single", "mixed"])
+def test_autotune_keeps_the_fastest_schedule(precision):
+    """s2l_autotune_local times every candidate schedule and keeps the fastest: the kept schedule is
+    the candidate with the least time, the context's later applies use it (equal to a context built
+    directly with that schedule, bitwise), and a refresh of the factors keeps it."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=20), 8, nev=8, overlap=2)
+    M = device.HipTwoLevelSchwarz(st, precision)
+    cands, best = M.autotune(nrhs=1, applies=5)
+    print("autotune", precision, cands, best, flush=True)
+    assert len(cands) >= 4 and 0 <= best < len(cands)
+    assert all(c["us"] > 0 for c in cands)
+    assert cands[best]["us"] == min(c["us"] for c in cands)
+    assert any(c["dense_top"] == 0 for c in cands) and any(c["dense_top"] > 0 for c in cands)
+    w = cands[best]
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    Mw = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=w["tasks"], dense_top=w["dense_top"])
+    if w["launch_mode"] != 0:
+        Mw.ctx.set_launch_mode(w["launch_mode"])
+    z = M.apply(r)
+    assert np.array_equal(z, Mw.apply(r))
+    M.refresh(st)
+    assert np.array_equal(M.apply(r), z) or w["launch_mode"] != 0

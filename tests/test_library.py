@@ -92,7 +92,8 @@ def test_null_context_is_rejected():
     for rc in (L.s2l_apply(None, None, None, 1, None), L.s2l_spmv(None, None, None, 1, None),
                L.s2l_finalize_local(None, 0), L.s2l_synchronize(None),
                L.s2l_solver_create(None, None, 1, None), L.s2l_solver_step(None, None),
-               L.s2l_solver_admit(None, 0, None, None, 1)):
+               L.s2l_solver_admit(None, 0, None, None, 1),
+               L.s2l_autotune_local(None, 1, 1, None, 0, None, None), L.s2l_set_dense_top(None, 0)):
         assert rc < 0
 
 
