@@ -315,6 +315,7 @@ struct s2l_ctx {
     DevBuf<int64_t> off_d, voff_d;
     DevBuf<int32_t> vrows_d, vrowptr_d, vrowg_d, blksub_d, blkrow_d, cperm_d;
     DevBuf<double> V_d, T1_d, T2_d;  // inverted factors of A00, row-major with leading dimension tri_ld
+    DevBuf<double> Einv_d;           // A00^-1 = T2 T1 (the rows of the permuted columns for LU), same layout
     int32_t tri_ld = 0;
     bool coarse_perm = false;
 
@@ -708,8 +709,8 @@ void applyImpl(s2l_ctx* c, const double* r, double* z, int nrhs, const int32_t* 
     launchCoarseRestrict(c->ca, r, c->n, W.cvec, W.cpart, nrhs, skip, s);
     phaseEnd(c, t, s);
     t = phaseBegin(c, S2L_PHASE_PREC_COARSE_SOLVE, s);
-    launchTriGemv(c->n0, c->tri_ld, c->T1_d.p, 1, c->coarse_perm ? c->cperm_d.p : nullptr, W.cvec, W.tvec, nrhs, skip, s);
-    launchTriGemv(c->n0, c->tri_ld, c->T2_d.p, 0, nullptr, W.tvec, W.yvec, nrhs, skip, s);
+    // y = A00^-1 c in one product with the inverse (T2 T1): one launch and no dependent second pass
+    launchTriGemv(c->n0, c->tri_ld, c->Einv_d.p, 2, c->coarse_perm ? c->cperm_d.p : nullptr, W.cvec, W.yvec, nrhs, skip, s);
     phaseEnd(c, t, s);
     t = phaseBegin(c, S2L_PHASE_PREC_COARSE_PROLONG, s);
     if (c->boolean) {
@@ -1407,6 +1408,23 @@ struct s2l_solver {
 };
 
 // ============================================================================ C-ABI
+// A00^-1 = T2 T1 on the device (rocBLAS dgemm over the row-major factors: in column-major terms
+// C = T1^T T2^T = (T2 T1)^T, which is T2 T1 row-major), the padding columns zero.
+void coarseInverse(s2l_ctx* c, hipStream_t s)
+{
+    const int32_t n0 = c->n0, ld = c->tri_ld;
+    c->Einv_d.reserve(static_cast<size_t>(n0) * ld);
+    HIPCHECK(hipMemsetAsync(c->Einv_d.p, 0, sizeof(double) * static_cast<size_t>(n0) * ld, s));
+    if (!c->rf_blas)
+        require(rocblas_create_handle(&c->rf_blas) == rocblas_status_success, S2L_EHIP, "coarse inverse: rocblas_create_handle failed");
+    require(rocblas_set_stream(c->rf_blas, s) == rocblas_status_success, S2L_EHIP, "coarse inverse: rocblas_set_stream failed");
+    const double one = 1.0, zero = 0.0;
+    require(rocblas_dgemm(c->rf_blas, rocblas_operation_none, rocblas_operation_none, n0, n0, n0, &one, c->T1_d.p, ld, c->T2_d.p,
+                          ld, &zero, c->Einv_d.p, ld) == rocblas_status_success,
+            S2L_EHIP, "coarse inverse: rocblas_dgemm failed");
+    HIPCHECK(hipGetLastError());
+}
+
 // Packs the uploaded local factors for the local solve in @p precision (the body of
 // s2l_finalize_local; the autotuner repacks through it).
 void finalizeLocal(s2l_ctx* c, int precision)
@@ -1837,6 +1855,7 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
                 std::copy(T2.begin() + static_cast<size_t>(i) * n0, T2.begin() + static_cast<size_t>(i + 1) * n0,
                           Tp.begin() + static_cast<size_t>(i) * c->tri_ld);
             c->T2_d.upload(Tp, c->stream);
+            coarseInverse(c, c->stream);
             HIPCHECK(hipStreamSynchronize(c->stream));
             CoarseArgs& a = c->ca;
             a.nsub = c->nsub;
@@ -2144,6 +2163,7 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
                     S2L_EHIP, "refactorize: rocsolver_dtrtri failed");
             launchTriLayout(c->rf_E.p, n0, c->tri_ld, c->T1_d.p, c->T2_d.p, s);
             HIPCHECK(hipGetLastError());
+            coarseInverse(c, s);
             HIPCHECK(hipMemcpyAsync(&pinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
             require(pinfo == 0, S2L_ENUMERIC, "refactorize: the factor of A_00 is singular");

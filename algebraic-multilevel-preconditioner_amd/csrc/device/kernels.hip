@@ -783,9 +783,9 @@ __global__ __launch_bounds__(kCrows) void k_coarse_restrict_mfma(CoarseArgs c, c
     }
 }
 
-// y = T x for 16 right-hand sides at a time (T an inverted triangular factor of A00, row-major
-// n0 x n0 with explicit zeros and rows padded to ld, a multiple of eight; x rows permuted by perm
-// when given). Workgroup (t, g) computes rows [16 t, 16 t + 16) with its kTriWaves wavefronts
+// y = T x for 16 right-hand sides at a time (T an inverted triangular factor of A00, lower 1 or
+// upper 0, or the full inverse, lower 2: row-major n0 x n0 with explicit zeros and rows padded to
+// ld, a multiple of eight; x rows permuted by perm when given). Workgroup (t, g) computes rows [16 t, 16 t + 16) with its kTriWaves wavefronts
 // splitting the columns the triangle holds into blocks of eight (wavefront w takes blocks w,
 // w + kTriWaves, ...). A lane reads two consecutive entries of its row with one 16-byte load; they
 // feed the two products over the columns k = 8 b + 2 q and 8 b + 2 q + 1 (q = lane / 16), and the
@@ -805,8 +805,8 @@ __global__ __launch_bounds__(kTriWaves * kWave) void k_tri_gemm_mfma(int32_t n0,
     const double* xv = x + static_cast<int64_t>(rok ? rhs : 0) * n0;
     const int row = min(r0 + (lane & 15), n0 - 1);  // rows past n0 repeat the last one; their sums are dropped
     const double* Trow = T + static_cast<int64_t>(row) * ld;
-    const int b0 = lower ? 0 : (r0 >> 3);
-    const int b1 = lower ? ((min(n0, r0 + 16) + 7) >> 3) : (ld >> 3);
+    const int b0 = lower == 0 ? (r0 >> 3) : 0;
+    const int b1 = lower == 1 ? ((min(n0, r0 + 16) + 7) >> 3) : (ld >> 3);
     mfmaAcc acc = {0.0, 0.0, 0.0, 0.0};
     for (int bb = b0 + wave; bb < b1; bb += kTriWaves * kTriBatch) {
         double2 a[kTriBatch];
@@ -915,9 +915,10 @@ __global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, c
     }
 }
 
-// y = T x for one right-hand side per grid row: one wavefront per row of the triangle, a lane
-// reading two consecutive entries with one 16-byte load (rows padded to ld, a multiple of eight,
-// with zeros), four such loads per batch; lane partial sums, then the wavefront sum.
+// y = T x for one right-hand side per grid row: one wavefront per row of the triangle (lower 1,
+// upper 0) or of a full matrix (lower 2), a lane reading two consecutive entries with one 16-byte
+// load (rows padded to ld, a multiple of eight, with zeros), four such loads per batch; lane
+// partial sums, then the wavefront sum.
 __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const double* Tm, int lower, const int32_t* perm,
                                                    const double* x, double* y, const int32_t* skip)
 {
@@ -927,7 +928,7 @@ __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const 
     const int i = blockIdx.x * (blockDim.x / kWave) + wave;
     if (i >= n0) return;
     const double* xv = x + static_cast<int64_t>(rr) * n0;
-    const int lo = lower ? 0 : (i & ~1), hi = lower ? i + 1 : n0;  // even start: 16-byte aligned
+    const int lo = lower == 0 ? (i & ~1) : 0, hi = lower == 1 ? i + 1 : n0;  // even start: 16-byte aligned
     const double* row = Tm + static_cast<int64_t>(i) * ld;
     double s = 0.0;
     for (int j0 = lo; j0 < hi; j0 += 8 * kWave) {
