@@ -219,8 +219,10 @@ __device__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, 
 {
     // FP64 factors: each pass takes two groups of R rows, nw * R rows apart, whose loads are
     // issued together (FP32 factors leave no registers for the second group). The block's
-    // entries (TF) are converted to the arithmetic type T.
-    constexpr int R = blockRows<TF>();
+    // entries (TF) are converted to the arithmetic type T. Eight-byte arithmetic types (FP64 sums
+    // of FP32 entries, a pair of FP32 right-hand sides) take the FP64 row count: four rows of them
+    // spill registers.
+    constexpr int R = sizeof(T) == 8 ? 2 : blockRows<TF>();
     constexpr int P = sizeof(TF) == 8 ? 2 : 1;
     for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
         const int rA = r0, rB = r0 + nw * R;

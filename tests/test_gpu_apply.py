@@ -263,8 +263,9 @@ def test_many_rhs_coarse_products_on_matrix_cores(precision):
 def test_fp32_pairs_of_rhs_equal_single_applies(name, kw, N, nrhs):
     """FP32 local solves of two right-hand sides in one workgroup (a float pair per row in LDS, one
     read of the factors for both; an odd count leaves the last pair half empty) give the results of
-    one right-hand side at a time (the same float operations per component: bitwise, or to 1e-6
-    should the compiler contract them differently), in every launch of the task queue."""
+    one right-hand side at a time to 1e-6 (the same float operations per component, except that the
+    dense blocks of wide supernodes and dense tops sum two rows per wavefront pass for a pair and four
+    for one right-hand side), and every batch is bitwise repeatable."""
     from amp_amd import device
     st = _setup(name, kw, N, nev=8, overlap=2)
     print(f"case {name} {kw} N={N} fp32 pairs nrhs={nrhs}", flush=True)
