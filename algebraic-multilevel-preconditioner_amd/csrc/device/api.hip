@@ -810,11 +810,12 @@ struct GmresRun {
     struct Slot {
         bool live = false;      ///< holds a solve
         bool conv = false, breakdown = false;
+        bool nonfinite = false; ///< a residual was not finite: the solve stops (no counterpart)
         int iters = 0;
         double bnorm = 0, beta = 0, rel = 0;
         double normF = 1, l1_0 = 0, l1 = 0, thr = 0;  ///< OpenFOAM test
         std::vector<double> hist;
-        bool active(int maxit) const { return live && !conv && iters < maxit; }
+        bool active(int maxit) const { return live && !conv && !nonfinite && iters < maxit; }
     };
     std::vector<Slot> sl;
     std::vector<char> fresh;  ///< admitted since the last cycle (begin() pending)
@@ -991,6 +992,10 @@ void GmresRun::begin(const std::vector<char>& on, const double* d_b, double* d_x
         S.beta = std::sqrt(sum(q, 0));
         S.rel = S.beta / S.bnorm;
         S.hist.push_back(S.rel);
+        if (!std::isfinite(S.rel)) {  // a non-finite right-hand side or initial guess
+            S.nonfinite = true;
+            continue;
+        }
         if (of) {
             S.l1_0 = S.l1 = sum(q, 1);
             S.thr = std::max(o.rtol * S.l1_0, o.atol * S.normF);
@@ -1097,6 +1102,12 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
         if (b2 == 0.0) cv = true;
         S.conv = cv;
         S.breakdown = S.breakdown || G.breakdown;
+        // a non-finite residual (the Arnoldi estimate or the true one) ends the solve unconverged
+        // instead of restarting until maxit, which is what the reference's loop would do
+        if (!std::isfinite(b2) || !std::isfinite(G.rel)) {
+            S.conv = false;
+            S.nonfinite = true;
+        }
     }
     return true;
 }
@@ -1106,7 +1117,7 @@ void GmresRun::result(int q, s2l_krylov_result* R, double* history) const
     const Slot& S = sl[static_cast<size_t>(q)];
     std::memset(R, 0, sizeof(*R));
     R->converged = S.conv;
-    R->breakdown = 0;
+    R->breakdown = S.nonfinite ? 1 : 0;
     R->iterations = S.iters;
     R->rel_residual = S.rel;
     R->true_rel_residual = S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0;
@@ -1408,6 +1419,34 @@ struct s2l_solver {
 };
 
 // ============================================================================ C-ABI
+// A_00 = W^T A W of the uploaded operator and coarse basis into rf_E (column-major n0 x n0), 16
+// columns at a time: prolongation of unit vectors, SpMV, restriction (coarse_space.cpp:95-105).
+void formCoarseMatrix(s2l_ctx* c, hipStream_t s)
+{
+    const int32_t n0 = c->n0, G = 16;
+    c->rf_E.reserve(static_cast<size_t>(n0) * n0);
+    c->rf_Y.reserve(static_cast<size_t>(n0) * G);
+    c->rf_C.reserve(static_cast<size_t>(n0) * G);
+    c->rf_X.reserve(static_cast<size_t>(c->n) * G);
+    c->rf_AX.reserve(static_cast<size_t>(c->n) * G);
+    c->rf_part.reserve(static_cast<size_t>(std::max(c->nsub, 1)) * std::max(c->ca.rchunks, 1) * std::max(c->ca.mmax, 1) * G);
+    if (!c->boolean) c->rf_loc.reserve(static_cast<size_t>(c->local_total) * G);
+    for (int32_t k0 = 0; k0 < n0; k0 += G) {
+        const int32_t cnt = std::min(G, n0 - k0);
+        launchUnitColumns(c->rf_Y.p, n0, k0, cnt, s);
+        if (c->boolean) {
+            launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_X.p, c->n, 1, cnt, nullptr, s);
+        } else {
+            launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_loc.p, c->local_total, 0, cnt, nullptr, s);
+            launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->rf_loc.p, c->local_total, nullptr, c->rf_X.p, cnt, nullptr, s);
+        }
+        launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->rf_X.p, c->rf_AX.p, nullptr, cnt, nullptr, s);
+        launchCoarseRestrict(c->ca, c->rf_AX.p, c->n, c->rf_C.p, c->rf_part.p, cnt, nullptr, s);
+        launchStoreColumns(c->rf_C.p, n0, k0, cnt, c->rf_E.p, n0, s);
+    }
+    HIPCHECK(hipGetLastError());
+}
+
 // A00^-1 = T2 T1 on the device (rocBLAS dgemm over the row-major factors: in column-major terms
 // C = T1^T T2^T = (T2 T1)^T, which is T2 T1 row-major), the padding columns zero.
 void coarseInverse(s2l_ctx* c, hipStream_t s)
@@ -1753,12 +1792,14 @@ int s2l_upload_local_factor(s2l_ctx* c, int32_t i, int32_t n, const int32_t* per
         F.lcp.assign(l_colptr, l_colptr + n + 1);
         F.lr.assign(l_row, l_row + l_colptr[n]);
         F.lv.assign(l_val, l_val + l_colptr[n]);
+        for (double v : F.lv) require(std::isfinite(v), S2L_ENUMERIC, "upload_local_factor: non-finite factor entry");
         F.cholesky = (ut_colptr == nullptr);
         if (!F.cholesky) {
             require(ut_row && ut_val, S2L_EINVAL, "upload_local_factor: incomplete U^T");
             F.ucp.assign(ut_colptr, ut_colptr + n + 1);
             F.ur.assign(ut_row, ut_row + ut_colptr[n]);
             F.uv.assign(ut_val, ut_val + ut_colptr[n]);
+            for (double v : F.uv) require(std::isfinite(v), S2L_ENUMERIC, "upload_local_factor: non-finite factor entry");
         } else {
             F.ucp.clear();
             F.ur.clear();
@@ -1786,7 +1827,8 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
         c->correction = correction;
         c->off_h.assign(offsets, offsets + c->nsub + 1);
         if (n0 > 0) {
-            require(V && E, S2L_EINVAL, "upload_coarse: null V or E");
+            require(V, S2L_EINVAL, "upload_coarse: null V");
+            require(E || c->system_ready, S2L_ESTATE, "upload_coarse: E = NULL forms A_00 from the operator: upload the system first");
             std::vector<int64_t> voff(static_cast<size_t>(c->nsub));
             std::vector<int32_t> vrows(static_cast<size_t>(c->nsub)), vrowptr(static_cast<size_t>(c->nsub)), vrowg, bsub, brow;
             int64_t vtot = 0;
@@ -1816,6 +1858,35 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
             c->vrowg_d.upload(vrowg, c->stream);
             c->blksub_d.upload(bsub, c->stream);
             c->blkrow_d.upload(brow, c->stream);
+            CoarseArgs& a = c->ca;
+            a.nsub = c->nsub;
+            a.n0 = n0;
+            a.mmax = mmax;
+            int32_t maxrows = 0;
+            for (int32_t v : vrows) maxrows = std::max(maxrows, v);
+            a.rchunks = std::max(1, (maxrows + 255) / 256);
+            a.off = c->off_d.p;
+            a.voff = c->voff_d.p;
+            a.vrows = c->vrows_d.p;
+            a.vrow_ptr = c->vrowptr_d.p;
+            a.vrow_g = c->vrowg_d.p;
+            a.V = c->V_d.p;
+            a.blk_sub = c->blksub_d.p;
+            a.blk_row = c->blkrow_d.p;
+            a.nblk = static_cast<int32_t>(bsub.size());
+            // A_00 given (row-major), or formed here on the device from the operator and the basis
+            std::vector<double> Eh;
+            if (!E) {
+                formCoarseMatrix(c, c->stream);
+                std::vector<double> Ec(static_cast<size_t>(n0) * n0);
+                HIPCHECK(hipMemcpyAsync(Ec.data(), c->rf_E.p, sizeof(double) * Ec.size(), hipMemcpyDeviceToHost, c->stream));
+                HIPCHECK(hipStreamSynchronize(c->stream));
+                Eh.resize(Ec.size());
+                for (int32_t i = 0; i < n0; ++i)
+                    for (int32_t j = 0; j < n0; ++j)
+                        Eh[static_cast<size_t>(i) * n0 + j] = Ec[static_cast<size_t>(j) * n0 + i];  // row-major
+                E = Eh.data();
+            }
             // A_00: Cholesky (symmetric) or LU with partial pivoting, FP64; the device applies the
             // inverses of the triangular factors as two triangular matrix-vector products.
             std::vector<double> M(E, E + static_cast<size_t>(n0) * n0);
@@ -1857,22 +1928,6 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
             c->T2_d.upload(Tp, c->stream);
             coarseInverse(c, c->stream);
             HIPCHECK(hipStreamSynchronize(c->stream));
-            CoarseArgs& a = c->ca;
-            a.nsub = c->nsub;
-            a.n0 = n0;
-            a.mmax = mmax;
-            int32_t maxrows = 0;
-            for (int32_t v : vrows) maxrows = std::max(maxrows, v);
-            a.rchunks = std::max(1, (maxrows + 255) / 256);
-            a.off = c->off_d.p;
-            a.voff = c->voff_d.p;
-            a.vrows = c->vrows_d.p;
-            a.vrow_ptr = c->vrowptr_d.p;
-            a.vrow_g = c->vrowg_d.p;
-            a.V = c->V_d.p;
-            a.blk_sub = c->blksub_d.p;
-            a.blk_row = c->blkrow_d.p;
-            a.nblk = static_cast<int32_t>(bsub.size());
         }
         c->work_nrhs = 0;
         c->coarse_ready = true;
@@ -2122,29 +2177,9 @@ int s2l_refactorize(s2l_ctx* c, const double* val, int host)
         c->rf_ms[1] = ms_since(t_local);
         const auto t_coarse = clk::now();
         if (c->n0 > 0) {
-            // A_00 = W^T A W, 16 columns at a time: prolongation of unit vectors, SpMV, restriction
-            const int32_t n0 = c->n0, G = 16;
-            c->rf_E.reserve(static_cast<size_t>(n0) * n0);
-            c->rf_Y.reserve(static_cast<size_t>(n0) * G);
-            c->rf_C.reserve(static_cast<size_t>(n0) * G);
-            c->rf_X.reserve(static_cast<size_t>(c->n) * G);
-            c->rf_AX.reserve(static_cast<size_t>(c->n) * G);
-            c->rf_part.reserve(static_cast<size_t>(std::max(c->nsub, 1)) * std::max(c->ca.rchunks, 1) * std::max(c->ca.mmax, 1) * G);
-            if (!c->boolean) c->rf_loc.reserve(static_cast<size_t>(c->local_total) * G);
-            for (int32_t k0 = 0; k0 < n0; k0 += G) {
-                const int32_t cnt = std::min(G, n0 - k0);
-                launchUnitColumns(c->rf_Y.p, n0, k0, cnt, s);
-                if (c->boolean) {
-                    launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_X.p, c->n, 1, cnt, nullptr, s);
-                } else {
-                    launchCoarseProlong(c->ca, c->rf_Y.p, c->rf_loc.p, c->local_total, 0, cnt, nullptr, s);
-                    launchRevGather(c->n, c->rg_ptr.p, c->rg_idx.p, c->rf_loc.p, c->local_total, nullptr, c->rf_X.p, cnt, nullptr, s);
-                }
-                launchSpmv(c->n, c->rp.p, c->col.p, c->val.p, c->rf_X.p, c->rf_AX.p, nullptr, cnt, nullptr, s);
-                launchCoarseRestrict(c->ca, c->rf_AX.p, c->n, c->rf_C.p, c->rf_part.p, cnt, nullptr, s);
-                launchStoreColumns(c->rf_C.p, n0, k0, cnt, c->rf_E.p, n0, s);
-            }
-            HIPCHECK(hipGetLastError());
+            // A_00 = W^T A W on the device
+            const int32_t n0 = c->n0;
+            formCoarseMatrix(c, s);
             // Cholesky factor and inverse of A_00 (dense FP64 library routines), then the layout of
             // the coarse solve
             if (!c->rf_blas) {

@@ -1548,6 +1548,7 @@ __device__ void gmGivensStep(const GmArgs& a, int j, int r, double hn)
         st.breakdown = 1;
         done = 1;
     }
+    if (!isfinite(st.rel) || !isfinite(hn)) done = 1;  // the host ends the solve (non-finite residual)
     if (st.iters >= st.maxit || j + 1 >= m) done = 1;
     st.done = done;
     if (!a.of) a.flags[r] = done;

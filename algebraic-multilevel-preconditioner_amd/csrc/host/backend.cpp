@@ -86,7 +86,8 @@ void HipTwoLevelSchwarz::upload(const SetupData& s)
     }
     check(ctx_, s2l_finalize_local(ctx_, s.precision == "single" ? 1 : (s.precision == "mixed" ? 2 : 0)));
     check(ctx_, s2l_upload_coarse(ctx_, s.n0, s.n0 > 0 ? s.coarse_offsets.data() : nullptr, s.n0 > 0 ? s.V.data() : nullptr,
-                                  s.n0 > 0 ? s.E.data() : nullptr, s.symmetric ? 1 : 0, s.correction == "additive" ? 1 : 0));
+                                  s.n0 > 0 && !s.E.empty() ? s.E.data() : nullptr, s.symmetric ? 1 : 0,
+                                  s.correction == "additive" ? 1 : 0));
     n_ = s.n;
     rowptr_ = s.rowptr;
     col_ = s.col;

@@ -248,11 +248,12 @@ class HipContext:
     # ------------------------------------------------------------------ uploads
     def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
                      correction: str | None = None, tasks_per_subdomain: int | None = None, coarse: bool = True,
-                     dense_top: int | None = None):
+                     dense_top: int | None = None, coarse_on_device: bool = False):
         """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products);
         ``coarse=False`` stops after the local factors (a setup whose coarse space is not built yet).
         ``dense_top``: widest skeleton replaced by a dense top (s2l_set_dense_top; None = the
-        library's default, 0 = none)."""
+        library's default, 0 = none). ``coarse_on_device``: A_00 = W^T A W formed by the library on
+        the device from the uploaded operator and basis (E = NULL) instead of the setup's."""
         if tasks_per_subdomain is not None:
             self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
         if dense_top is not None:
@@ -270,7 +271,7 @@ class HipContext:
             self.upload_factor(i, f)
         self.finalize(precision)
         if coarse:
-            self.upload_coarse(setup, d["correction"])
+            self.upload_coarse(setup, d["correction"], coarse_on_device)
         self._keep = d
 
     def upload_factor(self, i, f):
@@ -291,7 +292,7 @@ class HipContext:
     def finalize(self, precision: str = "double"):
         self._check(self.L.s2l_finalize_local(self.h, _PRECISIONS[precision]))
 
-    def upload_coarse(self, setup, correction: str = "deflated"):
+    def upload_coarse(self, setup, correction: str = "deflated", coarse_on_device: bool = False):
         cs = setup.coarse
         n0 = cs.n0
         off = np.ascontiguousarray(cs.offsets, np.int64)
@@ -301,10 +302,10 @@ class HipContext:
                 rows = V[:sd.n_owned] if sd.boolean else V
                 blocks.append(np.asfortranarray(rows).reshape(-1, order="F"))
             Vc = np.ascontiguousarray(np.concatenate(blocks), np.float64)
-            E = np.ascontiguousarray(cs.E, np.float64)
+            E = None if coarse_on_device else np.ascontiguousarray(cs.E, np.float64)
             self._check(self.L.s2l_upload_coarse(self.h, n0, _p(off, ctypes.c_int64), _p(Vc, ctypes.c_double),
-                                                 _p(E, ctypes.c_double), 1 if setup.dec.symmetric else 0,
-                                                 1 if correction == "additive" else 0))
+                                                 None if E is None else _p(E, ctypes.c_double),
+                                                 1 if setup.dec.symmetric else 0, 1 if correction == "additive" else 0))
             self._keep_coarse = (Vc, E, off)
         else:
             self._check(self.L.s2l_upload_coarse(self.h, 0, _p(off, ctypes.c_int64), None, None, 1,

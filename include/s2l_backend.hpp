@@ -86,7 +86,7 @@ struct SetupData {
     };
     std::vector<Factor> factors;
     // coarse space: V_i blocks (rows x m_i column-major; rows = owned rows for the Boolean PoU, all
-    // local rows otherwise) and A_00 (n0 x n0 row-major)
+    // local rows otherwise) and A_00 (n0 x n0 row-major; empty: formed on the device as W^T A W)
     int32_t n0 = 0;
     std::vector<int64_t> coarse_offsets; ///< nsub + 1
     std::vector<double> V, E;

@@ -86,7 +86,9 @@ int s2l_upload_local_factor(s2l_ctx* ctx, int32_t i, int32_t n, const int32_t* p
 int s2l_finalize_local(s2l_ctx* ctx, int precision);
 
 /* Coarse space: V_i blocks (rows x m_i, column-major, concatenated; rows = owned rows with the
- * Boolean PoU, all local rows otherwise), offsets (nsub + 1) and A_00 dense row-major (n0 x n0).
+ * Boolean PoU, all local rows otherwise), offsets (nsub + 1) and A_00 dense row-major (n0 x n0);
+ * @p E = NULL forms A_00 = W^T A W on the device from the uploaded operator and V
+ * (coarse_space.cpp:95-105; needs s2l_upload_system first), so that a caller need not assemble it.
  * @p symmetric: Cholesky of A_00, else LU with partial pivoting (both FP64, factored on the host
  * at upload). @p correction: 0 deflated, 1 additive (two_level_schwarz.hpp:69). n0 = 0 gives the
  * one-level preconditioner (two_level_schwarz.cpp:136-139). Replaces CoarseSpace
@@ -170,7 +172,9 @@ typedef struct s2l_krylov_opts {
 
 typedef struct s2l_krylov_result {
     int converged;
-    int breakdown;
+    int breakdown;    /* BiCGStab: (r^, A y) = 0 or not finite (bicgstab.cpp:97-100); GMRES: a residual
+                         was not finite and the solve stopped unconverged (the reference's GMRES would
+                         restart until maxit) */
     int iterations;
     int restarts;
     double rel_residual;

@@ -317,3 +317,22 @@ def test_spmv_groups_bitwise(nrhs):
     for k in range(nrhs):
         assert np.array_equal(Y[k], M.ctx.spmv(X[k]))
         assert np.array_equal(Y[k], O.spmv(X[k]))
+
+
+@pytest.mark.parametrize("name,kw,N,correction", [("poisson3d", dict(nx=16), 8, "deflated"),
+                                                   ("convdiff2d", dict(nx=24), 4, "additive")])
+def test_coarse_matrix_formed_on_the_device(name, kw, N, correction):
+    """s2l_upload_coarse with E = NULL: A_00 = W^T A W formed on the device from the uploaded operator
+    and basis (coarse_space.cpp:95-105) gives the apply of the host's A_00 (Cholesky for Poisson,
+    LU with pivoting for convection-diffusion) to 1e-12, and the oracle's."""
+    from amp_amd import device
+    st = _setup(name, kw, N, nev=8, correction=correction)
+    print(f"case device A_00 {name} {kw} N={N} {correction}", flush=True)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    zh = device.HipTwoLevelSchwarz(st, "double").apply(r)
+    ctx = device.HipContext()
+    ctx.upload_setup(st, "double", coarse_on_device=True)
+    zd = ctx.apply(r)
+    zo = orc.OracleSchwarz(st).apply(r)
+    assert np.linalg.norm(zd - zh) / np.linalg.norm(zh) < 1e-12
+    assert np.linalg.norm(zd - zo) / np.linalg.norm(zo) < 1e-12

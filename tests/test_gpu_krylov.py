@@ -399,3 +399,35 @@ def test_phase_timers_count_the_reference_timers():
     parts = sum(ph[k][0] for k in ("solve.preconditioner.coarse.restrict", "solve.preconditioner.coarse.solve",
                                    "solve.preconditioner.coarse.prolong"))
     assert 0 < parts <= ph["solve.preconditioner.coarse"][0] * 1.05
+
+
+@pytest.mark.parametrize("method", ["gmres", "bicgstab"])
+def test_non_finite_right_hand_side_stops_as_a_breakdown(method):
+    """A NaN in the right-hand side (and an Inf in another) ends the solve unconverged with the
+    breakdown flag after at most one iteration, instead of restarting until maxit (BiCGStab:
+    bicgstab.cpp:97-100; GMRES: no reference counterpart); the finite right-hand side solved in the
+    same batch is untouched (the same bits as alone)."""
+    st = _st(*CASES[1])
+    M = _ctx(st)
+    b = st.dec.b.copy()
+    bn = b.copy()
+    bn[len(b) // 3] = np.nan
+    bi = b.copy()
+    bi[len(b) // 2] = np.inf
+    X, R = M.ctx.krylov(np.stack([bn, b, bi]), None, method, 30, 1e-8, 1000)
+    for k in (0, 2):
+        assert not R[k].converged and R[k].breakdown and R[k].iterations <= 1, (k, R[k])
+    x, r = M.ctx.krylov(b, None, method, 30, 1e-8, 1000)
+    assert R[1].converged and R[1].iterations == r.iterations and np.array_equal(X[1], x)
+
+
+def test_non_finite_factor_is_refused():
+    """A local factor with a NaN entry is refused at upload (S2L_ENUMERIC)."""
+    from amp_amd import device
+    st = _st(*CASES[1])
+    f = st.factors[0]
+    bad = type(f)(**{**f.__dict__, "l_val": f.l_val.copy()})
+    bad.l_val[len(bad.l_val) // 2] = np.nan
+    ctx = device.HipContext()
+    with pytest.raises(RuntimeError, match="non-finite"):
+        ctx.upload_setup(type(st)(st.dec, st.subs, [bad] + list(st.factors[1:]), st.coarse), "double")
