@@ -1208,12 +1208,17 @@ __global__ __launch_bounds__(64) void k_bicg_control(BiArgs a, int stage)
             st.rho_old = st.alpha = st.omega = 1.0;
             st.first = 1;
             const bool zero = st.bnorm == 0.0;  // x = 0, converged (bicgstab.cpp:24-29)
-            if (zero) {
-                st.conv = st.done = 1;
-                biPush(a, st, q, 0.0);
+            // a non-finite right-hand side ends the solve as a breakdown (the reference's tests
+            // against rtol * ||b|| would pass for ||b|| = inf and report convergence)
+            const bool bad = !isfinite(st.bnorm);
+            if (zero || bad) {
+                st.done = 1;
+                st.conv = zero ? 1 : 0;
+                st.brk = bad ? 1 : 0;
+                biPush(a, st, q, zero ? 0.0 : st.bnorm);
             }
             M(kBiZero, q) = zero ? 0 : 1;
-            M(kBiLive, q) = zero ? 1 : 0;
+            M(kBiLive, q) = (zero || bad) ? 1 : 0;
             for (int m = kBiNom; m < kBiMasks; ++m) M(m, q) = 1;
             for (int m = 0; m < kBiMasks; ++m)
                 for (int k = 0; k < 4; ++k) C(m, q, k) = 0.0;
