@@ -236,11 +236,13 @@ struct s2l_ctx {
     bool queue_ok = false;
     Packed pk;
     DevBuf<SubDesc> sub_d;
-    DevBuf<int32_t> levw_d, levh_d, ridx_d, cptr_d, cidx_d, gmap_d, lperm_d;
+    DevBuf<int32_t> levw_d, levh_d, ridx_d, cptr_d, cend_d, cidx_d, gmap_d, lperm_d;
     DevBuf<SnDesc> levdesc_d;
     DevBuf<int32_t> levrow_d, levcb_d, levce_d;
     DevBuf<StepDesc> steps_d;
     size_t y_off = 0, st_off = 0, step_off = 0;
+    size_t slot_off = 0, slot_off2 = 0;  // LDS slots of a task without child tasks (one rhs / pair)
+    int32_t slot_cap = 0, slot_cap2 = 0;
     // FP32 factors, two right-hand sides per workgroup (a float2 vector in LDS): its LDS layout
     size_t y_off2 = 0, st_off2 = 0, step_off2 = 0, lds_bytes2 = 0;
     bool pair_ok = false;
@@ -470,6 +472,9 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     }
     a.ridx = c->ridx_d.p;
     a.cptr = c->cptr_d.p;
+    a.cend = c->cend_d.p;
+    a.slot_off = static_cast<int32_t>(pair ? c->slot_off2 : c->slot_off);
+    a.slot_cap = pair ? c->slot_cap2 : c->slot_cap;
     a.cidx = c->cidx_d.p;
     a.tbuf = reinterpret_cast<T*>(c->work.tbuf);
     a.gmap = c->gmap_d.p;
@@ -1496,11 +1501,20 @@ void finalizeLocal(s2l_ctx* c, int precision)
     c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
     c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
     c->queue_ok = c->lds_bytes <= c->max_lds;
+    // the slots of the largest task without child tasks that fit in the rest of the LDS
+    auto slotRoom = [&](size_t base, size_t elem, size_t& off, int32_t& cap) {
+        off = (base + 15) / 16 * 16;
+        const size_t room = c->max_lds > off ? (c->max_lds - off) / elem : 0;
+        cap = static_cast<int32_t>(std::min<size_t>(room, static_cast<size_t>(P.max_task_slots)));
+        return off + static_cast<size_t>(cap) * elem;
+    };
+    if (c->queue_ok) c->lds_bytes = slotRoom(c->lds_bytes, esz, c->slot_off, c->slot_cap);
     c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
     c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
     c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
     c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
     c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
+    if (c->pair_ok) c->lds_bytes2 = slotRoom(c->lds_bytes2, sizeof(float2), c->slot_off2, c->slot_cap2);
     if (c->pair_ok)
         HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(c->lds_bytes2)));
@@ -1535,6 +1549,7 @@ void finalizeLocal(s2l_ctx* c, int precision)
     c->steps_d.upload(P.steps, c->stream);
     c->ridx_d.upload(P.ridx, c->stream);
     c->cptr_d.upload(P.cptr, c->stream);
+    c->cend_d.upload(P.cend, c->stream);
     c->cidx_d.upload(P.cidx, c->stream);
     c->gmap_d.upload(gmap, c->stream);
     c->lperm_d.upload(lperm, c->stream);

@@ -108,34 +108,6 @@ template <typename T, typename P> __device__ inline T gatherDot8(const P* p, int
     return acc;
 }
 
-// x[row] minus the products pushed to this row by its descendants: one contiguous slot range,
-// subtracted in slot order, eight loads per batch.
-template <typename T, typename TF>
-__device__ inline T pullRow(const LocalArgs<T, TF>& a, const SubDesc& sd, const T* tb, const T* x, int row)
-{
-    T v = x[row];
-    const int cb = a.cptr[sd.cptr_off + row], ce = a.cptr[sd.cptr_off + row + 1];
-    int q = cb;
-    for (; q + kB <= ce; q += kB) {
-        T t[kB];
-#pragma unroll
-        for (int u = 0; u < kB; ++u) t[u] = tb[q + u];
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < kB; ++u) v -= t[u];
-    }
-    if (q < ce) {
-        T t[kB - 1];
-#pragma unroll
-        for (int u = 0; u < kB - 1; ++u) t[u] = q + u < ce ? tb[q + u] : T(0);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < kB - 1; ++u)
-            if (q + u < ce) v -= t[u];
-    }
-    return v;
-}
-
 // x[row] minus the products in the slots [cb, ce) pushed to it by its descendants, subtracted in
 // slot order, eight loads per batch (the slot range comes with the chunk's rows).
 template <typename T> __device__ inline T pullSlots(const T* tb, T v, int cb, int ce)
@@ -159,6 +131,34 @@ template <typename T> __device__ inline T pullSlots(const T* tb, T v, int cb, in
             if (q + u < ce) v -= t[u];
     }
     return v;
+}
+
+// Slots of a task without child tasks live in LDS (kLds: ls holds the task's slots [sb, sb + sc)
+// of its subdomain's T buffer): the task is the only one pushing to its rows. Otherwise all slots
+// are in global memory (tb). The choice is a template parameter, so each sweep's code sees one
+// address space per access.
+template <typename T, bool kLds> struct Slots {
+    T* tb;   ///< the subdomain's T buffer (global)
+    T* ls;   ///< kLds: the task's slots in LDS
+    int sb, sc;
+    /// x minus the slots [cb, ce) of a row, in slot order
+    __device__ T pull(T x, int cb, int ce) const { return kLds ? pullSlots(ls, x, cb - sb, ce - sb) : pullSlots(tb, x, cb, ce); }
+    /// a pushed product into its slot d
+    __device__ void push(int d, T v) const
+    {
+        if (kLds && static_cast<unsigned>(d - sb) < static_cast<unsigned>(sc))
+            ls[d - sb] = v;
+        else
+            tb[d] = v;
+    }
+};
+
+// x[row] minus the products pushed to this row by its descendants: the row's slot range
+// [cptr, cend), subtracted in slot order.
+template <typename T, typename TF, bool kLds>
+__device__ inline T pullRow(const LocalArgs<T, TF>& a, const SubDesc& sd, const Slots<T, kLds>& sl, const T* x, int row)
+{
+    return sl.pull(x[row], a.cptr[sd.cptr_off + row], a.cend[sd.cptr_off + row]);
 }
 
 // ------------------------------------------------------------------ wide supernodes (w > 64): whole workgroup
@@ -215,7 +215,7 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
 // row. kTri 1 skips the column quads right of the rows (lower triangle), 2 those left of them
 // (upper); the stored block is zero there. All lanes of a wavefront run the same rows.
 template <typename T, typename TF, int kTri, typename Out>
-__device__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
+__device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
 {
     // FP64 factors: each pass takes two groups of R rows, nw * R rows apart, whose loads are
     // issued together (FP32 factors leave no registers for the second group). The block's
@@ -282,7 +282,7 @@ __device__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, 
 // y = X x_C for the inverse X of a wide supernode's diagonal block (lower or upper triangle),
 // written over x_C.
 template <typename T, typename TF>
-__device__ void wideDiag(const TF* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw, int nt)
+__device__ __forceinline__ void wideDiag(const TF* X, int w, bool lower, int c0, T* x, T* ybuf, int tid, int lane, int wave, int nw, int nt)
 {
     const int ld = wideLd(w);
     if (lower)
@@ -295,23 +295,23 @@ __device__ void wideDiag(const TF* X, int w, bool lower, int c0, T* x, T* ybuf, 
 }
 
 // Forward: every row pulls its products, y = X x_C, then the rows below push B y to their slots.
-template <typename T, typename TF>
-__device__ void fwdWide(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, T* tb, int tid, int lane,
+template <typename T, typename TF, bool kLds>
+__device__ __forceinline__ void fwdWide(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, T* x, T* ybuf, const Slots<T, kLds>& sl, int tid, int lane,
                         int wave, int nt)
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave;
-    for (int j = tid; j < w; j += nt) x[c0 + j] = pullRow(a, sd, tb, x, c0 + j);
+    for (int j = tid; j < w; j += nt) x[c0 + j] = pullRow(a, sd, sl, x, c0 + j);
     __syncthreads();
     wideDiag(a.diag + s.diag, w, true, c0, x, ybuf, tid, lane, wave, nw, nt);
     const int32_t* dst = a.cidx + sd.dst_off + s.toff;
-    denseRows<T, TF, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { tb[dst[k]] = v; });
+    denseRows<T, TF, 0>(a.below + s.below, wideLd(w), h, w, x + c0, lane, wave, nw, [&](int k, T v) { sl.push(dst[k], v); });
 }
 
 // Backward: x_C -= B^T x_R with G groups of w threads, group g summing the rows k = g, g + G, ...
 // (consecutive threads on consecutive columns of a row), the partial sums combined in group
 // order through LDS. Then x_C = X x_C.
 template <typename T, typename TF>
-__device__ void bwdWide(const LocalArgs<T, TF>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
+__device__ __forceinline__ void bwdWide(const LocalArgs<T, TF>& a, const SnDesc s, T* x, T* ybuf, int tid, int lane, int wave, int nt)
 {
     const int w = s.w, h = s.h, c0 = s.c0, nw = nt / kWave, ld = wideLd(w);
     if (h > 0) {
@@ -336,12 +336,12 @@ __device__ void bwdWide(const LocalArgs<T, TF>& a, const SnDesc s, T* x, T* ybuf
 // Dense top, the rows [qlo, qhi) of part of a skeleton task: every row of the skeleton pulls the
 // products of the subtrees (b_S - L_SR y_R, staged in order in ybuf), then x_q = sum over c of
 // S^-1(q, c) b_c for the part's rows, four rows per wavefront pass with 16-byte loads of S^-1.
-template <typename T, typename TF>
-__device__ void denseTop(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, const int32_t* rows, int qlo, int qhi, T* x,
-                         T* ybuf, const T* tb, int tid, int lane, int wave, int nt)
+template <typename T, typename TF, bool kLds>
+__device__ __forceinline__ void denseTop(const LocalArgs<T, TF>& a, const SubDesc& sd, const SnDesc s, const int32_t* rows, int qlo, int qhi, T* x,
+                         T* ybuf, const Slots<T, kLds>& sl, int tid, int lane, int wave, int nt)
 {
     const int S = s.w, ld = wideLd(S), nw = nt / kWave;
-    for (int q = tid; q < S; q += nt) ybuf[q] = pullRow(a, sd, tb, x, rows[q]);
+    for (int q = tid; q < S; q += nt) ybuf[q] = pullRow(a, sd, sl, x, rows[q]);
     __syncthreads();
     denseRows<T, TF, 0>(a.diag + s.diag + static_cast<int64_t>(qlo) * ld, ld, qhi - qlo, S, ybuf, lane, wave, nw,
                         [&](int r, T v) { x[rows[qlo + r]] = v; });
@@ -371,10 +371,10 @@ __device__ inline int findItem(const int* prefix, int count, int t)
 //             their slots;
 //   backward: every column subtracts the products with the rows below; y = inverse(diagonal) * x.
 // The step's fields are uniform over the workgroup.
-template <typename T, typename TF>
-__device__ void runChunk(const LocalArgs<T, TF>& a, const SubDesc& sd, int q0, int cnt, int wbase, int hbase, int W, int H,
-                         int roff, bool forward, T* x, T* ybuf, SnDesc* st, int* wp, int* hp, T* tb, int tid, int lane,
-                         int wave, int nt)
+template <typename T, typename TF, bool kLds>
+__device__ __forceinline__ void runChunk(const LocalArgs<T, TF>& a, const SubDesc& sd, int q0, int cnt, int wbase, int hbase, int W, int H,
+                         int roff, bool forward, T* x, T* ybuf, SnDesc* st, int* wp, int* hp, const Slots<T, kLds>& sl, int tid,
+                         int lane, int wave, int nt)
 {
     for (int i = tid; i < cnt; i += nt) {
         st[i] = a.lev_desc[q0 + i];
@@ -384,7 +384,7 @@ __device__ void runChunk(const LocalArgs<T, TF>& a, const SubDesc& sd, int q0, i
     if (forward)
         for (int t = tid; t < W; t += nt) {
             const int row = a.lev_row[roff + t];
-            x[row] = pullSlots(tb, x[row], a.lev_cb[roff + t], a.lev_ce[roff + t]);
+            x[row] = sl.pull(x[row], a.lev_cb[roff + t], a.lev_ce[roff + t]);
         }
     __syncthreads();
     // the supernode of each thread's first row (t = tid) and first row below, found once
@@ -441,7 +441,7 @@ __device__ void runChunk(const LocalArgs<T, TF>& a, const SubDesc& sd, int q0, i
             const SnDesc& s = st[i];
             const int k = t - hp[i];
             const TF* B = a.below + s.below + static_cast<int64_t>(k) * s.w;
-            tb[a.cidx[sd.dst_off + s.toff + k]] = dot8(B, 1, x + s.c0, 1, s.w, T(0));
+            sl.push(a.cidx[sd.dst_off + s.toff + k], dot8(B, 1, x + s.c0, 1, s.w, T(0)));
         }
         __syncthreads();
     }
@@ -458,9 +458,9 @@ __device__ inline int64_t uniform64(int64_t v)
 // The steps of one sweep of a task, in order: chunks of the small supernodes of a level and the
 // level's wide supernodes, level after level. The steps are staged in LDS kStepBlock at a time;
 // every step ends with a barrier.
-template <typename T, typename TF>
-__device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0, int nsteps, bool forward, T* x, unsigned char* smem,
-                         T* tb, const int32_t* rows, int qlo, int qhi, int tid, int lane, int wave, int nt)
+template <typename T, typename TF, bool kLds>
+__device__ __forceinline__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0, int nsteps, bool forward, T* x, unsigned char* smem,
+                         const Slots<T, kLds>& sl, const int32_t* rows, int qlo, int qhi, int tid, int lane, int wave, int nt)
 {
     T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
     SnDesc* st = reinterpret_cast<SnDesc*>(smem + a.st_off);
@@ -475,7 +475,7 @@ __device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0
             const StepDesc& sp = sb[k];
             if (uniform(sp.kind) == 0) {
                 runChunk(a, sd, uniform(sp.q0), uniform(sp.cnt), uniform(sp.wbase), uniform(sp.hbase), uniform(sp.W),
-                         uniform(sp.H), uniform(sp.roff), forward, x, ybuf, st, wp, hp, tb, tid, lane, wave, nt);
+                         uniform(sp.H), uniform(sp.roff), forward, x, ybuf, st, wp, hp, sl, tid, lane, wave, nt);
             } else {
                 SnDesc d;
                 d.c0 = uniform(sp.wide.c0);
@@ -486,9 +486,9 @@ __device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0
                 d.below = uniform64(sp.wide.below);
                 d.ridx = uniform64(sp.wide.ridx);
                 if (uniform(sp.kind) == 2)
-                    denseTop(a, sd, d, rows, qlo, qhi, x, ybuf, tb, tid, lane, wave, nt);
+                    denseTop(a, sd, d, rows, qlo, qhi, x, ybuf, sl, tid, lane, wave, nt);
                 else if (forward)
-                    fwdWide(a, sd, d, x, ybuf, tb, tid, lane, wave, nt);
+                    fwdWide(a, sd, d, x, ybuf, sl, tid, lane, wave, nt);
                 else
                     bwdWide(a, d, x, ybuf, tid, lane, wave, nt);
                 __syncthreads();
@@ -502,7 +502,7 @@ __device__ void runSteps(const LocalArgs<T, TF>& a, const SubDesc& sd, int step0
 // one subdomain's entries run in order by a single workgroup); a later launch on the same stream
 // sees the T buffer and the shared results of earlier launches.
 template <typename T, typename TF>
-__device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
+__device__ __forceinline__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
 {
     // kPair: T = float2 holds right-hand sides 2 r and 2 r + 1 (the second absent when nrhs is odd)
     constexpr bool kPair = std::is_same_v<T, float2>;
@@ -521,6 +521,10 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
     T* tb = a.tbuf + static_cast<int64_t>(r) * a.t_total + sd.t_off;
     T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
     const int32_t* rows = a.task_rows + td.rows_first;
+    // a forward sweep of a task without child tasks keeps its rows' slots in LDS when they fit
+    const bool ldsSlots = !backward && td.slot_count > 0 && td.slot_count <= a.slot_cap;
+    const Slots<T, false> slg{tb, nullptr, 0, 0};
+    const Slots<T, true> sll{tb, reinterpret_cast<T*>(smem + a.slot_off), td.slot_base, td.slot_count};
     // the rows this entry computes: all of the task's, or a dense top part's range
     const int qlo = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * part / td.nparts) : 0;
     const int qhi = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * (part + 1) / td.nparts) : td.rows_count;
@@ -536,7 +540,10 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
                 x[row] = static_cast<T>(rhs[g]);
         }
         __syncthreads();
-        runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, tb, rows, qlo, qhi, tid, lane, wave, nt);
+        if (ldsSlots)
+            runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, sll, rows, qlo, qhi, tid, lane, wave, nt);
+        else
+            runSteps(a, sd, td.f_step, td.f_nsteps, true, x, smem, slg, rows, qlo, qhi, tid, lane, wave, nt);
         if (!combined) {
             for (int q = tid; q < td.rows_count; q += blockDim.x) xg[rows[q]] = x[rows[q]];
             __syncthreads();
@@ -548,7 +555,7 @@ __device__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned 
         for (int q = tid; q < td.ext_count; q += blockDim.x) x[ext[q]] = xg[ext[q]];
         __syncthreads();
     }
-    runSteps(a, sd, td.b_step, td.b_nsteps, false, x, smem, tb, rows, qlo, qhi, tid, lane, wave, nt);
+    runSteps(a, sd, td.b_step, td.b_nsteps, false, x, smem, slg, rows, qlo, qhi, tid, lane, wave, nt);
     if (td.nchild > 0)
         for (int q = qlo + tid; q < qhi; q += blockDim.x) xg[rows[q]] = x[rows[q]];
     // the owned rows of the result (z = x0 + y), or the local buffer (scaled by D_i) for a reverse add

@@ -57,10 +57,10 @@ namespace {
 // level.
 //
 // The same levels are also emitted as steps (StepDesc): every chunk, then every wide supernode of
-// a level, in the order the kernel runs them. @p cptr: the subdomain's slot pointers for the
-// forward sweep (the pulled range of every row), null for a backward sweep.
+// a level, in the order the kernel runs them. @p cptr / @p cend: the subdomain's slot ranges for
+// the forward sweep (the pulled range [cptr[r], cend[r]) of every row), null for a backward sweep.
 void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, int32_t& lev_off, int32_t& nlev,
-                  int32_t& step_off, int32_t& nsteps, const int32_t* cptr)
+                  int32_t& step_off, int32_t& nsteps, const int32_t* cptr, const int32_t* cend)
 {
     lev_off = static_cast<int32_t>(P.lev.size());
     nlev = static_cast<int32_t>(byLevel.size());
@@ -107,7 +107,7 @@ void appendLevels(Packed& P, const std::vector<std::vector<int32_t>>& byLevel, i
                 for (int32_t j = 0; j < d.w; ++j) {
                     P.lev_row.push_back(d.c0 + j);
                     P.lev_cb.push_back(cptr[d.c0 + j]);
-                    P.lev_ce.push_back(cptr[d.c0 + j + 1]);
+                    P.lev_ce.push_back(cend[d.c0 + j]);
                 }
             }
         for (int32_t ch = 0; ch < nchunk; ++ch) {
@@ -483,25 +483,85 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
                 dense[static_cast<std::size_t>(s)] = skel[static_cast<std::size_t>(s)] &&
                                                      width[static_cast<std::size_t>(rootOf[static_cast<std::size_t>(s)])] <= dense_top_max;
         }
-        // contributions: the product of row k below supernode d goes to a slot of its target row;
-        // the slots of one row are contiguous (ordered by d), so a row pulls one contiguous range.
-        // The supernodes of a dense top push nothing (their products are inside S^-1).
+        const int32_t t_base = static_cast<int32_t>(P.task.size());
+        std::vector<int32_t> taskOf(static_cast<std::size_t>(nsf), -1);
+        std::vector<int32_t> taskRoot;
+        for (int32_t s = nsf - 1; s >= 0; --s) {
+            const int32_t p = parent[static_cast<std::size_t>(s)];
+            if (!samePartition) {
+                // the backward sweep uses another supernode partition (LU with another
+                // structure): every subdomain is a single task
+                if (taskRoot.empty()) taskRoot.push_back(nsf - 1);
+                taskOf[static_cast<std::size_t>(s)] = 0;
+                continue;
+            }
+            if (p < 0) {
+                taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
+                taskRoot.push_back(s);
+            } else if (skel[static_cast<std::size_t>(s)] || !skel[static_cast<std::size_t>(p)]) {
+                // the skeleton is one task with the root; a subtree below a skeleton supernode
+                // was assigned when its parent was visited
+                if (taskOf[static_cast<std::size_t>(s)] < 0) taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(p)];
+            }
+            if (!skel[static_cast<std::size_t>(s)]) continue;
+            // pack the subtrees below skeleton supernode s into tasks; a skeleton child gets its
+            // own task when it is visited
+            std::vector<int32_t> sub;
+            for (int32_t c : kids[static_cast<std::size_t>(s)])
+                if (!skel[static_cast<std::size_t>(c)]) sub.push_back(c);
+            std::stable_sort(sub.begin(), sub.end(), [&](int32_t x, int32_t y) { return estimate(x) > estimate(y); });
+            const int64_t cap = sub.empty() ? 0 : std::max(limit, estimate(sub.front()));
+            std::vector<int64_t> load;
+            std::vector<int32_t> binTask;
+            for (int32_t c : sub) {
+                std::size_t b = 0;
+                while (b < load.size() && load[b] + estimate(c) > cap) ++b;
+                if (b == load.size()) {
+                    load.push_back(0);
+                    binTask.push_back(static_cast<int32_t>(taskRoot.size()));
+                    taskRoot.push_back(c);
+                }
+                load[b] += estimate(c);
+                taskOf[static_cast<std::size_t>(c)] = binTask[b];
+            }
+        }
+        const int32_t nt = static_cast<int32_t>(taskRoot.size());
+        std::vector<std::vector<int32_t>> tsn(static_cast<std::size_t>(nt));
+        for (int32_t s = 0; s < nsf; ++s) tsn[static_cast<std::size_t>(taskOf[static_cast<std::size_t>(s)])].push_back(s);
+        // contributions: the product of row k below supernode d goes to a slot of its target row.
+        // Slots are numbered task by task (the rows of a task in order, a row's slots ordered by
+        // d), so a row pulls one contiguous range [cptr, cend) and the slots of a task's rows are
+        // one contiguous range: a task without child tasks, the only one pushing to its rows, can
+        // keep them in LDS. The supernodes of a dense top push nothing (their products are inside
+        // S^-1).
+        std::vector<int32_t> tslotBase(static_cast<std::size_t>(nt), 0), tslotCount(static_cast<std::size_t>(nt), 0);
         {
-            std::vector<int32_t> cnt(static_cast<std::size_t>(n) + 1, 0);
+            std::vector<int32_t> cnt(static_cast<std::size_t>(n), 0);
             int64_t pushed = 0;
             for (int32_t s = 0; s < nsf; ++s) {
                 if (dense[static_cast<std::size_t>(s)]) continue;
                 const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
-                for (int32_t k = 0; k < d.h; ++k) ++cnt[static_cast<std::size_t>(P.ridx[static_cast<std::size_t>(d.ridx + k)]) + 1];
+                for (int32_t k = 0; k < d.h; ++k) ++cnt[static_cast<std::size_t>(P.ridx[static_cast<std::size_t>(d.ridx + k)])];
                 pushed += d.h;
             }
             sd.cptr_off = static_cast<int32_t>(P.cptr.size());
             sd.dst_off = static_cast<int32_t>(P.cidx.size());
-            std::vector<int32_t> ptr(static_cast<std::size_t>(n) + 1, 0);
-            for (int32_t r = 0; r < n; ++r) ptr[static_cast<std::size_t>(r) + 1] = ptr[static_cast<std::size_t>(r)] + cnt[static_cast<std::size_t>(r) + 1];
-            for (int32_t r = 0; r <= n; ++r) P.cptr.push_back(ptr[static_cast<std::size_t>(r)]);
+            std::vector<int32_t> beg(static_cast<std::size_t>(n), 0), end(static_cast<std::size_t>(n), 0);
+            int32_t pos = 0;
+            for (int32_t t = 0; t < nt; ++t) {
+                tslotBase[static_cast<std::size_t>(t)] = pos;
+                for (int32_t sn : tsn[static_cast<std::size_t>(t)])
+                    for (int32_t r = fst[sn]; r < fst[sn + 1]; ++r) {
+                        beg[static_cast<std::size_t>(r)] = pos;
+                        pos += cnt[static_cast<std::size_t>(r)];
+                        end[static_cast<std::size_t>(r)] = pos;
+                    }
+                tslotCount[static_cast<std::size_t>(t)] = pos - tslotBase[static_cast<std::size_t>(t)];
+            }
+            P.cptr.insert(P.cptr.end(), beg.begin(), beg.end());
+            P.cend.insert(P.cend.end(), end.begin(), end.end());
             P.cidx.resize(static_cast<std::size_t>(sd.dst_off) + static_cast<std::size_t>(toff), 0);
-            std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
+            std::vector<int32_t> next(beg);
             for (int32_t s = 0; s < nsf; ++s) {
                 if (dense[static_cast<std::size_t>(s)]) continue;
                 const SnDesc& d = P.sn[static_cast<std::size_t>(f_base + s)];
@@ -563,51 +623,6 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         }
         P.bytes_index += static_cast<int64_t>(nsf + nsb) * static_cast<int64_t>(sizeof(SnDesc));
 
-        const int32_t t_base = static_cast<int32_t>(P.task.size());
-        std::vector<int32_t> taskOf(static_cast<std::size_t>(nsf), -1);
-        std::vector<int32_t> taskRoot;
-        for (int32_t s = nsf - 1; s >= 0; --s) {
-            const int32_t p = parent[static_cast<std::size_t>(s)];
-            if (!samePartition) {
-                // the backward sweep uses another supernode partition (LU with another
-                // structure): every subdomain is a single task
-                if (taskRoot.empty()) taskRoot.push_back(nsf - 1);
-                taskOf[static_cast<std::size_t>(s)] = 0;
-                continue;
-            }
-            if (p < 0) {
-                taskOf[static_cast<std::size_t>(s)] = static_cast<int32_t>(taskRoot.size());
-                taskRoot.push_back(s);
-            } else if (skel[static_cast<std::size_t>(s)] || !skel[static_cast<std::size_t>(p)]) {
-                // the skeleton is one task with the root; a subtree below a skeleton supernode
-                // was assigned when its parent was visited
-                if (taskOf[static_cast<std::size_t>(s)] < 0) taskOf[static_cast<std::size_t>(s)] = taskOf[static_cast<std::size_t>(p)];
-            }
-            if (!skel[static_cast<std::size_t>(s)]) continue;
-            // pack the subtrees below skeleton supernode s into tasks; a skeleton child gets its
-            // own task when it is visited
-            std::vector<int32_t> sub;
-            for (int32_t c : kids[static_cast<std::size_t>(s)])
-                if (!skel[static_cast<std::size_t>(c)]) sub.push_back(c);
-            std::stable_sort(sub.begin(), sub.end(), [&](int32_t x, int32_t y) { return estimate(x) > estimate(y); });
-            const int64_t cap = sub.empty() ? 0 : std::max(limit, estimate(sub.front()));
-            std::vector<int64_t> load;
-            std::vector<int32_t> binTask;
-            for (int32_t c : sub) {
-                std::size_t b = 0;
-                while (b < load.size() && load[b] + estimate(c) > cap) ++b;
-                if (b == load.size()) {
-                    load.push_back(0);
-                    binTask.push_back(static_cast<int32_t>(taskRoot.size()));
-                    taskRoot.push_back(c);
-                }
-                load[b] += estimate(c);
-                taskOf[static_cast<std::size_t>(c)] = binTask[b];
-            }
-        }
-        const int32_t nt = static_cast<int32_t>(taskRoot.size());
-        std::vector<std::vector<int32_t>> tsn(static_cast<std::size_t>(nt));
-        for (int32_t s = 0; s < nsf; ++s) tsn[static_cast<std::size_t>(taskOf[static_cast<std::size_t>(s)])].push_back(s);
         std::vector<int32_t> heightT(static_cast<std::size_t>(nsf), 0);
         for (int32_t s = 0; s < nsf; ++s) {
             const int32_t p = parent[static_cast<std::size_t>(s)];
@@ -626,6 +641,8 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             const int32_t r = taskRoot[static_cast<std::size_t>(t)];
             TaskDesc td{};
             td.sub = static_cast<int32_t>(i);
+            td.slot_base = tslotBase[static_cast<std::size_t>(t)];
+            td.slot_count = tslotCount[static_cast<std::size_t>(t)];
             td.depth = samePartition ? depth[static_cast<std::size_t>(r)] : 0;
             const int32_t pr = samePartition ? parent[static_cast<std::size_t>(r)] : -1;
             td.parent = (pr >= 0) ? t_base + taskOf[static_cast<std::size_t>(pr)] : -1;
@@ -692,7 +709,8 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             for (int32_t s : list) hmax = std::max(hmax, heightT[static_cast<std::size_t>(s)]);
             std::vector<std::vector<int32_t>> byLev(static_cast<std::size_t>(hmax) + 1);
             for (int32_t s : list) byLev[static_cast<std::size_t>(heightT[static_cast<std::size_t>(s)])].push_back(f_base + s);
-            appendLevels(P, byLev, td.f_lev, td.f_nlev, td.f_step, td.f_nsteps, P.cptr.data() + sd.cptr_off);
+            appendLevels(P, byLev, td.f_lev, td.f_nlev, td.f_step, td.f_nsteps, P.cptr.data() + sd.cptr_off,
+                         P.cend.data() + sd.cptr_off);
             // backward levels: backward supernodes of the task (same columns when the partitions agree)
             std::vector<int32_t> blist;
             if (samePartition)
@@ -706,7 +724,7 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             }
             std::vector<std::vector<int32_t>> byLevB(static_cast<std::size_t>(dmax - dmin) + 1);
             for (int32_t s : blist) byLevB[static_cast<std::size_t>(depthB[static_cast<std::size_t>(s)] - dmin)].push_back(b_base + s);
-            appendLevels(P, byLevB, td.b_lev, td.b_nlev, td.b_step, td.b_nsteps, nullptr);
+            appendLevels(P, byLevB, td.b_lev, td.b_nlev, td.b_step, td.b_nsteps, nullptr, nullptr);
             // rows of other tasks read by the backward sweep
             td.ext_first = static_cast<int32_t>(P.ext_rows.size());
             std::vector<int32_t> ext;
@@ -727,6 +745,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             TaskDesc& td = P.task[static_cast<std::size_t>(t_base + t)];
             td.child_first = static_cast<int32_t>(P.task_child.size());
             td.nchild = static_cast<int32_t>(children[static_cast<std::size_t>(t)].size());
+            // other tasks push to the rows of a task with children: its slots stay in global memory
+            if (td.nchild > 0) td.slot_count = 0;
+            P.max_task_slots = std::max(P.max_task_slots, td.slot_count);
             P.task_child.insert(P.task_child.end(), children[static_cast<std::size_t>(t)].begin(), children[static_cast<std::size_t>(t)].end());
         }
         // tasks without child tasks (the subtrees) depend on no other task in the forward sweep:

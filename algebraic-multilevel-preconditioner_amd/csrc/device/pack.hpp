@@ -55,7 +55,7 @@ struct StepDesc {
 struct SubDesc {
     int32_t n;        ///< local size n_i
     int32_t map_off;  ///< offset of the n_i entries of gmap / lperm (= offset of Omega_i in the local buffers)
-    int32_t cptr_off; ///< offset of the n_i + 1 slot pointers (row r pulls T slots [cptr[r], cptr[r+1]))
+    int32_t cptr_off; ///< offset of the n_i slot ranges (row r pulls T slots [cptr[r], cend[r]))
     int32_t n_owned;  ///< n_I (set by the context)
     int64_t t_off;    ///< offset of the subdomain's T buffer region
     int32_t begin;    ///< first owned global row (set by the context)
@@ -78,6 +78,8 @@ struct TaskDesc {
     int32_t f_step, f_nsteps;       ///< forward steps (indices into steps)
     int32_t b_step, b_nsteps;       ///< backward steps
     int32_t nparts;                 ///< dense top: workgroups sharing its rows (one queue entry each); else 0
+    int32_t slot_base, slot_count;  ///< the slots of the task's rows (T buffer of its subdomain); count 0
+                                    ///< for a task with child tasks (they push to its rows too)
 };
 
 /// A dense top: the skeleton task of one elimination tree whose supernodes (all above the subtree
@@ -121,13 +123,14 @@ struct Packed {
     std::vector<StepDesc> steps;      ///< the steps of every task's sweeps
     std::vector<double> diag, below;  ///< values
     std::vector<int32_t> ridx;        ///< below-row indices
-    std::vector<int32_t> cptr, cidx;  ///< forward contributions: slot ranges per row; slot of every product
+    std::vector<int32_t> cptr, cend, cidx;  ///< forward contributions: slot range per row; slot of every product
     std::vector<SubDesc> sub;
     std::vector<int32_t> fwd_base, fwd_count, bwd_base; ///< per subdomain: its forward and backward supernodes in sn
     std::vector<DenseTop> dtop;
     std::vector<int32_t> dtop_sn;     ///< forward supernodes of the dense tops
     std::vector<int32_t> dtop_q;      ///< per subdomain with a dense top: row of every factor position in its top (-1 outside)
     int64_t entries_dense = 0;        ///< entries of the dense tops' inverses (S^2 each, read once per solve)
+    int32_t max_task_slots = 0;       ///< most slots of the rows of a task without child tasks
     int64_t entries_skel_fwd = 0, entries_skel_bwd = 0; ///< factor entries of the supernodes they replace
     int64_t t_total = 0;
     int64_t entries_fwd = 0, entries_bwd = 0;
