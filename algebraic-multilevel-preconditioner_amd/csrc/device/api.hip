@@ -24,7 +24,7 @@
 #include <rocsolver/rocsolver.h>
 
 namespace s2l {
-template <typename T, typename TF> const void* localSolveKernel();
+template <typename T, typename TF> const void* localSolveKernel(int threads);
 }
 
 namespace {
@@ -254,6 +254,8 @@ struct s2l_ctx {
     std::vector<int32_t> qsub_h;
     int tasks_per_sub = 0;  // 0: one subtree task per compute unit (compute units / subdomains)
     int dense_top_max = kDenseTopMax;  // widest skeleton replaced by a dense top (0: none)
+    int local_threads_req = kLocalThreads; // threads of a local-solve workgroup asked for: 1024, or 512 (two per compute unit)
+    int local_threads = kLocalThreads;     // in use (512 falls back to 1024 when half the LDS does not hold the layout)
     int cus = 0;
     int64_t local_total = 0;
     size_t lds_bytes = 0;
@@ -508,7 +510,7 @@ void launchQueue(s2l_ctx* c, LocalArgs<T, TF>& a, const int32_t* q, int32_t coun
     a.serial = serial ? 1 : 0;
     const bool pair = std::is_same_v<T, float2>;  // a workgroup per entry and pair of right-hand sides
     const int grid = serial ? 1 : count * (pair ? (a.nrhs + 1) / 2 : a.nrhs);
-    launchLocalSolve<T, TF>(a, grid, pair ? c->lds_bytes2 : c->lds_bytes, s);
+    launchLocalSolve<T, TF>(a, grid, pair ? c->lds_bytes2 : c->lds_bytes, c->local_threads, s);
 }
 
 template <typename T> LsArgs<T> lsArgs(s2l_ctx* c, const double* rhs, const double* x0, double* out, int mode,
@@ -1500,11 +1502,16 @@ void finalizeLocal(s2l_ctx* c, int precision)
     c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
     c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
     c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
-    c->queue_ok = c->lds_bytes <= c->max_lds;
+    // two 512-thread workgroups share a compute unit's LDS; when half of it does not hold the
+    // layout, the workgroups take 1024 threads
+    c->local_threads = c->local_threads_req;
+    if (c->local_threads == 512 && c->lds_bytes > c->max_lds / 2) c->local_threads = 1024;
+    const size_t lds_max = c->local_threads == 512 ? c->max_lds / 2 : c->max_lds;
+    c->queue_ok = c->lds_bytes <= lds_max;
     // the slots of the largest task without child tasks that fit in the rest of the LDS
     auto slotRoom = [&](size_t base, size_t elem, size_t& off, int32_t& cap) {
         off = (base + 15) / 16 * 16;
-        const size_t room = c->max_lds > off ? (c->max_lds - off) / elem : 0;
+        const size_t room = lds_max > off ? (lds_max - off) / elem : 0;
         cap = static_cast<int32_t>(std::min<size_t>(room, static_cast<size_t>(P.max_task_slots)));
         return off + static_cast<size_t>(cap) * elem;
     };
@@ -1513,15 +1520,15 @@ void finalizeLocal(s2l_ctx* c, int precision)
     c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
     c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
     c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
-    c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= c->max_lds;
+    c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= lds_max;
     if (c->pair_ok) c->lds_bytes2 = slotRoom(c->lds_bytes2, sizeof(float2), c->slot_off2, c->slot_cap2);
     if (c->pair_ok)
-        HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(), hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(c->local_threads), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(c->lds_bytes2)));
     if (c->queue_ok) {
-        const void* kern = precision == 1   ? localSolveKernel<float, float>()
-                           : precision == 2 ? localSolveKernel<double, float>()
-                                            : localSolveKernel<double, double>();
+        const void* kern = precision == 1   ? localSolveKernel<float, float>(c->local_threads)
+                           : precision == 2 ? localSolveKernel<double, float>(c->local_threads)
+                                            : localSolveKernel<double, double>(c->local_threads);
         HIPCHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(c->lds_bytes)));
     } else if (c->launch_mode != 2) {
         require(precision != 2, S2L_ELIMIT,
@@ -2696,6 +2703,16 @@ int s2l_set_tasks_per_subdomain(s2l_ctx* c, int tasks)
     });
 }
 
+int s2l_set_local_threads(s2l_ctx* c, int32_t threads)
+{
+    return guarded(c, [&] {
+        if (c) c->invalidateGraphs();
+        require(c && (threads == 512 || threads == 1024), S2L_EINVAL, "local_threads: 512 or 1024");
+        c->local_threads_req = threads;
+        c->local_ready = false;
+    });
+}
+
 int s2l_set_dense_top(s2l_ctx* c, int32_t max_width)
 {
     return guarded(c, [&] {
@@ -2728,9 +2745,15 @@ int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry
         for (int t : tset)
             for (int d : {static_cast<int>(kDenseTopMax), 0}) {
                 if (t == 1 && d != 0) continue;  // one task per subdomain: no skeleton
-                cand.push_back(s2l_tune_entry{t, d, 0, 0, 0.0});
+                cand.push_back(s2l_tune_entry{t, d, 0, 1024, 0.0});
             }
-        if (precision != 2) cand.push_back(s2l_tune_entry{b, static_cast<int>(kDenseTopMax), 2, 0, 0.0});
+        // two 512-thread workgroups per compute unit: b and 2b tasks per subdomain
+        for (int t : {b, 2 * b})
+            for (int d : {static_cast<int>(kDenseTopMax), 0}) {
+                if (t == 1 && d != 0) continue;
+                cand.push_back(s2l_tune_entry{t, d, 0, 512, 0.0});
+            }
+        if (precision != 2) cand.push_back(s2l_tune_entry{b, static_cast<int>(kDenseTopMax), 2, 1024, 0.0});
         require(static_cast<int32_t>(cand.size()) <= max_entries, S2L_ELIMIT,
                 "autotune_local: " + std::to_string(cand.size()) + " candidates, max_entries too small");
         // input: a fixed vector per right-hand side; output: the owned rows
@@ -2762,8 +2785,10 @@ int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry
                 s2l_tune_entry& e = cand[k];
                 c->tasks_per_sub = e.tasks;
                 c->dense_top_max = e.dense_top;
+                c->local_threads_req = e.threads;
                 c->launch_mode = 0;
                 finalizeLocal(c, precision);
+                e.threads = c->local_threads;  // 512 may fall back to 1024
                 if (e.launch_mode == 2 || !c->queue_ok) {
                     if (!c->ls_built) buildLs(c, precision);
                     c->launch_mode = 2;
@@ -2793,6 +2818,7 @@ int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry
         const s2l_tune_entry& w = cand[static_cast<size_t>(ibest)];
         c->tasks_per_sub = w.tasks;
         c->dense_top_max = w.dense_top;
+        c->local_threads_req = w.threads;
         c->launch_mode = 0;
         finalizeLocal(c, precision);
         if (w.launch_mode == 2) {

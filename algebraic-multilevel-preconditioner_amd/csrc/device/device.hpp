@@ -11,7 +11,7 @@
 
 namespace s2l {
 
-constexpr int kLocalThreads = 1024; ///< one workgroup (16 wavefronts) per queue entry
+constexpr int kLocalThreads = 1024; ///< threads of a local-solve workgroup (one per queue entry); 512 is the other choice
 constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
 constexpr int kStepBlock = 32;      ///< steps of a task's sweep staged in LDS at a time
 
@@ -117,7 +117,7 @@ void launchBicgControl(const BiArgs& a, int stage, hipStream_t s);
 /// y_r = v for the right-hand sides r that skip[r] does not exclude
 void launchFillMasked(int64_t n, double v, double* y, int nrhs, const int32_t* skip, hipStream_t s);
 
-template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, hipStream_t s);
+template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, int threads, hipStream_t s);
 template <typename T, typename TF> const void* localSolveKernel();
 
 /// One launch of the level-synchronous local solve: items [first, first + count) of kind LsKind

@@ -585,7 +585,12 @@ __device__ __forceinline__ void runEntry(const LocalArgs<T, TF>& a, long long e,
 // Batched local solve. Grid mode: workgroup b runs entry b of the launch (all subdomains' tasks
 // of one depth group, for every right-hand side). Serial mode: one workgroup runs the entries of
 // one subdomain in queue order (the one-launch-per-subdomain comparison and single solves).
-template <typename T, typename TF> __global__ __launch_bounds__(kLocalThreads) void k_local_solve(LocalArgs<T, TF> a)
+//
+// kThreads 1024: one workgroup per compute unit (16 wavefronts, 128 VGPRs each); 512: two
+// workgroups share a compute unit (8 wavefronts each, 4 per SIMD together: at most 128 VGPRs), each
+// hiding the other's memory round trips and barriers.
+template <typename T, typename TF, int kThreads>
+__global__ __launch_bounds__(kThreads, kThreads == 1024 ? 1 : 4) void k_local_solve(LocalArgs<T, TF> a)
 {
     extern __shared__ unsigned char smem_raw[];
     T* x = reinterpret_cast<T*>(smem_raw);
@@ -1698,23 +1703,27 @@ inline unsigned nblk(int64_t n, int t) { return static_cast<unsigned>((n + t - 1
 } // namespace
 
 // ------------------------------------------------------------------ launchers
-template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, hipStream_t s)
+template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, int threads, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_local_solve<T, TF>), dim3(grid), dim3(kLocalThreads), lds, s, a);
+    if (threads == 512)
+        hipLaunchKernelGGL((k_local_solve<T, TF, 512>), dim3(grid), dim3(512), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_local_solve<T, TF, 1024>), dim3(grid), dim3(1024), lds, s, a);
 }
-template void launchLocalSolve<float, float>(const LocalArgs<float, float>&, int, size_t, hipStream_t);
-template void launchLocalSolve<double, double>(const LocalArgs<double, double>&, int, size_t, hipStream_t);
-template void launchLocalSolve<double, float>(const LocalArgs<double, float>&, int, size_t, hipStream_t);
-template void launchLocalSolve<float2, float>(const LocalArgs<float2, float>&, int, size_t, hipStream_t);
+template void launchLocalSolve<float, float>(const LocalArgs<float, float>&, int, size_t, int, hipStream_t);
+template void launchLocalSolve<double, double>(const LocalArgs<double, double>&, int, size_t, int, hipStream_t);
+template void launchLocalSolve<double, float>(const LocalArgs<double, float>&, int, size_t, int, hipStream_t);
+template void launchLocalSolve<float2, float>(const LocalArgs<float2, float>&, int, size_t, int, hipStream_t);
 
-template <typename T, typename TF> const void* localSolveKernel()
+template <typename T, typename TF> const void* localSolveKernel(int threads)
 {
-    return reinterpret_cast<const void*>(&k_local_solve<T, TF>);
+    return threads == 512 ? reinterpret_cast<const void*>(&k_local_solve<T, TF, 512>)
+                          : reinterpret_cast<const void*>(&k_local_solve<T, TF, 1024>);
 }
-template const void* localSolveKernel<float, float>();
-template const void* localSolveKernel<double, double>();
-template const void* localSolveKernel<double, float>();
-template const void* localSolveKernel<float2, float>();
+template const void* localSolveKernel<float, float>(int);
+template const void* localSolveKernel<double, double>(int);
+template const void* localSolveKernel<double, float>(int);
+template const void* localSolveKernel<float2, float>(int);
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)

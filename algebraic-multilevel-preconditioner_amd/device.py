@@ -41,7 +41,7 @@ class KrylovResultC(ctypes.Structure):
 class TuneEntry(ctypes.Structure):
     """s2l_tune_entry of include/s2l_hip.h."""
     _fields_ = [("tasks", ctypes.c_int32), ("dense_top", ctypes.c_int32), ("launch_mode", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("us", ctypes.c_double)]
+                ("threads", ctypes.c_int32), ("us", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -67,7 +67,7 @@ EXPORTS = ["s2l_ctx_create", "s2l_ctx_destroy", "s2l_last_error", "s2l_upload_sy
            "s2l_set_tasks_per_subdomain", "s2l_set_graphs", "s2l_solver_create", "s2l_solver_destroy",
            "s2l_solver_admit", "s2l_solver_step", "s2l_solver_take", "s2l_phase_name", "s2l_phase_timing_enable",
            "s2l_phase_timing_read", "s2l_apply_noalloc", "s2l_refactorize", "s2l_local_eigensolve", "s2l_set_dense_top",
-           "s2l_autotune_local"]
+           "s2l_autotune_local", "s2l_set_local_threads"]
 #: S2L_NPHASES of include/s2l_hip.h
 NPHASES = 10
 
@@ -111,6 +111,7 @@ def hip_lib():
         "s2l_set_local_launch_mode": [vp, ctypes.c_int],
         "s2l_set_tasks_per_subdomain": [vp, ctypes.c_int],
         "s2l_set_dense_top": [vp, ctypes.c_int32],
+        "s2l_set_local_threads": [vp, ctypes.c_int32],
         "s2l_autotune_local": [vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TuneEntry), ctypes.c_int32,
                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
         "s2l_set_graphs": [vp, ctypes.c_int],
@@ -248,7 +249,7 @@ class HipContext:
     # ------------------------------------------------------------------ uploads
     def upload_setup(self, setup, precision: str = "double", one_level: str | None = None,
                      correction: str | None = None, tasks_per_subdomain: int | None = None, coarse: bool = True,
-                     dense_top: int | None = None, coarse_on_device: bool = False):
+                     dense_top: int | None = None, coarse_on_device: bool = False, local_threads: int | None = None):
         """Uploads everything a hostsetup.SchwarzSetup holds (the reference's setup products);
         ``coarse=False`` stops after the local factors (a setup whose coarse space is not built yet).
         ``dense_top``: widest skeleton replaced by a dense top (s2l_set_dense_top; None = the
@@ -258,6 +259,8 @@ class HipContext:
             self._check(self.L.s2l_set_tasks_per_subdomain(self.h, tasks_per_subdomain))
         if dense_top is not None:
             self._check(self.L.s2l_set_dense_top(self.h, dense_top))
+        if local_threads is not None:
+            self._check(self.L.s2l_set_local_threads(self.h, local_threads))
         d = setup_arrays(setup, precision, one_level, correction)
         rp, col, val, own = d["rowptr"], d["col"], d["val"], d["ownership"]
         N = len(own) - 1
@@ -455,7 +458,8 @@ class HipContext:
         cnt = ctypes.c_int32()
         best = ctypes.c_int32()
         self._check(self.L.s2l_autotune_local(self.h, nrhs, applies, ent, 16, ctypes.byref(cnt), ctypes.byref(best)))
-        out = [dict(tasks=e.tasks, dense_top=e.dense_top, launch_mode=e.launch_mode, us=e.us) for e in ent[:cnt.value]]
+        out = [dict(tasks=e.tasks, dense_top=e.dense_top, launch_mode=e.launch_mode, threads=e.threads, us=e.us)
+               for e in ent[:cnt.value]]
         return out, best.value
 
 
@@ -526,15 +530,18 @@ class HipTwoLevelSchwarz:
     """prec::Preconditioner of the reference, applied on the GPU (two_level_schwarz.cpp:134-163)."""
 
     def __init__(self, setup, precision: str = "double", device: int = 0, one_level: str | None = None,
-                 correction: str | None = None, tasks_per_subdomain: int | None = None, dense_top: int | None = None):
+                 correction: str | None = None, tasks_per_subdomain: int | None = None, dense_top: int | None = None,
+                 local_threads: int | None = None):
         self.setup = setup
         self.precision = precision
         self.one_level = one_level or setup.one_level
         self.correction = correction or setup.correction
         self.tasks_per_subdomain = tasks_per_subdomain
         self.dense_top = dense_top
+        self.local_threads = local_threads
         self.ctx = HipContext(device)
-        self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain, dense_top=dense_top)
+        self.ctx.upload_setup(setup, precision, self.one_level, self.correction, tasks_per_subdomain, dense_top=dense_top,
+                              local_threads=local_threads)
 
     def apply(self, r):
         return self.ctx.apply(r)
@@ -545,6 +552,7 @@ class HipTwoLevelSchwarz:
         cands, best = self.ctx.autotune(nrhs, applies)
         self.tasks_per_subdomain = cands[best]["tasks"]
         self.dense_top = cands[best]["dense_top"]
+        self.local_threads = cands[best]["threads"]
         return cands, best
 
     def name(self) -> str:
@@ -579,7 +587,7 @@ class HipTwoLevelSchwarz:
         A with the same pattern and coarse basis, hostsetup.refactorize) into this context."""
         self.setup = setup
         self.ctx.upload_setup(setup, self.precision, self.one_level, self.correction, self.tasks_per_subdomain,
-                              dense_top=self.dense_top)
+                              dense_top=self.dense_top, local_threads=self.local_threads)
         return "refactorize"
 
     def update(self, level: str = "reuse", setup=None) -> str:

@@ -114,7 +114,8 @@ def measure_traffic(args):
         cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
                os.path.abspath(__file__), "--profile-applies", str(PMC_APPLIES), "--profile-one-level", "--setup-workers", "1",
                "--workload", args.workload, "--tasks", str(args.tasks), "--no-autotune"] + \
-              ([] if args.dense_top is None else ["--dense-top", str(args.dense_top)])
+              ([] if args.dense_top is None else ["--dense-top", str(args.dense_top)]) + \
+              ([] if args.local_threads is None else ["--local-threads", str(args.local_threads)])
         log(f"traffic pass {counter}")
         try:
             p = subprocess.run(cmd, capture_output=True, text=True, timeout=300, cwd="/tmp",
@@ -307,6 +308,8 @@ def main():
                     help="tasks per subdomain of the batched local solve (0 = compute units / subdomains)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default local-solve schedule instead of the autotuner's choice (s2l_autotune_local)")
+    ap.add_argument("--local-threads", type=int, default=None, choices=(512, 1024),
+                    help="threads per local-solve workgroup (s2l_set_local_threads; default: the library's, 1024)")
     ap.add_argument("--dense-top", type=int, default=None,
                     help="widest skeleton replaced by a dense top (s2l_set_dense_top; default: the library's, 0 = none)")
     ap.add_argument("--leaf", type=int, default=0,
@@ -380,15 +383,16 @@ def main():
     t_setup = time.perf_counter() - t0
     log(f"setup {t_setup:.1f} s {st.times}; n0 = {st.coarse.n0}")
 
-    M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    M32 = device.HipTwoLevelSchwarz(st, "single", tasks_per_subdomain=args.tasks, dense_top=args.dense_top, local_threads=args.local_threads)
     tune = {}
-    if not args.no_autotune and args.profile_applies == 0 and args.tasks == 0 and args.dense_top is None:
+    if (not args.no_autotune and args.profile_applies == 0 and args.tasks == 0 and args.dense_top is None
+            and args.local_threads is None):
         t0 = time.perf_counter()
         cands, best = M32.autotune(nrhs, 20)
         tune["fp32"] = {"candidates": cands, "kept": best, "s": round(time.perf_counter() - t0, 2)}
         log(f"autotune fp32: kept {cands[best]} of {len(cands)} candidates")
         # the traffic passes measure the schedule kept
-        args.tasks, args.dense_top = cands[best]["tasks"], cands[best]["dense_top"]
+        args.tasks, args.dense_top, args.local_threads = cands[best]["tasks"], cands[best]["dense_top"], cands[best]["threads"]
     ctx = M32.ctx
     ctx.set_graphs(args.graphs != 0)
     if args.local_mode is not None:
@@ -413,7 +417,7 @@ def main():
                "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}
         if args.profile_precisions:
             for prec in ("double", "mixed"):
-                Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+                Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks, dense_top=args.dense_top, local_threads=args.local_threads)
                 Mq.ctx.set_graphs(False)
                 sq = Mq.ctx.stats()
                 ms_q, _ = time_applies(Mq.ctx, r, z, args.profile_applies, warm=PROFILE_WARM)
@@ -421,14 +425,14 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    M64 = device.HipTwoLevelSchwarz(st, "double", tasks_per_subdomain=args.tasks, dense_top=args.dense_top, local_threads=args.local_threads)
     if tune:
         cands, best = M64.autotune(nrhs, 20)
         tune["fp64"] = {"candidates": cands, "kept": best}
     ctx64 = M64.ctx
     ctx64.set_graphs(args.graphs != 0)
     # FP32 factor entries applied in FP64 arithmetic (no reference counterpart)
-    Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks, dense_top=args.dense_top)
+    Mmx = device.HipTwoLevelSchwarz(st, "mixed", tasks_per_subdomain=args.tasks, dense_top=args.dense_top, local_threads=args.local_threads)
     if tune:
         cands, best = Mmx.autotune(nrhs, 20)
         tune["mixed"] = {"candidates": cands, "kept": best}

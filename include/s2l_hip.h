@@ -311,19 +311,27 @@ int s2l_set_tasks_per_subdomain(s2l_ctx* ctx, int tasks);
  * s2l_finalize_local(). */
 int s2l_set_dense_top(s2l_ctx* ctx, int32_t max_width);
 
+/* Threads of a local-solve workgroup (no reference counterpart): 1024 (default, one workgroup per
+ * compute unit) or 512 (two per compute unit, each hiding the other's memory round trips; falls back
+ * to 1024 when half the LDS does not hold a subdomain's layout). Pair it with twice the tasks per
+ * subdomain. Takes effect at the next s2l_finalize_local(). */
+int s2l_set_local_threads(s2l_ctx* ctx, int32_t threads);
+
 /* Autotuner of the local-solve schedule (no reference counterpart; it picks the layout of the
  * batched LocalSolver::solve, src/linalg/local_solver.cpp:119-126). Repacks the uploaded factors
- * (in the context's precision) for every candidate schedule: tasks per subdomain around one
- * workgroup per compute unit (1, b/2, b-1, b, b+1, 2b with b = compute units / subdomains), each with
- * and without dense tops, and the level-synchronous schedule (launch mode 2; not with precision 2);
- * times @p applies batched local solves of @p nrhs right-hand sides of each with HIP events after 3
- * untimed ones, and keeps the fastest (its tasks, dense-top width and launch mode stay set for later
+ * (in the context's precision) for every candidate schedule: 1024-thread workgroups with tasks per
+ * subdomain around one workgroup per compute unit (1, b/2, b-1, b, b+1, 2b with b = compute units /
+ * subdomains), 512-thread workgroups with b and 2b tasks, each with and without dense tops, and the
+ * level-synchronous schedule (launch mode 2; not with precision 2); times @p applies batched local
+ * solves of @p nrhs right-hand sides of each with HIP events after 3 untimed ones, and keeps the
+ * fastest (its tasks, dense-top width, workgroup size and launch mode stay set for later
  * s2l_finalize_local calls). Writes the candidates to @p entries (at most @p max_entries: 16 always
  * suffice), their number to @p count and the index of the one kept to @p best. It repacks from the
  * uploaded host factors: call it before s2l_refactorize, whose new values it would replace. */
 typedef struct s2l_tune_entry {
-    int32_t tasks, dense_top, launch_mode, pad;
-    double us;  /* microseconds per batched local solve of nrhs right-hand sides */
+    int32_t tasks, dense_top, launch_mode;
+    int32_t threads;  /* threads per local-solve workgroup (s2l_set_local_threads) */
+    double us;        /* microseconds per batched local solve of nrhs right-hand sides */
 } s2l_tune_entry;
 int s2l_autotune_local(s2l_ctx* ctx, int32_t nrhs, int32_t applies, s2l_tune_entry* entries, int32_t max_entries,
                        int32_t* count, int32_t* best);

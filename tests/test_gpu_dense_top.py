@@ -88,12 +88,35 @@ def test_autotune_keeps_the_fastest_schedule(precision):
     assert all(c["us"] > 0 for c in cands)
     assert cands[best]["us"] == min(c["us"] for c in cands)
     assert any(c["dense_top"] == 0 for c in cands) and any(c["dense_top"] > 0 for c in cands)
+    assert {c["threads"] for c in cands} <= {512, 1024}
     w = cands[best]
     r = np.cos(np.linspace(0.0, 7.0, st.n))
-    Mw = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=w["tasks"], dense_top=w["dense_top"])
+    Mw = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=w["tasks"], dense_top=w["dense_top"],
+                                   local_threads=w["threads"])
     if w["launch_mode"] != 0:
         Mw.ctx.set_launch_mode(w["launch_mode"])
     z = M.apply(r)
     assert np.array_equal(z, Mw.apply(r))
     M.refresh(st)
     assert np.array_equal(M.apply(r), z) or w["launch_mode"] != 0
+
+
+@pytest.mark.parametrize("precision", ["double", "single", "mixed"])
+def test_512_thread_workgroups_agree(precision):
+    """Two 512-thread workgroups per compute unit (s2l_set_local_threads) run the same schedule
+    with half the threads: FP64 to 1e-13 of the 1024-thread workgroups and of the oracle, every
+    precision bitwise repeatable and in the serial launch mode equal to the grid mode."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=16), 8, nev=8, overlap=2)
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    for t in (2, 8):
+        print(f"case 512 threads {precision} tasks {t}", flush=True)
+        M5 = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t, local_threads=512)
+        M1 = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=t)
+        z5 = M5.apply(r)
+        assert np.array_equal(z5, M5.apply(r))
+        assert _rel(z5, M1.apply(r)) < (1e-13 if precision == "double" else 2e-6)
+        if precision == "double":
+            assert _rel(z5, orc.OracleSchwarz(st).apply(r)) < 1e-12
+        M5.ctx.set_launch_mode(1)
+        assert np.array_equal(M5.apply(r), z5)
