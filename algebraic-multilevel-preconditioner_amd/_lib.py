@@ -4,8 +4,10 @@
   declared in ``include/s2l_setup.h``.
 * ``libs2l_hip.so``: the MI355X apply path (device context, uploads, HIP kernels, device
   Krylov loops), declared in ``include/s2l_hip.h``.
+* ``libs2l_backend.so``: the C++ host classes; its C-ABI of the reuse policy is declared in
+  ``include/s2l_reuse.h``.
 
-Both are built in-tree by ``__graft_entry__.build()``. A missing library raises
+All are built in-tree by ``__graft_entry__.build()``. A missing library raises
 ``RuntimeError`` at first use: there is no fallback path.
 """
 import ctypes
@@ -35,6 +37,7 @@ class GenOpts(ctypes.Structure):
 
 _setup = None
 _hip = None
+_backend = None
 
 #: set once this process has created a device context (the HIP runtime is initialised)
 gpu_initialised = False
@@ -104,6 +107,37 @@ def setup_lib():
     L.s2l_factor_copy.argtypes = [ctypes.c_void_p, _c_i32p, _c_i64p, _c_i32p, _c_dp, _c_i64p, _c_i32p, _c_dp]
     L.s2l_factor_supernodes.argtypes = [ctypes.c_void_p, _c_i32p]
     _setup = L
+    return L
+
+
+def backend_lib():
+    """Handle of libs2l_backend.so with the argument types of its reuse-policy C-ABI declared."""
+    global _backend
+    if _backend is not None:
+        return _backend
+    L = _load("libs2l_backend.so")
+    P = ctypes.c_void_p
+    L.s2l_reuse_create.restype = P
+    L.s2l_reuse_create.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
+    L.s2l_reuse_destroy.argtypes = [P]
+    L.s2l_reuse_last_error.restype = ctypes.c_char_p
+    L.s2l_reuse_decide.restype = ctypes.c_int
+    L.s2l_reuse_decide.argtypes = [P, ctypes.c_int]
+    L.s2l_reuse_record.restype = ctypes.c_int
+    L.s2l_reuse_record.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int]
+    for f in ("s2l_reuse_last", "s2l_reuse_wasted", "s2l_reuse_rebuild_premium"):
+        getattr(L, f).restype = ctypes.c_double
+        getattr(L, f).argtypes = [P]
+    L.s2l_reuse_reference.restype = ctypes.c_double
+    L.s2l_reuse_reference.argtypes = [P, ctypes.c_int]
+    L.s2l_reuse_age.restype = ctypes.c_int
+    L.s2l_reuse_age.argtypes = [P]
+    L.s2l_reuse_per_decade.restype = ctypes.c_double
+    L.s2l_reuse_per_decade.argtypes = [ctypes.c_int, ctypes.c_double]
+    L.s2l_reuse_lower_median.restype = ctypes.c_double
+    L.s2l_reuse_lower_median.argtypes = [_c_dp, ctypes.c_int]
+    _backend = L
     return L
 
 

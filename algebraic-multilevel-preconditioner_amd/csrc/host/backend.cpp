@@ -1,5 +1,6 @@
 // backend.cpp -- C++17 host classes over the C-ABI (include/s2l_backend.hpp).
 #include "s2l_backend.hpp"
+#include "s2l_reuse.h"
 #include <cmath>
 #include <algorithm>
 
@@ -335,3 +336,63 @@ std::unique_ptr<KrylovSolver> makeKrylovSolver(const core::SolverConfig& cfg) { 
 } // namespace solv
 
 } // namespace s2l
+
+// ---- C-ABI of the reuse policy (include/s2l_reuse.h): forwards to prec::ReusePolicy
+struct s2l_reuse {
+    s2l::prec::ReusePolicy policy;
+};
+
+namespace {
+thread_local std::string g_reuse_error;
+}
+
+extern "C" {
+
+s2l_reuse* s2l_reuse_create(const char* mode, double growth, int max_age)
+{
+    try {
+        if (mode == nullptr) throw std::invalid_argument("no reuse mode (off|refactor|auto)");
+        return new s2l_reuse{s2l::prec::ReusePolicy(s2l::prec::ReusePolicy::parseMode(mode), growth, max_age)};
+    } catch (const std::exception& e) {
+        g_reuse_error = e.what();
+        return nullptr;
+    }
+}
+
+void s2l_reuse_destroy(s2l_reuse* p) { delete p; }
+
+const char* s2l_reuse_last_error(void) { return g_reuse_error.c_str(); }
+
+int s2l_reuse_decide(const s2l_reuse* p, int matrix_changed)
+{
+    if (p == nullptr) return -1;
+    return static_cast<int>(p->policy.decide(matrix_changed != 0));
+}
+
+int s2l_reuse_record(s2l_reuse* p, int done, int iterations, double reduction, int converged, double t_update,
+                     double t_solve, int kind)
+{
+    if (p == nullptr || done < 0 || done > 2) return -1;
+    p->policy.record(static_cast<s2l::prec::UpdateLevel>(done), iterations, reduction, converged != 0, t_update, t_solve,
+                     kind);
+    return 0;
+}
+
+double s2l_reuse_last(const s2l_reuse* p) { return p ? p->policy.last() : 0.0; }
+double s2l_reuse_reference(const s2l_reuse* p, int kind) { return p ? p->policy.reference(kind) : 0.0; }
+int s2l_reuse_age(const s2l_reuse* p) { return p ? p->policy.age() : 0; }
+double s2l_reuse_wasted(const s2l_reuse* p) { return p ? p->policy.wasted() : 0.0; }
+double s2l_reuse_rebuild_premium(const s2l_reuse* p) { return p ? p->policy.rebuildPremium() : 0.0; }
+
+double s2l_reuse_per_decade(int iterations, double reduction)
+{
+    return s2l::prec::ReusePolicy::perDecade(iterations, reduction);
+}
+
+double s2l_reuse_lower_median(const double* v, int n)
+{
+    if (v == nullptr || n <= 0) return 0.0;
+    return s2l::prec::ReusePolicy::lowerMedian(std::vector<double>(v, v + n));
+}
+
+} // extern "C"

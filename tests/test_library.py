@@ -23,7 +23,8 @@ def declared(header):
     return sorted(set(re.findall(r"\b(s2l_\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header,lib", [("s2l_hip.h", "libs2l_hip.so"), ("s2l_setup.h", "libs2l_setup.so")])
+@pytest.mark.parametrize("header,lib", [("s2l_hip.h", "libs2l_hip.so"), ("s2l_setup.h", "libs2l_setup.so"),
+                                        ("s2l_reuse.h", "libs2l_backend.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = declared(header)
     assert len(names) > 10

@@ -1,7 +1,8 @@
 """The reuse policy (prec::ReusePolicy, include/preconditioner/reuse_policy.hpp:20-136) against the
-cases of the reference's own test (test/test_reuse.cpp:50-110), in its Python form and in the C++
-backend (a test program built with g++ and run on the CPU). A planted mistake (the break-even
-comparison without the safety nets' reference) must fail one of the cases."""
+cases of the reference's own test (test/test_reuse.cpp:50-110), through the Python binding of its
+C-ABI (include/s2l_reuse.h) and in C++ (a test program built with g++ and run on the CPU): one
+implementation, two hosts. A planted mistake (the break-even comparison without the premium) must
+fail one of the cases."""
 import os
 import shutil
 import subprocess
@@ -101,9 +102,30 @@ def test_planted_break_even_mistake_is_caught(monkeypatch):
             return REBUILD
         if not matrix_changed:
             return REUSE
-        return REBUILD if self._wasted > 0.0 else REFACTORIZE
+        return REBUILD if self.wasted() > 0.0 else REFACTORIZE
     monkeypatch.setattr(P, "decide", decide)
     assert "break-even" in policy_cases(P)
+
+
+def test_c_abi_rejects_bad_arguments():
+    """Unknown modes and levels are errors; a NULL policy answers -1 / 0 and is not dereferenced."""
+    from amp_amd import _lib
+    L = _lib.backend_lib()
+    assert not L.s2l_reuse_create(b"sometimes", 0.0, 0)
+    assert "sometimes" in L.s2l_reuse_last_error().decode()
+    assert not L.s2l_reuse_create(None, 0.0, 0)
+    assert L.s2l_reuse_decide(None, 1) == -1
+    assert L.s2l_reuse_record(None, 0, 10, 1e-8, 1, 0.0, 1.0, 0) == -1
+    assert L.s2l_reuse_age(None) == 0 and L.s2l_reuse_wasted(None) == 0.0
+    h = L.s2l_reuse_create(b"auto", 0.0, 0)
+    try:
+        assert L.s2l_reuse_record(h, 3, 10, 1e-8, 1, 0.0, 1.0, 0) == -1
+        assert L.s2l_reuse_record(h, 2, 10, 1e-8, 1, 2.0, 1.0, 0) == 0
+        assert _near(L.s2l_reuse_reference(h, 0), 1.25)
+    finally:
+        L.s2l_reuse_destroy(h)
+    with pytest.raises(ValueError):
+        P("auto").record("sometimes", 1, 0.1, True, 0.0, 1.0)
 
 
 def test_cpp_policy_matches_reference_cases(tmp_path):
