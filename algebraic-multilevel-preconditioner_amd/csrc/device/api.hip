@@ -2576,6 +2576,15 @@ int s2l_device_footprint(s2l_ctx* c, int64_t* bytes)
     });
 }
 
+// Slots the task queue keeps in LDS: the first slot_cap of every task without child tasks.
+static int64_t ldsSlots(const s2l_ctx* c)
+{
+    if (!c->queue_ok) return 0;
+    int64_t k = 0;
+    for (const TaskDesc& t : c->pk.task) k += std::min(t.slot_count, c->slot_cap);
+    return k;
+}
+
 int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
 {
     return guarded(c, [&] {
@@ -2616,10 +2625,14 @@ int s2l_get_stats(s2l_ctx* c, s2l_stats* o)
                     static_cast<double>(c->ls.bwd_below_rows) * st;
         } else {
             // task queue: factor values of both sweeps, index data, rhs gather (8 B value + 4 B map),
-            // permutation and owned-row output (read x0 8 B + write z 8 B)
+            // permutation and owned-row output (read x0 8 B + write z 8 B), the slots outside LDS
+            // written and read once
             local = (c->pk.entries_fwd + c->pk.entries_bwd) * sv + static_cast<double>(c->pk.bytes_index) +
                     static_cast<double>(c->local_total) * (8.0 + 4.0 + 4.0) + static_cast<double>(c->n) * 16.0 +
-                    static_cast<double>(c->pk.t_total) * st * 2.0;
+                    static_cast<double>(c->pk.t_total - ldsSlots(c)) * st * 2.0;
+            o->lds_slot_cap = c->queue_ok ? c->slot_cap : 0;
+            o->max_task_slots = c->pk.max_task_slots;
+            o->lds_slots = ldsSlots(c);
         }
         o->bytes_local_solve = local;
         const double spmv = static_cast<double>(c->nnz) * 12.0 + (static_cast<double>(c->n) + 1) * 4.0 + static_cast<double>(c->n) * 16.0;

@@ -37,7 +37,7 @@ template <typename T, typename TF = T> struct LocalArgs {
     const int32_t* cend;  ///< end of every row's slots
     const int32_t* cidx;  ///< slot of every pushed product
     int32_t slot_off;     ///< byte offset in the dynamic LDS of the slots of a task without child tasks
-    int32_t slot_cap;     ///< slots that fit there (a task with more keeps them in global memory)
+    int32_t slot_cap;     ///< slots that fit there (a task with more keeps the rest in global memory)
     T* tbuf;              ///< T buffer, t_total entries per right-hand side
     const int32_t* gmap;  ///< factor position -> global row (gather)
     const int32_t* lperm; ///< factor position -> local index

@@ -39,7 +39,9 @@ template <typename T> __device__ inline T waveSum(T v)
 
 // Global loads issued per batch by the dot products of the local solve: each batch is one memory
 // round trip, so a thread's dot product of length n costs ceil(n / 8) dependent round trips. The
-// remainder is one predicated batch rather than a loop with a round trip per element.
+// remainder is one batch rather than a loop with a round trip per element: its loads past the end
+// read the last element again (no branch around a load: the compiler would move a conversion of the
+// loaded value into the branch and wait for each load there) and their products are skipped.
 constexpr int kB = 8;
 
 // sum over k < n of p[k * ps] * q[k * qs], in order k = 0, 1, ..., with the loads of a batch issued
@@ -63,9 +65,9 @@ template <typename T, typename P> __device__ inline T dot8(const P* p, int ps, c
         P a[kB - 1];
         T b[kB - 1];
 #pragma unroll
-        for (int u = 0; u < kB - 1; ++u) a[u] = k + u < n ? p[(k + u) * ps] : P(0);
+        for (int u = 0; u < kB - 1; ++u) a[u] = p[min(k + u, n - 1) * ps];
 #pragma unroll
-        for (int u = 0; u < kB - 1; ++u) b[u] = k + u < n ? q[(k + u) * qs] : T(0);
+        for (int u = 0; u < kB - 1; ++u) b[u] = q[min(k + u, n - 1) * qs];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
@@ -97,8 +99,8 @@ template <typename T, typename P> __device__ inline T gatherDot8(const P* p, int
         int32_t i[kB - 1];
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u) {
-            a[u] = k + u < n ? p[(k + u) * ps] : P(0);
-            i[u] = k + u < n ? r[(k + u) * rs] : 0;
+            a[u] = p[min(k + u, n - 1) * ps];
+            i[u] = r[min(k + u, n - 1) * rs];
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -124,7 +126,7 @@ template <typename T> __device__ inline T pullSlots(const T* tb, T v, int cb, in
     if (q < ce) {
         T t[kB - 1];
 #pragma unroll
-        for (int u = 0; u < kB - 1; ++u) t[u] = q + u < ce ? tb[q + u] : T(0);
+        for (int u = 0; u < kB - 1; ++u) t[u] = tb[min(q + u, ce - 1)];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kB - 1; ++u)
@@ -133,16 +135,23 @@ template <typename T> __device__ inline T pullSlots(const T* tb, T v, int cb, in
     return v;
 }
 
-// Slots of a task without child tasks live in LDS (kLds: ls holds the task's slots [sb, sb + sc)
-// of its subdomain's T buffer): the task is the only one pushing to its rows. Otherwise all slots
-// are in global memory (tb). The choice is a template parameter, so each sweep's code sees one
-// address space per access.
+// Slots of a task without child tasks live in LDS (kLds: ls holds the first sc of the task's
+// slots, [sb, sb + sc) of its subdomain's T buffer, the rest stay in global memory): the task is
+// the only one pushing to its rows. Otherwise all slots are in global memory (tb). The choice is a
+// template parameter, so each sweep's code sees one address space per access.
 template <typename T, bool kLds> struct Slots {
     T* tb;   ///< the subdomain's T buffer (global)
-    T* ls;   ///< kLds: the task's slots in LDS
+    T* ls;   ///< kLds: the task's first sc slots in LDS
     int sb, sc;
-    /// x minus the slots [cb, ce) of a row, in slot order
-    __device__ T pull(T x, int cb, int ce) const { return kLds ? pullSlots(ls, x, cb - sb, ce - sb) : pullSlots(tb, x, cb, ce); }
+    /// x minus the slots [cb, ce) of a row, in slot order (its LDS part first)
+    __device__ T pull(T x, int cb, int ce) const
+    {
+        if (!kLds) return pullSlots(tb, x, cb, ce);
+        const int e = sb + sc;
+        if (ce <= e) return pullSlots(ls, x, cb - sb, ce - sb);
+        if (cb >= e) return pullSlots(tb, x, cb, ce);
+        return pullSlots(tb, pullSlots(ls, x, cb - sb, sc), e, ce);
+    }
     /// a pushed product into its slot d
     __device__ void push(int d, T v) const
     {
@@ -217,13 +226,13 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
 template <typename T, typename TF, int kTri, typename Out>
 __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
 {
-    // FP64 factors: each pass takes two groups of R rows, nw * R rows apart, whose loads are
-    // issued together (FP32 factors leave no registers for the second group). The block's
-    // entries (TF) are converted to the arithmetic type T. Eight-byte arithmetic types (FP64 sums
-    // of FP32 entries, a pair of FP32 right-hand sides) take the FP64 row count: four rows of them
-    // spill registers.
+    // Two rows per group with an eight-byte factor or arithmetic type (FP64 factors, FP64 sums of
+    // FP32 entries, a pair of FP32 right-hand sides): four rows of them spill registers. Those
+    // passes take two groups of R rows, nw * R rows apart, whose loads are issued together; FP32
+    // throughout takes one group of four rows (no registers are left for a second). The block's
+    // entries (TF) are converted to the arithmetic type T.
     constexpr int R = sizeof(T) == 8 ? 2 : blockRows<TF>();
-    constexpr int P = sizeof(TF) == 8 ? 2 : 1;
+    constexpr int P = R == 2 ? 2 : 1;
     for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
         const int rA = r0, rB = r0 + nw * R;
         const int nA = min(R, rows - rA), nB = P == 2 ? max(0, min(R, rows - rB)) : 0;
@@ -245,19 +254,24 @@ __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int col
             TF mA[R][4], mB[R][4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) xv[i] = c + i < cols ? xs[c + i] : T(0);
+            bool okA[R], okB[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if (needA && r < nA) {
-                    load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[r]);
-                } else {
+                okA[r] = needA && r < nA;
+                okB[r] = needB && r < nB;
+                if constexpr (std::is_same_v<T, TF>) {
+                    // loads under a branch, the quads not needed set to zero
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) mA[r][i] = TF(0);
-                }
-                if (needB && r < nB) {
-                    load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[r]);
+                    for (int i = 0; i < 4; ++i) mA[r][i] = mB[r][i] = TF(0);
+                    if (okA[r]) load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[r]);
+                    if (okB[r]) load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[r]);
                 } else {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) mB[r][i] = TF(0);
+                    // entries converted to T: every lane loads all its quads, those it does not
+                    // need from the block's first (cached) quad, and zeroes them after the loads
+                    // (under a branch the conversion joins the load, which then waits for each
+                    // load before issuing the next)
+                    load4(M + (okA[r] ? static_cast<int64_t>(rA + r) * ld + c : 0), mA[r]);
+                    load4(M + (okB[r] ? static_cast<int64_t>(rB + r) * ld + c : 0), mB[r]);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -265,8 +279,13 @@ __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int col
             for (int r = 0; r < R; ++r)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    accA[r] += widen<T>(mA[r][i]) * xv[i];
-                    accB[r] += widen<T>(mB[r][i]) * xv[i];
+                    if constexpr (std::is_same_v<T, TF>) {
+                        accA[r] += mA[r][i] * xv[i];
+                        accB[r] += mB[r][i] * xv[i];
+                    } else {
+                        accA[r] += (okA[r] ? widen<T>(mA[r][i]) : T(0)) * xv[i];
+                        accB[r] += (okB[r] ? widen<T>(mB[r][i]) : T(0)) * xv[i];
+                    }
                 }
         }
         const T yA = transposeSum<T, R>(accA, lane);
@@ -521,10 +540,10 @@ __device__ __forceinline__ void runEntry(const LocalArgs<T, TF>& a, long long e,
     T* tb = a.tbuf + static_cast<int64_t>(r) * a.t_total + sd.t_off;
     T* xg = a.xg + static_cast<int64_t>(r) * a.n_local_total + sd.map_off;
     const int32_t* rows = a.task_rows + td.rows_first;
-    // a forward sweep of a task without child tasks keeps its rows' slots in LDS when they fit
-    const bool ldsSlots = !backward && td.slot_count > 0 && td.slot_count <= a.slot_cap;
+    // a forward sweep of a task without child tasks keeps its rows' slots in LDS, as many as fit
+    const bool ldsSlots = !backward && td.slot_count > 0 && a.slot_cap > 0;
     const Slots<T, false> slg{tb, nullptr, 0, 0};
-    const Slots<T, true> sll{tb, reinterpret_cast<T*>(smem + a.slot_off), td.slot_base, td.slot_count};
+    const Slots<T, true> sll{tb, reinterpret_cast<T*>(smem + a.slot_off), td.slot_base, min(td.slot_count, a.slot_cap)};
     // the rows this entry computes: all of the task's, or a dense top part's range
     const int qlo = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * part / td.nparts) : 0;
     const int qhi = td.nparts > 0 ? static_cast<int>(static_cast<int64_t>(td.rows_count) * (part + 1) / td.nparts) : td.rows_count;

@@ -529,11 +529,13 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         std::vector<std::vector<int32_t>> tsn(static_cast<std::size_t>(nt));
         for (int32_t s = 0; s < nsf; ++s) tsn[static_cast<std::size_t>(taskOf[static_cast<std::size_t>(s)])].push_back(s);
         // contributions: the product of row k below supernode d goes to a slot of its target row.
-        // Slots are numbered task by task (the rows of a task in order, a row's slots ordered by
-        // d), so a row pulls one contiguous range [cptr, cend) and the slots of a task's rows are
-        // one contiguous range: a task without child tasks, the only one pushing to its rows, can
-        // keep them in LDS. The supernodes of a dense top push nothing (their products are inside
-        // S^-1).
+        // Slots are numbered task by task (a row's slots ordered by d), so a row pulls one
+        // contiguous range [cptr, cend) and the slots of a task's rows are one contiguous range: a
+        // task without child tasks, the only one pushing to its rows, can keep them in LDS. A task
+        // numbers its rows from its top supernode down: when only a prefix of its slots fits in
+        // LDS, that prefix holds the rows pulled most (the top supernodes receive the products of
+        // the whole subtree). The supernodes of a dense top push nothing (their products are
+        // inside S^-1).
         std::vector<int32_t> tslotBase(static_cast<std::size_t>(nt), 0), tslotCount(static_cast<std::size_t>(nt), 0);
         {
             std::vector<int32_t> cnt(static_cast<std::size_t>(n), 0);
@@ -550,8 +552,9 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
             int32_t pos = 0;
             for (int32_t t = 0; t < nt; ++t) {
                 tslotBase[static_cast<std::size_t>(t)] = pos;
-                for (int32_t sn : tsn[static_cast<std::size_t>(t)])
-                    for (int32_t r = fst[sn]; r < fst[sn + 1]; ++r) {
+                const std::vector<int32_t>& ts = tsn[static_cast<std::size_t>(t)];
+                for (auto it = ts.rbegin(); it != ts.rend(); ++it)
+                    for (int32_t r = fst[*it]; r < fst[*it + 1]; ++r) {
                         beg[static_cast<std::size_t>(r)] = pos;
                         pos += cnt[static_cast<std::size_t>(r)];
                         end[static_cast<std::size_t>(r)] = pos;

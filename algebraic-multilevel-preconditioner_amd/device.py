@@ -51,7 +51,8 @@ class Stats(ctypes.Structure):
                 ("coarse_rows", ctypes.c_int64), ("levels_fwd_max", ctypes.c_int32),
                 ("levels_bwd_max", ctypes.c_int32), ("supernodes", ctypes.c_int32),
                 ("bytes_local_solve", ctypes.c_double), ("bytes_apply", ctypes.c_double),
-                ("bytes_spmv", ctypes.c_double), ("refactor_ms", ctypes.c_double * 4)]
+                ("bytes_spmv", ctypes.c_double), ("refactor_ms", ctypes.c_double * 4),
+                ("lds_slot_cap", ctypes.c_int32), ("max_task_slots", ctypes.c_int32), ("lds_slots", ctypes.c_int64)]
 
 
 _hip = None

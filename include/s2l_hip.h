@@ -243,6 +243,9 @@ typedef struct s2l_stats {
     double bytes_apply;           /* algorithmic HBM bytes of one preconditioner application (1 rhs) */
     double bytes_spmv;            /* algorithmic HBM bytes of one SpMV (1 rhs) */
     double refactor_ms[4];        /* the last s2l_refactorize: new values of A, local factors, A_00, total [ms] */
+    int32_t lds_slot_cap;         /* task queue: slots of a task without child tasks kept in LDS (its others in HBM) */
+    int32_t max_task_slots;       /* task queue: most slots of a task without child tasks */
+    int64_t lds_slots;            /* task queue: slots kept in LDS, summed over tasks (not counted as HBM bytes) */
 } s2l_stats;
 int s2l_get_stats(s2l_ctx* ctx, s2l_stats* out);
 /* Bytes of device memory the context holds (its uploaded system, subdomains, factors, coarse

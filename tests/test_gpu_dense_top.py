@@ -56,6 +56,35 @@ def test_dense_top_local_solves_match_the_oracle():
         assert _rel(M.ctx.local_solve(i, b), orc.OracleLocalFactor(f, "double").solve(b)) < 1e-12, i
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed", "single"])
+@pytest.mark.parametrize("tasks", [1, 2])
+def test_slots_partly_in_lds_match_the_oracle(precision, tasks):
+    """Subdomains of about 7000 rows in one or two subtree tasks: a task's slots outnumber the LDS
+    room (stats: lds_slot_cap < max_task_slots; in FP32 only with one task), so the first slot_cap of
+    them stay in LDS and the rest in HBM. The local solves match the oracle's (LocalSolver::solve,
+    local_solver.cpp:119-126), and a second apply gives the same bits."""
+    from amp_amd import device
+    st = _setup("poisson3d", dict(nx=24), 2, nev=4, overlap=1)
+    print(f"case partial LDS slots {precision} tasks {tasks}", flush=True)
+    M = device.HipTwoLevelSchwarz(st, precision, tasks_per_subdomain=tasks)
+    s = M.ctx.stats()
+    assert s.lds_slot_cap > 0 and s.lds_slots > 0
+    if tasks == 1 or precision != "single":
+        assert s.lds_slot_cap < s.max_task_slots, (s.lds_slot_cap, s.max_task_slots)
+    rng = np.random.default_rng(11)
+    # FP32 storage: the device rounds the inverted diagonal blocks, the oracle the factor entries
+    tol = {"double": 1e-12, "mixed": 2e-6, "single": 1e-4}[precision]
+    for i in range(2):
+        f = st.factors[i]
+        b = rng.standard_normal(f.n)
+        assert _rel(M.ctx.local_solve(i, b), orc.OracleLocalFactor(f, precision).solve(b)) < tol, i
+    r = np.cos(np.linspace(0.0, 7.0, st.n))
+    z = M.apply(r)
+    assert np.array_equal(z, M.apply(r))
+    if precision == "double":
+        assert _rel(z, orc.OracleSchwarz(st).apply(r)) < 1e-12
+
+
 def test_dense_top_gmres_iterations_and_paired_rhs():
     """GMRES with dense tops takes the oracle's iterations (FP64); in FP32 two right-hand sides
     sharing a workgroup give the bits of each alone."""
