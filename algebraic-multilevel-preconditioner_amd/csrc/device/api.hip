@@ -2760,8 +2760,10 @@ int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry
                 if (t == 1 && d != 0) continue;  // one task per subdomain: no skeleton
                 cand.push_back(s2l_tune_entry{t, d, 0, 1024, 0.0});
             }
-        // two 512-thread workgroups per compute unit: b and 2b tasks per subdomain
-        for (int t : {b, 2 * b})
+        // two 512-thread workgroups per compute unit: b, 3b/2 and 2b tasks per subdomain
+        std::vector<int> tset2{b, b + b / 2, 2 * b};
+        tset2.erase(std::unique(tset2.begin(), tset2.end()), tset2.end());
+        for (int t : tset2)
             for (int d : {static_cast<int>(kDenseTopMax), 0}) {
                 if (t == 1 && d != 0) continue;
                 cand.push_back(s2l_tune_entry{t, d, 0, 512, 0.0});

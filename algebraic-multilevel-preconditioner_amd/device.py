@@ -455,10 +455,10 @@ class HipContext:
     def autotune(self, nrhs: int = 1, applies: int = 20):
         """s2l_autotune_local: times the candidate local-solve schedules and keeps the fastest.
         Returns (candidates as dicts with tasks, dense_top, launch_mode, us; index of the kept one)."""
-        ent = (TuneEntry * 16)()
+        ent = (TuneEntry * 32)()
         cnt = ctypes.c_int32()
         best = ctypes.c_int32()
-        self._check(self.L.s2l_autotune_local(self.h, nrhs, applies, ent, 16, ctypes.byref(cnt), ctypes.byref(best)))
+        self._check(self.L.s2l_autotune_local(self.h, nrhs, applies, ent, 32, ctypes.byref(cnt), ctypes.byref(best)))
         out = [dict(tasks=e.tasks, dense_top=e.dense_top, launch_mode=e.launch_mode, threads=e.threads, us=e.us)
                for e in ent[:cnt.value]]
         return out, best.value

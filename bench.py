@@ -61,7 +61,9 @@ COPY_COUNT = 1 << 25   # doubles of the calibration copy (256 MiB read + 256 MiB
 PMC_APPLIES = 3        # applies of a counter pass
 PROFILE_WARM = 2       # untimed applies before the timed ones in --profile-applies runs
 LATENCY_SOLVES = 100   # solves of the latency pass (p50 / p99)
-LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float>")
+# kernel names as rocprofv3 reports them; the task-queue kernel's name also carries its workgroup size
+# (k_local_solve<float, float, 1024> or <float, float, 512>)
+LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float,")
 
 
 def time_applies(ctx, r, z, count, warm=3):
@@ -82,15 +84,15 @@ def time_applies(ctx, r, z, count, warm=3):
     return ms / max(nl, 1), wall * 1e3
 
 
-def roofline(bytes_per_apply, avg_ms, applies):
-    """The dominant kernel's roofline: k_local_solve<float, float>, whose launches of one apply (the subtree
+def roofline(bytes_per_apply, avg_ms, applies, threads=None):
+    """The dominant kernel's roofline: k_local_solve<float, float, threads>, whose launches of one apply (the subtree
     forward launch, the top tasks, the subtree backward launch) are timed together with HIP events
     on their stream; avg_us is their time per apply, bytes the algorithmic bytes of one apply's
     local solves."""
     achieved = bytes_per_apply / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_local_solve<float, float> (task queue, the launches of one apply)", "applies_timed": int(applies),
+            "kernel": f"k_local_solve<float, float, {threads or 1024}> (task queue, the launches of one apply)", "applies_timed": int(applies),
             "avg_us": round(avg_ms * 1e3, 2),
             "bytes_per_apply": int(bytes_per_apply)}
 
@@ -414,14 +416,14 @@ def main():
         ctx.synchronize()
         out = {"mode": "profile", "applies": args.profile_applies, "leaf": args.leaf, "tasks": args.tasks,
                "levels": [stats.levels_fwd_max, stats.levels_bwd_max], "supernodes": stats.supernodes,
-               "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies)}
+               "roofline": roofline(stats.bytes_local_solve, avg_ms, args.profile_applies, args.local_threads)}
         if args.profile_precisions:
             for prec in ("double", "mixed"):
                 Mq = device.HipTwoLevelSchwarz(st, prec, tasks_per_subdomain=args.tasks, dense_top=args.dense_top, local_threads=args.local_threads)
                 Mq.ctx.set_graphs(False)
                 sq = Mq.ctx.stats()
                 ms_q, _ = time_applies(Mq.ctx, r, z, args.profile_applies, warm=PROFILE_WARM)
-                out[prec] = roofline(sq.bytes_local_solve, ms_q, args.profile_applies)
+                out[prec] = roofline(sq.bytes_local_solve, ms_q, args.profile_applies, args.local_threads)
         print(json.dumps(out), flush=True)
         return
 
@@ -555,7 +557,7 @@ def main():
     mixed_ms = time_applies(ctxm, r, z, 20, warm=3)
 
     # ---- roofline of the dominant kernel (batched local solve, FP32 factors)
-    roof = roofline(stats.bytes_local_solve, mode_ms[0][0], 20)
+    roof = roofline(stats.bytes_local_solve, mode_ms[0][0], 20, args.local_threads)
     if not args.no_traffic:
         roof["traffic"], roof["traffic_note"] = measure_traffic(args)
 
