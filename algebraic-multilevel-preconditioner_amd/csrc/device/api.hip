@@ -252,6 +252,13 @@ struct s2l_ctx {
     DevBuf<TaskDesc> task_d;
     DevBuf<int32_t> tchild_d, trows_d, ext_d, queue_d, qsubq_d;
     std::vector<int32_t> qsub_h;
+    // dense tops in a launch of their own: per skeleton row (factor position, global row, local
+    // index, first slot), its end of slots, each task's offset in the table (-1: not a dense top);
+    // per depth group of the queue whether all its entries are dense tops; LDS of the widest b_S
+    DevBuf<int4> dtrow_d;
+    DevBuf<int32_t> dtce_d, dtb_d;
+    std::vector<char> qgroup_dense;
+    int32_t dt_maxS = 0;
     int tasks_per_sub = 0;  // 0: one subtree task per compute unit (compute units / subdomains)
     int dense_top_max = kDenseTopMax;  // widest skeleton replaced by a dense top (0: none)
     int local_threads_req = kLocalThreads; // threads of a local-solve workgroup asked for: 1024, or 512 (two per compute unit)
@@ -606,8 +613,21 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
             require(false, S2L_ESTATE, "the level-synchronous schedule has no FP32-storage / FP64-arithmetic mode");
     } else if (c->launch_mode == 0) {
         // level-ordered queue: one launch per depth group, every subdomain batched
-        for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g)
-            launchQueue<T, TF>(c, a, c->queue_d.p + c->pk.qgroup[g], c->pk.qgroup[g + 1] - c->pk.qgroup[g], false, s);
+        for (size_t g = 0; g + 1 < c->pk.qgroup.size(); ++g) {
+            const int32_t* q = c->queue_d.p + c->pk.qgroup[g];
+            const int32_t count = c->pk.qgroup[g + 1] - c->pk.qgroup[g];
+            if (c->qgroup_dense[g]) {
+                // a group of dense tops: the dense-top kernel (fewer registers, more wavefronts)
+                a.queue = q;
+                a.qcount = count;
+                a.serial = 0;
+                const bool pair = std::is_same_v<T, float2>;
+                launchDenseTop<T, TF>(a, count * (pair ? (a.nrhs + 1) / 2 : a.nrhs), c->dtrow_d.p, c->dtce_d.p, c->dtb_d.p,
+                                      static_cast<size_t>(c->dt_maxS) * sizeof(T), s);
+            } else {
+                launchQueue<T, TF>(c, a, q, count, false, s);
+            }
+        }
     } else {
         for (int i = 0; i < c->nsub; ++i)
             launchQueue<T, TF>(c, a, c->qsubq_d.p + c->qsub_h[static_cast<size_t>(i)],
@@ -1567,6 +1587,37 @@ void finalizeLocal(s2l_ctx* c, int precision)
     c->queue_d.upload(P.queue, c->stream);
     c->qsubq_d.upload(P.queue_sub, c->stream);
     c->qsub_h = P.qsub;
+    {
+        // the row table of the dense tops and the depth groups made of dense tops only
+        std::vector<int4> dt;
+        std::vector<int32_t> dtce, dtb(P.task.size(), -1);
+        int32_t maxS = 0;
+        for (const DenseTop& d : P.dtop) {
+            const TaskDesc& td = P.task[static_cast<size_t>(d.task)];
+            const SubDesc& sd = P.sub[static_cast<size_t>(td.sub)];
+            dtb[static_cast<size_t>(d.task)] = static_cast<int32_t>(dt.size());
+            for (int32_t q = 0; q < td.rows_count; ++q) {
+                const int32_t row = P.task_rows[static_cast<size_t>(td.rows_first + q)];
+                const size_t m = static_cast<size_t>(sd.map_off + row);
+                dt.push_back(int4{row, gmap[m], lperm[m], P.cptr[static_cast<size_t>(sd.cptr_off + row)]});
+                dtce.push_back(P.cend[static_cast<size_t>(sd.cptr_off + row)]);
+            }
+            maxS = std::max(maxS, td.rows_count);
+        }
+        c->qgroup_dense.assign(P.qgroup.size() > 0 ? P.qgroup.size() - 1 : 0, 0);
+        for (size_t g = 0; g + 1 < P.qgroup.size(); ++g) {
+            bool all = P.qgroup[g + 1] > P.qgroup[g];
+            for (int32_t k = P.qgroup[g]; k < P.qgroup[g + 1] && all; ++k) {
+                const int32_t tk = codeTask(P.queue[static_cast<size_t>(k)]);
+                all = dtb[static_cast<size_t>(tk)] >= 0 && P.task[static_cast<size_t>(tk)].parent < 0;
+            }
+            c->qgroup_dense[g] = all ? 1 : 0;
+        }
+        c->dtrow_d.upload(dt, c->stream);
+        c->dtce_d.upload(dtce, c->stream);
+        c->dtb_d.upload(dtb, c->stream);
+        c->dt_maxS = maxS;
+    }
     if (!c->queue_ok) {
         c->diag_f.release();
         c->below_f.release();

@@ -118,7 +118,13 @@ void launchBicgControl(const BiArgs& a, int stage, hipStream_t s);
 void launchFillMasked(int64_t n, double v, double* y, int nrhs, const int32_t* skip, hipStream_t s);
 
 template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>& a, int grid, size_t lds, int threads, hipStream_t s);
-template <typename T, typename TF> const void* localSolveKernel();
+template <typename T, typename TF> const void* localSolveKernel(int threads);
+/// A launch of @p grid dense tops (every entry of a.queue a dense top): dt / dt_ce / dtb the row
+/// table of the skeletons (factor position, global row, local index, first slot; end of slots;
+/// offset of a task's rows in the table), lds the bytes of the widest skeleton's b_S.
+template <typename T, typename TF>
+void launchDenseTop(const LocalArgs<T, TF>& a, int grid, const int4* dt, const int32_t* dt_ce, const int32_t* dtb, size_t lds,
+                    hipStream_t s);
 
 /// One launch of the level-synchronous local solve: items [first, first + count) of kind LsKind
 /// (widest supernode maxw) for nrhs right-hand sides.
