@@ -62,8 +62,10 @@ PMC_APPLIES = 3        # applies of a counter pass
 PROFILE_WARM = 2       # untimed applies before the timed ones in --profile-applies runs
 LATENCY_SOLVES = 100   # solves of the latency pass (p50 / p99)
 # kernel names as rocprofv3 reports them; the task-queue kernel's name also carries its workgroup size
-# (k_local_solve<float, float, 1024> or <float, float, 512>)
-LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float,")
+# (k_local_solve<float, float, 1024> or <float, float, 512>); the dense tops of an apply run in
+# k_dense_top<float, float>
+LOCAL_KERNELS = ("k_ls_forward<float>", "k_ls_back_partial<float>", "k_ls_backward<float>", "k_local_solve<float, float,",
+                 "k_dense_top<float, float>")
 
 
 def time_applies(ctx, r, z, count, warm=3):
@@ -85,14 +87,15 @@ def time_applies(ctx, r, z, count, warm=3):
 
 
 def roofline(bytes_per_apply, avg_ms, applies, threads=None):
-    """The dominant kernel's roofline: k_local_solve<float, float, threads>, whose launches of one apply (the subtree
-    forward launch, the top tasks, the subtree backward launch) are timed together with HIP events
-    on their stream; avg_us is their time per apply, bytes the algorithmic bytes of one apply's
-    local solves."""
+    """The dominant kernel's roofline: the local solve's launches of one apply (k_local_solve<float, float,
+    threads> for the subtree forward and backward sweeps, k_dense_top<float, float> for the top tasks
+    when they are all dense tops), timed together with HIP events on their stream; avg_us is their
+    time per apply, bytes the algorithmic bytes of one apply's local solves."""
     achieved = bytes_per_apply / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_local_solve<float, float, {threads or 1024}> (task queue, the launches of one apply)", "applies_timed": int(applies),
+            "kernel": f"k_local_solve<float, float, {threads or 1024}> + k_dense_top<float, float> (task queue, the launches of one apply)",
+            "applies_timed": int(applies),
             "avg_us": round(avg_ms * 1e3, 2),
             "bytes_per_apply": int(bytes_per_apply)}
 
