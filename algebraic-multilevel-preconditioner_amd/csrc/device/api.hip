@@ -324,7 +324,7 @@ struct s2l_ctx {
     CoarseArgs ca{};
     std::vector<int64_t> off_h;
     DevBuf<int64_t> off_d, voff_d;
-    DevBuf<int32_t> vrows_d, vrowptr_d, vrowg_d, blksub_d, blkrow_d, cperm_d;
+    DevBuf<int32_t> vrows_d, vrowptr_d, vrowg_d, vbase_d, blksub_d, blkrow_d, cperm_d;
     DevBuf<double> V_d, T1_d, T2_d;  // inverted factors of A00, row-major with leading dimension tri_ld
     DevBuf<double> Einv_d;           // A00^-1 = T2 T1 (the rows of the permuted columns for LU), same layout
     int32_t tri_ld = 0;
@@ -1929,6 +1929,19 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
             c->vrows_d.upload(vrows, c->stream);
             c->vrowptr_d.upload(vrowptr, c->stream);
             c->vrowg_d.upload(vrowg, c->stream);
+            {
+                // blocks whose rows are consecutive global rows (the owned rows of a Boolean
+                // partition after the renumbering): the restriction reads r without the row map
+                std::vector<int32_t> vbase(static_cast<size_t>(c->nsub), -1);
+                for (int i = 0; i < c->nsub; ++i) {
+                    const int32_t p0 = vrowptr[static_cast<size_t>(i)], rows = vrows[static_cast<size_t>(i)];
+                    bool run = rows > 0;
+                    for (int32_t l = 1; l < rows && run; ++l)
+                        run = vrowg[static_cast<size_t>(p0 + l)] == vrowg[static_cast<size_t>(p0)] + l;
+                    if (run) vbase[static_cast<size_t>(i)] = vrowg[static_cast<size_t>(p0)];
+                }
+                c->vbase_d.upload(vbase, c->stream);
+            }
             c->blksub_d.upload(bsub, c->stream);
             c->blkrow_d.upload(brow, c->stream);
             CoarseArgs& a = c->ca;
@@ -1943,6 +1956,7 @@ int s2l_upload_coarse(s2l_ctx* c, int32_t n0, const int64_t* offsets, const doub
             a.vrows = c->vrows_d.p;
             a.vrow_ptr = c->vrowptr_d.p;
             a.vrow_g = c->vrowg_d.p;
+            a.vbase = c->vbase_d.p;
             a.V = c->V_d.p;
             a.blk_sub = c->blksub_d.p;
             a.blk_row = c->blkrow_d.p;

@@ -72,6 +72,7 @@ struct CoarseArgs {
     const int32_t* vrows;  ///< rows of each V_i block
     const int32_t* vrow_ptr; ///< offset of each block's rows in vrow_g
     const int32_t* vrow_g; ///< global row of each V row
+    const int32_t* vbase;  ///< per block: its global rows are vbase + 0, 1, ... (contiguous), or -1
     const double* V;
     const int32_t* blk_sub; ///< prolongation blocks: subdomain and first row
     const int32_t* blk_row;

@@ -778,8 +778,12 @@ constexpr int kCrows = 256;  // rows of V_i per workgroup of the restriction (on
 // workgroup (i, group, rhs) owns kCg coarse columns of subdomain i over all its rows (no partial
 // sums across workgroups). Each thread sums its rows (tid, tid + blockDim, ...) in order, the
 // wavefronts' sums are combined in wavefront order.
-constexpr int kCg = 8;           // coarse columns per workgroup
+constexpr int kCg = 4;           // coarse columns per workgroup
 constexpr int kCgThreads = 512;
+constexpr int kCgRows = 4;       // rows per thread whose loads are issued together
+// c_q = sum over the rows l of block i of V_i(l, q) r(g(l)), for the kCg columns q of the workgroup:
+// each thread takes kCgRows rows at a time (their V entries, their rows of r: through the row map,
+// or directly when the block's rows are consecutive global rows), then wavefront and workgroup sums.
 __global__ __launch_bounds__(kCgThreads) void k_coarse_restrict_cols(CoarseArgs c, const double* r, int64_t n, double* cvec,
                                                                    const int32_t* skip)
 {
@@ -791,27 +795,32 @@ __global__ __launch_bounds__(kCgThreads) void k_coarse_restrict_cols(CoarseArgs 
     const int rows = c.vrows[i];
     const double* V = c.V + c.voff[i] + static_cast<int64_t>(q0) * rows;
     const int32_t* g = c.vrow_g + c.vrow_ptr[i];
+    const int base = c.vbase[i];
     const double* rv = r + static_cast<int64_t>(rr) * n;
     const int nq = min(kCg, m - q0);
     double acc[kCg];
 #pragma unroll
     for (int q = 0; q < kCg; ++q) acc[q] = 0.0;
-    // two rows per batch: their map entries, then r and the V entries
-    for (int l0 = threadIdx.x; l0 < rows; l0 += 2 * kCgThreads) {
-        const int l1 = l0 + kCgThreads;
-        const int g0 = g[l0], g1 = l1 < rows ? g[l1] : 0;
-        double va[kCg], vb[kCg];
+    for (int l0 = threadIdx.x; l0 < rows; l0 += kCgRows * kCgThreads) {
+        // loads past the block's rows or columns read its last row or column, then count zero
+        int gl[kCgRows];
+        double va[kCgRows][kCg];
 #pragma unroll
-        for (int q = 0; q < kCg; ++q) {
-            va[q] = q < nq ? V[static_cast<int64_t>(q) * rows + l0] : 0.0;
-            vb[q] = q < nq && l1 < rows ? V[static_cast<int64_t>(q) * rows + l1] : 0.0;
+        for (int u = 0; u < kCgRows; ++u) {
+            const int l = min(l0 + u * kCgThreads, rows - 1);
+            gl[u] = base >= 0 ? base + l : g[l];
+#pragma unroll
+            for (int q = 0; q < kCg; ++q) va[u][q] = V[static_cast<int64_t>(min(q, nq - 1)) * rows + l];
         }
-        __builtin_amdgcn_sched_barrier(0);
-        const double x0 = rv[g0], x1 = l1 < rows ? rv[g1] : 0.0;
+        double xv[kCgRows];
 #pragma unroll
-        for (int q = 0; q < kCg; ++q) {
-            acc[q] += va[q] * x0;
-            if (l1 < rows) acc[q] += vb[q] * x1;
+        for (int u = 0; u < kCgRows; ++u) xv[u] = rv[gl[u]];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < kCgRows; ++u) {
+            if (l0 + u * kCgThreads >= rows) continue;
+#pragma unroll
+            for (int q = 0; q < kCg; ++q) acc[q] += (q < nq ? va[u][q] : 0.0) * xv[u];
         }
     }
     __shared__ double red[kCgThreads / kWave][kCg];
