@@ -838,6 +838,7 @@ struct GmresRun {
         bool live = false;      ///< holds a solve
         bool conv = false, breakdown = false;
         bool nonfinite = false; ///< a residual was not finite: the solve stops (no counterpart)
+        double rate = 0.0;      ///< residual estimate's factor per iteration so far (0 = unknown)
         int iters = 0;
         double bnorm = 0, beta = 0, rel = 0;
         double normF = 1, l1_0 = 0, l1 = 0, thr = 0;  ///< OpenFOAM test
@@ -1065,8 +1066,26 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
     const int32_t* live = flags.p;
     const int32_t* idl = flags.p + cap;
     const int64_t N = static_cast<int64_t>(n) * cap;
-    const int batch = 4;  // iterations between host polls of the flags
-    for (int j0 = 0; j0 < m; j0 += batch) {
+    // Iterations between host polls of the flags: as many as the slowest active slot still needs to
+    // reach the tolerance at the rate its residual estimate has fallen so far, 1 to kMaxBatch
+    // (kMaxBatch while no rate is known, and with the OpenFOAM test). A poll costs one
+    // synchronization; every iteration launched past a slot's convergence costs the launches of an
+    // iteration that find it finished (a restart cycle after the true-residual check often needs
+    // one or two iterations).
+    constexpr int kMaxBatch = 4;
+    auto itersNeeded = [&](double rel, double rate) {
+        if (of || !(rel > 0.0) || !(rate > 0.0 && rate < 1.0)) return kMaxBatch;
+        if (rel <= o.rtol) return 1;
+        const double k = std::ceil(std::log(o.rtol / rel) / std::log(rate));
+        return static_cast<int>(std::min<double>(kMaxBatch, std::max(1.0, k)));
+    };
+    int batch = 1;
+    for (int q = 0; q < cap; ++q)
+        if (act[static_cast<size_t>(q)]) {
+            const Slot& S = sl[static_cast<size_t>(q)];
+            batch = std::max(batch, itersNeeded(S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0, S.rate));
+        }
+    for (int j0 = 0; j0 < m;) {
         const int j1 = std::min(m, j0 + batch);
         for (int j = j0; j < j1; ++j) {
             const double* vj = V.p + static_cast<int64_t>(j) * N;
@@ -1086,10 +1105,21 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
         }
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(pflags.p, flags.p, sizeof(int32_t) * cap, hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(pst.p, st.p, sizeof(GmState) * cap, hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
         bool all = true;
-        for (int q = 0; q < cap; ++q) all = all && pflags.p[q];
+        batch = 1;
+        for (int q = 0; q < cap; ++q) {
+            all = all && pflags.p[q];
+            if (pflags.p[q]) continue;
+            Slot& S = sl[static_cast<size_t>(q)];
+            const GmState& G = pst.p[q];
+            const double rel0 = S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0;
+            if (G.k > 0 && rel0 > 0.0 && G.rel > 0.0) S.rate = std::pow(G.rel / rel0, 1.0 / G.k);
+            batch = std::max(batch, itersNeeded(G.rel, S.rate));
+        }
         if (all) break;
+        j0 = j1;
     }
     // x += M^{-1} V_k y_k; true residual (gmres.cpp:135-156)
     gmLsq(a, s);
