@@ -623,7 +623,7 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
                 a.serial = 0;
                 const bool pair = std::is_same_v<T, float2>;
                 launchDenseTop<T, TF>(a, count * (pair ? (a.nrhs + 1) / 2 : a.nrhs), c->dtrow_d.p, c->dtce_d.p, c->dtb_d.p,
-                                      static_cast<size_t>(c->dt_maxS) * sizeof(T), s);
+                                      static_cast<size_t>(c->dt_maxS) * sizeof(T), c->cus, s);
             } else {
                 launchQueue<T, TF>(c, a, q, count, false, s);
             }
@@ -770,18 +770,21 @@ void spmvImpl(s2l_ctx* c, const double* x, double* y, const double* b, int nrhs,
     HIPCHECK(hipGetLastError());
 }
 
-// z = M^{-1} in, w = A z: one launch of a HIP graph of all their kernels (about a dozen launches
-// otherwise), captured on the first call for these buffers and replayed afterwards. Without graphs
-// (or while the local solve is timed with events) the kernels are launched one by one.
+// z = M^{-1} in, w = A z (w null: the apply alone): one launch of a HIP graph of all their kernels
+// (about a dozen launches otherwise), captured on the first call for these buffers and replayed
+// afterwards. Without graphs (or while the local solve is timed with events) the kernels are
+// launched one by one.
 void applySpmv(s2l_ctx* c, const double* in, double* z, double* w, int nrhs, const int32_t* skip, hipStream_t s)
 {
     if (!c->use_graphs || c->timing || c->phase_on) {
         int t = phaseBegin(c, S2L_PHASE_PRECONDITIONER, s);  // gmres.cpp:67
         applyImpl(c, in, z, nrhs, skip, s);
         phaseEnd(c, t, s);
-        t = phaseBegin(c, S2L_PHASE_SPMV, s);                // gmres.cpp:71
-        spmvImpl(c, z, w, nullptr, nrhs, skip, s);
-        phaseEnd(c, t, s);
+        if (w) {
+            t = phaseBegin(c, S2L_PHASE_SPMV, s);            // gmres.cpp:71
+            spmvImpl(c, z, w, nullptr, nrhs, skip, s);
+            phaseEnd(c, t, s);
+        }
         return;
     }
     c->ensureWork(nrhs);  // no allocation while capturing
@@ -795,7 +798,7 @@ void applySpmv(s2l_ctx* c, const double* in, double* z, double* w, int nrhs, con
     HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
         applyImpl(c, in, z, nrhs, skip, s);
-        spmvImpl(c, z, w, nullptr, nrhs, skip, s);
+        if (w) spmvImpl(c, z, w, nullptr, nrhs, skip, s);
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -883,7 +886,7 @@ void GmresRun::configure(const s2l_krylov_opts& opts, int slots)
     require(slots >= 1, S2L_EINVAL, "at least one right-hand side");
     of = o.stop_norm == 1;
     flex = o.flexible != 0;
-    const int nb = reduceBlocks(n);
+    const int nb = gmBlocks(n);
     const size_t N = static_cast<size_t>(n) * cap;
     V.reserve(static_cast<size_t>(m + 1) * N);
     if (flex) Z.reserve(static_cast<size_t>(m) * N);
@@ -1085,14 +1088,27 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
             const Slot& S = sl[static_cast<size_t>(q)];
             batch = std::max(batch, itersNeeded(S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0, S.rate));
         }
+    // one right-hand side whose rows fit one per thread of the GMRES grid: the SpMV of the Arnoldi
+    // step and the first CGS2 pass in one launch (the same bits as the two launches)
+    const bool fuse = cap == 1 && static_cast<int64_t>(a.nblocks) * kGmThreads >= n;
     for (int j0 = 0; j0 < m;) {
         const int j1 = std::min(m, j0 + batch);
         for (int j = j0; j < j1; ++j) {
             const double* vj = V.p + static_cast<int64_t>(j) * N;
             double* zj = flex ? Z.p + static_cast<int64_t>(j) * N : z.p;
-            applySpmv(c, vj, zj, w.p, cap, live, s);  // z = M^{-1} v_j (kept: flexible); w = A z
+            if (fuse) {
+                applySpmv(c, vj, zj, nullptr, cap, live, s);  // z = M^{-1} v_j (kept: flexible)
+                const int ts = phaseBegin(c, S2L_PHASE_SPMV, s);  // gmres.cpp:71, with CGS2 pass 1's products
+                gmSpmvDots(a, j, c->rp.p, c->col.p, c->val.p, zj, s);  // w = A z, partials of V^T w
+                phaseEnd(c, ts, s);
+            } else {
+                applySpmv(c, vj, zj, w.p, cap, live, s);  // z = M^{-1} v_j (kept: flexible); w = A z
+            }
             const int t = phaseBegin(c, S2L_PHASE_ORTHOGONALIZATION, s);  // gmres.cpp:77
-            gmDots(a, j, s);                          // CGS2 pass 1
+            if (fuse)
+                gmFinalH(a, j, s);                    // CGS2 pass 1: h
+            else
+                gmDots(a, j, s);                      // CGS2 pass 1
             gmUpdateDots(a, j, s);                    // w -= V h; pass 2
             gmUpdateNorm(a, j, s);                    // w -= V h2; partials of ||w||^2
             gmNormalize(a, j, s);                     // ||w||, Givens, v_{j+1} = w / ||w||

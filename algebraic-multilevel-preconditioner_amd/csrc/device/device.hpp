@@ -13,6 +13,7 @@ namespace s2l {
 
 constexpr int kLocalThreads = 1024; ///< threads of a local-solve workgroup (one per queue entry); 512 is the other choice
 constexpr int kGmMaxK = 65;         ///< GMRES restart length + 1 handled by the device loop
+constexpr int kGmThreads = 256;     ///< threads per workgroup of the GMRES vector kernels
 constexpr int kStepBlock = 32;      ///< steps of a task's sweep staged in LDS at a time
 
 /// Arguments of the batched local solve (all pointers are device pointers).
@@ -122,10 +123,12 @@ template <typename T, typename TF> void launchLocalSolve(const LocalArgs<T, TF>&
 template <typename T, typename TF> const void* localSolveKernel(int threads);
 /// A launch of @p grid dense tops (every entry of a.queue a dense top): dt / dt_ce / dtb the row
 /// table of the skeletons (factor position, global row, local index, first slot; end of slots;
-/// offset of a task's rows in the table), lds the bytes of the widest skeleton's b_S.
+/// offset of a task's rows in the table), lds the bytes of the widest skeleton's b_S; cus the compute
+/// units of the device (a launch of at most that many workgroups takes the variant with more loads
+/// in flight).
 template <typename T, typename TF>
 void launchDenseTop(const LocalArgs<T, TF>& a, int grid, const int4* dt, const int32_t* dt_ce, const int32_t* dtb, size_t lds,
-                    hipStream_t s);
+                    int cus, hipStream_t s);
 
 /// One launch of the level-synchronous local solve: items [first, first + count) of kind LsKind
 /// (widest supernode maxw) for nrhs right-hand sides.
@@ -171,6 +174,7 @@ void launchFill(int64_t n, double v, double* y, hipStream_t s);
 void launchReduce(int kind, int64_t n, const double* x, const double* y, double* partial, double* out, int nrhs,
                   const int32_t* skip, hipStream_t s);
 int reduceBlocks(int64_t n);
+int gmBlocks(int64_t n);
 
 // ---- GMRES
 struct GmArgs {
@@ -197,6 +201,11 @@ struct GmArgs {
     int32_t of;         ///< 1: OpenFOAM stopping test (gmres.cpp:106-122), run by gmOpenFoamTest
 };
 void gmDots(const GmArgs& a, int j, hipStream_t s);
+/// w = A z (one right-hand side per grid row) and the first CGS2 pass's partials in one launch;
+/// gmFinalH sums them into h (together: gmDots after an SpMV into w)
+void gmSpmvDots(const GmArgs& a, int j, const int32_t* rp, const int32_t* col, const double* val, const double* z,
+                hipStream_t s);
+void gmFinalH(const GmArgs& a, int j, hipStream_t s);
 void gmUpdateDots(const GmArgs& a, int j, hipStream_t s);
 void gmUpdateNorm(const GmArgs& a, int j, hipStream_t s);
 void gmNormalize(const GmArgs& a, int j, hipStream_t s);

@@ -223,14 +223,16 @@ template <typename T, int R> __device__ inline T transposeSum(T (&v)[R], int lan
 // columns (one vector load per row and 256 columns), and out(r, y_r) is called by one lane per
 // row. kTri 1 skips the column quads right of the rows (lower triangle), 2 those left of them
 // (upper); the stored block is zero there. All lanes of a wavefront run the same rows.
-template <typename T, typename TF, int kTri, typename Out>
+template <typename T, typename TF, int kTri, int NB = 1, typename Out>
 __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int cols, const T* xs, int lane, int wave, int nw, Out out)
 {
     // Two rows per group with an eight-byte factor or arithmetic type (FP64 factors, FP64 sums of
     // FP32 entries, a pair of FP32 right-hand sides): four rows of them spill registers. Those
     // passes take two groups of R rows, nw * R rows apart, whose loads are issued together; FP32
     // throughout takes one group of four rows (no registers are left for a second). The block's
-    // entries (TF) are converted to the arithmetic type T.
+    // entries (TF) are converted to the arithmetic type T. NB blocks of 4 * 64 columns have their
+    // loads issued together (a caller with the registers of one workgroup per compute unit); the
+    // products are summed in column order whatever NB is.
     constexpr int R = sizeof(T) == 8 ? 2 : blockRows<TF>();
     constexpr int P = R == 2 ? 2 : 1;
     for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
@@ -239,54 +241,59 @@ __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int col
         T accA[R], accB[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) accA[r] = accB[r] = T(0);
-        for (int cb = 0; cb < cols; cb += 4 * kWave) {
-            const int c = cb + 4 * lane;
-            bool needA = c < cols, needB = c < cols && nB > 0;
-            if (kTri == 1) {
-                needA = needA && c <= rA + nA - 1;
-                needB = needB && c <= rB + nB - 1;
-            }
-            if (kTri == 2) {
-                needA = needA && c + 3 >= rA;
-                needB = needB && c + 3 >= rB;
-            }
-            T xv[4];
-            TF mA[R][4], mB[R][4];
+        for (int cb0 = 0; cb0 < cols; cb0 += NB * 4 * kWave) {
+            T xv[NB][4];
+            TF mA[NB][R][4], mB[NB][R][4];
+            bool okA[NB][R], okB[NB][R];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) xv[i] = c + i < cols ? xs[c + i] : T(0);
-            bool okA[R], okB[R];
+            for (int nb = 0; nb < NB; ++nb) {
+                const int c = cb0 + nb * 4 * kWave + 4 * lane;
+                bool needA = c < cols, needB = c < cols && nB > 0;
+                if (kTri == 1) {
+                    needA = needA && c <= rA + nA - 1;
+                    needB = needB && c <= rB + nB - 1;
+                }
+                if (kTri == 2) {
+                    needA = needA && c + 3 >= rA;
+                    needB = needB && c + 3 >= rB;
+                }
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                okA[r] = needA && r < nA;
-                okB[r] = needB && r < nB;
-                if constexpr (std::is_same_v<T, TF>) {
-                    // loads under a branch, the quads not needed set to zero
+                for (int i = 0; i < 4; ++i) xv[nb][i] = c + i < cols ? xs[c + i] : T(0);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) mA[r][i] = mB[r][i] = TF(0);
-                    if (okA[r]) load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[r]);
-                    if (okB[r]) load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[r]);
-                } else {
-                    // entries converted to T: every lane loads all its quads, those it does not
-                    // need from the block's first (cached) quad, and zeroes them after the loads
-                    // (under a branch the conversion joins the load, which then waits for each
-                    // load before issuing the next)
-                    load4(M + (okA[r] ? static_cast<int64_t>(rA + r) * ld + c : 0), mA[r]);
-                    load4(M + (okB[r] ? static_cast<int64_t>(rB + r) * ld + c : 0), mB[r]);
+                for (int r = 0; r < R; ++r) {
+                    okA[nb][r] = needA && r < nA;
+                    okB[nb][r] = needB && r < nB;
+                    if constexpr (std::is_same_v<T, TF>) {
+                        // loads under a branch, the quads not needed set to zero
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) mA[nb][r][i] = mB[nb][r][i] = TF(0);
+                        if (okA[nb][r]) load4(M + static_cast<int64_t>(rA + r) * ld + c, mA[nb][r]);
+                        if (okB[nb][r]) load4(M + static_cast<int64_t>(rB + r) * ld + c, mB[nb][r]);
+                    } else {
+                        // entries converted to T: every lane loads all its quads, those it does not
+                        // need from the block's first (cached) quad, and zeroes them after the loads
+                        // (under a branch the conversion joins the load, which then waits for each
+                        // load before issuing the next)
+                        load4(M + (okA[nb][r] ? static_cast<int64_t>(rA + r) * ld + c : 0), mA[nb][r]);
+                        load4(M + (okB[nb][r] ? static_cast<int64_t>(rB + r) * ld + c : 0), mB[nb][r]);
+                    }
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < R; ++r)
+            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if constexpr (std::is_same_v<T, TF>) {
-                        accA[r] += mA[r][i] * xv[i];
-                        accB[r] += mB[r][i] * xv[i];
-                    } else {
-                        accA[r] += (okA[r] ? widen<T>(mA[r][i]) : T(0)) * xv[i];
-                        accB[r] += (okB[r] ? widen<T>(mB[r][i]) : T(0)) * xv[i];
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if constexpr (std::is_same_v<T, TF>) {
+                            accA[r] += mA[nb][r][i] * xv[nb][i];
+                            accB[r] += mB[nb][r][i] * xv[nb][i];
+                        } else {
+                            accA[r] += (okA[nb][r] ? widen<T>(mA[nb][r][i]) : T(0)) * xv[nb][i];
+                            accB[r] += (okB[nb][r] ? widen<T>(mB[nb][r][i]) : T(0)) * xv[nb][i];
+                        }
                     }
-                }
         }
         const T yA = transposeSum<T, R>(accA, lane);
         const T yB = transposeSum<T, R>(accB, lane);
@@ -632,11 +639,14 @@ __global__ __launch_bounds__(kThreads, kThreads == 1024 ? 1 : 4) void k_local_so
 // S^-1 in flight (3D Poisson 64^3, 64 skeletons of about 820 rows: 48.5 -> 43.7 us per launch).
 // dt[dtb[task] + q] = (factor position, global row, local index, first slot) of the skeleton's row
 // q, dt_ce its end of slots.
+// kNB > 1 (a launch of at most one workgroup per compute unit): the registers of one workgroup per
+// compute unit, with the loads of kNB blocks of 256 columns of S^-1 issued together (the same sums
+// in the same order: the same bits).
 constexpr int kDtThreads = 1024;
 
-template <typename T, typename TF>
-__global__ __launch_bounds__(kDtThreads, 2) void k_dense_top(LocalArgs<T, TF> a, const int4* dt, const int32_t* dt_ce,
-                                                             const int32_t* dtb)
+template <typename T, typename TF, int kNB>
+__global__ __launch_bounds__(kDtThreads, kNB == 1 ? 2 : 1) void k_dense_top(LocalArgs<T, TF> a, const int4* dt,
+                                                                          const int32_t* dt_ce, const int32_t* dtb)
 {
     extern __shared__ unsigned char smem_raw[];
     T* bS = reinterpret_cast<T*>(smem_raw);
@@ -694,22 +704,17 @@ __global__ __launch_bounds__(kDtThreads, 2) void k_dense_top(LocalArgs<T, TF> a,
             store(rA, w.z, static_cast<double>(v));
         }
     };
-    denseRows<T, TF, 0>(a.diag + sp.wide.diag + static_cast<int64_t>(qlo) * ld, ld, qhi - qlo, S, bS, lane, wave, nt / kWave, out);
+    denseRows<T, TF, 0, kNB>(a.diag + sp.wide.diag + static_cast<int64_t>(qlo) * ld, ld, qhi - qlo, S, bS, lane, wave, nt / kWave,
+                             out);
 }
 
 // ------------------------------------------------------------------ SpMV: one thread per row,
 // sequential sum in the stored (local) column order with fma, as DistributedMatrix::apply.
-__global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
-                       const double* b, const int32_t* skip)
+// Row i of A x: the row's entries eight at a time, columns and values in one round trip, x in the
+// next.
+__device__ __forceinline__ double spmvRow(const int32_t* rp, const int32_t* col, const double* val, const double* xr, int i)
 {
-    const int r = blockIdx.y;
-    if (skip && skip[r]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t off = static_cast<int64_t>(r) * n;
-    const double* xr = x + off;
     double s = 0.0;
-    // the row's entries eight at a time: columns and values in one round trip, x in the next
     const int ke = rp[i + 1];
     for (int k0 = rp[i]; k0 < ke; k0 += 8) {
         int c[8];
@@ -727,6 +732,18 @@ __global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const d
         for (int u = 0; u < 8; ++u)
             if (k0 + u < ke) s = fma(v[u], xv[u], s);
     }
+    return s;
+}
+
+__global__ void k_spmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                       const double* b, const int32_t* skip)
+{
+    const int r = blockIdx.y;
+    if (skip && skip[r]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t off = static_cast<int64_t>(r) * n;
+    const double s = spmvRow(rp, col, val, x + off, i);
     y[off + i] = b ? b[off + i] - s : s;
 }
 
@@ -1032,10 +1049,14 @@ __global__ void k_coarse_final(CoarseArgs c, const double* part, double* cvec, c
     }
 }
 
+// 16-byte loads per lane and batch of k_tri_gemv: a row of up to 1536 entries in one round trip
+constexpr int kGemvBatch = 12;
+
 // y = T x for one right-hand side per grid row: one wavefront per row of the triangle (lower 1,
 // upper 0) or of a full matrix (lower 2), a lane reading two consecutive entries with one 16-byte
-// load (rows padded to ld, a multiple of eight, with zeros), four such loads per batch; lane
-// partial sums, then the wavefront sum.
+// load (rows padded to ld, a multiple of eight, with zeros), kGemvBatch such loads per batch; lane
+// partial sums (entries 2 lane + 128 k in increasing k), then the wavefront sum.
+
 __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const double* Tm, int lower, const int32_t* perm,
                                                    const double* x, double* y, const int32_t* skip)
 {
@@ -1048,11 +1069,11 @@ __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const 
     const int lo = lower == 0 ? (i & ~1) : 0, hi = lower == 1 ? i + 1 : n0;  // even start: 16-byte aligned
     const double* row = Tm + static_cast<int64_t>(i) * ld;
     double s = 0.0;
-    for (int j0 = lo; j0 < hi; j0 += 8 * kWave) {
-        double2 t[4];
-        double xa[4], xb[4];
+    for (int j0 = lo; j0 < hi; j0 += 2 * kGemvBatch * kWave) {
+        double2 t[kGemvBatch];
+        double xa[kGemvBatch], xb[kGemvBatch];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kGemvBatch; ++u) {
             const int j = j0 + 2 * lane + 2 * kWave * u;
             const bool ok = j < hi;
             t[u] = ok ? *reinterpret_cast<const double2*>(row + j) : double2{0.0, 0.0};
@@ -1061,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_tri_gemv(int32_t n0, int32_t ld, const 
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) s += t[u].x * xa[u] + t[u].y * xb[u];
+        for (int u = 0; u < kGemvBatch; ++u) s += t[u].x * xa[u] + t[u].y * xb[u];
     }
     s = waveSum(s);
     if (lane == 0) y[static_cast<int64_t>(rr) * n0 + i] = s;
@@ -1514,7 +1535,6 @@ __global__ void k_bicg_half(int64_t n, double* x, const double* y, const double*
 }
 
 // ------------------------------------------------------------------ GMRES
-constexpr int kGmThreads = 256;
 
 // Basis layout: V[(k * nrhs + r) * n + i], so that column k of every right-hand side is one
 // contiguous n x nrhs block (the input layout of the preconditioner and the SpMV).
@@ -1558,6 +1578,22 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_dots(GmArgs a, int j)
     const int r = blockIdx.y;
     if (a.flags[r]) return;
     gmDotsBody(a, j, r, a.w + static_cast<int64_t>(r) * a.n);
+}
+
+// w = A z with the rows of k_spmv (the same fma order: the same bits), then the partials of
+// k_gm_dots over the same grid (the same bits): one launch for the SpMV of the Arnoldi step and the
+// first pass of CGS2. A thread's rows are the ones it reads back for the dots.
+__global__ __launch_bounds__(kGmThreads) void k_gm_spmv_dots(GmArgs a, int j, const int32_t* rp, const int32_t* col,
+                                                             const double* val, const double* z)
+{
+    const int r = blockIdx.y;
+    if (a.flags[r]) return;
+    const int64_t off = static_cast<int64_t>(r) * a.n;
+    double* w = a.w + off;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n; i += stride)
+        w[i] = spmvRow(rp, col, val, z + off, static_cast<int>(i));
+    gmDotsBody(a, j, r, w);
 }
 
 // w -= V h, then partials of h2 = V^T w.
@@ -1605,11 +1641,13 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_update_norm(GmArgs a, int j)
     nrm = waveSum(nrm);
     if (lane == 0) red[wave] = nrm;
     __syncthreads();
-    double* pr = a.partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kGmMaxK;
+    // the partials of ||w||^2 one after another in this right-hand side's region (its h2 partials
+    // there were summed by the previous launch), so each workgroup of k_gm_givens_normalize reads
+    // them as one contiguous block
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int wv = 0; wv < kGmThreads / kWave; ++wv) s += red[wv];
-        pr[0] = s;
+        a.partial[static_cast<int64_t>(r) * gridDim.x * kGmMaxK + blockIdx.x] = s;
     }
 }
 
@@ -1621,6 +1659,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_final(GmArgs a, int j, int
     if (a.flags[r]) return;
     const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK + k;
     double v = 0.0;
+#pragma unroll 4
     for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
     const double t = blockSum(v);
     if (threadIdx.x == 0) (second ? a.h2 : a.h)[r * (a.m + 1) + k] = t;
@@ -1684,7 +1723,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_givens_normalize(GmArgs a,
     if (a.flags[r]) return;
     const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
     double v = 0.0;
-    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+#pragma unroll 4
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[b];
     const double ss = blockSum(v);
     __shared__ double hs;
     if (threadIdx.x == 0) hs = sqrt(ss);
@@ -1728,7 +1768,7 @@ __global__ __launch_bounds__(kGmThreads) void k_gm_of_partial(GmArgs a, int j)
         }
     }
     const double t = blockSum(s1);
-    if (threadIdx.x == 0) a.partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kGmMaxK] = t;
+    if (threadIdx.x == 0) a.partial[static_cast<int64_t>(r) * gridDim.x * kGmMaxK + blockIdx.x] = t;
 }
 
 // sum|r_j| = |g_{j+1}| x (the partials in block order); convergence and the right-hand side's flag.
@@ -1738,7 +1778,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_gm_of_final(GmArgs a, int j)
     if (a.flags[r]) return;
     const double* base = a.partial + static_cast<int64_t>(r) * a.nblocks * kGmMaxK;
     double v = 0.0;
-    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[static_cast<int64_t>(b) * kGmMaxK];
+#pragma unroll 4
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x) v += base[b];
     const double t = blockSum(v);
     if (threadIdx.x == 0) {
         GmState& st = a.st[r];
@@ -1830,18 +1871,23 @@ template const void* localSolveKernel<float2, float>(int);
 
 template <typename T, typename TF>
 void launchDenseTop(const LocalArgs<T, TF>& a, int grid, const int4* dt, const int32_t* dt_ce, const int32_t* dtb, size_t lds,
-                    hipStream_t s)
+                    int cus, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_dense_top<T, TF>), dim3(grid), dim3(kDtThreads), lds, s, a, dt, dt_ce, dtb);
+    // FP64 entries: two blocks of columns per batch (three need more than the 128 registers)
+    constexpr int kWideNB = sizeof(TF) == 8 ? 2 : 3;
+    if (grid <= cus)
+        hipLaunchKernelGGL((k_dense_top<T, TF, kWideNB>), dim3(grid), dim3(kDtThreads), lds, s, a, dt, dt_ce, dtb);
+    else
+        hipLaunchKernelGGL((k_dense_top<T, TF, 1>), dim3(grid), dim3(kDtThreads), lds, s, a, dt, dt_ce, dtb);
 }
 template void launchDenseTop<float, float>(const LocalArgs<float, float>&, int, const int4*, const int32_t*, const int32_t*, size_t,
-                                           hipStream_t);
+                                           int, hipStream_t);
 template void launchDenseTop<double, double>(const LocalArgs<double, double>&, int, const int4*, const int32_t*, const int32_t*,
-                                             size_t, hipStream_t);
+                                             size_t, int, hipStream_t);
 template void launchDenseTop<double, float>(const LocalArgs<double, float>&, int, const int4*, const int32_t*, const int32_t*,
-                                            size_t, hipStream_t);
+                                            size_t, int, hipStream_t);
 template void launchDenseTop<float2, float>(const LocalArgs<float2, float>&, int, const int4*, const int32_t*, const int32_t*,
-                                            size_t, hipStream_t);
+                                            size_t, int, hipStream_t);
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)
@@ -1905,6 +1951,15 @@ void launchAxpby(int64_t n, double a, const double* x, double b, double* y, int 
 void launchCopy(int64_t n, const double* x, double* y, int nrhs, hipStream_t s)
 {
     hipLaunchKernelGGL(k_copy, dim3(nblk(n, 256), nrhs), dim3(256), 0, s, n, x, y);
+}
+
+// Workgroups of the GMRES vector kernels: one element per thread up to 2048 workgroups of
+// kGmThreads (a thread's elements are dependent round trips: four per thread cost a dot product
+// about three times one), then a grid stride.
+int gmBlocks(int64_t n)
+{
+    const int64_t b = (n + kGmThreads - 1) / kGmThreads;
+    return static_cast<int>(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
 int reduceBlocks(int64_t n)
@@ -1976,6 +2031,15 @@ void launchFill(int64_t n, double v, double* y, hipStream_t s)
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, n, v, y);
 }
 
+void gmSpmvDots(const GmArgs& a, int j, const int32_t* rp, const int32_t* col, const double* val, const double* z,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gm_spmv_dots, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j, rp, col, val, z);
+}
+void gmFinalH(const GmArgs& a, int j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gm_final, dim3(j + 1, a.nrhs), dim3(kBlockThreads), 0, s, a, j, 0);
+}
 void gmDots(const GmArgs& a, int j, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_dots, dim3(a.nblocks, a.nrhs), dim3(kGmThreads), 0, s, a, j);

@@ -74,10 +74,11 @@ KERNEL_TYPES = {"single": ("float", "float"), "mixed": ("double", "float"), "dou
 def local_kernels(precision):
     """Kernel names of one apply's local solve as rocprofv3 reports them: the task-queue kernel's name
     also carries its workgroup size (k_local_solve<T, TF, 1024> or <T, TF, 512>); the dense tops of an
-    apply run in k_dense_top<T, TF>; the level-synchronous kernels in k_ls_*<T>."""
+    apply run in k_dense_top<T, TF, NB> (NB column blocks per batch of loads); the level-synchronous
+    kernels in k_ls_*<T>."""
     t, tf = KERNEL_TYPES[precision]
     return (f"k_ls_forward<{t}>", f"k_ls_back_partial<{t}>", f"k_ls_backward<{t}>", f"k_local_solve<{t}, {tf},",
-            f"k_dense_top<{t}, {tf}>")
+            f"k_dense_top<{t}, {tf},")
 
 
 def time_applies(ctx, r, z, count, warm=3):
@@ -107,7 +108,7 @@ def roofline(bytes_per_apply, avg_ms, applies, threads=None, precision="mixed"):
     t, tf = KERNEL_TYPES[precision]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_local_solve<{t}, {tf}, {threads or 1024}> + k_dense_top<{t}, {tf}> (task queue, the launches of one apply)",
+            "kernel": f"k_local_solve<{t}, {tf}, {threads or 1024}> + k_dense_top<{t}, {tf}, NB> (task queue, the launches of one apply)",
             "applies_timed": int(applies),
             "avg_us": round(avg_ms * 1e3, 2),
             "bytes_per_apply": int(bytes_per_apply)}
