@@ -241,6 +241,7 @@ struct s2l_ctx {
     DevBuf<int32_t> levrow_d, levcb_d, levce_d;
     DevBuf<StepDesc> steps_d;
     size_t y_off = 0, st_off = 0, step_off = 0;
+    int32_t st_cap = 0;  // supernode descriptors of the staging area (the most of a chunk)
     size_t slot_off = 0, slot_off2 = 0;  // LDS slots of a task without child tasks (one rhs / pair)
     int32_t slot_cap = 0, slot_cap2 = 0;
     // FP32 factors, two right-hand sides per workgroup (a float2 vector in LDS): its LDS layout
@@ -467,6 +468,7 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     a.y_off = static_cast<int32_t>(pair ? c->y_off2 : c->y_off);
     a.st_off = static_cast<int32_t>(pair ? c->st_off2 : c->st_off);
     a.step_off = static_cast<int32_t>(pair ? c->step_off2 : c->step_off);
+    a.st_cap = c->st_cap;
     a.lev_desc = c->levdesc_d.p;
     a.lev_row = c->levrow_d.p;
     a.lev_cb = c->levcb_d.p;
@@ -836,7 +838,7 @@ struct GmresRun {
     DevBuf<int32_t> flags, mask;
     PinBuf<GmState> pst;
     PinBuf<int32_t> pflags, pmask;
-    PinBuf<double> pred, pcoef;
+    PinBuf<double> pred, pred0, pcoef;  ///< pred0: the sums of ||b||^2 at the start of a solve
     struct Slot {
         bool live = false;      ///< holds a solve
         bool conv = false, breakdown = false;
@@ -912,6 +914,7 @@ void GmresRun::configure(const s2l_krylov_opts& opts, int slots)
     pflags.reserve(static_cast<size_t>(2) * cap);
     pmask.reserve(static_cast<size_t>(cap));
     pred.reserve(static_cast<size_t>(3) * cap);
+    pred0.reserve(static_cast<size_t>(3) * cap);
     pcoef.reserve(static_cast<size_t>(4) * cap);
     sl.assign(static_cast<size_t>(cap), Slot{});
     fresh.assign(static_cast<size_t>(cap), 0);
@@ -951,12 +954,22 @@ void GmresRun::begin(const std::vector<char>& on, const double* d_b, double* d_x
     }
     if (!any) return;
     setMask(on, s);
-    reduce(0, d_b, nullptr, s);
+    if (!of) {
+        // ||b||^2, r = b - A x0 and ||r||^2 behind one synchronization (a slot with b = 0 computes
+        // an r it does not use)
+        launchReduce(0, n, d_b, nullptr, part.p, red.p, cap, mask.p, s);
+        HIPCHECK(hipMemcpyAsync(pred0.p, red.p, sizeof(double) * 3 * cap, hipMemcpyDeviceToHost, s));
+        spmvImpl(c, d_x, r.p, d_b, cap, mask.p, s);
+        reduce(0, r.p, nullptr, s);
+    } else {
+        reduce(0, d_b, nullptr, s);
+        std::memcpy(pred0.p, pred.p, sizeof(double) * 3 * cap);
+    }
     std::vector<char> nz(static_cast<size_t>(cap), 0);
     for (int q = 0; q < cap; ++q) {
         if (!on[static_cast<size_t>(q)]) continue;
         Slot& S = sl[static_cast<size_t>(q)];
-        S.bnorm = std::sqrt(sum(q, 0));
+        S.bnorm = std::sqrt(pred0.p[3 * q]);
         if (S.bnorm == 0.0) {  // x = 0, converged (gmres.cpp:28-33)
             HIPCHECK(hipMemsetAsync(d_x + static_cast<int64_t>(q) * n, 0, sizeof(double) * n, s));
             S.conv = true;
@@ -968,11 +981,7 @@ void GmresRun::begin(const std::vector<char>& on, const double* d_b, double* d_x
     bool anynz = false;
     for (char v : nz) anynz = anynz || v;
     if (!anynz) return;
-    setMask(nz, s);
-    if (!of) {
-        spmvImpl(c, d_x, r.p, d_b, cap, mask.p, s);  // r = b - A x
-        reduce(0, r.p, nullptr, s);
-    } else {
+    if (of) {
         // F = sum|A x0 - xm A 1| + sum|b - xm A 1| + 1e-20, xm = mean(x0) (krylov.cpp:47-57)
         spmvImpl(c, d_x, r.p, nullptr, cap, mask.p, s);  // r = A x0
         reduce(5, d_x, nullptr, s);
@@ -1016,6 +1025,8 @@ void GmresRun::begin(const std::vector<char>& on, const double* d_b, double* d_x
         setMask(nz, s);
         launchAxpby(n, 1.0, d_b, -1.0, r.p, cap, mask.p, s);  // r = b - A x0
         reduce(1, r.p, nullptr, s);
+    } else {
+        setMask(nz, s);  // the slots that iterate
     }
     for (int q = 0; q < cap; ++q) {
         if (!nz[static_cast<size_t>(q)]) continue;
@@ -1563,10 +1574,15 @@ void finalizeLocal(s2l_ctx* c, int precision)
         P.sub[static_cast<size_t>(i)].n_owned = c->ownership[i + 1] - c->ownership[i];
         P.sub[static_cast<size_t>(i)].begin = c->ownership[i];
     }
-    // LDS: the subdomain vector, the result buffer of one chunk, the staged descriptors
+    // LDS: the subdomain vector, the result buffer of one step, the staged descriptors, sized to the
+    // packed steps (the rest holds slots). The result buffer keeps at least kMinStepRows rows: the
+    // row groups of a backward step (G x w <= threads of a workgroup) use it too.
+    const size_t yRows = static_cast<size_t>(std::max(P.max_step_rows, kMinStepRows));
+    c->st_cap = std::max(P.max_step_sn, 1);
+    const size_t stBytes = static_cast<size_t>(c->st_cap) * (sizeof(SnDesc) + 2 * sizeof(int32_t));
     c->y_off = (static_cast<size_t>(P.max_n) * esz + 15) / 16 * 16;
-    c->st_off = c->y_off + (static_cast<size_t>(kChunkRows) * esz + 15) / 16 * 16;
-    c->step_off = (c->st_off + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+    c->st_off = c->y_off + (yRows * esz + 15) / 16 * 16;
+    c->step_off = (c->st_off + stBytes + 15) / 16 * 16;
     c->lds_bytes = c->step_off + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
     // two 512-thread workgroups share a compute unit's LDS; when half of it does not hold the
     // layout, the workgroups take 1024 threads
@@ -1583,8 +1599,8 @@ void finalizeLocal(s2l_ctx* c, int precision)
     };
     if (c->queue_ok) c->lds_bytes = slotRoom(c->lds_bytes, esz, c->slot_off, c->slot_cap);
     c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
-    c->st_off2 = c->y_off2 + (static_cast<size_t>(kChunkRows) * sizeof(float2) + 15) / 16 * 16;
-    c->step_off2 = (c->st_off2 + static_cast<size_t>(kChunkSn) * (sizeof(SnDesc) + 2 * sizeof(int32_t)) + 15) / 16 * 16;
+    c->st_off2 = c->y_off2 + (yRows * sizeof(float2) + 15) / 16 * 16;
+    c->step_off2 = (c->st_off2 + stBytes + 15) / 16 * 16;
     c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
     c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= lds_max;
     if (c->pair_ok) c->lds_bytes2 = slotRoom(c->lds_bytes2, sizeof(float2), c->slot_off2, c->slot_cap2);

@@ -26,6 +26,7 @@ template <typename T, typename TF = T> struct LocalArgs {
     int32_t y_off;            ///< byte offset of the result buffer in the dynamic LDS
     int32_t st_off;           ///< byte offset of the staged supernode descriptors
     int32_t step_off;         ///< byte offset of the staged steps (kStepBlock StepDesc)
+    int32_t st_cap;           ///< supernode descriptors the staging area holds (the most of a chunk)
     const SnDesc* lev_desc;   ///< descriptors in level order
     const int32_t* lev_row;   ///< forward: factor position of each small row, level order
     const int32_t* lev_cb;    ///< forward: first pulled slot of each small row

@@ -490,8 +490,8 @@ __device__ __forceinline__ void runSteps(const LocalArgs<T, TF>& a, const SubDes
 {
     T* ybuf = reinterpret_cast<T*>(smem + a.y_off);
     SnDesc* st = reinterpret_cast<SnDesc*>(smem + a.st_off);
-    int* wp = reinterpret_cast<int*>(smem + a.st_off + kChunkSn * sizeof(SnDesc));
-    int* hp = wp + kChunkSn;
+    int* wp = reinterpret_cast<int*>(smem + a.st_off + a.st_cap * sizeof(SnDesc));
+    int* hp = wp + a.st_cap;
     StepDesc* sb = reinterpret_cast<StepDesc*>(smem + a.step_off);
     for (int k0 = 0; k0 < nsteps; k0 += kStepBlock) {
         const int kn = min(kStepBlock, nsteps - k0);

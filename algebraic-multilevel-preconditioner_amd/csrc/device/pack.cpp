@@ -787,6 +787,10 @@ Packed packFactors(const std::vector<HostFactor>& fs, const std::vector<int32_t>
         P.max_levels_fwd = std::max(P.max_levels_fwd, td.f_nlev);
         P.max_levels_bwd = std::max(P.max_levels_bwd, td.b_nlev);
     }
+    for (const StepDesc& sp : P.steps) {
+        P.max_step_rows = std::max(P.max_step_rows, sp.kind == 0 ? sp.W : sp.wide.w);
+        if (sp.kind == 0) P.max_step_sn = std::max(P.max_step_sn, sp.cnt);
+    }
     runDiagonalJobs(P.diag, jobs);
     runDenseJobs(P.diag, djobs, fs);
     return P;

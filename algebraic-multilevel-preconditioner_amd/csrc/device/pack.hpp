@@ -10,7 +10,8 @@ namespace s2l {
 
 constexpr int32_t kSmallWidth = 64;   ///< supernodes up to one wavefront wide are "small"
 constexpr int32_t kChunkSn = 512;     ///< small supernodes staged in LDS at a time
-constexpr int32_t kChunkRows = 4096;  ///< rows of one chunk (LDS result buffer)
+constexpr int32_t kChunkRows = 4096;  ///< most rows of one chunk (and widest supernode)
+constexpr int32_t kMinStepRows = 1024; ///< least rows of the LDS result buffer (the largest workgroup)
 constexpr int32_t kMaxSkeleton = 16;       ///< most supernodes cut from the subtrees into the top task
 constexpr int32_t kDenseTopMax = 2048;     ///< widest skeleton replaced by a dense inverse (default limit)
 constexpr int32_t kMaxParts = 32;          ///< most workgroups sharing the rows of a dense top
@@ -135,6 +136,8 @@ struct Packed {
     int64_t t_total = 0;
     int64_t entries_fwd = 0, entries_bwd = 0;
     int32_t max_levels_fwd = 0, max_levels_bwd = 0, max_n = 0;
+    int32_t max_step_rows = 0;        ///< most rows of a step (a chunk's rows, a wide supernode's or a dense top's width)
+    int32_t max_step_sn = 0;          ///< most small supernodes of a chunk
     int64_t bytes_index = 0;          ///< bytes of index data read per solve (ridx + contributions)
 };
 
