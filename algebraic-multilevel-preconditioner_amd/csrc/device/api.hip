@@ -852,6 +852,7 @@ struct GmresRun {
     };
     std::vector<Slot> sl;
     std::vector<char> fresh;  ///< admitted since the last cycle (begin() pending)
+    double rate_hint = 0.0;   ///< residual factor per iteration of the last converged slot (0 = none yet)
 
     GmresRun(s2l_ctx* ctx, const s2l_krylov_opts& opts, int slots) : c(ctx) { configure(opts, slots); }
     /// options and slot count of the next solves; the device buffers only grow
@@ -1081,14 +1082,15 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
     const int32_t* idl = flags.p + cap;
     const int64_t N = static_cast<int64_t>(n) * cap;
     // Iterations between host polls of the flags: as many as the slowest active slot still needs to
-    // reach the tolerance at the rate its residual estimate has fallen so far, 1 to kMaxBatch
-    // (kMaxBatch while no rate is known, and with the OpenFOAM test). A poll costs one
+    // reach the tolerance at the rate its residual estimate has fallen so far (a slot without
+    // iterations yet takes the rate of the last converged slot of this run), 1 to kMaxBatch
+    // (kMaxUnknown while no rate is known, and with the OpenFOAM test). A poll costs one
     // synchronization; every iteration launched past a slot's convergence costs the launches of an
     // iteration that find it finished (a restart cycle after the true-residual check often needs
-    // one or two iterations).
-    constexpr int kMaxBatch = 4;
+    // one or two iterations). The results do not depend on the batches.
+    constexpr int kMaxBatch = 8, kMaxUnknown = 4;
     auto itersNeeded = [&](double rel, double rate) {
-        if (of || !(rel > 0.0) || !(rate > 0.0 && rate < 1.0)) return kMaxBatch;
+        if (of || !(rel > 0.0) || !(rate > 0.0 && rate < 1.0)) return kMaxUnknown;
         if (rel <= o.rtol) return 1;
         const double k = std::ceil(std::log(o.rtol / rel) / std::log(rate));
         return static_cast<int>(std::min<double>(kMaxBatch, std::max(1.0, k)));
@@ -1097,7 +1099,7 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
     for (int q = 0; q < cap; ++q)
         if (act[static_cast<size_t>(q)]) {
             const Slot& S = sl[static_cast<size_t>(q)];
-            batch = std::max(batch, itersNeeded(S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0, S.rate));
+            batch = std::max(batch, itersNeeded(S.bnorm > 0.0 ? S.beta / S.bnorm : 0.0, S.rate > 0.0 ? S.rate : rate_hint));
         }
     // one right-hand side whose rows fit one per thread of the GMRES grid: the SpMV of the Arnoldi
     // step and the first CGS2 pass in one launch (the same bits as the two launches)
@@ -1185,6 +1187,7 @@ bool GmresRun::cycle(const double* d_b, double* d_x, hipStream_t s)
         if (cv && !within_slack) cv = false;
         if (b2 == 0.0) cv = true;
         S.conv = cv;
+        if (cv && S.rate > 0.0 && S.rate < 1.0) rate_hint = S.rate;
         S.breakdown = S.breakdown || G.breakdown;
         // a non-finite residual (the Arnoldi estimate or the true one) ends the solve unconverged
         // instead of restarting until maxit, which is what the reference's loop would do
