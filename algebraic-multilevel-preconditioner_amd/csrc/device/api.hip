@@ -25,6 +25,9 @@
 
 namespace s2l {
 template <typename T, typename TF> const void* localSolveKernel(int threads);
+// two right-hand sides per workgroup: a float pair (FP32 arithmetic) or a double pair (FP64
+// arithmetic of FP32 entries) per row
+template <typename T> constexpr bool kIsPair = std::is_same_v<T, float2> || std::is_same_v<T, double2>;
 }
 
 namespace {
@@ -244,7 +247,7 @@ struct s2l_ctx {
     int32_t st_cap = 0;  // supernode descriptors of the staging area (the most of a chunk)
     size_t slot_off = 0, slot_off2 = 0;  // LDS slots of a task without child tasks (one rhs / pair)
     int32_t slot_cap = 0, slot_cap2 = 0;
-    // FP32 factors, two right-hand sides per workgroup (a float2 vector in LDS): its LDS layout
+    // two right-hand sides per workgroup (a float2 or double2 vector in LDS): its LDS layout
     size_t y_off2 = 0, st_off2 = 0, step_off2 = 0, lds_bytes2 = 0;
     bool pair_ok = false;
     DevBuf<float> diag_f, below_f;
@@ -464,7 +467,7 @@ LocalArgs<T, TF> localArgs(s2l_ctx* c, const double* rhs, const double* x0, doub
     a.sub = c->sub_d.p;
     a.lev_wpre = c->levw_d.p;
     a.lev_hpre = c->levh_d.p;
-    const bool pair = std::is_same_v<T, float2>;
+    const bool pair = kIsPair<T>;
     a.y_off = static_cast<int32_t>(pair ? c->y_off2 : c->y_off);
     a.st_off = static_cast<int32_t>(pair ? c->st_off2 : c->st_off);
     a.step_off = static_cast<int32_t>(pair ? c->step_off2 : c->step_off);
@@ -517,7 +520,7 @@ void launchQueue(s2l_ctx* c, LocalArgs<T, TF>& a, const int32_t* q, int32_t coun
     a.queue = q;
     a.qcount = count;
     a.serial = serial ? 1 : 0;
-    const bool pair = std::is_same_v<T, float2>;  // a workgroup per entry and pair of right-hand sides
+    const bool pair = kIsPair<T>;  // a workgroup per entry and pair of right-hand sides
     const int grid = serial ? 1 : count * (pair ? (a.nrhs + 1) / 2 : a.nrhs);
     launchLocalSolve<T, TF>(a, grid, pair ? c->lds_bytes2 : c->lds_bytes, c->local_threads, s);
 }
@@ -623,7 +626,7 @@ void localSolve(s2l_ctx* c, const double* rhs, const double* x0, double* out, in
                 a.queue = q;
                 a.qcount = count;
                 a.serial = 0;
-                const bool pair = std::is_same_v<T, float2>;
+                const bool pair = kIsPair<T>;
                 launchDenseTop<T, TF>(a, count * (pair ? (a.nrhs + 1) / 2 : a.nrhs), c->dtrow_d.p, c->dtce_d.p, c->dtb_d.p,
                                       static_cast<size_t>(c->dt_maxS) * sizeof(T), c->cus, s);
             } else {
@@ -709,6 +712,8 @@ void oneLevel(s2l_ctx* c, const double* rhs, const double* x0, double* z, int nr
     double* out = direct ? z : c->work.loc;
     if (c->precision == 1 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
         localSolve<float2, float>(c, rhs, x0, out, mode, nrhs, skip, s);
+    else if (c->precision == 2 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
+        localSolve<double2, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else if (c->precision == 1)
         localSolve<float, float>(c, rhs, x0, out, mode, nrhs, skip, s);
     else if (c->precision == 2)
@@ -1601,14 +1606,18 @@ void finalizeLocal(s2l_ctx* c, int precision)
         return off + static_cast<size_t>(cap) * elem;
     };
     if (c->queue_ok) c->lds_bytes = slotRoom(c->lds_bytes, esz, c->slot_off, c->slot_cap);
-    c->y_off2 = (static_cast<size_t>(P.max_n) * sizeof(float2) + 15) / 16 * 16;
-    c->st_off2 = c->y_off2 + (yRows * sizeof(float2) + 15) / 16 * 16;
+    // pairs of right-hand sides: a float pair per row (FP32), a double pair (FP64 arithmetic of FP32 entries)
+    const size_t psz = precision == 2 ? sizeof(double2) : sizeof(float2);
+    c->y_off2 = (static_cast<size_t>(P.max_n) * psz + 15) / 16 * 16;
+    c->st_off2 = c->y_off2 + (yRows * psz + 15) / 16 * 16;
     c->step_off2 = (c->st_off2 + stBytes + 15) / 16 * 16;
     c->lds_bytes2 = c->step_off2 + static_cast<size_t>(kStepBlock) * sizeof(StepDesc);
-    c->pair_ok = precision == 1 && c->queue_ok && c->lds_bytes2 <= lds_max;
-    if (c->pair_ok) c->lds_bytes2 = slotRoom(c->lds_bytes2, sizeof(float2), c->slot_off2, c->slot_cap2);
+    c->pair_ok = (precision == 1 || precision == 2) && c->queue_ok && c->lds_bytes2 <= lds_max;
+    if (c->pair_ok) c->lds_bytes2 = slotRoom(c->lds_bytes2, psz, c->slot_off2, c->slot_cap2);
     if (c->pair_ok)
-        HIPCHECK(hipFuncSetAttribute(localSolveKernel<float2, float>(c->local_threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+        HIPCHECK(hipFuncSetAttribute(precision == 2 ? localSolveKernel<double2, float>(c->local_threads)
+                                                    : localSolveKernel<float2, float>(c->local_threads),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(c->lds_bytes2)));
     if (c->queue_ok) {
         const void* kern = precision == 1   ? localSolveKernel<float, float>(c->local_threads)
@@ -2914,6 +2923,8 @@ int s2l_autotune_local(s2l_ctx* c, int32_t nrhs, int32_t applies, s2l_tune_entry
             for (int k = 0; k < cnt; ++k) {
                 if (precision == 1 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
                     localSolve<float2, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
+                else if (precision == 2 && nrhs >= 2 && c->pair_ok && c->launch_mode == 0)
+                    localSolve<double2, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
                 else if (precision == 1)
                     localSolve<float, float>(c, in.p, nullptr, out.p, 0, nrhs, nullptr, s);
                 else if (precision == 2)

@@ -19,17 +19,40 @@ constexpr int kWave = 64;
 constexpr int kPullSplit = 4;  // a backward chunk of at most blockDim / kPullSplit columns splits its rows below
 constexpr int kMaxGroups = 32; // row groups of such a split
 
-// Lane exchange of a value; a pair of floats (two right-hand sides solved together) moves as two.
+// A pair of right-hand sides solved together: T = float2 (FP32 arithmetic) or double2 (FP64
+// arithmetic of FP32 entries) holds right-hand sides 2 r and 2 r + 1 in its two components.
+template <typename T> struct PairOf {
+    static constexpr bool value = false;
+};
+template <> struct PairOf<float2> {
+    static constexpr bool value = true;
+    using E = float;
+};
+template <> struct PairOf<double2> {
+    static constexpr bool value = true;
+    using E = double;
+};
+
+// Lane exchange of a value; a pair moves as two.
 template <typename T> __device__ inline T shflXor(T v, int m) { return __shfl_xor(v, m, kWave); }
 template <> __device__ inline float2 shflXor<float2>(float2 v, int m)
 {
     return float2(__shfl_xor(v.x, m, kWave), __shfl_xor(v.y, m, kWave));
+}
+template <> __device__ inline double2 shflXor<double2>(double2 v, int m)
+{
+    return double2(__shfl_xor(v.x, m, kWave), __shfl_xor(v.y, m, kWave));
 }
 
 // A factor entry in the arithmetic type; for a pair of right-hand sides the same entry multiplies
 // both (float2's one-argument constructor would leave the second component zero).
 template <typename T, typename P> __device__ inline T widen(P v) { return static_cast<T>(v); }
 template <> __device__ inline float2 widen<float2, float>(float v) { return float2(v, v); }
+template <> __device__ inline double2 widen<double2, float>(float v)
+{
+    const double d = static_cast<double>(v);
+    return double2(d, d);
+}
 
 template <typename T> __device__ inline T waveSum(T v)
 {
@@ -233,7 +256,7 @@ __device__ __forceinline__ void denseRows(const TF* M, int ld, int rows, int col
     // entries (TF) are converted to the arithmetic type T. NB blocks of 4 * 64 columns have their
     // loads issued together (a caller with the registers of one workgroup per compute unit); the
     // products are summed in column order whatever NB is.
-    constexpr int R = sizeof(T) == 8 ? 2 : blockRows<TF>();
+    constexpr int R = sizeof(T) >= 8 ? 2 : blockRows<TF>();
     constexpr int P = R == 2 ? 2 : 1;
     for (int r0 = wave * R; r0 < rows; r0 += P * nw * R) {
         const int rA = r0, rB = r0 + nw * R;
@@ -530,8 +553,8 @@ __device__ __forceinline__ void runSteps(const LocalArgs<T, TF>& a, const SubDes
 template <typename T, typename TF>
 __device__ __forceinline__ void runEntry(const LocalArgs<T, TF>& a, long long e, T* x, unsigned char* smem, int tid, int lane, int wave, int nt)
 {
-    // kPair: T = float2 holds right-hand sides 2 r and 2 r + 1 (the second absent when nrhs is odd)
-    constexpr bool kPair = std::is_same_v<T, float2>;
+    // kPair: T holds right-hand sides 2 r and 2 r + 1 (the second absent when nrhs is odd)
+    constexpr bool kPair = PairOf<T>::value;
     const int r = static_cast<int>(e / a.qcount);
     const int rA = kPair ? 2 * r : r, rB = rA + 1;
     const bool onA = !(a.skip && a.skip[rA]);
@@ -560,9 +583,10 @@ __device__ __forceinline__ void runEntry(const LocalArgs<T, TF>& a, long long e,
         for (int q = tid; q < td.rows_count; q += blockDim.x) {
             const int row = rows[q];
             const int g = a.local_in ? sd.map_off + a.lperm[sd.map_off + row] : a.gmap[sd.map_off + row];
-            if constexpr (kPair)
-                x[row] = float2(static_cast<float>(rhs[g]), onB ? static_cast<float>(rhs[ld + g]) : 0.0f);
-            else
+            if constexpr (kPair) {
+                using E = typename PairOf<T>::E;
+                x[row] = T(static_cast<E>(rhs[g]), onB ? static_cast<E>(rhs[ld + g]) : E(0));
+            } else
                 x[row] = static_cast<T>(rhs[g]);
         }
         __syncthreads();
@@ -650,7 +674,7 @@ __global__ __launch_bounds__(kDtThreads, kNB == 1 ? 2 : 1) void k_dense_top(Loca
 {
     extern __shared__ unsigned char smem_raw[];
     T* bS = reinterpret_cast<T*>(smem_raw);
-    constexpr bool kPair = std::is_same_v<T, float2>;
+    constexpr bool kPair = PairOf<T>::value;
     const long long e = blockIdx.x;
     const int r = static_cast<int>(e / a.qcount);
     const int rA = kPair ? 2 * r : r, rB = rA + 1;
@@ -674,9 +698,10 @@ __global__ __launch_bounds__(kDtThreads, kNB == 1 ? 2 : 1) void k_dense_top(Loca
         const int4 w = rowq[q];
         const int g = a.local_in ? sd.map_off + w.z : w.y;
         T v;
-        if constexpr (kPair)
-            v = float2(static_cast<float>(rhs[g]), onB ? static_cast<float>(rhs[ldr + g]) : 0.0f);
-        else
+        if constexpr (kPair) {
+            using E = typename PairOf<T>::E;
+            v = T(static_cast<E>(rhs[g]), onB ? static_cast<E>(rhs[ldr + g]) : E(0));
+        } else
             v = static_cast<T>(rhs[g]);
         bS[q] = pullSlots(tb, v, w.w, ceq[q]);
     }
@@ -1858,6 +1883,7 @@ template void launchLocalSolve<float, float>(const LocalArgs<float, float>&, int
 template void launchLocalSolve<double, double>(const LocalArgs<double, double>&, int, size_t, int, hipStream_t);
 template void launchLocalSolve<double, float>(const LocalArgs<double, float>&, int, size_t, int, hipStream_t);
 template void launchLocalSolve<float2, float>(const LocalArgs<float2, float>&, int, size_t, int, hipStream_t);
+template void launchLocalSolve<double2, float>(const LocalArgs<double2, float>&, int, size_t, int, hipStream_t);
 
 template <typename T, typename TF> const void* localSolveKernel(int threads)
 {
@@ -1868,13 +1894,15 @@ template const void* localSolveKernel<float, float>(int);
 template const void* localSolveKernel<double, double>(int);
 template const void* localSolveKernel<double, float>(int);
 template const void* localSolveKernel<float2, float>(int);
+template const void* localSolveKernel<double2, float>(int);
 
 template <typename T, typename TF>
 void launchDenseTop(const LocalArgs<T, TF>& a, int grid, const int4* dt, const int32_t* dt_ce, const int32_t* dtb, size_t lds,
                     int cus, hipStream_t s)
 {
-    // FP64 entries: two blocks of columns per batch (three need more than the 128 registers)
-    constexpr int kWideNB = sizeof(TF) == 8 ? 2 : 3;
+    // FP64 entries or pairs of FP64 right-hand sides: two blocks of columns per batch (three need
+    // more than the 128 registers)
+    constexpr int kWideNB = (sizeof(TF) == 8 || sizeof(T) == 16) ? 2 : 3;
     if (grid <= cus)
         hipLaunchKernelGGL((k_dense_top<T, TF, kWideNB>), dim3(grid), dim3(kDtThreads), lds, s, a, dt, dt_ce, dtb);
     else
@@ -1888,6 +1916,8 @@ template void launchDenseTop<double, float>(const LocalArgs<double, float>&, int
                                             size_t, int, hipStream_t);
 template void launchDenseTop<float2, float>(const LocalArgs<float2, float>&, int, const int4*, const int32_t*, const int32_t*,
                                             size_t, int, hipStream_t);
+template void launchDenseTop<double2, float>(const LocalArgs<double2, float>&, int, const int4*, const int32_t*, const int32_t*,
+                                             size_t, int, hipStream_t);
 
 void launchSpmv(int32_t n, const int32_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                 const double* b, int nrhs, const int32_t* skip, hipStream_t s)
