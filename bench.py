@@ -484,6 +484,10 @@ def main():
     for k in range(nsteps):
         dB[k].upload(B[k * nrhs:(k + 1) * nrhs].ravel())
     zero = np.zeros(n * nrhs)
+    # the initial guess x0 = 0 comes from a resident buffer too: a device copy on the solving
+    # context's stream, not an upload from the host, inside the timed region
+    dZero = ctx.alloc(n * nrhs)
+    dZero.upload(zero)
 
     class Batch:
         """Results of one step: mean iterations, all converged."""
@@ -492,7 +496,7 @@ def main():
             self.converged = all(bool(r.converged) for r in rs)
 
     def run(c, k, flexible=False):
-        dX[k].upload(zero)
+        c.copy_device(dZero, dX[k])
         t = time.perf_counter()
         res = c.gmres_device(dB[k], dX[k], nrhs, restart, args.rtol, 2000, flexible=flexible)
         return time.perf_counter() - t, Batch(res)
